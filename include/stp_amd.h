@@ -1,0 +1,281 @@
+/*
+ * stp_amd.h -- C ABI of the MI355X-native SuperTerrain+ heightfield hot path.
+ *
+ * The reference exposes this path only as C++ classes (STPHeightfieldGenerator,
+ * STPDiversityGenerator, STPSingleHistogramFilter, STPPermutationGenerator, ...). This header is
+ * the FFI boundary a harness binds with ctypes/cgo/JNI: plain pointers and sizes, `int` status
+ * codes, a thread-local last-error string, no exceptions across the boundary. Every entry
+ * point cites the reference interface it replaces (paths relative to the reference checkout).
+ *
+ * Conventions
+ *  - Functions return STP_OK (0) or an STP_ERROR_* code; stp_last_error() describes the last
+ *    failure on the calling thread.
+ *  - `d_` pointers are device (HBM) pointers, `h_` pointers are host pointers.
+ *  - `stream` is a hipStream_t passed as void*; NULL is the default stream. Device entry points
+ *    only enqueue work unless documented as synchronous.
+ *  - Pixel formats follow STPWorldMapPixelFormat.hpp:12-21: biome sample uint16, height float,
+ *    fixed-point height uint16, seed uint64.
+ */
+#ifndef STP_AMD_H
+#define STP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum {
+	STP_OK = 0,
+	STP_ERROR_INVALID_ENVIRONMENT = 1, /* STPInvalidEnvironment (STPEnvironment.cpp:17-65) */
+	STP_ERROR_NUMERIC_DOMAIN = 2,      /* STPNumericDomainError (STPSingleHistogramFilter.cpp:874-880) */
+	STP_ERROR_HIP = 3,                 /* STPCUDAError (STPDeviceErrorHandler.hpp:22-58), HIP edition */
+	STP_ERROR_CAPACITY = 4,            /* a histogram needs more bins than the configured capacity */
+	STP_ERROR_COMPILATION = 5,         /* runtime compilation failed (STPDeviceRuntimeBinary.cpp:80-104) */
+	STP_ERROR_INVALID_ARGUMENT = 6,
+	STP_ERROR_INTERNAL = 7
+};
+
+/* Description of the last error raised on this thread ("" if none). */
+const char* stp_last_error(void);
+
+/* STPEngineInitialiser::initialise (SuperTerrain+/SuperTerrain+/Private/STPEngineInitialiser.cpp:37-58):
+ * selects the device and creates its context. */
+int stp_engine_initialise(int device);
+
+/* ------------------------------------------------------------------ device memory plumbing
+ * For callers without their own allocator (the reference wraps these in STPSmartDeviceMemory,
+ * SuperTerrain+/SuperTerrain+/Public/SuperTerrain+/Utility/Memory/STPSmartDeviceMemory.inl:47-68). */
+int stp_device_malloc(void** d_ptr, size_t bytes);
+int stp_device_free(void* d_ptr);
+int stp_host_malloc_pinned(void** h_ptr, size_t bytes);
+int stp_host_free_pinned(void* h_ptr);
+/* Copy of `bytes` bytes in any direction (unified addressing). Asynchronous on `stream`. */
+int stp_memcpy_async(void* dst, const void* src, size_t bytes, void* stream);
+int stp_stream_create(void** stream);
+int stp_stream_destroy(void* stream);
+int stp_stream_synchronize(void* stream);
+
+/* ------------------------------------------------------------------ settings */
+/* STPSimplexNoiseSetting (SuperTerrain+/SuperAlgorithm+/Host/Public/SuperAlgorithm+Host/STPSimplexNoiseSetting.h:14-27)
+ * + Engine: 0 = std::mt19937_64 (reference), 1 = std::mt19937. */
+typedef struct {
+	uint64_t Seed;
+	uint32_t Distribution;
+	double Offset;
+	uint32_t Engine;
+} stp_simplex_noise_setting;
+
+/* STPRainDropSetting (SuperTerrain+/SuperTerrain+/Public/SuperTerrain+/Environment/STPRainDropSetting.h:15-53)
+ * + MaxLifetime: steps per drop, 0 = unlimited (reference behaviour). */
+typedef struct {
+	uint64_t Seed;
+	uint32_t RainDropCount;
+	float Inertia;
+	float SedimentCapacityFactor;
+	float minSedimentCapacity;
+	float initWaterVolume;
+	float minWaterVolume;
+	float Friction;
+	float initSpeed;
+	float ErodeSpeed;
+	float DepositSpeed;
+	float EvaporateSpeed;
+	float Gravity;
+	uint32_t ErosionBrushRadius;
+	uint32_t MaxLifetime;
+} stp_raindrop_setting;
+
+/* STPChunkSetting (SuperTerrain+/SuperTerrain+/Public/SuperTerrain+/Environment/STPChunkSetting.h:15-45). */
+typedef struct {
+	uint32_t ChunkSize[2];
+	uint32_t MapSize[2];
+	double ChunkOffset[3];
+	double ChunkScale[2];
+	double MapOffset[2];
+	uint32_t DiversityNearestNeighbour[2];
+	uint32_t ErosionNearestNeighbour[2];
+	uint32_t SplatNearestNeighbour[2];
+	uint32_t RenderDistance[2];
+} stp_chunk_setting;
+
+/* STPNearestNeighbourInformation (SuperTerrain+/SuperTerrain+/Public/SuperTerrain+/World/Chunk/STPNearestNeighbourInformation.hpp:13-25). */
+typedef struct {
+	uint32_t MapSize[2];
+	uint32_t ChunkNearestNeighbour[2];
+	uint32_t TotalMapSize[2];
+} stp_nn_information;
+
+/* STPSimplexNoise::STPFractalSimplexInformation (SuperTerrain+/SuperAlgorithm+/Device/Public/SuperAlgorithm+Device/STPSimplexNoise.cuh:29-52). */
+typedef struct {
+	float Persistence;
+	float Lacunarity;
+	uint32_t Octave;
+	float Scale;
+	float Offset[2];
+	float HalfDimension[2];
+	float Amplitude;
+	float Frequency;
+} stp_fractal_simplex_information;
+
+/* STPDemo::STPBiomeProperty (SuperDemo+/World/Biomes/STPBiomeProperty.hpp:10-27). */
+typedef struct {
+	float Scale;
+	uint32_t Octave;
+	float Persistence;
+	float Lacunarity;
+	float Depth;
+	float Variation;
+} stp_biome_property;
+
+/* STPPermutation (SuperTerrain+/SuperAlgorithm+/Host/Public/SuperAlgorithm+Host/STPPermutation.hpp:9-22): device view. */
+typedef struct {
+	const uint8_t* Permutation;
+	const float* Gradient2D;
+	uint32_t Gradient2DSize;
+} stp_permutation;
+
+/* STPSingleHistogram (SuperTerrain+/SuperAlgorithm+/Host/Public/SuperAlgorithm+Host/STPSingleHistogram.hpp:15-50).
+ * Bin is an array of 8-byte records {uint16 Item; 2 pad bytes; float Weight}. */
+typedef struct {
+	const void* Bin;
+	const uint32_t* HistogramStartOffset;
+} stp_single_histogram;
+
+/* STP*Setting::validate() (SuperTerrain+/SuperTerrain+/Private/STPEnvironment.cpp:17-65,
+ * SuperTerrain+/SuperAlgorithm+/Host/Private/STPAlgorithmEnvironment.cpp:13-17). The rain drop
+ * check also rejects settings whose drops could live for more than 65535 steps. */
+int stp_simplex_noise_setting_validate(const stp_simplex_noise_setting* setting);
+int stp_raindrop_setting_validate(const stp_raindrop_setting* setting);
+int stp_chunk_setting_validate(const stp_chunk_setting* setting);
+
+/* ------------------------------------------------------------------ host-side tables */
+/* STPPermutationGenerator::generate (SuperTerrain+/SuperAlgorithm+/Host/Private/STPPermutationGenerator.cpp:40-94):
+ * h_permutation[512], h_gradient[2 * Distribution] (gradient_capacity floats available). */
+int stp_permutation_generate(const stp_simplex_noise_setting* setting, uint8_t* h_permutation, float* h_gradient,
+	uint32_t gradient_capacity);
+
+/* Same, but the tables are uploaded to device memory owned by the returned handle;
+ * `out_view` receives the device view (STPPermutationGenerator::STPPermutationTable,
+ * SuperTerrain+/SuperAlgorithm+/Host/Public/SuperAlgorithm+Host/STPPermutationGenerator.h:23-43). */
+typedef struct stp_permutation_table stp_permutation_table;
+int stp_permutation_table_create(const stp_simplex_noise_setting* setting, stp_permutation_table** out,
+	stp_permutation* out_view);
+int stp_permutation_table_destroy(stp_permutation_table* table);
+
+/* generateErosionBrush (SuperTerrain+/SuperTerrain+/Private/World/Chunk/STPHeightfieldGenerator.cpp:44-99). */
+int stp_erosion_brush_generate(uint32_t dimension_x, uint32_t radius, int32_t* h_index, float* h_weight,
+	uint32_t capacity, uint32_t* brush_size);
+
+/* ------------------------------------------------------------------ chunk math
+ * STPChunk (SuperTerrain+/SuperTerrain+/Private/World/Chunk/STPChunk.cpp:63-108). */
+int stp_chunk_world_coordinate(const double point[3], const uint32_t chunk_size[2], const double scale[2],
+	int32_t out[2]);
+int stp_chunk_local_coordinate(uint32_t chunk_id, const uint32_t chunk_range[2], uint32_t out[2]);
+int stp_chunk_local_origin(const int32_t centre[2], const uint32_t chunk_size[2], const uint32_t neighbour_size[2],
+	int32_t out[2]);
+int stp_chunk_map_offset(const int32_t chunk[2], const uint32_t chunk_size[2], const uint32_t map_size[2],
+	const double map_offset[2], double out[2]);
+int stp_chunk_offset(const int32_t chunk[2], const uint32_t chunk_size[2], const int32_t offset[2], int32_t out[2]);
+/* out receives region_size[0] * region_size[1] (x, y) pairs. */
+int stp_chunk_neighbour_offset(const uint32_t chunk_size[2], const uint32_t region_size[2], int32_t* out,
+	uint32_t capacity);
+
+/* ------------------------------------------------------------------ device kernels (asynchronous) */
+/* Single-biome heightmap: h = simplex2DFractal(x, y) * variation + depth, row-major width x height
+ * (STPSimplexNoise.cu:84-110 via SuperDemo+/Script/STPMultiHeightGenerator.cu:57-71). */
+int stp_simplex_heightmap(const stp_permutation* perm, const stp_fractal_simplex_information* info, float variation,
+	float depth, float* d_height, uint32_t width, uint32_t height, void* stream);
+
+/* generateMultiBiomeHeightmap (SuperDemo+/Script/STPMultiHeightGenerator.cu:35-55): blends the biomes
+ * of `histogram` (device, jagged) with the host biome table (<= 64 entries). */
+int stp_multi_biome_heightmap(const stp_permutation* perm, const stp_biome_property* h_biome_table,
+	uint32_t biome_count, const stp_single_histogram* histogram, float* d_height, uint32_t width, uint32_t height,
+	float offset_x, float offset_y, void* stream);
+
+/* formatHeightmap (SuperTerrain+/SuperTerrain+/Private/GPGPU/STPHeightfieldKernel.cu:51-57,112-122). */
+int stp_format_heightmap(const float* d_height, uint16_t* d_fixed, uint32_t width, uint32_t height, void* stream);
+
+/* hydraulicErosion (SuperTerrain+/SuperTerrain+/Private/GPGPU/STPHeightfieldKernel.cu:37-49,80-108) on a merged
+ * free-slip map. mode 0 = parallel (FP32 atomics), 1 = serialized (bit-exact). stream_id is the
+ * Philox counter word that distinguishes repeated erosions with one seed. d_ledger (optional,
+ * RainDropCount floats) receives each drop's final carried sediment; d_drop_info (optional,
+ * 2 x uint32 per drop) receives {steps | erode_steps << 16, deposit_steps | killed << 16}. */
+int stp_hydraulic_erosion(float* d_height, const stp_nn_information* nn, const stp_raindrop_setting* setting,
+	const int32_t* d_brush_index, const float* d_brush_weight, uint32_t brush_size, uint32_t stream_id, int mode,
+	float* d_ledger, uint32_t* d_drop_info, void* stream);
+
+/* STPNearestNeighbourTextureBuffer merge / unmerge (SuperTerrain+/SuperTerrain+/Private/World/Chunk/
+ * STPNearestNeighbourTextureBuffer.cpp:69-113) as device gather / scatter. h_chunk_ptrs is a host
+ * array of NN.x * NN.y device pointers in row-major neighbour order; element_size is 1, 2 or 4. */
+int stp_nn_merge(const void* const* h_chunk_ptrs, void* d_merged, uint32_t element_size, const stp_nn_information* nn,
+	void* stream);
+int stp_nn_unmerge(const void* d_merged, void* const* h_chunk_ptrs, uint32_t element_size,
+	const stp_nn_information* nn, void* stream);
+
+/* Synthetic Voronoi biome map (no reference counterpart; input generator for the multi-biome
+ * workloads): nearest of one jittered site per cell_size cell, biome = hash % biome_count. */
+int stp_voronoi_biomemap(uint64_t seed, uint32_t biome_count, uint32_t cell_size, int64_t origin_x, int64_t origin_y,
+	uint16_t* d_out, uint32_t width, uint32_t height, uint32_t pitch, void* stream);
+
+/* ------------------------------------------------------------------ single histogram filter
+ * STPSingleHistogramFilter + STPFilterBuffer (SuperTerrain+/SuperAlgorithm+/Host/Public/SuperAlgorithm+Host/
+ * STPSingleHistogramFilter.h:27-208). The buffer owns the device workspace and output. */
+typedef struct stp_filter_buffer stp_filter_buffer;
+/* bin_capacity (1..64): most distinct items one pixel's histogram may hold. */
+int stp_filter_buffer_create(uint32_t bin_capacity, stp_filter_buffer** out);
+int stp_filter_buffer_destroy(stp_filter_buffer* buffer);
+/* STPSingleHistogramFilter::operator() (STPSingleHistogramFilter.cpp:870-902) on a DEVICE samplemap laid
+ * out as the merged nearest-neighbour map of `nn`. Synchronous on `stream` (it checks the
+ * capacity flag). `out` receives device pointers valid until the next call on `buffer`;
+ * bin_total receives the bin count. */
+int stp_single_histogram_filter(stp_filter_buffer* buffer, const uint16_t* d_samplemap, const stp_nn_information* nn,
+	uint32_t radius, stp_single_histogram* out, uint64_t* bin_total, void* stream);
+/* Same with a HOST samplemap and HOST (pinned) output, the reference's own calling convention. */
+int stp_single_histogram_filter_host(stp_filter_buffer* buffer, const uint16_t* h_samplemap,
+	const stp_nn_information* nn, uint32_t radius, stp_single_histogram* out, uint64_t* bin_total);
+
+/* ------------------------------------------------------------------ heightfield generator
+ * STPHeightfieldGenerator (SuperTerrain+/SuperTerrain+/Public/SuperTerrain+/World/Chunk/STPHeightfieldGenerator.h:26-161)
+ * with a built-in STPDiversityGenerator (STPDiversityGenerator.hpp:14-37). */
+typedef struct stp_diversity_generator stp_diversity_generator;
+typedef struct stp_heightfield_generator stp_heightfield_generator;
+
+/* Single-biome plugin: h = simplex2DFractal * variation + depth over one chunk. */
+int stp_diversity_simplex_create(const stp_simplex_noise_setting* noise, const stp_biome_property* biome,
+	const uint32_t map_size[2], stp_diversity_generator** out);
+/* Multi-biome plugin (STPBiomefieldGenerator, SuperDemo+/World/Biomes/STPBiomefieldGenerator.cpp:79-152):
+ * device single histogram filter of `radius` over the diversity neighbourhood, then the blend. */
+int stp_diversity_biomefield_create(const stp_simplex_noise_setting* noise, const stp_biome_property* h_table,
+	uint32_t biome_count, const uint32_t map_size[2], uint32_t radius, uint32_t bin_capacity,
+	stp_diversity_generator** out);
+int stp_diversity_destroy(stp_diversity_generator* generator);
+
+int stp_heightfield_generator_create(const stp_chunk_setting* chunk, const stp_raindrop_setting* raindrop,
+	stp_diversity_generator* diversity, stp_heightfield_generator** out);
+int stp_heightfield_generator_destroy(stp_heightfield_generator* generator);
+/* 0 = parallel (default, as the reference's release build), 1 = serialized. */
+int stp_heightfield_generator_set_erosion_mode(stp_heightfield_generator* generator, int mode);
+
+/* STPHeightfieldGenerator::generate (STPHeightfieldGenerator.h:145, .cpp:158-169): HOST heightfield of one
+ * chunk, HOST biomemaps of the diversity neighbourhood (row-major neighbour order). Synchronous. */
+int stp_heightfield_generate(stp_heightfield_generator* generator, float* h_heightfield,
+	const uint16_t* const* h_biomemap, float offset_x, float offset_y);
+/* STPHeightfieldGenerator::erode (STPHeightfieldGenerator.h:159, .cpp:171-197): HOST maps of the erosion
+ * neighbourhood, eroded in place, plus their UINT16 copies. Synchronous. */
+int stp_heightfield_erode(stp_heightfield_generator* generator, float* const* h_heightfield,
+	uint16_t* const* h_fixed);
+/* Device-resident variants: every pointer is a DEVICE pointer (the pointer arrays are host
+ * arrays). Asynchronous on `stream`; the call returns once the work is enqueued. */
+int stp_heightfield_generate_device(stp_heightfield_generator* generator, float* d_heightfield,
+	const uint16_t* const* d_biomemap, float offset_x, float offset_y, void* stream);
+int stp_heightfield_erode_device(stp_heightfield_generator* generator, float* const* d_heightfield,
+	uint16_t* const* d_fixed, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STP_AMD_H */

@@ -1,0 +1,272 @@
+"""ctypes binding of the C ABI in ``include/stp_amd.h``.
+
+This is the harness-side binding a Python caller (the tests, ``bench.py``) uses; it is also the
+reference shape for a cgo/JNI/N-API binding. It loads the in-tree
+``libstp_amd.so`` and raises if it is missing: there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libstp_amd.so")
+
+STATUS = {0: "STP_OK", 1: "STP_ERROR_INVALID_ENVIRONMENT", 2: "STP_ERROR_NUMERIC_DOMAIN",
+          3: "STP_ERROR_HIP", 4: "STP_ERROR_CAPACITY", 5: "STP_ERROR_COMPILATION",
+          6: "STP_ERROR_INVALID_ARGUMENT", 7: "STP_ERROR_INTERNAL"}
+
+
+class STPError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"{STATUS.get(code, code)}: {message}")
+        self.code = code
+
+
+class SimplexNoiseSetting(C.Structure):
+    _fields_ = [("Seed", C.c_uint64), ("Distribution", C.c_uint32), ("Offset", C.c_double),
+                ("Engine", C.c_uint32)]
+
+
+class RainDropSetting(C.Structure):
+    _fields_ = [("Seed", C.c_uint64), ("RainDropCount", C.c_uint32), ("Inertia", C.c_float),
+                ("SedimentCapacityFactor", C.c_float), ("minSedimentCapacity", C.c_float),
+                ("initWaterVolume", C.c_float), ("minWaterVolume", C.c_float),
+                ("Friction", C.c_float), ("initSpeed", C.c_float), ("ErodeSpeed", C.c_float),
+                ("DepositSpeed", C.c_float), ("EvaporateSpeed", C.c_float),
+                ("Gravity", C.c_float), ("ErosionBrushRadius", C.c_uint32),
+                ("MaxLifetime", C.c_uint32)]
+
+
+class ChunkSetting(C.Structure):
+    _fields_ = [("ChunkSize", C.c_uint32 * 2), ("MapSize", C.c_uint32 * 2),
+                ("ChunkOffset", C.c_double * 3), ("ChunkScale", C.c_double * 2),
+                ("MapOffset", C.c_double * 2), ("DiversityNearestNeighbour", C.c_uint32 * 2),
+                ("ErosionNearestNeighbour", C.c_uint32 * 2), ("SplatNearestNeighbour", C.c_uint32 * 2),
+                ("RenderDistance", C.c_uint32 * 2)]
+
+
+class NNInformation(C.Structure):
+    _fields_ = [("MapSize", C.c_uint32 * 2), ("ChunkNearestNeighbour", C.c_uint32 * 2),
+                ("TotalMapSize", C.c_uint32 * 2)]
+
+
+class FractalInformation(C.Structure):
+    _fields_ = [("Persistence", C.c_float), ("Lacunarity", C.c_float), ("Octave", C.c_uint32),
+                ("Scale", C.c_float), ("Offset", C.c_float * 2), ("HalfDimension", C.c_float * 2),
+                ("Amplitude", C.c_float), ("Frequency", C.c_float)]
+
+
+class BiomeProperty(C.Structure):
+    _fields_ = [("Scale", C.c_float), ("Octave", C.c_uint32), ("Persistence", C.c_float),
+                ("Lacunarity", C.c_float), ("Depth", C.c_float), ("Variation", C.c_float)]
+
+
+class Permutation(C.Structure):
+    _fields_ = [("Permutation", C.c_void_p), ("Gradient2D", C.c_void_p),
+                ("Gradient2DSize", C.c_uint32)]
+
+
+class SingleHistogram(C.Structure):
+    _fields_ = [("Bin", C.c_void_p), ("HistogramStartOffset", C.c_void_p)]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_U32 = C.c_uint32
+_U64 = C.c_uint64
+_F = C.c_float
+
+_SIGNATURES = {
+    "stp_last_error": (C.c_char_p, []),
+    "stp_engine_initialise": (_I, [_I]),
+    "stp_device_malloc": (_I, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "stp_device_free": (_I, [_P]),
+    "stp_host_malloc_pinned": (_I, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "stp_host_free_pinned": (_I, [_P]),
+    "stp_memcpy_async": (_I, [_P, _P, C.c_size_t, _P]),
+    "stp_stream_create": (_I, [C.POINTER(C.c_void_p)]),
+    "stp_stream_destroy": (_I, [_P]),
+    "stp_stream_synchronize": (_I, [_P]),
+    "stp_simplex_noise_setting_validate": (_I, [C.POINTER(SimplexNoiseSetting)]),
+    "stp_raindrop_setting_validate": (_I, [C.POINTER(RainDropSetting)]),
+    "stp_chunk_setting_validate": (_I, [C.POINTER(ChunkSetting)]),
+    "stp_permutation_generate": (_I, [C.POINTER(SimplexNoiseSetting), _P, _P, _U32]),
+    "stp_permutation_table_create": (_I, [C.POINTER(SimplexNoiseSetting), C.POINTER(C.c_void_p),
+                                          C.POINTER(Permutation)]),
+    "stp_permutation_table_destroy": (_I, [_P]),
+    "stp_erosion_brush_generate": (_I, [_U32, _U32, _P, _P, _U32, C.POINTER(_U32)]),
+    "stp_chunk_world_coordinate": (_I, [_P, _P, _P, _P]),
+    "stp_chunk_local_coordinate": (_I, [_U32, _P, _P]),
+    "stp_chunk_local_origin": (_I, [_P, _P, _P, _P]),
+    "stp_chunk_map_offset": (_I, [_P, _P, _P, _P, _P]),
+    "stp_chunk_offset": (_I, [_P, _P, _P, _P]),
+    "stp_chunk_neighbour_offset": (_I, [_P, _P, _P, _U32]),
+    "stp_simplex_heightmap": (_I, [C.POINTER(Permutation), C.POINTER(FractalInformation), _F, _F,
+                                   _P, _U32, _U32, _P]),
+    "stp_multi_biome_heightmap": (_I, [C.POINTER(Permutation), _P, _U32, C.POINTER(SingleHistogram),
+                                       _P, _U32, _U32, _F, _F, _P]),
+    "stp_format_heightmap": (_I, [_P, _P, _U32, _U32, _P]),
+    "stp_hydraulic_erosion": (_I, [_P, C.POINTER(NNInformation), C.POINTER(RainDropSetting), _P,
+                                   _P, _U32, _U32, _I, _P, _P, _P]),
+    "stp_nn_merge": (_I, [_P, _P, _U32, C.POINTER(NNInformation), _P]),
+    "stp_nn_unmerge": (_I, [_P, _P, _U32, C.POINTER(NNInformation), _P]),
+    "stp_voronoi_biomemap": (_I, [_U64, _U32, _U32, C.c_int64, C.c_int64, _P, _U32, _U32, _U32,
+                                  _P]),
+    "stp_filter_buffer_create": (_I, [_U32, C.POINTER(C.c_void_p)]),
+    "stp_filter_buffer_destroy": (_I, [_P]),
+    "stp_single_histogram_filter": (_I, [_P, _P, C.POINTER(NNInformation), _U32,
+                                         C.POINTER(SingleHistogram), C.POINTER(_U64), _P]),
+    "stp_single_histogram_filter_host": (_I, [_P, _P, C.POINTER(NNInformation), _U32,
+                                              C.POINTER(SingleHistogram), C.POINTER(_U64)]),
+    "stp_diversity_simplex_create": (_I, [C.POINTER(SimplexNoiseSetting), C.POINTER(BiomeProperty),
+                                          _P, C.POINTER(C.c_void_p)]),
+    "stp_diversity_biomefield_create": (_I, [C.POINTER(SimplexNoiseSetting), _P, _U32, _P, _U32,
+                                             _U32, C.POINTER(C.c_void_p)]),
+    "stp_diversity_destroy": (_I, [_P]),
+    "stp_heightfield_generator_create": (_I, [C.POINTER(ChunkSetting), C.POINTER(RainDropSetting),
+                                              _P, C.POINTER(C.c_void_p)]),
+    "stp_heightfield_generator_destroy": (_I, [_P]),
+    "stp_heightfield_generator_set_erosion_mode": (_I, [_P, _I]),
+    "stp_heightfield_generate": (_I, [_P, _P, _P, _F, _F]),
+    "stp_heightfield_erode": (_I, [_P, _P, _P]),
+    "stp_heightfield_generate_device": (_I, [_P, _P, _P, _F, _F, _P]),
+    "stp_heightfield_erode_device": (_I, [_P, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded C ABI. Raises if the HIP library has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library missing: {LIB_PATH}; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise STPError(rc, lib().stp_last_error().decode())
+    return rc
+
+
+def call(name, *args):
+    return check(getattr(lib(), name)(*args))
+
+
+def ptr(a):
+    """Host address of a contiguous numpy array."""
+    assert a.flags.c_contiguous
+    return C.c_void_p(a.ctypes.data)
+
+
+# ---------------------------------------------------------------- device memory
+
+class DeviceArray:
+    """A device allocation owned by the library's allocator (hipMalloc)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        call("stp_device_malloc", C.byref(p), max(self.nbytes, 1))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a, stream)
+        return d
+
+    def upload(self, a, stream=None):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.nbytes == self.nbytes
+        call("stp_memcpy_async", C.c_void_p(self.ptr), ptr(a), self.nbytes, stream)
+        call("stp_stream_synchronize", stream)
+
+    def download(self, stream=None):
+        out = np.empty(self.shape, self.dtype)
+        call("stp_memcpy_async", ptr(out), C.c_void_p(self.ptr), self.nbytes, stream)
+        call("stp_stream_synchronize", stream)
+        return out
+
+    def offset(self, elements):
+        return C.c_void_p(self.ptr + elements * self.dtype.itemsize)
+
+    @property
+    def vp(self):
+        return C.c_void_p(self.ptr)
+
+    def free(self):
+        if self.ptr:
+            lib().stp_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize(stream=None):
+    call("stp_stream_synchronize", stream)
+
+
+# ---------------------------------------------------------------- helpers
+
+def nn_info(map_size, nn):
+    return NNInformation((C.c_uint32 * 2)(*map_size), (C.c_uint32 * 2)(*nn),
+                         (C.c_uint32 * 2)(map_size[0] * nn[0], map_size[1] * nn[1]))
+
+
+def permutation_host(seed, distribution, offset, engine=0):
+    s = SimplexNoiseSetting(seed, distribution, offset, engine)
+    perm = np.zeros(512, np.uint8)
+    grad = np.zeros(2 * distribution, np.float32)
+    call("stp_permutation_generate", C.byref(s), ptr(perm), ptr(grad), grad.size)
+    return perm, grad
+
+
+class PermutationTable:
+    """Device permutation + gradient tables (STPPermutationGenerator::generate)."""
+
+    def __init__(self, seed, distribution, offset, engine=0):
+        s = SimplexNoiseSetting(seed, distribution, offset, engine)
+        h = C.c_void_p()
+        self.view = Permutation()
+        call("stp_permutation_table_create", C.byref(s), C.byref(h), C.byref(self.view))
+        self.handle = h.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().stp_permutation_table_destroy(C.c_void_p(self.handle))
+        except Exception:
+            pass
+
+
+def brush(dim_x, radius):
+    n = _U32()
+    call("stp_erosion_brush_generate", dim_x, radius, None, None, 0, C.byref(n))
+    idx = np.zeros(n.value, np.int32)
+    w = np.zeros(n.value, np.float32)
+    call("stp_erosion_brush_generate", dim_x, radius, ptr(idx), ptr(w), n.value, C.byref(n))
+    return idx, w
+
+
+def biome_array(props):
+    arr = (BiomeProperty * len(props))()
+    for i, p in enumerate(props):
+        arr[i] = BiomeProperty(*p)
+    return arr

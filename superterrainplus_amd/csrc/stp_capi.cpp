@@ -1,0 +1,558 @@
+// extern "C" boundary (include/stp_amd.h): converts exceptions into status codes plus a
+// thread-local message, and plain C structs into the C++ API types.
+#include "stp_internal.hpp"
+
+#include <stp_amd.h>
+
+#include <cstring>
+#include <string>
+
+using namespace SuperTerrainPlus;
+
+namespace {
+
+	thread_local std::string LastError;
+
+	template <class F>
+	int guarded(F&& f) noexcept {
+		try {
+			f();
+			LastError.clear();
+			return STP_OK;
+		} catch (const STPException::STPFundamentalException& e) {
+			LastError = e.what();
+			return static_cast<int>(e.Status);
+		} catch (const std::bad_alloc& e) {
+			LastError = std::string("out of memory: ") + e.what();
+			return STP_ERROR_INTERNAL;
+		} catch (const std::exception& e) {
+			LastError = e.what();
+			return STP_ERROR_INTERNAL;
+		} catch (...) {
+			LastError = "unknown error";
+			return STP_ERROR_INTERNAL;
+		}
+	}
+
+	void require(const bool cond, const char* what) {
+		if (!cond) {
+			STP_THROW(STPInvalidArgument, what);
+		}
+	}
+
+	STPEnvironment::STPSimplexNoiseSetting toNoise(const stp_simplex_noise_setting* s) {
+		require(s, "null simplex noise setting");
+		STPEnvironment::STPSimplexNoiseSetting n;
+		n.Seed = s->Seed;
+		n.Distribution = s->Distribution;
+		n.Offset = s->Offset;
+		n.Engine = static_cast<STPEnvironment::STPPermutationEngine>(s->Engine);
+		return n;
+	}
+
+	STPEnvironment::STPRainDropSetting toRain(const stp_raindrop_setting* s) {
+		require(s, "null raindrop setting");
+		STPEnvironment::STPRainDropSetting r;
+		r.Seed = s->Seed;
+		r.RainDropCount = s->RainDropCount;
+		r.Inertia = s->Inertia;
+		r.SedimentCapacityFactor = s->SedimentCapacityFactor;
+		r.minSedimentCapacity = s->minSedimentCapacity;
+		r.initWaterVolume = s->initWaterVolume;
+		r.minWaterVolume = s->minWaterVolume;
+		r.Friction = s->Friction;
+		r.initSpeed = s->initSpeed;
+		r.ErodeSpeed = s->ErodeSpeed;
+		r.DepositSpeed = s->DepositSpeed;
+		r.EvaporateSpeed = s->EvaporateSpeed;
+		r.Gravity = s->Gravity;
+		r.ErosionBrushRadius = s->ErosionBrushRadius;
+		r.MaxLifetime = s->MaxLifetime;
+		return r;
+	}
+
+	STPRainDropParameter toParam(const STPEnvironment::STPRainDropSetting& s) {
+		STPRainDropParameter p;
+		p.Seed = s.Seed;
+		p.RainDropCount = s.RainDropCount;
+		p.Inertia = s.Inertia;
+		p.SedimentCapacityFactor = s.SedimentCapacityFactor;
+		p.minSedimentCapacity = s.minSedimentCapacity;
+		p.initWaterVolume = s.initWaterVolume;
+		p.minWaterVolume = s.minWaterVolume;
+		p.Friction = s.Friction;
+		p.initSpeed = s.initSpeed;
+		p.ErodeSpeed = s.ErodeSpeed;
+		p.DepositSpeed = s.DepositSpeed;
+		p.EvaporateSpeed = s.EvaporateSpeed;
+		p.Gravity = s.Gravity;
+		p.ErosionBrushRadius = s.ErosionBrushRadius;
+		p.MaxLifetime = s.MaxLifetime;
+		return p;
+	}
+
+	STPEnvironment::STPChunkSetting toChunk(const stp_chunk_setting* s) {
+		require(s, "null chunk setting");
+		STPEnvironment::STPChunkSetting c;
+		c.ChunkSize = make_uint2(s->ChunkSize[0], s->ChunkSize[1]);
+		c.MapSize = make_uint2(s->MapSize[0], s->MapSize[1]);
+		c.ChunkOffset = make_double3(s->ChunkOffset[0], s->ChunkOffset[1], s->ChunkOffset[2]);
+		c.ChunkScale = make_double2(s->ChunkScale[0], s->ChunkScale[1]);
+		c.MapOffset = make_double2(s->MapOffset[0], s->MapOffset[1]);
+		c.DiversityNearestNeighbour = make_uint2(s->DiversityNearestNeighbour[0], s->DiversityNearestNeighbour[1]);
+		c.ErosionNearestNeighbour = make_uint2(s->ErosionNearestNeighbour[0], s->ErosionNearestNeighbour[1]);
+		c.SplatNearestNeighbour = make_uint2(s->SplatNearestNeighbour[0], s->SplatNearestNeighbour[1]);
+		c.RenderDistance = make_uint2(s->RenderDistance[0], s->RenderDistance[1]);
+		return c;
+	}
+
+	STPNearestNeighbourInformation toNN(const stp_nn_information* s) {
+		require(s, "null nearest-neighbour information");
+		STPNearestNeighbourInformation n;
+		n.MapSize = make_uint2(s->MapSize[0], s->MapSize[1]);
+		n.ChunkNearestNeighbour = make_uint2(s->ChunkNearestNeighbour[0], s->ChunkNearestNeighbour[1]);
+		n.TotalMapSize = make_uint2(s->TotalMapSize[0], s->TotalMapSize[1]);
+		require(n.TotalMapSize.x == n.MapSize.x * n.ChunkNearestNeighbour.x
+				&& n.TotalMapSize.y == n.MapSize.y * n.ChunkNearestNeighbour.y && n.MapSize.x > 0u && n.MapSize.y > 0u,
+			"inconsistent nearest-neighbour information");
+		return n;
+	}
+
+	STPAlgorithm::STPPermutation toPerm(const stp_permutation* p) {
+		require(p && p->Permutation && p->Gradient2D && p->Gradient2DSize > 0u, "invalid permutation view");
+		return STPAlgorithm::STPPermutation {p->Permutation, p->Gradient2D, p->Gradient2DSize};
+	}
+
+	STPBiomeProperty toBiome(const stp_biome_property& b) {
+		return STPBiomeProperty {b.Scale, b.Octave, b.Persistence, b.Lacunarity, b.Depth, b.Variation};
+	}
+
+	hipStream_t toStream(void* const s) {
+		return static_cast<hipStream_t>(s);
+	}
+
+} // namespace
+
+struct stp_permutation_table {
+	STPAlgorithm::STPPermutationGenerator::STPPermutationTable Table;
+};
+
+struct stp_filter_buffer {
+	STPAlgorithm::STPSingleHistogramFilter Filter;
+	STPAlgorithm::STPSingleHistogramFilter::STPFilterBuffer Buffer;
+	explicit stp_filter_buffer(const unsigned int cap) :
+		Buffer(STPAlgorithm::STPSingleHistogramFilter::STPFilterBuffer::STPExecutionType::Parallel, cap) {}
+};
+
+struct stp_diversity_generator {
+	std::unique_ptr<STPDiversityGenerator> Generator;
+};
+
+struct stp_heightfield_generator {
+	std::unique_ptr<STPHeightfieldGenerator> Generator;
+	STPEnvironment::STPChunkSetting Chunk;
+};
+
+extern "C" {
+
+const char* stp_last_error(void) {
+	return LastError.c_str();
+}
+
+int stp_engine_initialise(const int device) {
+	return guarded([&] {
+		STP_CHECK_HIP(hipSetDevice(device));
+		STP_CHECK_HIP(hipFree(nullptr));
+	});
+}
+
+int stp_device_malloc(void** const d_ptr, const size_t bytes) {
+	return guarded([&] {
+		require(d_ptr, "null output pointer");
+		*d_ptr = STPSmartDeviceMemory::allocateDevice(bytes);
+	});
+}
+
+int stp_device_free(void* const d_ptr) {
+	return guarded([&] { STP_CHECK_HIP(hipFree(d_ptr)); });
+}
+
+int stp_host_malloc_pinned(void** const h_ptr, const size_t bytes) {
+	return guarded([&] {
+		require(h_ptr, "null output pointer");
+		*h_ptr = STPSmartDeviceMemory::allocatePinned(bytes);
+	});
+}
+
+int stp_host_free_pinned(void* const h_ptr) {
+	return guarded([&] { STP_CHECK_HIP(hipHostFree(h_ptr)); });
+}
+
+int stp_memcpy_async(void* const dst, const void* const src, const size_t bytes, void* const stream) {
+	return guarded([&] { STP_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, toStream(stream))); });
+}
+
+int stp_stream_create(void** const stream) {
+	return guarded([&] {
+		require(stream, "null output pointer");
+		hipStream_t s = nullptr;
+		STP_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+		*stream = s;
+	});
+}
+
+int stp_stream_destroy(void* const stream) {
+	return guarded([&] { STP_CHECK_HIP(hipStreamDestroy(toStream(stream))); });
+}
+
+int stp_stream_synchronize(void* const stream) {
+	return guarded([&] { STP_CHECK_HIP(hipStreamSynchronize(toStream(stream))); });
+}
+
+int stp_simplex_noise_setting_validate(const stp_simplex_noise_setting* const setting) {
+	return guarded([&] { toNoise(setting).validate(); });
+}
+
+int stp_raindrop_setting_validate(const stp_raindrop_setting* const setting) {
+	return guarded([&] { toRain(setting).validate(); });
+}
+
+int stp_chunk_setting_validate(const stp_chunk_setting* const setting) {
+	return guarded([&] { toChunk(setting).validate(); });
+}
+
+int stp_permutation_generate(const stp_simplex_noise_setting* const setting, uint8_t* const h_permutation,
+	float* const h_gradient, const uint32_t gradient_capacity) {
+	return guarded([&] {
+		const STPEnvironment::STPSimplexNoiseSetting n = toNoise(setting);
+		n.validate();
+		require(h_permutation && h_gradient, "null output table");
+		require(gradient_capacity >= 2u * n.Distribution, "gradient capacity must hold 2 * Distribution floats");
+		STPAlgorithm::STPPermutationGenerator::generateHost(n, h_permutation, h_gradient);
+	});
+}
+
+int stp_permutation_table_create(const stp_simplex_noise_setting* const setting, stp_permutation_table** const out,
+	stp_permutation* const out_view) {
+	return guarded([&] {
+		require(out && out_view, "null output");
+		auto t = std::make_unique<stp_permutation_table>();
+		t->Table = STPAlgorithm::STPPermutationGenerator::generate(toNoise(setting));
+		out_view->Permutation = t->Table.Permutation.Permutation;
+		out_view->Gradient2D = t->Table.Permutation.Gradient2D;
+		out_view->Gradient2DSize = t->Table.Permutation.Gradient2DSize;
+		*out = t.release();
+	});
+}
+
+int stp_permutation_table_destroy(stp_permutation_table* const table) {
+	return guarded([&] { delete table; });
+}
+
+int stp_erosion_brush_generate(const uint32_t dimension_x, const uint32_t radius, int32_t* const h_index,
+	float* const h_weight, const uint32_t capacity, uint32_t* const brush_size) {
+	return guarded([&] {
+		std::vector<int> idx;
+		std::vector<float> w;
+		STPErosionBrushGenerator::generate(dimension_x, radius, idx, w);
+		if (brush_size) {
+			*brush_size = static_cast<uint32_t>(idx.size());
+		}
+		if (h_index || h_weight) {
+			require(capacity >= idx.size(), "brush capacity too small");
+			if (h_index) {
+				std::memcpy(h_index, idx.data(), idx.size() * sizeof(int));
+			}
+			if (h_weight) {
+				std::memcpy(h_weight, w.data(), w.size() * sizeof(float));
+			}
+		}
+	});
+}
+
+int stp_chunk_world_coordinate(const double point[3], const uint32_t chunk_size[2], const double scale[2], int32_t out[2]) {
+	return guarded([&] {
+		require(point && chunk_size && scale && out, "null argument");
+		const int2 r = STPChunk::calcWorldChunkCoordinate(make_double3(point[0], point[1], point[2]),
+			make_uint2(chunk_size[0], chunk_size[1]), make_double2(scale[0], scale[1]));
+		out[0] = r.x;
+		out[1] = r.y;
+	});
+}
+
+int stp_chunk_local_coordinate(const uint32_t chunk_id, const uint32_t chunk_range[2], uint32_t out[2]) {
+	return guarded([&] {
+		require(chunk_range && out && chunk_range[0] > 0u, "invalid argument");
+		const uint2 r = STPChunk::calcLocalChunkCoordinate(chunk_id, make_uint2(chunk_range[0], chunk_range[1]));
+		out[0] = r.x;
+		out[1] = r.y;
+	});
+}
+
+int stp_chunk_local_origin(const int32_t centre[2], const uint32_t chunk_size[2], const uint32_t neighbour_size[2],
+	int32_t out[2]) {
+	return guarded([&] {
+		require(centre && chunk_size && neighbour_size && out, "null argument");
+		const int2 r = STPChunk::calcLocalChunkOrigin(make_int2(centre[0], centre[1]), make_uint2(chunk_size[0], chunk_size[1]),
+			make_uint2(neighbour_size[0], neighbour_size[1]));
+		out[0] = r.x;
+		out[1] = r.y;
+	});
+}
+
+int stp_chunk_map_offset(const int32_t chunk[2], const uint32_t chunk_size[2], const uint32_t map_size[2],
+	const double map_offset[2], double out[2]) {
+	return guarded([&] {
+		require(chunk && chunk_size && map_size && map_offset && out, "null argument");
+		const double2 r = STPChunk::calcChunkMapOffset(make_int2(chunk[0], chunk[1]), make_uint2(chunk_size[0], chunk_size[1]),
+			make_uint2(map_size[0], map_size[1]), make_double2(map_offset[0], map_offset[1]));
+		out[0] = r.x;
+		out[1] = r.y;
+	});
+}
+
+int stp_chunk_offset(const int32_t chunk[2], const uint32_t chunk_size[2], const int32_t offset[2], int32_t out[2]) {
+	return guarded([&] {
+		require(chunk && chunk_size && offset && out, "null argument");
+		const int2 r = STPChunk::offsetChunk(make_int2(chunk[0], chunk[1]), make_uint2(chunk_size[0], chunk_size[1]),
+			make_int2(offset[0], offset[1]));
+		out[0] = r.x;
+		out[1] = r.y;
+	});
+}
+
+int stp_chunk_neighbour_offset(const uint32_t chunk_size[2], const uint32_t region_size[2], int32_t* const out,
+	const uint32_t capacity) {
+	return guarded([&] {
+		require(chunk_size && region_size && out, "null argument");
+		const std::vector<int2> r = STPChunk::calcChunkNeighbourOffset(make_uint2(chunk_size[0], chunk_size[1]),
+			make_uint2(region_size[0], region_size[1]));
+		require(capacity >= r.size(), "output capacity too small");
+		for (size_t i = 0u; i < r.size(); i++) {
+			out[2u * i] = r[i].x;
+			out[2u * i + 1u] = r[i].y;
+		}
+	});
+}
+
+int stp_simplex_heightmap(const stp_permutation* const perm, const stp_fractal_simplex_information* const info,
+	const float variation, const float depth, float* const d_height, const uint32_t width, const uint32_t height,
+	void* const stream) {
+	return guarded([&] {
+		require(info, "null fractal information");
+		STPAlgorithm::STPSimplexNoise::STPFractalSimplexInformation f;
+		f.Persistence = info->Persistence;
+		f.Lacunarity = info->Lacunarity;
+		f.Octave = info->Octave;
+		f.Scale = info->Scale;
+		f.Offset = make_float2(info->Offset[0], info->Offset[1]);
+		f.HalfDimension = make_float2(info->HalfDimension[0], info->HalfDimension[1]);
+		f.Amplitude = info->Amplitude;
+		f.Frequency = info->Frequency;
+		STPHeightfieldKernel::simplexHeightmap(toPerm(perm), f, variation, depth, d_height, width, height, toStream(stream));
+	});
+}
+
+int stp_multi_biome_heightmap(const stp_permutation* const perm, const stp_biome_property* const h_biome_table,
+	const uint32_t biome_count, const stp_single_histogram* const histogram, float* const d_height, const uint32_t width,
+	const uint32_t height, const float offset_x, const float offset_y, void* const stream) {
+	return guarded([&] {
+		require(h_biome_table && histogram, "null argument");
+		std::vector<STPBiomeProperty> t;
+		for (uint32_t i = 0u; i < biome_count; i++) {
+			t.push_back(toBiome(h_biome_table[i]));
+		}
+		const STPAlgorithm::STPSingleHistogram h {static_cast<const STPAlgorithm::STPSingleHistogram::STPBin*>(histogram->Bin),
+			histogram->HistogramStartOffset};
+		STPHeightfieldKernel::multiBiomeHeightmap(toPerm(perm), t.data(), biome_count, h, d_height,
+			make_uint2(width, height), make_float2(offset_x, offset_y), toStream(stream));
+	});
+}
+
+int stp_format_heightmap(const float* const d_height, uint16_t* const d_fixed, const uint32_t width, const uint32_t height,
+	void* const stream) {
+	return guarded([&] {
+		STPHeightfieldKernel::formatHeightmap(d_height, d_fixed, static_cast<unsigned long long>(width) * height,
+			toStream(stream));
+	});
+}
+
+int stp_hydraulic_erosion(float* const d_height, const stp_nn_information* const nn, const stp_raindrop_setting* const setting,
+	const int32_t* const d_brush_index, const float* const d_brush_weight, const uint32_t brush_size,
+	const uint32_t stream_id, const int mode, float* const d_ledger, uint32_t* const d_drop_info, void* const stream) {
+	return guarded([&] {
+		const STPEnvironment::STPRainDropSetting r = toRain(setting);
+		r.validate();
+		require(mode == 0 || mode == 1, "erosion mode must be 0 (parallel) or 1 (serialized)");
+		const STPNearestNeighbourInformation n = toNN(nn);
+		// the brush must be the one generated for this map width and radius (its extent is what
+		// the in-bounds argument of the kernel relies on)
+		std::vector<int> idx;
+		std::vector<float> w;
+		STPErosionBrushGenerator::generate(n.TotalMapSize.x, r.ErosionBrushRadius, idx, w);
+		require(brush_size == idx.size(), "brush size does not match the brush of this radius and map width");
+		// the kernel's in-bounds argument holds for this brush's taps only: check the indices
+		require(d_brush_index != nullptr && d_brush_weight != nullptr, "null brush");
+		std::vector<int> given(idx.size());
+		STP_CHECK_HIP(hipMemcpyAsync(given.data(), d_brush_index, idx.size() * sizeof(int), hipMemcpyDeviceToHost,
+			toStream(stream)));
+		STP_CHECK_HIP(hipStreamSynchronize(toStream(stream)));
+		require(given == idx, "brush indices differ from the brush of this radius and map width");
+		STPHeightfieldKernel::hydraulicErosion(d_height, n, toParam(r), d_brush_index, d_brush_weight, brush_size,
+			stream_id, static_cast<STPHeightfieldKernel::STPErosionMode>(mode), d_ledger,
+			reinterpret_cast<uint2*>(d_drop_info), toStream(stream));
+	});
+}
+
+int stp_nn_merge(const void* const* const h_chunk_ptrs, void* const d_merged, const uint32_t element_size,
+	const stp_nn_information* const nn, void* const stream) {
+	return guarded([&] {
+		require(h_chunk_ptrs, "null chunk pointer array");
+		STPHeightfieldKernel::mergeNeighbour(h_chunk_ptrs, d_merged, element_size, toNN(nn), toStream(stream));
+	});
+}
+
+int stp_nn_unmerge(const void* const d_merged, void* const* const h_chunk_ptrs, const uint32_t element_size,
+	const stp_nn_information* const nn, void* const stream) {
+	return guarded([&] {
+		require(h_chunk_ptrs, "null chunk pointer array");
+		STPHeightfieldKernel::unmergeNeighbour(d_merged, h_chunk_ptrs, element_size, toNN(nn), toStream(stream));
+	});
+}
+
+int stp_voronoi_biomemap(const uint64_t seed, const uint32_t biome_count, const uint32_t cell_size, const int64_t origin_x,
+	const int64_t origin_y, uint16_t* const d_out, const uint32_t width, const uint32_t height, const uint32_t pitch,
+	void* const stream) {
+	return guarded([&] {
+		STPHeightfieldKernel::voronoiBiomemap(seed, biome_count, cell_size, origin_x, origin_y, d_out, width, height, pitch,
+			toStream(stream));
+	});
+}
+
+int stp_filter_buffer_create(const uint32_t bin_capacity, stp_filter_buffer** const out) {
+	return guarded([&] {
+		require(out, "null output");
+		*out = new stp_filter_buffer(bin_capacity);
+	});
+}
+
+int stp_filter_buffer_destroy(stp_filter_buffer* const buffer) {
+	return guarded([&] { delete buffer; });
+}
+
+int stp_single_histogram_filter(stp_filter_buffer* const buffer, const uint16_t* const d_samplemap,
+	const stp_nn_information* const nn, const uint32_t radius, stp_single_histogram* const out, uint64_t* const bin_total,
+	void* const stream) {
+	return guarded([&] {
+		require(buffer && d_samplemap && out, "null argument");
+		const STPAlgorithm::STPSingleHistogram h = buffer->Filter.filterDevice(d_samplemap, toNN(nn), buffer->Buffer, radius,
+			toStream(stream));
+		out->Bin = h.Bin;
+		out->HistogramStartOffset = h.HistogramStartOffset;
+		if (bin_total) {
+			*bin_total = buffer->Buffer.size().first;
+		}
+	});
+}
+
+int stp_single_histogram_filter_host(stp_filter_buffer* const buffer, const uint16_t* const h_samplemap,
+	const stp_nn_information* const nn, const uint32_t radius, stp_single_histogram* const out, uint64_t* const bin_total) {
+	return guarded([&] {
+		require(buffer && h_samplemap && out, "null argument");
+		const STPAlgorithm::STPSingleHistogram h = buffer->Filter(h_samplemap, toNN(nn), buffer->Buffer, radius);
+		out->Bin = h.Bin;
+		out->HistogramStartOffset = h.HistogramStartOffset;
+		if (bin_total) {
+			*bin_total = buffer->Buffer.size().first;
+		}
+	});
+}
+
+int stp_diversity_simplex_create(const stp_simplex_noise_setting* const noise, const stp_biome_property* const biome,
+	const uint32_t map_size[2], stp_diversity_generator** const out) {
+	return guarded([&] {
+		require(biome && map_size && out, "null argument");
+		auto g = std::make_unique<stp_diversity_generator>();
+		g->Generator = std::make_unique<STPSimplexHeightGenerator>(toNoise(noise), toBiome(*biome),
+			make_uint2(map_size[0], map_size[1]));
+		*out = g.release();
+	});
+}
+
+int stp_diversity_biomefield_create(const stp_simplex_noise_setting* const noise, const stp_biome_property* const h_table,
+	const uint32_t biome_count, const uint32_t map_size[2], const uint32_t radius, const uint32_t bin_capacity,
+	stp_diversity_generator** const out) {
+	return guarded([&] {
+		require(h_table && map_size && out, "null argument");
+		std::vector<STPBiomeProperty> t;
+		for (uint32_t i = 0u; i < biome_count; i++) {
+			t.push_back(toBiome(h_table[i]));
+		}
+		auto g = std::make_unique<stp_diversity_generator>();
+		g->Generator = std::make_unique<STPBiomefieldGenerator>(toNoise(noise), std::move(t),
+			make_uint2(map_size[0], map_size[1]), radius, bin_capacity);
+		*out = g.release();
+	});
+}
+
+int stp_diversity_destroy(stp_diversity_generator* const generator) {
+	return guarded([&] { delete generator; });
+}
+
+int stp_heightfield_generator_create(const stp_chunk_setting* const chunk, const stp_raindrop_setting* const raindrop,
+	stp_diversity_generator* const diversity, stp_heightfield_generator** const out) {
+	return guarded([&] {
+		require(diversity && diversity->Generator && out, "null argument");
+		auto g = std::make_unique<stp_heightfield_generator>();
+		g->Chunk = toChunk(chunk);
+		const STPEnvironment::STPRainDropSetting r = toRain(raindrop);
+		const STPHeightfieldGenerator::STPGeneratorSetup setup {&g->Chunk, &r, diversity->Generator.get()};
+		g->Generator = std::make_unique<STPHeightfieldGenerator>(setup);
+		*out = g.release();
+	});
+}
+
+int stp_heightfield_generator_destroy(stp_heightfield_generator* const generator) {
+	return guarded([&] { delete generator; });
+}
+
+int stp_heightfield_generator_set_erosion_mode(stp_heightfield_generator* const generator, const int mode) {
+	return guarded([&] {
+		require(generator && (mode == 0 || mode == 1), "invalid argument");
+		generator->Generator->setErosionMode(static_cast<STPHeightfieldGenerator::STPErosionMode>(mode));
+	});
+}
+
+int stp_heightfield_generate(stp_heightfield_generator* const generator, float* const h_heightfield,
+	const uint16_t* const* const h_biomemap, const float offset_x, const float offset_y) {
+	return guarded([&] {
+		require(generator && h_heightfield && h_biomemap, "null argument");
+		generator->Generator->generate(h_heightfield, h_biomemap, make_float2(offset_x, offset_y));
+	});
+}
+
+int stp_heightfield_erode(stp_heightfield_generator* const generator, float* const* const h_heightfield,
+	uint16_t* const* const h_fixed) {
+	return guarded([&] {
+		require(generator && h_heightfield && h_fixed, "null argument");
+		generator->Generator->erode(h_heightfield, h_fixed);
+	});
+}
+
+int stp_heightfield_generate_device(stp_heightfield_generator* const generator, float* const d_heightfield,
+	const uint16_t* const* const d_biomemap, const float offset_x, const float offset_y, void* const stream) {
+	return guarded([&] {
+		require(generator && d_heightfield && d_biomemap, "null argument");
+		generator->Generator->generateDevice(d_heightfield, d_biomemap, make_float2(offset_x, offset_y), toStream(stream));
+	});
+}
+
+int stp_heightfield_erode_device(stp_heightfield_generator* const generator, float* const* const d_heightfield,
+	uint16_t* const* const d_fixed, void* const stream) {
+	return guarded([&] {
+		require(generator && d_heightfield && d_fixed, "null argument");
+		generator->Generator->erodeDevice(d_heightfield, d_fixed, toStream(stream));
+	});
+}
+
+} // extern "C"

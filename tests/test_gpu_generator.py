@@ -1,0 +1,136 @@
+"""STPHeightfieldGenerator through the C ABI (host-pointer API like the reference, and the
+device-resident API) against the oracle pipeline: per-chunk noise, free-slip merge,
+serialized erosion, unmerge, UINT16 format -- all bit-exact."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from superterrainplus_amd import capi, workloads as W
+
+from helpers import log, oracle_desc, oracle_rain
+
+pytestmark = pytest.mark.gpu
+
+MAP = 128
+NOISE = capi.SimplexNoiseSetting(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+
+
+def chunk_setting(diversity_nn=(1, 1), erosion_nn=(3, 3)):
+    c = capi.ChunkSetting()
+    c.ChunkSize[:] = [MAP, MAP]
+    c.MapSize[:] = [MAP, MAP]
+    c.ChunkScale[:] = [1.0, 1.0]
+    c.DiversityNearestNeighbour[:] = list(diversity_nn)
+    c.ErosionNearestNeighbour[:] = list(erosion_nn)
+    c.SplatNearestNeighbour[:] = [1, 1]
+    c.RenderDistance[:] = [3, 3]
+    return c
+
+
+class Generator:
+    def __init__(self, diversity, chunk, rain, mode):
+        h = C.c_void_p()
+        capi.call("stp_heightfield_generator_create", C.byref(chunk), C.byref(rain), C.c_void_p(diversity),
+                  C.byref(h))
+        self.h = h.value
+        capi.call("stp_heightfield_generator_set_erosion_mode", C.c_void_p(self.h), mode)
+
+    def __del__(self):
+        capi.lib().stp_heightfield_generator_destroy(C.c_void_p(self.h))
+
+
+def simplex_plugin():
+    d = C.c_void_p()
+    biome = capi.BiomeProperty(*W.Fractal().biome())
+    capi.call("stp_diversity_simplex_create", C.byref(NOISE), C.byref(biome), (C.c_uint32 * 2)(MAP, MAP), C.byref(d))
+    return d.value
+
+
+OFFSETS = [(float(cx * MAP), float(cy * MAP)) for cy in (-1, 0, 1) for cx in (-1, 0, 1)]
+
+
+def oracle_chunks(oracle):
+    perm, g32, _ = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+    return [oracle.fractal_map(perm, g32, oracle_desc(oracle, MAP, MAP, off), MAP, MAP) for off in OFFSETS]
+
+
+def oracle_erode(oracle, chunks, rain, stream_id):
+    merged = oracle.nn_merge(chunks, (3, 3))
+    oracle.erode(merged, (3, 3), (MAP, MAP), oracle_rain(oracle, rain), rain.RainDropCount, rain.Seed,
+                 stream_id=stream_id)
+    fixed = oracle.format_heightmap(merged)
+    return oracle.nn_unmerge(merged, (3, 3), (MAP, MAP)), oracle.nn_unmerge(fixed, (3, 3), (MAP, MAP))
+
+
+def test_generator_host_api_bit_exact(gpu, oracle):
+    log("generator host api")
+    plugin = simplex_plugin()
+    rain = W.erosion_setting(2000)
+    gen = Generator(plugin, chunk_setting(), rain, mode=1)
+    chunks = [np.zeros((MAP, MAP), np.float32) for _ in range(9)]
+    none_biome = (C.c_void_p * 1)(None)
+    for c, off in zip(chunks, OFFSETS):
+        capi.call("stp_heightfield_generate", C.c_void_p(gen.h), capi.ptr(c), none_biome, off[0], off[1])
+    want = oracle_chunks(oracle)
+    assert all(c.tobytes() == w.tobytes() for c, w in zip(chunks, want))
+
+    fixed = [np.zeros((MAP, MAP), np.uint16) for _ in range(9)]
+    hp = (C.c_void_p * 9)(*[c.ctypes.data for c in chunks])
+    fp = (C.c_void_p * 9)(*[f.ctypes.data for f in fixed])
+    for call_index in range(2):
+        capi.call("stp_heightfield_erode", C.c_void_p(gen.h), hp, fp)
+        want, want_fixed = oracle_erode(oracle, want, rain, call_index)
+        assert all(c.tobytes() == w.tobytes() for c, w in zip(chunks, want)), call_index
+        assert all((f == w).all() for f, w in zip(fixed, want_fixed)), call_index
+    del gen
+    capi.call("stp_diversity_destroy", C.c_void_p(plugin))
+
+
+def test_generator_device_api_matches_host_api(gpu, oracle):
+    log("generator device api")
+    plugin = simplex_plugin()
+    rain = W.erosion_setting(2000)
+    gen = Generator(plugin, chunk_setting(), rain, mode=1)
+    d_chunks = [capi.DeviceArray((MAP, MAP), np.float32) for _ in range(9)]
+    d_fixed = [capi.DeviceArray((MAP, MAP), np.uint16) for _ in range(9)]
+    none_biome = (C.c_void_p * 1)(None)
+    for d, off in zip(d_chunks, OFFSETS):
+        capi.call("stp_heightfield_generate_device", C.c_void_p(gen.h), d.vp, none_biome, off[0], off[1], None)
+    capi.call("stp_heightfield_erode_device", C.c_void_p(gen.h), (C.c_void_p * 9)(*[d.ptr for d in d_chunks]),
+              (C.c_void_p * 9)(*[d.ptr for d in d_fixed]), None)
+    capi.synchronize()
+    want, want_fixed = oracle_erode(oracle, oracle_chunks(oracle), rain, 0)
+    assert all(d.download().tobytes() == w.tobytes() for d, w in zip(d_chunks, want))
+    assert all((d.download() == w).all() for d, w in zip(d_fixed, want_fixed))
+    del gen
+    capi.call("stp_diversity_destroy", C.c_void_p(plugin))
+
+
+def test_generator_biomefield_plugin(gpu, oracle):
+    """Multi-biome plugin over a 3x3 diversity neighbourhood of Voronoi biome maps."""
+    log("generator biomefield plugin")
+    props = W.biome_table(4, 7)
+    d = C.c_void_p()
+    capi.call("stp_diversity_biomefield_create", C.byref(NOISE), capi.biome_array(props), 4,
+              (C.c_uint32 * 2)(MAP, MAP), 16, 8, C.byref(d))
+    plugin = d.value
+    gen = Generator(plugin, chunk_setting(diversity_nn=(3, 3)), W.erosion_setting(100), mode=0)
+    biome_chunks = [oracle.voronoi_map(7, 4, 64, (int(off[0]), int(off[1])), MAP, MAP) for off in OFFSETS]
+    bp = (C.c_void_p * 9)(*[b.ctypes.data for b in biome_chunks])
+    out = np.zeros((MAP, MAP), np.float32)
+    capi.call("stp_heightfield_generate", C.c_void_p(gen.h), capi.ptr(out), bp, 0.0, 0.0)
+    perm, g32, _ = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+    merged = oracle.nn_merge(biome_chunks, (3, 3))
+    item, weight, off = oracle.shf(merged, (3, 3), (MAP, MAP), 16)
+    want = oracle.multi_biome(perm, g32, props, item, weight, off, (MAP, MAP), (0.0, 0.0))
+    assert out.tobytes() == want.tobytes()
+    # device-resident biome chunks give the same heightfield
+    d_biome = [capi.DeviceArray.from_host(b) for b in biome_chunks]
+    d_out = capi.DeviceArray((MAP, MAP), np.float32)
+    capi.call("stp_heightfield_generate_device", C.c_void_p(gen.h), d_out.vp,
+              (C.c_void_p * 9)(*[b.ptr for b in d_biome]), 0.0, 0.0, None)
+    capi.synchronize()
+    assert d_out.download().tobytes() == want.tobytes()
+    del gen
+    capi.call("stp_diversity_destroy", C.c_void_p(plugin))

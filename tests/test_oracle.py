@@ -1,0 +1,405 @@
+"""Pins the CPU oracle before it is trusted as the parity reference (CPU only).
+
+What pins each oracle function (beyond restating it):
+  permutation/gradients  reference KAT (SuperTest+/SuperAlgorithm+/STPTestPermutation.cpp:34-146:
+                         seed 6666, N 10, 6.5 deg -> grad 0/3/7), permutation invariants,
+                         closed-form angles, committed seed-42 tables.
+  Philox4x32-10          the published Random123 known-answer vectors.
+  simplex / fractal      closed forms: zero at every simplex lattice vertex, |n| <= 1, fractal
+                         in [0, 1], floori special cases; FP32 vs FP64 <= 1e-5; golden samples.
+  erosion brush          closed-form tap counts (25 at r=3, 45 at r=4), symmetry, unit sum.
+  erosion                flat plane (every drop dies at once), tilted plane (closed-form first
+                         step), FP64 mass ledger, golden checksum.
+  single histogram       reference KAT (SuperTest+/SuperAlgorithm+/STPTestHistogram.cpp:44-104:
+                         items AND order at pixels 0/8/15), reference domain errors, brute-force
+                         counts, and an independent derivation of the bin order (chain starts).
+  free-slip mapping      reference KAT (SuperTest+/SuperTerrain+/STPTestNearestNeighbour.cpp:50-91).
+  chunk math             reference KAT (SuperTest+/SuperTerrain+/STPTest2DChunk.cpp:30-89).
+  format                 closed form with numpy's IEEE round-half-even.
+"""
+import hashlib
+import math
+import os
+
+import numpy as np
+import pytest
+
+from superterrainplus_amd import workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = np.load(os.path.join(HERE, "golden", "golden.npz"))
+
+# SuperTest+/SuperAlgorithm+/STPTestHistogram.cpp:44-57 (reference texture, 4x4 chunks, 3x3 NN)
+SHF_KAT_TEXTURE = np.array([
+    2, 0, 2, 0, 1, 1, 0, 3, 3, 2, 1, 2,
+    0, 1, 2, 1, 1, 1, 3, 2, 2, 0, 0, 1,
+    0, 1, 0, 2, 1, 3, 1, 1, 1, 2, 1, 0,
+    3, 3, 0, 1, 1, 2, 2, 2, 2, 0, 0, 3,
+    3, 2, 3, 3, 0, 3, 2, 2, 1, 0, 0, 3,
+    2, 0, 0, 1, 2, 0, 2, 2, 0, 2, 0, 3,
+    1, 2, 3, 0, 3, 2, 1, 2, 2, 3, 0, 2,
+    1, 1, 0, 3, 0, 2, 1, 0, 3, 2, 2, 1,
+    2, 2, 0, 1, 0, 2, 0, 0, 0, 0, 1, 1,
+    0, 1, 3, 3, 0, 3, 3, 3, 1, 0, 1, 1,
+    3, 1, 3, 0, 2, 1, 1, 0, 2, 2, 2, 0,
+    2, 0, 1, 1, 2, 3, 1, 2, 3, 2, 0, 1], np.uint16).reshape(12, 12)
+# STPTestHistogram.cpp:77-104 (pixel -> ordered (item, weight))
+SHF_KAT_EXPECTED = {
+    0: [(0, 0.24), (3, 0.24), (2, 0.28), (1, 0.24)],
+    8: [(0, 0.36), (3, 0.24), (1, 0.16), (2, 0.24)],
+    15: [(0, 0.32), (3, 0.2), (1, 0.12), (2, 0.36)],
+}
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+def c0_desc(oracle, side, offset=(0.0, 0.0), octave=8):
+    d = oracle.FractalDesc()
+    d.Persistence, d.Lacunarity, d.Octave, d.Scale = 0.5, 2.0, octave, 200.0
+    d.OffsetX, d.OffsetY = offset
+    d.HalfX = d.HalfY = side / 2.0
+    d.Amplitude = d.Frequency = 1.0
+    return d
+
+
+def rain(oracle, drops, **kw):
+    s = W.erosion_setting(drops)
+    r = oracle.RainDropSetting()
+    for f, _ in oracle.RainDropSetting._fields_:
+        setattr(r, f, getattr(s, f))
+    for k, v in kw.items():
+        setattr(r, k, v)
+    return r
+
+
+# ------------------------------------------------------------------ Philox
+def test_philox_known_answers(oracle):
+    # Random123 kat_vectors, philox4x32 R=10
+    cases = [
+        ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+        ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+        ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+         [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+    ]
+    for ctr, key, want in cases:
+        assert list(oracle.philox(ctr, key)) == want
+        assert W.philox(ctr, key) == want
+
+
+# ------------------------------------------------------------------ permutation
+def test_permutation_reference_kat(oracle):
+    perm, g32, _ = oracle.permutation(6666, 10, math.radians(6.5), 0)
+    for k, deg in ((0, 6.5), (3, 114.5), (7, 258.5)):
+        for c, f in ((0, math.cos), (1, math.sin)):
+            want = f(math.radians(deg))
+            assert abs(g32[2 * k + c] - want) <= 10 * EPS32 * abs(want)
+    assert (perm[:256] == perm[256:]).all()
+    assert sorted(perm[:256].tolist()) == list(range(256))
+    perm2, _, _ = oracle.permutation(6666, 10, math.radians(6.5), 0)
+    assert (perm == perm2).all()
+    perm3, _, _ = oracle.permutation(183457092, 10, math.radians(6.5), 0)
+    assert (perm != perm3).any()
+    with pytest.raises(ValueError):
+        oracle.permutation(1, 0, 0.0)
+    with pytest.raises(ValueError):
+        oracle.permutation(1, 8, 2 * math.pi)
+
+
+@pytest.mark.parametrize("n", [1, 6, 8, 10, 11, 12, 13, 24, 97])
+def test_gradient_closed_form_and_counter_bound(oracle, n):
+    _, g32, g64 = oracle.permutation(5, n, 0.3, 0)
+    angle = 0.0
+    step = 2 * math.pi / n
+    for k in range(n):
+        assert g64[2 * k] == math.cos(angle + 0.3) and g64[2 * k + 1] == math.sin(angle + 0.3)
+        assert abs(g64[2 * k] - math.cos(k * step + 0.3)) < 1e-12
+        angle += step
+    assert (g32 == g64.astype(np.float32)).all()
+
+
+def test_permutation_golden(oracle):
+    for eng, name in ((0, "mt19937_64"), (1, "mt19937")):
+        p, g32, g64 = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, eng)
+        assert (p == GOLDEN[f"perm42_{name}"]).all()
+        assert (g32 == GOLDEN[f"grad12_f32_{name}"]).all()
+        assert (g64 == GOLDEN[f"grad12_f64_{name}"]).all()
+
+
+# ------------------------------------------------------------------ simplex
+def test_floori_special_cases(oracle):
+    assert oracle.floori(0.0) == -1
+    assert oracle.floori(-1.0) == -2
+    assert oracle.floori(1.5) == 1
+    assert oracle.floori(-0.5) == -1
+    assert oracle.floori(2.0) == 2
+
+
+def test_simplex_zero_on_lattice_vertices(oracle):
+    perm, _, g64 = oracle.permutation(42, 12, W.GRADIENT_OFFSET, 1)
+    G2 = 0.2113248654
+    for i in range(-5, 6):
+        for j in range(-5, 6):
+            t = (i + j) * G2
+            v = oracle.simplex2d(perm, g64, i - t, j - t, f64=True)
+            assert abs(v) < 1e-9, (i, j, v)
+
+
+def test_simplex_bounds_and_fractal_range(oracle):
+    perm, g32, g64 = oracle.permutation(7, 12, W.GRADIENT_OFFSET, 0)
+    rng = np.random.default_rng(0)
+    for x, y in rng.uniform(-300, 300, (2000, 2)):
+        assert abs(oracle.simplex2d(perm, g64, x, y, f64=True)) <= 1.0
+    m = oracle.fractal_map(perm, g32, c0_desc(oracle, 64, (-1000.5, 377.25)), 64, 64)
+    assert m.min() >= 0.0 and m.max() <= 1.0 and m.std() > 0.01
+
+
+def test_fractal_fp32_vs_fp64_tolerance(oracle):
+    # 1e-5 max-abs holds for world coordinates within a few thousand pixels of the origin (the
+    # configurations' range). Further out FP32 loses coordinate precision: the top octave
+    # samples at |x| / 200 * 128, so the error grows ~linearly with |x| (bound below).
+    perm, g32, g64 = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, 1)
+    for off in ((0.0, 0.0), (-512.0, 512.0), (2048.0, 2048.0), (8948.5, -955.5), (1e5, 0.0)):
+        a = oracle.fractal_map(perm, g32, c0_desc(oracle, 512, off), 512, 512)
+        b = oracle.fractal_map(perm, g64, c0_desc(oracle, 512, off), 512, 512, f64=True)
+        err = np.abs(a - b).max()
+        reach = max(abs(off[0]), abs(off[1])) + 512
+        assert err <= (1e-5 if reach <= 4096 else 1e-6 + 2e-9 * reach), (off, err)
+
+
+def test_fractal_golden(oracle):
+    perm, g32, g64 = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, 1)
+    idx = GOLDEN["c0_sample_index"]
+    a = oracle.fractal_map(perm, g32, c0_desc(oracle, 512), 512, 512).ravel()[idx]
+    b = oracle.fractal_map(perm, g64, c0_desc(oracle, 512), 512, 512, f64=True).ravel()[idx]
+    assert (a == GOLDEN["c0_noise_f32"]).all()
+    assert np.abs(b - GOLDEN["c0_noise_f64"]).max() < 1e-12
+
+
+# ------------------------------------------------------------------ brush
+@pytest.mark.parametrize("r,taps", [(1, 1), (2, 9), (3, 25), (4, 45), (5, 69)])
+def test_brush_closed_form(oracle, r, taps):
+    W_ = 97
+    idx, w = oracle.brush(W_, r)
+    idx64, w64 = oracle.brush(W_, r, f64=True)
+    want = [(bx, by) for by in range(-r, r + 1) for bx in range(-r, r + 1) if bx * bx + by * by < r * r]
+    assert len(idx) == taps == len(want)
+    assert idx.tolist() == [by * W_ + bx for bx, by in want]
+    assert (w >= 0).all() and abs(w.astype(np.float64).sum() - 1.0) < 1e-6
+    assert abs(w64.sum() - 1.0) < 1e-12
+    wm = dict(zip(idx.tolist(), w.tolist()))
+    for bx, by in want:
+        assert wm[by * W_ + bx] == wm[-by * W_ - bx] == wm[bx * W_ + by]
+
+
+# ------------------------------------------------------------------ erosion
+def test_erosion_flat_plane_kills_every_drop(oracle):
+    h = np.full((64, 64), 0.5, np.float64)
+    ledger, stats = oracle.erode(h, (1, 1), (64, 64), rain(oracle, 500), 500, 9, f64=True)
+    assert (h == 0.5).all() and (ledger == 0).all()
+    assert stats[3] == 500 and stats[0] == stats[3]
+
+
+def test_erosion_tilted_plane_first_step(oracle):
+    a = 0.001
+    x = np.arange(64, dtype=np.float64)
+    h = np.tile(a * x + 0.5, (64, 1))
+    before = h.sum()
+    ledger, stats = oracle.erode(h, (1, 1), (64, 64), rain(oracle, 1, MaxLifetime=1), 1, 3, f64=True)
+    # first step: direction (-1, 0), dh = -a, capacity max(a * 4, 0.01) = 0.01,
+    # erosion min(0.01 * 0.3, a) = a spread over the 25-tap brush (sum of weights 1)
+    assert stats[0] == 1 and stats[1] == 1 and stats[3] == 0
+    assert abs(ledger[0] - a) < 1e-12
+    assert abs((before - h.sum()) - a) < 1e-12
+
+
+def test_erosion_fp64_mass_ledger(oracle):
+    perm, _, g64 = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, 1)
+    h = oracle.fractal_map(perm, g64, c0_desc(oracle, 256), 256, 256, f64=True)
+    before = h.sum()
+    ledger, _ = oracle.erode(h, (1, 1), (256, 256), rain(oracle, 4096), 4096, 42, f64=True)
+    assert abs((h.sum() - before) + ledger.sum()) < 1e-9 * max(1.0, ledger.sum())
+
+
+def test_erosion_golden(oracle):
+    perm, g32, _ = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, 1)
+    h = oracle.fractal_map(perm, g32, c0_desc(oracle, 512), 512, 512)
+    ledger, stats = oracle.erode(h, (1, 1), (512, 512), rain(oracle, 8192), 8192, 42)
+    assert (stats == GOLDEN["c0_erosion_stats"]).all()
+    assert (h.ravel()[GOLDEN["c0_sample_index"]] == GOLDEN["c0_eroded_f32"]).all()
+    assert hashlib.sha256(h.tobytes()).digest() == GOLDEN["c0_eroded_sha256"].tobytes()
+
+
+def height_histogram(h, bins=256):
+    return np.histogram(np.clip(h, 0, 1), bins=bins, range=(0.0, 1.0))[0]
+
+
+def test_erosion_fp32_vs_fp64(oracle):
+    """FP32 and FP64 runs of the same drops agree tightly until a drop's path flips on a
+    rounding-level comparison; after that the runs diverge cell-wise (the algorithm is
+    chaotic) while staying statistically identical. Measured on this workload: RMS 6e-8 at
+    100 drops, 1.3e-5 at 1000, 2.7e-4 at 8192 (vs an erosion signal of 1.9e-3 RMS)."""
+    perm, g32, g64 = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, 1)
+    base32 = oracle.fractal_map(perm, g32, c0_desc(oracle, 512), 512, 512)
+    base64 = oracle.fractal_map(perm, g64, c0_desc(oracle, 512), 512, 512, f64=True)
+    for drops, rms_bound in ((100, 1e-6), (8192, 5e-4)):
+        h32, h64 = base32.copy(), base64.copy()
+        l32, _ = oracle.erode(h32, (1, 1), (512, 512), rain(oracle, drops), drops, 42)
+        l64, _ = oracle.erode(h64, (1, 1), (512, 512), rain(oracle, drops), drops, 42, f64=True)
+        rms = float(np.sqrt(np.mean((h32.astype(np.float64) - h64) ** 2)))
+        assert rms <= rms_bound, (drops, rms)
+        # same total sediment moved, same height distribution
+        assert abs(l32.astype(np.float64).sum() - l64.sum()) <= 0.01 * l64.sum()
+        assert np.abs(height_histogram(h32) - height_histogram(h64)).sum() <= 0.01 * h32.size
+
+
+# ------------------------------------------------------------------ format
+def test_format_closed_form(oracle):
+    rng = np.random.default_rng(3)
+    ties = (np.arange(0, 65535, 97, dtype=np.float32) + np.float32(0.5)) / np.float32(65535)
+    h = np.concatenate([rng.uniform(-0.5, 1.5, 5000).astype(np.float32), ties,
+                        np.array([0, 1, -0.0, 1e-30, np.nan, np.inf, -np.inf], np.float32)])
+    s = np.where(h > 1, np.float32(1), np.where(h > 0, h, np.float32(0))).astype(np.float32)
+    want = np.rint(s * np.float32(65535)).astype(np.uint16)
+    assert (oracle.format_heightmap(h) == want).all()
+
+
+# ------------------------------------------------------------------ single histogram filter
+def test_shf_reference_kat(oracle):
+    item, weight, off = oracle.shf(SHF_KAT_TEXTURE, (3, 3), (4, 4), 2)
+    assert off.size == 17 and off[-1] == item.size
+    for p, exp in SHF_KAT_EXPECTED.items():
+        got = list(zip(item[off[p]:off[p + 1]].tolist(), weight[off[p]:off[p + 1]].tolist()))
+        assert [g[0] for g in got] == [e[0] for e in exp]
+        for (_, gw), (_, ew) in zip(got, exp):
+            assert abs(gw - ew) <= 5 * EPS32 * ew
+
+
+def test_shf_reference_domain_errors(oracle):
+    for r in (0, 128, 3):
+        with pytest.raises(ValueError):
+            oracle.shf(SHF_KAT_TEXTURE, (3, 3), (4, 4), r)
+
+
+def brute_counts(sample, start, map_size, r):
+    mx, my = map_size
+    out = []
+    for y in range(my):
+        for x in range(mx):
+            win = sample[start[1] + y - r:start[1] + y + r + 1, start[0] + x - r:start[0] + x + r + 1]
+            vals, cnt = np.unique(win, return_counts=True)
+            out.append(dict(zip(vals.tolist(), cnt.tolist())))
+    return out
+
+
+def chain_order(sample, start, map_size, r):
+    """Bin order derived independently of the accumulator: an item's position is decided by
+    the start of its current run (a run breaks after a gap of >= 2r+2 lines), first along each
+    column (vertical pass), then along each row of column histograms (horizontal pass), with
+    ties inside one line broken by that line's own order."""
+    mx, my = map_size
+    sx, sy = start
+    ncols = mx + 2 * r
+
+    def run_start(occ, last):
+        # occ: sorted positions of the item up to `last`; returns the start of the run of `last`
+        i = occ.index(last)
+        while i > 0 and occ[i] - occ[i - 1] <= 2 * r + 1:
+            i -= 1
+        return occ[i]
+
+    vorder = {}
+    for c in range(ncols):
+        col = sample[sy - r:sy + my + r, sx - r + c]
+        occ = {}
+        for yy, v in enumerate(col.tolist()):
+            occ.setdefault(v, []).append(yy)
+        for y in range(my):
+            keys = []
+            for v, o in occ.items():
+                inwin = [q for q in o if y <= q <= y + 2 * r]
+                if inwin:
+                    keys.append((run_start([q for q in o if q <= y + 2 * r], inwin[-1]), v))
+            vorder[(c, y)] = [v for _, v in sorted(keys)]
+    order = []
+    for y in range(my):
+        occ = {}
+        for c in range(ncols):
+            for v in vorder[(c, y)]:
+                occ.setdefault(v, []).append(c)
+        for x in range(mx):
+            keys = []
+            for v, o in occ.items():
+                inwin = [q for q in o if x <= q <= x + 2 * r]
+                if inwin:
+                    cs = run_start([q for q in o if q <= x + 2 * r], inwin[-1])
+                    keys.append((cs, vorder[(cs, y)].index(v), v))
+            order.append([v for *_, v in sorted(keys)])
+    return order
+
+
+@pytest.mark.parametrize("seed,chunk,r,k", [(1, 6, 2, 4), (2, 5, 4, 3), (3, 9, 2, 6), (4, 8, 8, 5)])
+def test_shf_counts_and_order_independent(oracle, seed, chunk, r, k):
+    rng = np.random.default_rng(seed)
+    # blocky maps give long runs (the interesting case for the order rule)
+    coarse = rng.integers(0, k, (3 * chunk // 2 + 2, 3 * chunk // 2 + 2))
+    sample = np.kron(coarse, np.ones((2, 2), np.int64))[:3 * chunk, :3 * chunk].astype(np.uint16)
+    noise = rng.random(sample.shape) < 0.2
+    sample[noise] = rng.integers(0, k, noise.sum())
+    item, weight, off = oracle.shf(sample, (3, 3), (chunk, chunk), r)
+    counts = brute_counts(sample, (chunk, chunk), (chunk, chunk), r)
+    order = chain_order(sample, (chunk, chunk), (chunk, chunk), r)
+    total = (2 * r + 1) ** 2
+    norm = np.float32(1) / np.float32(total)
+    for p in range(chunk * chunk):
+        its = item[off[p]:off[p + 1]].tolist()
+        ws = weight[off[p]:off[p + 1]]
+        assert its == order[p]
+        assert {i: int(round(float(w) * total)) for i, w in zip(its, ws)} == counts[p]
+        assert all(w == np.float32(counts[p][i]) * norm for i, w in zip(its, ws))
+
+
+# ------------------------------------------------------------------ free-slip mapping
+def test_nearest_neighbour_reference_kat(oracle):
+    # STPTestNearestNeighbour.cpp:50-91: 2x2 maps in a 3x3 neighbourhood, pixel = chunk index
+    chunks = [np.full((2, 2), i, np.float32) for i in range(9)]
+    merged = oracle.nn_merge(chunks, (3, 3))
+    want = np.array([[(x // 2) + (y // 2) * 3 for x in range(6)] for y in range(6)], np.float32)
+    assert (merged == want).all()
+    back = oracle.nn_unmerge(merged, (3, 3), (2, 2))
+    assert all((b == c).all() for b, c in zip(back, chunks))
+
+
+def test_nearest_neighbour_round_trip(oracle):
+    rng = np.random.default_rng(5)
+    chunks = [rng.integers(0, 60000, (7, 5)).astype(np.uint16) for _ in range(15)]
+    merged = oracle.nn_merge(chunks, (5, 3))
+    assert merged.shape == (21, 25)
+    for i, c in enumerate(chunks):
+        lx, ly = i % 5, i // 5
+        assert (merged[ly * 7:(ly + 1) * 7, lx * 5:(lx + 1) * 5] == c).all()
+    assert all((b == c).all() for b, c in zip(oracle.nn_unmerge(merged, (5, 3), (5, 7)), chunks))
+
+
+# ------------------------------------------------------------------ chunk math
+def test_chunk_math_reference_kat(oracle):
+    # STPTest2DChunk.cpp:30-52
+    assert oracle.chunk_world_coord((-573.74, 845.982), (8, 5), (2.5, 4.0)) == (-232, 210)
+    assert oracle.chunk_map_offset((-232, 210), (8, 5), (512, 256), (25.5, -57.5)) == (-14822.5, 10694.5)
+
+
+# ------------------------------------------------------------------ synthetic biome map
+def test_voronoi_golden_and_nearest_site(oracle):
+    mb = W.multi_biome()
+    vm = oracle.voronoi_map(mb.biome_seed, mb.biome_count, mb.cell, (0, 0), mb.side, mb.side)
+    assert hashlib.sha256(vm.tobytes()).digest() == GOLDEN["voronoi_c2_sha256"].tobytes()
+    assert (np.bincount(vm.ravel(), minlength=8) == GOLDEN["voronoi_c2_histogram"]).all()
+    # brute force: every sampled pixel takes the biome of its nearest site among all sites
+    sites = []
+    for cy in range(-1, 9):
+        for cx in range(-1, 9):
+            r = W.philox((cx & 0xFFFFFFFF, cy & 0xFFFFFFFF, 0x564F524F, 0), (mb.biome_seed, 0))
+            sites.append((cx * mb.cell + r[0] % mb.cell, cy * mb.cell + r[1] % mb.cell, r[2] % mb.biome_count))
+    s = np.array(sites, np.int64)
+    rng = np.random.default_rng(11)
+    for x, y in rng.integers(0, mb.side, (300, 2)):
+        d = (s[:, 0] - x) ** 2 + (s[:, 1] - y) ** 2
+        assert vm[y, x] == s[np.argmin(d), 2] or np.sum(d == d.min()) > 1
