@@ -1,0 +1,207 @@
+"""Benchmark of the heightfield path on one MI355X.
+
+Workload (the batched configuration): 256 square chunks with sides drawn from {128, 256, 512,
+1024} (p = 0.4/0.3/0.2/0.1, seed 11), each through the whole path -- 4-biome Voronoi biome map
+with a radius-16 halo, single histogram filter (r = 16), 4-biome 8-octave simplex blend,
+parallel erosion at 0.25 drops per pixel, UINT16 format -- all chunks resident in HBM.
+
+* value: eroded Mpixel/s of the batched path (one captured hipGraph per step, one launch per
+  stage for all chunks).
+* plain: the same kernels with one chunk per launch sequence and a host synchronisation after
+  every chunk (the reference's generate()/erode() calling pattern); chunk latencies p50/p99.
+* roofline: the dominant kernel (erosion), algorithmic bytes per launch from the per-drop step
+  counters, divided by its HIP-event duration inside the timed graph replays.
+* cpu_baseline: the FP32 oracle running the same per-chunk pipeline on the host cores, on a
+  time-bounded sample of the same chunks.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--plain-steps P] [--cpu-seconds S]
+"""
+import argparse
+import concurrent.futures as cf
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "eroded heightfield Mpixel/s + p50/p99 chunk latency, batched vs plain, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+STAGES = ["biome", "filter_vertical", "filter_horizontal", "blend", "erosion", "format"]
+# algorithmic bytes of one drop step (r = 3 brush, 25 taps): two 2x2 bilinear height reads;
+# an erode step adds 25 x (4 B read + 8 B atomic read-modify-write); a deposit step 4 x 8 B.
+BYTES_STEP, BYTES_ERODE, BYTES_DEPOSIT = 32, 25 * 12, 4 * 8
+
+
+def workload():
+    from superterrainplus_amd import workloads as W
+    cfg = W.batched()
+    sides = cfg.chunk_sides()
+    origins = cfg.chunk_origins()
+    props = [(p[0], cfg.fractal.octave, p[2], p[3], p[4], p[5]) for p in cfg.table()]
+    return cfg, sides, origins, props
+
+
+def make_batch(cfg, sides, origins, props):
+    from superterrainplus_amd import capi, workloads as W
+    noise = capi.SimplexNoiseSetting(cfg.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, cfg.engine)
+    # a pixel's histogram can hold at most biome_count distinct biomes
+    setting = capi.BatchSetting(cfg.biome_seed, cfg.biome_count, cfg.cell, cfg.radius, cfg.biome_count,
+                                cfg.drops_per_pixel)
+    chunks = (capi.BatchChunk * len(sides))(*[capi.BatchChunk(s, (C.c_int32 * 2)(*o)) for s, o in zip(sides, origins)])
+    h = C.c_void_p()
+    capi.call("stp_batch_create", C.byref(noise), capi.biome_array(props), len(props),
+              C.byref(W.erosion_setting(1, seed=cfg.erosion_seed)), C.byref(setting), chunks, len(sides), C.byref(h))
+    return h
+
+
+def cpu_baseline(cfg, sides, origins, props, seconds, threads):
+    """FP32 oracle pipeline per chunk on `threads` host threads, chunks in workload order until
+    `seconds` of wall time have passed."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from superterrainplus_amd import workloads as W
+    perm, g32, _ = O.permutation(cfg.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, cfg.engine)
+    s = W.erosion_setting(1, seed=cfg.erosion_seed)
+    rain = O.RainDropSetting()
+    for f, _ in O.RainDropSetting._fields_:
+        setattr(rain, f, getattr(s, f))
+    r = cfg.radius
+    deadline = time.perf_counter() + seconds
+
+    def one(i):
+        if time.perf_counter() > deadline:
+            return 0
+        side, o = sides[i], origins[i]
+        halo = O.voronoi_map(cfg.biome_seed, cfg.biome_count, cfg.cell, (o[0] - r, o[1] - r), side + 2 * r, side + 2 * r)
+        item, weight, off = O.shf_window(halo, (r, r), (side, side), r, cap_per_pixel=8)
+        h = O.multi_biome(perm, g32, props, item, weight, off, (side, side), (o[0] + side / 2, o[1] + side / 2))
+        drops = int(side * side * cfg.drops_per_pixel)
+        O.erode(h, (1, 1), (side, side), rain, drops, s.Seed, stream_id=i)
+        O.format_heightmap(h)
+        return side * side
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        done = list(ex.map(one, range(len(sides))))
+    wall = time.perf_counter() - t0
+    px = sum(done)
+    n = sum(1 for d in done if d)
+    return {"value": px / wall / 1e6, "unit": "Mpixel/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} of {len(sides)} chunks ({px / 1e6:.2f} Mpixel) of the same workload, FP32 oracle "
+                      f"(noise, filter, blend, serialized erosion, format), {wall:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from superterrainplus_amd import capi
+    capi.call("stp_engine_initialise", 0)
+    cfg, sides, origins, props = workload()
+    print(f"[bench] {len(sides)} chunks, {sum(s * s for s in sides) / 1e6:.1f} Mpixel", file=sys.stderr, flush=True)
+    batch = make_batch(cfg, sides, origins, props)
+    stream = C.c_void_p()
+    capi.call("stp_stream_create", C.byref(stream))
+
+    # warm-up (captures the graph; the event records between stages are part of the graph)
+    timed_graph = True
+    capi.call("stp_batch_set_timing", batch, 1)
+    try:
+        capi.call("stp_batch_run", batch, 0, stream)
+    except capi.STPError as e:
+        print(f"[bench] event records cannot be captured ({e}); stage times come from eager runs",
+              file=sys.stderr, flush=True)
+        timed_graph = False
+        capi.call("stp_batch_set_timing", batch, 0)
+    for _ in range(args.warmup):
+        capi.call("stp_batch_run", batch, 0, stream)
+    capi.call("stp_stream_synchronize", stream)
+    print("[bench] warm-up done", file=sys.stderr, flush=True)
+
+    stage_ms = np.zeros(6)
+    capi.call("stp_stream_synchronize", stream)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        capi.call("stp_batch_run", batch, 0, stream)
+        if timed_graph:
+            capi.call("stp_stream_synchronize", stream)
+            ms = (C.c_float * 6)()
+            capi.call("stp_batch_stage_times", batch, ms, 6)
+            stage_ms += np.array(ms, np.float64)
+    capi.call("stp_stream_synchronize", stream)
+    wall = time.perf_counter() - t0
+    if not timed_graph:
+        capi.call("stp_batch_set_timing", batch, 1)
+        for _ in range(args.steps):
+            capi.call("stp_batch_run", batch, 1, stream)
+            capi.call("stp_stream_synchronize", stream)
+            ms = (C.c_float * 6)()
+            capi.call("stp_batch_stage_times", batch, ms, 6)
+            stage_ms += np.array(ms, np.float64)
+    stage_ms /= args.steps
+    counters = (C.c_uint64 * 6)()
+    capi.call("stp_batch_counters", batch, counters, stream)
+    pixels, drops, steps, erode, deposit, killed = [int(v) for v in counters]
+    ms_per_step = wall / args.steps * 1e3
+    batched_mpx = pixels / (wall / args.steps) / 1e6
+    print(f"[bench] batched {batched_mpx:.1f} Mpixel/s ({ms_per_step:.2f} ms/step)", file=sys.stderr, flush=True)
+
+    # plain: one chunk per launch sequence + host sync per chunk
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        capi.call("stp_batch_run", batch, 2, stream)
+        arr = (C.c_double * len(sides))()
+        capi.call("stp_batch_chunk_latencies", batch, arr, len(sides))
+        lat.extend(arr)
+    plain_wall = (time.perf_counter() - t0) / args.plain_steps
+    plain_mpx = pixels / plain_wall / 1e6
+    lat = np.array(lat)
+    print(f"[bench] plain {plain_mpx:.1f} Mpixel/s", file=sys.stderr, flush=True)
+
+    erosion_bytes = steps * BYTES_STEP + erode * BYTES_ERODE + deposit * BYTES_DEPOSIT
+    erosion_s = stage_ms[4] / 1e3
+    achieved = erosion_bytes / erosion_s / 1e9 if erosion_s > 0 else 0.0
+    result = {
+        "metric": METRIC, "value": round(batched_mpx, 2), "unit": "Mpixel/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": "batched: 256 chunks, sides {128,256,512,1024} p={.4,.3,.2,.1} seed 11, 4-biome Voronoi "
+                        "(seed 7, cell 256) blend r=16, 8-octave simplex, 0.25 drops/px parallel erosion, UINT16",
+            "chunks": len(sides), "mpixel": round(pixels / 1e6, 3), "drops": drops,
+            "plain_mpixel_s": round(plain_mpx, 2), "batched_vs_plain": round(batched_mpx / plain_mpx, 2),
+            "plain_chunk_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                       "p99": round(float(np.percentile(lat, 99)), 3)},
+            "batched_step_latency_ms": round(ms_per_step, 3),
+            "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES, stage_ms)},
+            "erosion_counters": {"steps": steps, "erode_steps": erode, "deposit_steps": deposit, "killed": killed},
+        },
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "erosionKERNEL", "algorithmic_bytes": erosion_bytes},
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        result["cpu_baseline"] = cpu_baseline(cfg, sides, origins, props, args.cpu_seconds, threads)
+    capi.call("stp_batch_destroy", batch)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

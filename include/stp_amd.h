@@ -274,6 +274,52 @@ int stp_heightfield_generate_device(stp_heightfield_generator* generator, float*
 int stp_heightfield_erode_device(stp_heightfield_generator* generator, float* const* d_heightfield,
 	uint16_t* const* d_fixed, void* stream);
 
+/* ------------------------------------------------------------------ batched chunks
+ * No reference counterpart (the reference makes one generate()/erode() call per chunk, each ending
+ * with a synchronisation, STPHeightfieldGenerator.cpp:158-197). A batch holds many square chunks of
+ * uneven size resident in device memory and runs the whole path for all of them: Voronoi biome
+ * map with a radius halo -> single histogram filter -> multi-biome blend -> parallel erosion on
+ * the chunk alone -> UINT16. */
+typedef struct {
+	uint64_t BiomeSeed;
+	uint32_t BiomeCount;
+	uint32_t BiomeCell;
+	uint32_t Radius;       /* filter radius, even */
+	uint32_t BinCapacity;  /* 1..64 */
+	float DropsPerPixel;   /* drops of a chunk = side^2 * DropsPerPixel */
+} stp_batch_setting;
+
+typedef struct {
+	uint32_t Side;         /* square chunk side in pixels */
+	int32_t Origin[2];     /* world pixel of the chunk's first pixel */
+} stp_batch_chunk;
+
+typedef struct stp_heightfield_batch stp_heightfield_batch;
+
+/* raindrop->RainDropCount is ignored (per-chunk counts come from DropsPerPixel). */
+int stp_batch_create(const stp_simplex_noise_setting* noise, const stp_biome_property* h_table, uint32_t biome_count,
+	const stp_raindrop_setting* raindrop, const stp_batch_setting* setting, const stp_batch_chunk* h_chunks,
+	uint32_t chunk_count, stp_heightfield_batch** out);
+int stp_batch_destroy(stp_heightfield_batch* batch);
+/* mode 0: replay of one captured hipGraph (asynchronous); 1: the same launches without a graph
+ * (asynchronous); 2: plain -- one chunk per launch sequence with a host synchronisation after each
+ * chunk, the reference's calling pattern (synchronous). */
+int stp_batch_run(stp_heightfield_batch* batch, int mode, void* stream);
+/* Record HIP events between the 6 stages (biome, vertical filter, horizontal filter, blend,
+ * erosion, format) of later runs. */
+int stp_batch_set_timing(stp_heightfield_batch* batch, int enable);
+/* Stage milliseconds of the last timed run; call after the run's stream has completed. */
+int stp_batch_stage_times(stp_heightfield_batch* batch, float* h_ms, uint32_t capacity);
+/* Host wall milliseconds of each chunk of the last plain run. */
+int stp_batch_chunk_latencies(stp_heightfield_batch* batch, double* h_ms, uint32_t capacity);
+/* h_out = {pixels, drops, drop steps, erode steps, deposit steps, killed drops} of the last run
+ * (synchronises `stream`); fails with STP_ERROR_CAPACITY if a histogram overflowed. */
+int stp_batch_counters(stp_heightfield_batch* batch, uint64_t h_out[6], void* stream);
+/* Copies chunk `index`'s outputs to host memory (each pointer optional): heights and UINT16
+ * (side^2), halo biome map ((side + 2 * Radius)^2), per-drop final sediment (drops). */
+int stp_batch_read_chunk(stp_heightfield_batch* batch, uint32_t index, float* h_height, uint16_t* h_fixed,
+	uint16_t* h_biome_halo, float* h_ledger);
+
 #ifdef __cplusplus
 }
 #endif

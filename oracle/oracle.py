@@ -82,6 +82,10 @@ def lib():
         L.orc_shf_f32.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                   C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_uint64]
+        L.orc_shf_window_f32.restype = C.c_int64
+        L.orc_shf_window_f32.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_uint64]
         for n in ("orc_nn_merge", "orc_nn_unmerge"):
             getattr(L, n).restype = None
             getattr(L, n).argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -194,6 +198,23 @@ def shf(sample, nn, map_size, radius):
     off = np.zeros(mx * my + 1, np.uint32)
     n = lib().orc_shf_f32(s.ctypes.data, s.shape[1], nn[0], nn[1], mx, my, radius, item.ctypes.data,
                           weight.ctypes.data, off.ctypes.data, cap)
+    if n == -1:
+        raise ValueError("single histogram filter domain error")
+    if n < 0:
+        raise RuntimeError("oracle SHF output capacity exceeded")
+    return item[:n].copy(), weight[:n].copy(), off
+
+
+def shf_window(sample, start, map_size, radius, cap_per_pixel=64):
+    """Single histogram filter of the map_size window whose first pixel is `start` in `sample`."""
+    s = np.ascontiguousarray(sample, np.uint16)
+    mx, my = map_size
+    cap = mx * my * cap_per_pixel
+    item = np.zeros(cap, np.uint16)
+    weight = np.zeros(cap, np.float32)
+    off = np.zeros(mx * my + 1, np.uint32)
+    n = lib().orc_shf_window_f32(s.ctypes.data, s.shape[1], start[0], start[1], mx, my, radius,
+                                 item.ctypes.data, weight.ctypes.data, off.ctypes.data, cap)
     if n == -1:
         raise ValueError("single histogram filter domain error")
     if n < 0:

@@ -408,14 +408,16 @@ struct UBin {
 };
 
 // Returns 0 on success, 1 on a domain error (STPSingleHistogramFilter.cpp:874-880).
+// The reference derives the window start from the neighbourhood (start = Map * (NN / 2),
+// STPSingleHistogramFilter.cpp:881-892); this form takes the start directly so that a
+// halo-padded tile (start = (r, r)) can be filtered as well.
 template <class Real>
-int singleHistogramFilter(const uint16_t* sample, const uint32_t totalX, const uint32_t nnX, const uint32_t nnY,
-	const uint32_t mapX, const uint32_t mapY, const uint32_t radius, std::vector<uint16_t>& outItem,
-	std::vector<Real>& outWeight, std::vector<uint32_t>& outOffset) {
+int singleHistogramFilterWindow(const uint16_t* sample, const uint32_t totalX, const uint32_t startX,
+	const uint32_t startY, const uint32_t mapX, const uint32_t mapY, const uint32_t radius,
+	std::vector<uint16_t>& outItem, std::vector<Real>& outWeight, std::vector<uint32_t>& outOffset) {
 	if (!(radius > 0u && (radius & 1u) == 0u)) {
 		return 1;
 	}
-	const uint32_t startX = mapX * (nnX / 2u), startY = mapY * (nnY / 2u);
 	if (!(radius <= startX && radius <= startY)) {
 		return 1;
 	}
@@ -492,6 +494,14 @@ int singleHistogramFilter(const uint16_t* sample, const uint32_t totalX, const u
 	}
 	outOffset.push_back(static_cast<uint32_t>(outItem.size()));
 	return 0;
+}
+
+template <class Real>
+int singleHistogramFilter(const uint16_t* sample, const uint32_t totalX, const uint32_t nnX, const uint32_t nnY,
+	const uint32_t mapX, const uint32_t mapY, const uint32_t radius, std::vector<uint16_t>& outItem,
+	std::vector<Real>& outWeight, std::vector<uint32_t>& outOffset) {
+	return singleHistogramFilterWindow<Real>(sample, totalX, mapX * (nnX / 2u), mapY * (nnY / 2u), mapX, mapY, radius,
+		outItem, outWeight, outOffset);
 }
 
 // ------------------------------------------------------------------------------------
@@ -722,6 +732,24 @@ int64_t orc_shf_f32(const uint16_t* sample, const uint32_t totalX, const uint32_
 	std::vector<float> w;
 	std::vector<uint32_t> off;
 	if (singleHistogramFilter<float>(sample, totalX, nnX, nnY, mapX, mapY, radius, it, w, off) != 0) {
+		return -1;
+	}
+	if (it.size() > cap) {
+		return -2;
+	}
+	std::copy(it.begin(), it.end(), item);
+	std::copy(w.begin(), w.end(), weight);
+	std::copy(off.begin(), off.end(), offset);
+	return static_cast<int64_t>(it.size());
+}
+
+int64_t orc_shf_window_f32(const uint16_t* sample, const uint32_t totalX, const uint32_t startX, const uint32_t startY,
+	const uint32_t mapX, const uint32_t mapY, const uint32_t radius, uint16_t* item, float* weight, uint32_t* offset,
+	const uint64_t cap) {
+	std::vector<uint16_t> it;
+	std::vector<float> w;
+	std::vector<uint32_t> off;
+	if (singleHistogramFilterWindow<float>(sample, totalX, startX, startY, mapX, mapY, radius, it, w, off) != 0) {
 		return -1;
 	}
 	if (it.size() > cap) {

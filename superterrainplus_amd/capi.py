@@ -62,6 +62,15 @@ class BiomeProperty(C.Structure):
                 ("Lacunarity", C.c_float), ("Depth", C.c_float), ("Variation", C.c_float)]
 
 
+class BatchSetting(C.Structure):
+    _fields_ = [("BiomeSeed", C.c_uint64), ("BiomeCount", C.c_uint32), ("BiomeCell", C.c_uint32),
+                ("Radius", C.c_uint32), ("BinCapacity", C.c_uint32), ("DropsPerPixel", C.c_float)]
+
+
+class BatchChunk(C.Structure):
+    _fields_ = [("Side", C.c_uint32), ("Origin", C.c_int32 * 2)]
+
+
 class Permutation(C.Structure):
     _fields_ = [("Permutation", C.c_void_p), ("Gradient2D", C.c_void_p),
                 ("Gradient2DSize", C.c_uint32)]
@@ -132,6 +141,15 @@ _SIGNATURES = {
     "stp_heightfield_erode": (_I, [_P, _P, _P]),
     "stp_heightfield_generate_device": (_I, [_P, _P, _P, _F, _F, _P]),
     "stp_heightfield_erode_device": (_I, [_P, _P, _P, _P]),
+    "stp_batch_create": (_I, [C.POINTER(SimplexNoiseSetting), _P, _U32, C.POINTER(RainDropSetting),
+                              C.POINTER(BatchSetting), _P, _U32, C.POINTER(C.c_void_p)]),
+    "stp_batch_destroy": (_I, [_P]),
+    "stp_batch_run": (_I, [_P, _I, _P]),
+    "stp_batch_set_timing": (_I, [_P, _I]),
+    "stp_batch_stage_times": (_I, [_P, _P, _U32]),
+    "stp_batch_chunk_latencies": (_I, [_P, _P, _U32]),
+    "stp_batch_counters": (_I, [_P, _P, _P]),
+    "stp_batch_read_chunk": (_I, [_P, _U32, _P, _P, _P, _P]),
 }
 
 _lib = None

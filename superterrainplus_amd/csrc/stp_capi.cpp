@@ -1,5 +1,6 @@
 // extern "C" boundary (include/stp_amd.h): converts exceptions into status codes plus a
 // thread-local message, and plain C structs into the C++ API types.
+#include "stp_batch.hpp"
 #include "stp_internal.hpp"
 
 #include <stp_amd.h>
@@ -146,6 +147,10 @@ struct stp_filter_buffer {
 
 struct stp_diversity_generator {
 	std::unique_ptr<STPDiversityGenerator> Generator;
+};
+
+struct stp_heightfield_batch {
+	std::unique_ptr<STPHeightfieldBatch> Batch;
 };
 
 struct stp_heightfield_generator {
@@ -552,6 +557,90 @@ int stp_heightfield_erode_device(stp_heightfield_generator* const generator, flo
 	return guarded([&] {
 		require(generator && d_heightfield && d_fixed, "null argument");
 		generator->Generator->erodeDevice(d_heightfield, d_fixed, toStream(stream));
+	});
+}
+
+int stp_batch_create(const stp_simplex_noise_setting* const noise, const stp_biome_property* const h_table,
+	const uint32_t biome_count, const stp_raindrop_setting* const raindrop, const stp_batch_setting* const setting,
+	const stp_batch_chunk* const h_chunks, const uint32_t chunk_count, stp_heightfield_batch** const out) {
+	return guarded([&] {
+		require(h_table && setting && h_chunks && out, "null argument");
+		std::vector<STPBiomeProperty> t;
+		for (uint32_t i = 0u; i < biome_count; i++) {
+			t.push_back(toBiome(h_table[i]));
+		}
+		STPBatchSetting bs;
+		bs.BiomeSeed = setting->BiomeSeed;
+		bs.BiomeCount = setting->BiomeCount;
+		bs.BiomeCell = setting->BiomeCell;
+		bs.Radius = setting->Radius;
+		bs.BinCapacity = setting->BinCapacity;
+		bs.DropsPerPixel = setting->DropsPerPixel;
+		std::vector<STPBatchChunk> c(chunk_count);
+		for (uint32_t i = 0u; i < chunk_count; i++) {
+			c[i] = STPBatchChunk {h_chunks[i].Side, h_chunks[i].Origin[0], h_chunks[i].Origin[1]};
+		}
+		auto b = std::make_unique<stp_heightfield_batch>();
+		b->Batch = std::make_unique<STPHeightfieldBatch>(toNoise(noise), t, toRain(raindrop), bs, c);
+		*out = b.release();
+	});
+}
+
+int stp_batch_destroy(stp_heightfield_batch* const batch) {
+	return guarded([&] { delete batch; });
+}
+
+int stp_batch_run(stp_heightfield_batch* const batch, const int mode, void* const stream) {
+	return guarded([&] {
+		require(batch && mode >= 0 && mode <= 2, "invalid argument");
+		batch->Batch->run(static_cast<STPHeightfieldBatch::STPRunMode>(mode), toStream(stream));
+	});
+}
+
+int stp_batch_set_timing(stp_heightfield_batch* const batch, const int enable) {
+	return guarded([&] {
+		require(batch, "null batch");
+		batch->Batch->setTiming(enable != 0);
+	});
+}
+
+int stp_batch_stage_times(stp_heightfield_batch* const batch, float* const h_ms, const uint32_t capacity) {
+	return guarded([&] {
+		require(batch && h_ms, "null argument");
+		const std::vector<float> t = batch->Batch->stageTimes();
+		require(capacity >= t.size(), "capacity must hold 6 stages");
+		std::memcpy(h_ms, t.data(), t.size() * sizeof(float));
+	});
+}
+
+int stp_batch_chunk_latencies(stp_heightfield_batch* const batch, double* const h_ms, const uint32_t capacity) {
+	return guarded([&] {
+		require(batch && h_ms, "null argument");
+		const std::vector<double>& l = batch->Batch->latency();
+		require(!l.empty(), "no plain run yet");
+		require(capacity >= l.size(), "capacity must hold one latency per chunk");
+		std::memcpy(h_ms, l.data(), l.size() * sizeof(double));
+	});
+}
+
+int stp_batch_counters(stp_heightfield_batch* const batch, uint64_t h_out[6], void* const stream) {
+	return guarded([&] {
+		require(batch && h_out, "null argument");
+		batch->Batch->check(toStream(stream));
+		const auto c = batch->Batch->counters(toStream(stream));
+		h_out[0] = batch->Batch->pixels();
+		h_out[1] = batch->Batch->drops();
+		for (int k = 0; k < 4; k++) {
+			h_out[2 + k] = c[k];
+		}
+	});
+}
+
+int stp_batch_read_chunk(stp_heightfield_batch* const batch, const uint32_t index, float* const h_height,
+	uint16_t* const h_fixed, uint16_t* const h_biome_halo, float* const h_ledger) {
+	return guarded([&] {
+		require(batch, "null batch");
+		batch->Batch->readChunk(index, h_height, h_fixed, h_biome_halo, h_ledger);
 	});
 }
 

@@ -193,13 +193,12 @@ std::vector<int2> STPChunk::calcChunkNeighbourOffset(const uint2 chunkSize, cons
 // ------------------------------------------------------------------ erosion brush
 // STPHeightfieldGenerator.cpp:44-99: weights 1 - |d| / r in double, stored as float, then
 // normalised as float((double)w_float * (1 / sum_double)).
-void STPErosionBrushGenerator::generate(const unsigned int dimension_x, const unsigned int radius,
-	std::vector<int>& index, std::vector<float>& weight) {
+void STPErosionBrushGenerator::generateOffsets(const unsigned int radius, std::vector<int2>& offset,
+	std::vector<float>& weight) {
 	STP_ASSERT_DOMAIN(radius > 0u, "Erosion brush radius must be a positive integer");
-	STP_ASSERT_DOMAIN(dimension_x > 0u, "Dimension row count should be positive");
 	const int r = static_cast<int>(radius), r2 = r * r;
 	double sum = 0.0;
-	index.clear();
+	offset.clear();
 	weight.clear();
 	for (int by = -r; by <= r; by++) {
 		for (int bx = -r; bx <= r; bx++) {
@@ -207,7 +206,7 @@ void STPErosionBrushGenerator::generate(const unsigned int dimension_x, const un
 			if (d2 < r2) {
 				const double w = 1.0 - std::sqrt(d2) / r;
 				sum += w;
-				index.push_back(by * static_cast<int>(dimension_x) + bx);
+				offset.push_back(make_int2(bx, by));
 				weight.push_back(static_cast<float>(w));
 			}
 		}
@@ -215,5 +214,16 @@ void STPErosionBrushGenerator::generate(const unsigned int dimension_x, const un
 	const double factor = 1.0 / sum;
 	for (float& w : weight) {
 		w = static_cast<float>(w * factor);
+	}
+}
+
+void STPErosionBrushGenerator::generate(const unsigned int dimension_x, const unsigned int radius,
+	std::vector<int>& index, std::vector<float>& weight) {
+	STP_ASSERT_DOMAIN(dimension_x > 0u, "Dimension row count should be positive");
+	std::vector<int2> offset;
+	generateOffsets(radius, offset, weight);
+	index.resize(offset.size());
+	for (size_t k = 0u; k < offset.size(); k++) {
+		index[k] = offset[k].y * static_cast<int>(dimension_x) + offset[k].x;
 	}
 }

@@ -44,6 +44,8 @@ namespace SuperTerrainPlus {
 	// (2r+1)^2 square, strictly inside the radius) as linear offsets in a map of width dimension_x.
 	namespace STPErosionBrushGenerator {
 		void generate(unsigned int dimension_x, unsigned int radius, std::vector<int>& index, std::vector<float>& weight);
+		// The same taps as (dx, dy) offsets, independent of the map width.
+		void generateOffsets(unsigned int radius, std::vector<int2>& offset, std::vector<float>& weight);
 	}
 
 	namespace STPHeightfieldKernel {

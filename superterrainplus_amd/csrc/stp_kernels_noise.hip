@@ -5,12 +5,11 @@
 // histogram read for the blend), so these kernels are compute-bound. The permutation table is
 // staged in LDS once per workgroup (4.5 KB) and every thread produces PixelPerThread pixels
 // so the staging cost is amortised.
-#include "stp_internal.hpp"
-#include <stp/STPRandom.hpp>
+#include "stp_stage_noise.hpp"
 
 using namespace SuperTerrainPlus;
 using namespace SuperTerrainPlus::STPAlgorithm;
-namespace Noise = SuperTerrainPlus::STPAlgorithm::STPSimplexNoise;
+using namespace SuperTerrainPlus::STPStage;
 
 namespace {
 
@@ -36,24 +35,6 @@ namespace {
 				height[x + static_cast<size_t>(y) * width] = n * variation + depth;
 			}
 		}
-	}
-
-	struct BiomeTable {
-		STPBiomeProperty Property[STPHeightfieldKernel::MaxBiome];
-	};
-
-	__device__ inline float biomeSample(const Noise::STPStagedPermutation& table, const STPBiomeProperty& bp,
-		const float x, const float y, const float2 offset, const float2 half) {
-		Noise::STPFractalSimplexInformation f;
-		f.Persistence = bp.Persistence;
-		f.Lacunarity = bp.Lacunarity;
-		f.Octave = bp.Octave;
-		f.Scale = bp.Scale;
-		f.Offset = offset;
-		f.HalfDimension = half;
-		f.Amplitude = 1.0f;
-		f.Frequency = 1.0f;
-		return Noise::simplex2DFractal(table, x, y, f) * bp.Variation + bp.Depth;
 	}
 
 	// generateMultiBiomeHeightmap (SuperDemo+/Script/STPMultiHeightGenerator.cu:35-55): sums
@@ -118,25 +99,9 @@ namespace {
 				break;
 			}
 			const size_t index = x + static_cast<size_t>(y) * map_size.x;
-			const unsigned int n = bin_count[index];
-			float h = 0.0f;
-			for (unsigned int b = 0u; b < n; b++) {
-				const unsigned short item = bin_item[index * stride + b];
-				const float w = bin_weight[index * stride + b];
-				if (item < biome_count) {
-					h += w * biomeSample(table, prop[item], 1.0f * static_cast<float>(x), 1.0f * static_cast<float>(y), offset, half);
-				}
-			}
-			height[index] = h;
+			height[index] = blendPixel(table, prop, biome_count, bin_count[index], bin_item + index * stride,
+				bin_weight + index * stride, x, y, offset, half);
 		}
-	}
-
-	__device__ inline long long floorDiv(const long long a, const long long b) {
-		long long q = a / b;
-		if ((a % b != 0) && ((a < 0) != (b < 0))) {
-			q--;
-		}
-		return q;
 	}
 
 	// Synthetic biome map: nearest jittered site over the 3x3 cell neighbourhood, exact int64
@@ -149,43 +114,11 @@ namespace {
 		if (x >= width || y >= rows) {
 			return;
 		}
-		const long long wx = origin_x + x, wy = origin_y + y;
-		const long long cx = floorDiv(wx, cell), cy = floorDiv(wy, cell);
-		long long best = 0x7FFFFFFFFFFFFFFFll;
-		unsigned int biome = 0u;
-		for (int dy = -1; dy <= 1; dy++) {
-			for (int dx = -1; dx <= 1; dx++) {
-				const long long ncx = cx + dx, ncy = cy + dy;
-				const STPRandom::Philox4 r = STPRandom::philox4x32_10(static_cast<unsigned int>(ncx),
-					static_cast<unsigned int>(ncy), 0x564F524Fu, 0u, static_cast<unsigned int>(seed),
-					static_cast<unsigned int>(seed >> 32));
-				const long long sx = ncx * cell + (r.v[0] % cell), sy = ncy * cell + (r.v[1] % cell);
-				const long long d = (wx - sx) * (wx - sx) + (wy - sy) * (wy - sy);
-				if (d < best) {
-					best = d;
-					biome = r.v[2] % biome_count;
-				}
-			}
-		}
-		out[x + static_cast<size_t>(y) * pitch] = static_cast<STPSample_t>(biome);
+		out[x + static_cast<size_t>(y) * pitch] = voronoiBiome(seed, biome_count, cell, origin_x + x, origin_y + y);
 	}
 
 	inline dim3 tileGrid(const unsigned int width, const unsigned int rows) {
 		return dim3((width + TileX - 1u) / TileX, (rows + TileY * PixelPerThread - 1u) / (TileY * PixelPerThread));
-	}
-
-	BiomeTable packBiomes(const STPBiomeProperty* table, const unsigned int count) {
-		if (count == 0u || count > STPHeightfieldKernel::MaxBiome) {
-			STP_THROW(STPInvalidArgument, "biome count must be in [1, 64]");
-		}
-		BiomeTable t = {};
-		for (unsigned int i = 0u; i < count; i++) {
-			if (table[i].Octave == 0u || !(table[i].Scale != 0.0f)) {
-				STP_THROW(STPInvalidArgument, "biome property needs Octave > 0 and Scale != 0");
-			}
-			t.Property[i] = table[i];
-		}
-		return t;
 	}
 
 	void checkPermutation(const STPPermutation& perm) {
@@ -215,7 +148,7 @@ void STPHeightfieldKernel::multiBiomeHeightmap(const STPPermutation& perm, const
 	const unsigned int biome_count, const STPSingleHistogram& histogram, float* const height, const uint2 map_size,
 	const float2 offset, const hipStream_t stream) {
 	checkPermutation(perm);
-	const BiomeTable t = packBiomes(table_host, biome_count);
+	const BiomeTable t = packBiomeTable(table_host, biome_count);
 	if (!histogram.Bin || !histogram.HistogramStartOffset || !height) {
 		STP_THROW(STPInvalidArgument, "histogram or heightmap pointer is null");
 	}
@@ -229,7 +162,7 @@ void STPHeightfieldKernel::multiBiomeHeightmapDense(const STPPermutation& perm, 
 	const float* const bin_weight, const unsigned int bin_stride, float* const height, const uint2 map_size,
 	const float2 offset, const hipStream_t stream) {
 	checkPermutation(perm);
-	const BiomeTable t = packBiomes(table_host, biome_count);
+	const BiomeTable t = packBiomeTable(table_host, biome_count);
 	multiBiomeDenseKERNEL<<<tileGrid(map_size.x, map_size.y), dim3(TileX, TileY), 0, stream>>>(perm, t, biome_count,
 		bin_count, bin_item, bin_weight, bin_stride, height, map_size, offset);
 	STP_CHECK_HIP(hipGetLastError());

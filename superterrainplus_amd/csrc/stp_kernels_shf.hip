@@ -18,172 +18,41 @@
 // of lines of all chunks in flight. The dense output [pixel][slot] feeds the fused blend
 // directly; the jagged reference layout is produced by a scan + scatter.
 #include "stp_internal.hpp"
+#include "stp_stage_shf.hpp"
 
 using namespace SuperTerrainPlus;
 using namespace SuperTerrainPlus::STPHeightfieldKernel;
 
 namespace {
 
-	constexpr unsigned int LineBlock = 64u;
+	constexpr unsigned int LineBlock = STPStage::ShfLineBlock;
 	constexpr unsigned int ScanBlock = 256u, ScanItems = 8u, ScanTile = ScanBlock * ScanItems;
 
-	// ------------------------------------------------------------ vertical pass
-	struct VerticalAcc {
-		unsigned int* slot; // packed item | count << 16, slot k at slot[k * LineBlock]
-		unsigned int n;
-		unsigned int cap;
-		bool overflow;
-
-		__device__ inline void increment(const unsigned int item) {
-			for (unsigned int k = 0u; k < n; k++) {
-				const unsigned int e = slot[k * LineBlock];
-				if ((e & 0xFFFFu) == item) {
-					slot[k * LineBlock] = e + 0x10000u;
-					return;
-				}
-			}
-			if (n < cap) {
-				slot[n * LineBlock] = item | 0x10000u;
-				n++;
-			} else {
-				overflow = true;
-			}
-		}
-		__device__ inline void decrement(const unsigned int item) {
-			for (unsigned int k = 0u; k < n; k++) {
-				const unsigned int e = slot[k * LineBlock];
-				if ((e & 0xFFFFu) == item) {
-					if ((e >> 16) <= 1u) {
-						for (unsigned int m = k; m + 1u < n; m++) {
-							slot[m * LineBlock] = slot[(m + 1u) * LineBlock];
-						}
-						n--;
-					} else {
-						slot[k * LineBlock] = e - 0x10000u;
-					}
-					return;
-				}
-			}
-		}
-	};
-
+	// ------------------------------------------------------------ line passes
 	__global__ void __launch_bounds__(LineBlock) verticalKERNEL(const STPSample_t* const sample, const unsigned int pitch,
 		const uint2 start, const uint2 map, const unsigned int radius, const unsigned int vcap, unsigned int* const vbin,
 		unsigned char* const vcount, unsigned int* const overflow) {
 		extern __shared__ unsigned int lds[];
-		const unsigned int ncols = map.x + 2u * radius;
 		const unsigned int c = blockIdx.x * LineBlock + threadIdx.x;
-		if (c >= ncols) {
+		if (c >= map.x + 2u * radius) {
 			return;
 		}
-		VerticalAcc acc {lds + threadIdx.x, 0u, vcap, false};
-		const STPSample_t* const col = sample + (start.x - radius + c);
-		const size_t top = start.y - radius;
-		for (unsigned int j = 0u; j <= 2u * radius; j++) {
-			acc.increment(col[(top + j) * pitch]);
-		}
-		for (unsigned int y = 0u; y < map.y; y++) {
-			if (y > 0u) {
-				acc.increment(col[(static_cast<size_t>(start.y) + y + radius) * pitch]);
-				acc.decrement(col[(static_cast<size_t>(start.y) + y - radius - 1u) * pitch]);
-			}
-			vcount[static_cast<size_t>(y) * ncols + c] = static_cast<unsigned char>(acc.n);
-			for (unsigned int k = 0u; k < acc.n; k++) {
-				vbin[(static_cast<size_t>(y) * vcap + k) * ncols + c] = acc.slot[k * LineBlock];
-			}
-		}
-		if (acc.overflow) {
+		if (STPStage::shfVerticalLine(sample, pitch, start, map, radius, vcap, lds + threadIdx.x, vbin, vcount, c)) {
 			atomicOr(overflow, 1u);
 		}
 	}
-
-	// ------------------------------------------------------------ horizontal pass
-	struct HorizontalAcc {
-		unsigned int* item;  // slot k at item[k * LineBlock]
-		unsigned int* count;
-		unsigned int n;
-		unsigned int cap;
-		bool overflow;
-
-		__device__ inline void increment(const unsigned int it, const unsigned int c) {
-			for (unsigned int k = 0u; k < n; k++) {
-				if (item[k * LineBlock] == it) {
-					count[k * LineBlock] += c;
-					return;
-				}
-			}
-			if (n < cap) {
-				item[n * LineBlock] = it;
-				count[n * LineBlock] = c;
-				n++;
-			} else {
-				overflow = true;
-			}
-		}
-		__device__ inline void decrement(const unsigned int it, const unsigned int c) {
-			for (unsigned int k = 0u; k < n; k++) {
-				if (item[k * LineBlock] == it) {
-					const unsigned int q = count[k * LineBlock];
-					if (q <= c) {
-						for (unsigned int m = k; m + 1u < n; m++) {
-							item[m * LineBlock] = item[(m + 1u) * LineBlock];
-							count[m * LineBlock] = count[(m + 1u) * LineBlock];
-						}
-						n--;
-					} else {
-						count[k * LineBlock] = q - c;
-					}
-					return;
-				}
-			}
-		}
-	};
 
 	__global__ void __launch_bounds__(LineBlock) horizontalKERNEL(const uint2 map, const unsigned int radius,
 		const unsigned int vcap, const unsigned int cap, const unsigned int* const vbin,
 		const unsigned char* const vcount, unsigned char* const bin_count, unsigned short* const bin_item,
 		float* const bin_weight, unsigned int* const overflow) {
 		extern __shared__ unsigned int lds[];
-		const unsigned int ncols = map.x + 2u * radius;
 		const unsigned int y = blockIdx.x * LineBlock + threadIdx.x;
 		if (y >= map.y) {
 			return;
 		}
-		HorizontalAcc acc {lds + threadIdx.x, lds + cap * LineBlock + threadIdx.x, 0u, cap, false};
-		const unsigned char* const rowCount = vcount + static_cast<size_t>(y) * ncols;
-		const unsigned int* const rowBin = vbin + static_cast<size_t>(y) * vcap * ncols;
-		const auto add = [&](const unsigned int column, const bool inc) {
-			const unsigned int nc = rowCount[column];
-			for (unsigned int k = 0u; k < nc; k++) {
-				const unsigned int e = rowBin[static_cast<size_t>(k) * ncols + column];
-				if (inc) {
-					acc.increment(e & 0xFFFFu, e >> 16);
-				} else {
-					acc.decrement(e & 0xFFFFu, e >> 16);
-				}
-			}
-		};
-		for (unsigned int c = 0u; c <= 2u * radius; c++) {
-			add(c, true);
-		}
-		for (unsigned int x = 0u; x < map.x; x++) {
-			if (x > 0u) {
-				add(x + 2u * radius, true);
-				add(x - 1u, false);
-			}
-			const size_t p = static_cast<size_t>(y) * map.x + x;
-			unsigned int sum = 0u;
-			for (unsigned int k = 0u; k < acc.n; k++) {
-				sum += acc.count[k * LineBlock];
-			}
-			const float norm = 1.0f / static_cast<float>(sum);
-			bin_count[p] = static_cast<unsigned char>(acc.n);
-			for (unsigned int k = 0u; k < acc.n; k++) {
-				bin_item[p * cap + k] = static_cast<unsigned short>(acc.item[k * LineBlock]);
-				bin_weight[p * cap + k] = static_cast<float>(acc.count[k * LineBlock]) * norm;
-			}
-		}
-		if (acc.overflow) {
+		if (STPStage::shfHorizontalLine(map, radius, vcap, cap, vbin, vcount, lds + threadIdx.x, bin_count, bin_item,
+				bin_weight, y)) {
 			atomicOr(overflow, 1u);
 		}
 	}

@@ -1,0 +1,199 @@
+// Device stage functions of free-slip hydraulic erosion, shared by the per-chunk kernels
+// (stp_kernels_erosion.hip) and the batched chunk-table kernels (stp_batch.hip).
+//
+// Behaviour follows SuperTerrain+/SuperTerrain+/Private/GPGPU/STPHeightfieldKernel.cu:80-108 (spawn)
+// and SuperTerrain+/SuperTerrain+/Private/GPGPU/STPRainDrop.cu:18-164 (drop), plus: MaxLifetime
+// (0 = unlimited), a stateless Philox spawn, a NaN-safe kill test and a footprint guard that keeps
+// every access inside the map (see cellInside / spawnInside).
+#pragma once
+#ifndef STP_AMD_STAGE_EROSION_HPP
+#define STP_AMD_STAGE_EROSION_HPP
+
+#include "stp_internal.hpp"
+#include <stp/STPRandom.hpp>
+
+namespace SuperTerrainPlus::STPStage {
+
+	struct ErosionArgs {
+		STPRainDropParameter Rain;
+		uint2 MapSize;  // one chunk
+		uint2 NN;       // neighbourhood in chunks
+		uint2 Total;    // merged map = MapSize * NN
+		unsigned int BrushSize;
+		unsigned int StreamId;
+	};
+
+	__device__ inline float loadHeight(const float* p) {
+		// relaxed agent-scope load: served by L2, never by a stale per-CU L1 line
+		return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	}
+
+	template <bool Atomic>
+	__device__ inline float readH(const float* map, const unsigned int i) {
+		if constexpr (Atomic) {
+			return loadHeight(map + i);
+		} else {
+			return map[i];
+		}
+	}
+
+	// STPRainDrop.cu:23-51 -> (height, gradient x, gradient y); glm::mix(a, b, t) = a*(1-t) + b*t
+	template <bool Atomic>
+	__device__ inline float3 heightGradient(const float* map, const unsigned int W, const float px, const float py) {
+		const unsigned int rx = static_cast<unsigned int>(px), ry = static_cast<unsigned int>(py);
+		const float cx = px - static_cast<float>(rx), cy = py - static_cast<float>(ry);
+		const unsigned int b = ry * W + rx;
+		const float hx = readH<Atomic>(map, b), hy = readH<Atomic>(map, b + 1u), hz = readH<Atomic>(map, b + W),
+			hw = readH<Atomic>(map, b + W + 1u);
+		float3 r;
+		r.x = hx * (1 - cx) * (1 - cy) + hy * cx * (1 - cy) + hz * (1 - cx) * cy + hw * cx * cy;
+		r.y = (hy - hx) * (1 - cy) + (hw - hz) * cy;
+		r.z = (hz - hx) * (1 - cx) + (hw - hy) * cx;
+		return r;
+	}
+
+	struct DropState {
+		float px, py, dx, dy, speed, volume, sediment;
+	};
+
+	__device__ inline DropState spawn(const ErosionArgs& a, const unsigned int drop) {
+		// STPHeightfieldKernel.cu:93-102: centre chunk only, u * (Map - 1) + (NN / 2) * Map
+		const STPRandom::Philox4 r = STPRandom::philox4x32_10(drop, a.StreamId, 0u, 0u,
+			static_cast<unsigned int>(a.Rain.Seed), static_cast<unsigned int>(a.Rain.Seed >> 32));
+		DropState d;
+		d.px = STPRandom::unitUniform(r.v[0]) * static_cast<float>(a.MapSize.x - 1u)
+			+ static_cast<float>((a.NN.x / 2u) * a.MapSize.x);
+		d.py = STPRandom::unitUniform(r.v[1]) * static_cast<float>(a.MapSize.y - 1u)
+			+ static_cast<float>((a.NN.y / 2u) * a.MapSize.y);
+		d.dx = 0.0f;
+		d.dy = 0.0f;
+		d.speed = a.Rain.initSpeed;
+		d.volume = a.Rain.initWaterVolume;
+		d.sediment = 0.0f;
+		return d;
+	}
+
+	// Moves the drop one cell; returns false when it dies (STPRainDrop.cu:78-105).
+	template <bool Atomic>
+	__device__ inline bool moveDrop(const ErosionArgs& a, const float* map, DropState& d, float3& hg) {
+		hg = heightGradient<Atomic>(map, a.Total.x, d.px, d.py);
+		const float inertia = a.Rain.Inertia;
+		d.dx = (-hg.y) * (1 - inertia) + d.dx * inertia;
+		d.dy = (-hg.z) * (1 - inertia) + d.dy * inertia;
+		const float length = sqrtf(d.dx * d.dx + d.dy * d.dy);
+		if (length != 0.0f) {
+			d.dx /= length;
+			d.dy /= length;
+		}
+		d.px += d.dx;
+		d.py += d.dy;
+		const float radius = static_cast<float>(a.Rain.ErosionBrushRadius);
+		// STPRainDrop.cu:96-105, written so that a NaN position also dies
+		return !(d.dx == 0.0f && d.dy == 0.0f) && d.px >= radius && d.px < 1.0f * static_cast<float>(a.Total.x) - radius
+			&& d.py >= radius && d.py < 1.0f * static_cast<float>(a.Total.y) - radius;
+	}
+
+	// Footprint guard (no reference counterpart): the cell a drop erodes/deposits at must keep its
+	// brush (reach r-1) and its 2x2 cell (reach +1) inside the map. In exact arithmetic the kill
+	// test already implies this; it can only fail when a unit direction rounds above 1 in
+	// magnitude. Spawns whose 2x2 cell leaves the map (u == 1 on a 1x1 neighbourhood, which the
+	// reference reads out of bounds for) die before their first step.
+	__device__ inline bool cellInside(const ErosionArgs& a, const unsigned int cx, const unsigned int cy) {
+		const unsigned int r = a.Rain.ErosionBrushRadius, reach = r > 2u ? r - 1u : 1u;
+		return cx + 1u >= r && cy + 1u >= r && cx + reach <= a.Total.x - 1u && cy + reach <= a.Total.y - 1u;
+	}
+
+	__device__ inline bool spawnInside(const ErosionArgs& a, const DropState& d) {
+		return static_cast<unsigned int>(d.px) + 1u <= a.Total.x - 1u
+			&& static_cast<unsigned int>(d.py) + 1u <= a.Total.y - 1u;
+	}
+
+	__device__ inline void advance(const ErosionArgs& a, DropState& d, const float deltaHeight) {
+		d.speed = sqrtf(fmaxf(0.0f, d.speed * d.speed + deltaHeight * a.Rain.Gravity));
+		d.speed *= 1.0f - a.Rain.Friction;
+		d.volume *= (1.0f - a.Rain.EvaporateSpeed);
+	}
+
+	__device__ inline bool alive(const ErosionArgs& a, const DropState& d, const unsigned int life) {
+		return (a.Rain.MaxLifetime == 0u || life < a.Rain.MaxLifetime) && d.volume >= a.Rain.minWaterVolume;
+	}
+
+	// Taps whose heights are fetched together: one memory round trip covers the r = 3 brush.
+	constexpr unsigned int TapGroup = 32u;
+
+	// One drop with FP32 atomics (parallel mode). `brush` is the LDS copy (index, weight bits).
+	// The brush taps of one step are distinct cells, so fetching all their heights before
+	// issuing the atomics leaves this drop's own result unchanged (the reference reads and
+	// updates them one by one) while needing one round trip instead of one per tap.
+	__device__ inline void erodeDropParallel(float* const map, const ErosionArgs& a, const int2* const brush,
+		const unsigned int drop, float* const ledger, uint2* const drop_info) {
+		DropState d = spawn(a, drop);
+		const unsigned int W = a.Total.x;
+		float lost = 0.0f;
+		unsigned int steps = 0u, erodeSteps = 0u, depositSteps = 0u, killed = 0u;
+		if (!spawnInside(a, d)) {
+			d.volume = 0.0f;
+			killed = 1u;
+		}
+		for (unsigned int life = 0u; alive(a, d, life); life++) {
+			steps++;
+			const unsigned int cellX = static_cast<unsigned int>(d.px), cellY = static_cast<unsigned int>(d.py);
+			const unsigned int mapIndex = cellY * W + cellX;
+			const float ox = d.px - static_cast<float>(static_cast<int>(d.px)),
+				oy = d.py - static_cast<float>(static_cast<int>(d.py));
+			float3 hg;
+			if (!moveDrop<true>(a, map, d, hg) || !cellInside(a, cellX, cellY)) {
+				lost = d.sediment;
+				d.volume = 0.0f;
+				d.sediment = 0.0f;
+				killed = 1u;
+				break;
+			}
+			const float deltaHeight = heightGradient<true>(map, W, d.px, d.py).x - hg.x;
+			const float cap = fmaxf(-deltaHeight * d.speed * d.volume * a.Rain.SedimentCapacityFactor,
+				a.Rain.minSedimentCapacity);
+			if (d.sediment > cap || deltaHeight > 0.0f) {
+				const float amount = (deltaHeight > 0.0f) ? fminf(deltaHeight, d.sediment)
+														  : (d.sediment - cap) * a.Rain.DepositSpeed;
+				d.sediment -= amount;
+				unsafeAtomicAdd(map + mapIndex, amount * (1.0f - ox) * (1.0f - oy));
+				unsafeAtomicAdd(map + mapIndex + 1u, amount * ox * (1.0f - oy));
+				unsafeAtomicAdd(map + mapIndex + W, amount * (1.0f - ox) * oy);
+				unsafeAtomicAdd(map + mapIndex + W + 1u, amount * ox * oy);
+				depositSteps++;
+			} else {
+				const float amount = fminf((cap - d.sediment) * a.Rain.ErodeSpeed, -deltaHeight);
+				for (unsigned int base = 0u; base < a.BrushSize; base += TapGroup) {
+					float h[TapGroup];
+#pragma unroll
+					for (unsigned int j = 0u; j < TapGroup; j++) {
+						if (base + j < a.BrushSize) {
+							h[j] = loadHeight(map + mapIndex + static_cast<unsigned int>(brush[base + j].x));
+						}
+					}
+#pragma unroll
+					for (unsigned int j = 0u; j < TapGroup; j++) {
+						if (base + j < a.BrushSize) {
+							const int2 b = brush[base + j];
+							const float wa = amount * __int_as_float(b.y);
+							const float ds = (h[j] < wa) ? h[j] : wa;
+							unsafeAtomicAdd(map + mapIndex + static_cast<unsigned int>(b.x), -ds);
+							d.sediment += ds;
+						}
+					}
+				}
+				erodeSteps++;
+			}
+			advance(a, d, deltaHeight);
+		}
+		if (ledger) {
+			ledger[drop] = lost + d.sediment;
+		}
+		if (drop_info) {
+			drop_info[drop] = make_uint2(steps | (erodeSteps << 16), depositSteps | (killed << 16));
+		}
+	}
+
+} // namespace SuperTerrainPlus::STPStage
+
+#endif // STP_AMD_STAGE_EROSION_HPP

@@ -1,0 +1,160 @@
+"""Batched path (one launch per stage for all chunks, hipGraph replay) against the oracle and
+against the plain per-chunk path: bit-exact without erosion, statistically equal with it."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from superterrainplus_amd import capi, workloads as W
+
+from helpers import height_histogram, log, oracle_rain
+
+pytestmark = pytest.mark.gpu
+
+NOISE = capi.SimplexNoiseSetting(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+RADIUS, COUNT, CELL, SEED = 8, 4, 96, 7
+SIDES = [64, 128, 64, 96, 256, 64]
+ORIGINS = [(-1000, 500), (0, 0), (333, -777), (4096, 4096), (-2048, 1024), (64, 64)]
+
+
+def props():
+    return [(p[0], 8, p[2], p[3], p[4], p[5]) for p in W.biome_table(COUNT, SEED)]
+
+
+class Batch:
+    def __init__(self, dpp, sides=SIDES, origins=ORIGINS, radius=RADIUS):
+        self.sides, self.origins, self.radius = sides, origins, radius
+        setting = capi.BatchSetting(SEED, COUNT, CELL, radius, 8, dpp)
+        chunks = (capi.BatchChunk * len(sides))(*[capi.BatchChunk(s, (C.c_int32 * 2)(*o))
+                                                   for s, o in zip(sides, origins)])
+        h = C.c_void_p()
+        capi.call("stp_batch_create", C.byref(NOISE), capi.biome_array(props()), COUNT,
+                  C.byref(W.erosion_setting(1)), C.byref(setting), chunks, len(sides), C.byref(h))
+        self.h = h.value
+
+    def run(self, mode):
+        capi.call("stp_batch_run", C.c_void_p(self.h), mode, None)
+        capi.synchronize()
+
+    def read(self, i):
+        s, r = self.sides[i], self.radius
+        h = np.zeros((s, s), np.float32)
+        f = np.zeros((s, s), np.uint16)
+        b = np.zeros((s + 2 * r, s + 2 * r), np.uint16)
+        capi.call("stp_batch_read_chunk", C.c_void_p(self.h), i, capi.ptr(h), capi.ptr(f), capi.ptr(b), None)
+        return h, f, b
+
+    def ledger(self, i, drops):
+        out = np.zeros(max(drops, 1), np.float32)
+        capi.call("stp_batch_read_chunk", C.c_void_p(self.h), i, None, None, None, capi.ptr(out))
+        return out[:drops]
+
+    def counters(self):
+        out = (C.c_uint64 * 6)()
+        capi.call("stp_batch_counters", C.c_void_p(self.h), out, None)
+        return list(out)
+
+    def __del__(self):
+        capi.lib().stp_batch_destroy(C.c_void_p(self.h))
+
+
+def oracle_chunk(oracle, i):
+    s, o, r = SIDES[i], ORIGINS[i], RADIUS
+    halo = oracle.voronoi_map(SEED, COUNT, CELL, (o[0] - r, o[1] - r), s + 2 * r, s + 2 * r)
+    item, weight, off = oracle.shf_window(halo, (r, r), (s, s), r)
+    perm, g32, _ = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+    h = oracle.multi_biome(perm, g32, props(), item, weight, off, (s, s), (o[0] + s / 2, o[1] + s / 2))
+    return halo, h
+
+
+def test_batch_without_erosion_bit_exact(gpu, oracle):
+    log("batch no erosion")
+    b = Batch(0.0)
+    results = {}
+    for mode in (0, 1, 2):
+        b.run(mode)
+        results[mode] = [b.read(i) for i in range(len(SIDES))]
+    for i in range(len(SIDES)):
+        halo, h = oracle_chunk(oracle, i)
+        gh, gf, gb = results[0][i]
+        assert (gb == halo).all(), i
+        assert gh.tobytes() == h.tobytes(), i
+        assert (gf == oracle.format_heightmap(h)).all(), i
+        for mode in (1, 2):
+            assert all((x == y).all() for x, y in zip(results[mode][i], results[0][i])), (mode, i)
+
+
+def test_batch_graph_replays_are_stable(gpu):
+    """Replays of the captured graph with eager work in between give identical results."""
+    log("batch graph replays")
+    b = Batch(0.0)
+    b.run(0)
+    first = [b.read(i) for i in range(len(SIDES))]
+    for _ in range(3):
+        b.run(1)
+        b.run(0)
+        again = [b.read(i) for i in range(len(SIDES))]
+        assert all((x == y).all() for c0, c1 in zip(first, again) for x, y in zip(c0, c1))
+
+
+def test_batch_erosion_against_serialized_oracle(gpu, oracle):
+    """Per chunk: parallel erosion inside the batch vs the serialized FP32 oracle erosion of the
+    same pre-erosion heightfield, same seed and stream id (= chunk index)."""
+    log("batch erosion statistics")
+    dpp = 0.25
+    pre = Batch(0.0)
+    pre.run(0)
+    b = Batch(dpp)
+    b.run(0)
+    cnt = b.counters()
+    drops_total = sum(int(s * s * dpp) for s in SIDES)
+    assert cnt[0] == sum(s * s for s in SIDES) and cnt[1] == drops_total
+    assert cnt[2] <= 30 * drops_total and cnt[5] <= drops_total
+    setting = W.erosion_setting(1)
+    for i, s in enumerate(SIDES):
+        drops = int(s * s * dpp)
+        before = pre.read(i)[0]
+        after, fixed, _ = b.read(i)
+        ledger = b.ledger(i, drops)
+        moved = float(ledger.astype(np.float64).sum())
+        assert abs((after.astype(np.float64).sum() - before.astype(np.float64).sum()) + moved) <= 2e-4 * moved + 1e-6
+        ser = before.copy()
+        sled, _ = oracle.erode(ser, (1, 1), (s, s), oracle_rain(oracle, setting), drops, setting.Seed, stream_id=i)
+        assert abs(moved - float(sled.astype(np.float64).sum())) <= 0.03 * moved
+        if s >= 128:
+            erosion_rms = np.sqrt(np.mean((ser.astype(np.float64) - before) ** 2))
+            diff_rms = np.sqrt(np.mean((after.astype(np.float64) - ser) ** 2))
+            assert diff_rms <= 0.5 * erosion_rms, (i, diff_rms, erosion_rms)
+            assert np.abs(height_histogram(after) - height_histogram(ser)).sum() <= 0.01 * after.size
+        assert (fixed == oracle.format_heightmap(after)).all()
+
+
+def test_batch_plain_matches_graph_statistically(gpu):
+    log("batch plain vs graph")
+    b = Batch(0.25)
+    b.run(2)
+    plain = [b.read(i)[0] for i in range(len(SIDES))]
+    lat = (C.c_double * len(SIDES))()
+    capi.call("stp_batch_chunk_latencies", C.c_void_p(b.h), lat, len(SIDES))
+    assert all(v > 0 for v in lat)
+    b.run(0)
+    graph = [b.read(i)[0] for i in range(len(SIDES))]
+    for p, g in zip(plain, graph):
+        assert np.abs(height_histogram(p) - height_histogram(g)).sum() <= 0.02 * p.size
+
+
+def test_batch_stage_timing(gpu):
+    b = Batch(0.25)
+    capi.call("stp_batch_set_timing", C.c_void_p(b.h), 1)
+    for mode in (0, 1):
+        b.run(mode)
+        ms = (C.c_float * 6)()
+        capi.call("stp_batch_stage_times", C.c_void_p(b.h), ms, 6)
+        assert all(v >= 0 for v in ms) and sum(ms) > 0
+
+
+def test_batch_rejects_bad_settings(gpu):
+    with pytest.raises(capi.STPError):
+        Batch(0.25, radius=3)
+    with pytest.raises(capi.STPError):
+        Batch(0.25, sides=[4], origins=[(0, 0)])
