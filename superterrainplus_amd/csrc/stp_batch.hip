@@ -316,6 +316,14 @@ STPHeightfieldBatch::STPHeightfieldBatch(const STPEnvironment::STPSimplexNoiseSe
 }
 
 STPHeightfieldBatch::~STPHeightfieldBatch() {
+	for (int k = 0; k < StageCount; k++) {
+		if (this->StageExec[k]) {
+			(void)hipGraphExecDestroy(this->StageExec[k]);
+		}
+		if (this->StageGraph[k]) {
+			(void)hipGraphDestroy(this->StageGraph[k]);
+		}
+	}
 	if (this->GraphExec) {
 		(void)hipGraphExecDestroy(this->GraphExec);
 	}
@@ -330,58 +338,53 @@ STPHeightfieldBatch::~STPHeightfieldBatch() {
 	}
 }
 
-void STPHeightfieldBatch::enqueue(const hipStream_t stream, const unsigned int first, const unsigned int last,
-	const bool timed, const bool clear_flag) {
+void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage, const unsigned int first,
+	const unsigned int last, const bool clear_flag) {
 	const unsigned int radius = this->Setting.Radius, cap = this->Setting.BinCapacity;
 	const auto range = [&](const int s) {
 		const unsigned int b = this->Range[first].Begin[s], e = this->Range[last - 1u].End[s];
 		return std::make_pair(this->Tile[s].get() + b, e - b);
 	};
-	const auto mark = [&](const int k) {
-		if (timed) {
-			STP_CHECK_HIP(hipEventRecord(this->Event[k], stream));
+	switch (stage) {
+	case 0: {
+		if (clear_flag) {
+			zeroKERNEL<<<1, 64, 0, stream>>>(this->Flag.get(), 4u);
 		}
-	};
-	mark(0);
-	if (clear_flag) {
-		zeroKERNEL<<<1, 64, 0, stream>>>(this->Flag.get(), 4u);
-	}
-	{
 		const auto [t, n] = range(0);
 		if (n) {
 			biomeKERNEL<<<n, BiomeTile, 0, stream>>>(t, this->ChunkRecord.get(), this->Setting.BiomeSeed,
 				this->Setting.BiomeCount, this->Setting.BiomeCell, radius, this->BiomeMap.get());
 		}
+		break;
 	}
-	mark(1);
-	{
+	case 1: {
 		const auto [t, n] = range(1);
 		if (n) {
 			verticalKERNEL<<<n, ShfLineBlock, this->VCap * ShfLineBlock * sizeof(unsigned int), stream>>>(t,
 				this->ChunkRecord.get(), radius, this->VCap, this->BiomeMap.get(), this->VBin.get(), this->VCount.get(),
 				this->Flag.get());
 		}
+		break;
 	}
-	mark(2);
-	{
+	case 2: {
 		const auto [t, n] = range(2);
 		if (n) {
 			horizontalKERNEL<<<n, ShfLineBlock, 2u * cap * ShfLineBlock * sizeof(unsigned int), stream>>>(t,
 				this->ChunkRecord.get(), radius, this->VCap, cap, this->VBin.get(), this->VCount.get(), this->BinCount.get(),
 				this->BinItem.get(), this->BinWeight.get(), this->Flag.get());
 		}
+		break;
 	}
-	mark(3);
-	{
+	case 3: {
 		const auto [t, n] = range(3);
 		if (n) {
 			blendKERNEL<<<n, dim3(BlendX, BlendY), 0, stream>>>(t, this->ChunkRecord.get(), this->Permutation.Permutation,
 				this->Biome, this->BiomeCount, cap, this->BinCount.get(), this->BinItem.get(), this->BinWeight.get(),
 				this->Height.get());
 		}
+		break;
 	}
-	mark(4);
-	{
+	case 4: {
 		const auto [t, n] = range(4);
 		if (n) {
 			ErosionArgs args;
@@ -395,16 +398,63 @@ void STPHeightfieldBatch::enqueue(const hipStream_t stream, const unsigned int f
 				this->BrushOffset.get(), this->BrushWeight.get(), this->Height.get(), this->Ledger.get(),
 				this->DropInfo.get());
 		}
+		break;
 	}
-	mark(5);
-	{
+	case 5: {
 		const unsigned long long b = this->Record[first].Pixel;
 		const Chunk& l = this->Record[last - 1u];
 		const unsigned long long e = l.Pixel + static_cast<unsigned long long>(l.Side) * l.Side;
 		STPHeightfieldKernel::formatHeightmap(this->Height.get() + b, this->Fixed.get() + b, e - b, stream);
+		break;
 	}
-	mark(6);
+	default:
+		STP_THROW(STPInvalidArgument, "unknown stage");
+	}
 	STP_CHECK_HIP(hipGetLastError());
+}
+
+void STPHeightfieldBatch::enqueue(const hipStream_t stream, const unsigned int first, const unsigned int last,
+	const bool timed, const bool clear_flag) {
+	for (int s = 0; s < StageCount; s++) {
+		if (timed) {
+			STP_CHECK_HIP(hipEventRecord(this->Event[s], stream));
+		}
+		this->enqueueStage(stream, s, first, last, clear_flag);
+	}
+	if (timed) {
+		STP_CHECK_HIP(hipEventRecord(this->Event[StageCount], stream));
+	}
+}
+
+hipGraphExec_t STPHeightfieldBatch::capture(const int stage_begin, const int stage_end, hipGraph_t& graph) {
+	const unsigned int n = static_cast<unsigned int>(this->Record.size());
+	STP_CHECK_HIP(hipStreamBeginCapture(this->CaptureStream, hipStreamCaptureModeThreadLocal));
+	bool ok = true;
+	try {
+		for (int s = stage_begin; s < stage_end; s++) {
+			this->enqueueStage(this->CaptureStream, s, 0u, n, true);
+		}
+	} catch (...) {
+		ok = false;
+	}
+	hipGraph_t g = nullptr;
+	const hipError_t end = hipStreamEndCapture(this->CaptureStream, &g);
+	if (!ok || end != hipSuccess) {
+		if (g) {
+			(void)hipGraphDestroy(g);
+		}
+		(void)hipGetLastError();
+		STP_THROW(STPHIPError, "graph capture of the batch failed");
+	}
+	hipGraphExec_t exec = nullptr;
+	const hipError_t inst = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+	if (inst != hipSuccess) {
+		(void)hipGraphDestroy(g);
+		(void)hipGetLastError();
+		STP_THROW(STPHIPError, std::string("graph instantiation failed: ") + hipGetErrorString(inst));
+	}
+	graph = g;
+	return exec;
 }
 
 void STPHeightfieldBatch::run(const STPRunMode mode, const hipStream_t stream) {
@@ -412,28 +462,26 @@ void STPHeightfieldBatch::run(const STPRunMode mode, const hipStream_t stream) {
 	this->LastTimed = false;
 	switch (mode) {
 	case STPRunMode::Graph:
-		if (!this->GraphExec) {
-			STP_CHECK_HIP(hipStreamBeginCapture(this->CaptureStream, hipStreamCaptureModeThreadLocal));
-			bool ok = true;
-			try {
-				this->enqueue(this->CaptureStream, 0u, n, this->Timing, true);
-			} catch (...) {
-				ok = false;
-			}
-			hipGraph_t g = nullptr;
-			const hipError_t end = hipStreamEndCapture(this->CaptureStream, &g);
-			if (!ok || end != hipSuccess) {
-				if (g) {
-					(void)hipGraphDestroy(g);
+		if (this->Timing) {
+			// one graph per stage; the events are recorded on the stream between the graph launches
+			// (an event captured inside a graph cannot be queried afterwards)
+			if (!this->StageExec[0]) {
+				for (int s = 0; s < StageCount; s++) {
+					this->StageExec[s] = this->capture(s, s + 1, this->StageGraph[s]);
 				}
-				STP_THROW(STPHIPError, "graph capture of the batch failed");
 			}
-			this->Graph = g;
-			STP_CHECK_HIP(hipGraphInstantiate(&this->GraphExec, this->Graph, nullptr, nullptr, 0));
-			this->GraphTimed = this->Timing;
+			for (int s = 0; s < StageCount; s++) {
+				STP_CHECK_HIP(hipEventRecord(this->Event[s], stream));
+				STP_CHECK_HIP(hipGraphLaunch(this->StageExec[s], stream));
+			}
+			STP_CHECK_HIP(hipEventRecord(this->Event[StageCount], stream));
+			this->LastTimed = true;
+		} else {
+			if (!this->GraphExec) {
+				this->GraphExec = this->capture(0, StageCount, this->Graph);
+			}
+			STP_CHECK_HIP(hipGraphLaunch(this->GraphExec, stream));
 		}
-		STP_CHECK_HIP(hipGraphLaunch(this->GraphExec, stream));
-		this->LastTimed = this->GraphTimed;
 		break;
 	case STPRunMode::Eager:
 		this->enqueue(stream, 0u, n, this->Timing, true);
@@ -457,13 +505,6 @@ void STPHeightfieldBatch::run(const STPRunMode mode, const hipStream_t stream) {
 }
 
 void STPHeightfieldBatch::setTiming(const bool enable) {
-	if (enable != this->Timing && this->GraphExec) {
-		// the captured graph records events or not; re-capture on the next graph run
-		(void)hipGraphExecDestroy(this->GraphExec);
-		(void)hipGraphDestroy(this->Graph);
-		this->GraphExec = nullptr;
-		this->Graph = nullptr;
-	}
 	this->Timing = enable;
 }
 

@@ -54,7 +54,8 @@ namespace SuperTerrainPlus {
 		~STPHeightfieldBatch();
 
 		void run(STPRunMode mode, hipStream_t stream);
-		// Record HIP events between the stages of later runs (graph runs are re-captured).
+		// Record HIP events between the stages of later runs. Timed graph runs launch one graph per
+		// stage with the events recorded on the stream between them.
 		void setTiming(bool enable);
 		// Per-stage milliseconds of the last timed run (after the stream has been synchronised).
 		std::vector<float> stageTimes() const;
@@ -103,11 +104,15 @@ namespace SuperTerrainPlus {
 		hipStream_t CaptureStream = nullptr;
 		hipGraph_t Graph = nullptr;
 		hipGraphExec_t GraphExec = nullptr;
-		bool Timing = false, GraphTimed = false, LastTimed = false;
+		hipGraph_t StageGraph[StageCount] {};
+		hipGraphExec_t StageExec[StageCount] {};
+		bool Timing = false, LastTimed = false;
 		hipEvent_t Event[StageCount + 1] {};
 		std::vector<double> Latency;
 
+		void enqueueStage(hipStream_t stream, int stage, unsigned int first, unsigned int last, bool clear_flag);
 		void enqueue(hipStream_t stream, unsigned int first, unsigned int last, bool timed, bool clear_flag);
+		hipGraphExec_t capture(int stage_begin, int stage_end, hipGraph_t& graph);
 	};
 
 } // namespace SuperTerrainPlus

@@ -23,11 +23,15 @@ namespace SuperTerrainPlus {
 		if (!(EXPR)) STP_THROW(STPNumericDomainError, MSG);                                                  \
 	} while (0)
 // STPDeviceErrorHandler.hpp:58 (STP_CHECK_CUDA), HIP edition.
+// The runtime keeps the last error per thread until hipGetLastError() reads it; it is read here so
+// that an error reported by one call is not reported again by the next launch check.
 #define STP_CHECK_HIP(EXPR)                                                                                 \
 	do {                                                                                                 \
 		const hipError_t stp_err_ = (EXPR);                                                              \
-		if (stp_err_ != hipSuccess)                                                                      \
+		if (stp_err_ != hipSuccess) {                                                                    \
+			(void)hipGetLastError();                                                                     \
 			STP_THROW(STPHIPError, std::string(#EXPR) + " -> " + hipGetErrorString(stp_err_));             \
+		}                                                                                                \
 	} while (0)
 
 	// The float/integer fields of STPRainDropSetting.h:15-53 in the order the kernels read them,

@@ -121,6 +121,12 @@ namespace SuperTerrainPlus::STPStage {
 	// Taps whose heights are fetched together: one memory round trip covers the r = 3 brush.
 	constexpr unsigned int TapGroup = 32u;
 
+	// Cell of a brush tap: the (possibly negative) linear tap offset is added to the cell index
+	// in 32-bit arithmetic. The footprint guard keeps the result inside the map.
+	__device__ inline unsigned int tapIndex(const unsigned int map_index, const int tap) {
+		return static_cast<unsigned int>(static_cast<int>(map_index) + tap);
+	}
+
 	// One drop with FP32 atomics (parallel mode). `brush` is the LDS copy (index, weight bits).
 	// The brush taps of one step are distinct cells, so fetching all their heights before
 	// issuing the atomics leaves this drop's own result unchanged (the reference reads and
@@ -168,16 +174,20 @@ namespace SuperTerrainPlus::STPStage {
 #pragma unroll
 					for (unsigned int j = 0u; j < TapGroup; j++) {
 						if (base + j < a.BrushSize) {
-							h[j] = loadHeight(map + mapIndex + static_cast<unsigned int>(brush[base + j].x));
+							// the tap's cell index is formed in 32 bits first (brush offsets are
+							// negative); only then is it added to the map pointer
+							const unsigned int ei = tapIndex(mapIndex, brush[base + j].x);
+							h[j] = loadHeight(map + ei);
 						}
 					}
 #pragma unroll
 					for (unsigned int j = 0u; j < TapGroup; j++) {
 						if (base + j < a.BrushSize) {
 							const int2 b = brush[base + j];
+							const unsigned int ei = tapIndex(mapIndex, b.x);
 							const float wa = amount * __int_as_float(b.y);
 							const float ds = (h[j] < wa) ? h[j] : wa;
-							unsafeAtomicAdd(map + mapIndex + static_cast<unsigned int>(b.x), -ds);
+							unsafeAtomicAdd(map + ei, -ds);
 							d.sediment += ds;
 						}
 					}
