@@ -31,7 +31,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "eroded heightfield Mpixel/s + p50/p99 chunk latency, batched vs plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
-STAGES = ["biome", "filter_vertical", "filter_horizontal", "blend", "erosion", "format"]
+STAGES = ["biome", "filter_vertical", "filter_horizontal", "blend", "erosion_bin", "erosion", "format"]
 # algorithmic bytes of one drop step (r = 3 brush, 25 taps): two 2x2 bilinear height reads;
 # an erode step adds 25 x (4 B read + 8 B atomic read-modify-write); a deposit step 4 x 8 B.
 BYTES_STEP, BYTES_ERODE, BYTES_DEPOSIT = 32, 25 * 12, 4 * 8
@@ -104,6 +104,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default erosion launch shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -117,6 +118,13 @@ def main():
     batch = make_batch(cfg, sides, origins, props)
     stream = C.c_void_p()
     capi.call("stp_stream_create", C.byref(stream))
+    tuning = capi.BatchTuning()
+    if args.no_autotune:
+        capi.call("stp_batch_get_tuning", batch, C.byref(tuning))
+    else:
+        capi.call("stp_batch_autotune", batch, 2, stream, C.byref(tuning))
+    print(f"[bench] erosion tile {tuning.ErosionTile}, {tuning.ErosionThreads} threads"
+          f"{'' if args.no_autotune else ' (autotuned)'}", file=sys.stderr, flush=True)
 
     # timed region: one graph per stage, HIP events recorded on the stream between the graph
     # launches (so the erosion kernel's duration is measured in the same run as the throughput)
@@ -125,13 +133,13 @@ def main():
         capi.call("stp_batch_run", batch, 0, stream)
     capi.call("stp_stream_synchronize", stream)
     print("[bench] warm-up done", file=sys.stderr, flush=True)
-    stage_ms = np.zeros(6)
+    stage_ms = np.zeros(len(STAGES))
     t0 = time.perf_counter()
     for _ in range(args.steps):
         capi.call("stp_batch_run", batch, 0, stream)
         capi.call("stp_stream_synchronize", stream)
-        ms = (C.c_float * 6)()
-        capi.call("stp_batch_stage_times", batch, ms, 6)
+        ms = (C.c_float * len(STAGES))()
+        capi.call("stp_batch_stage_times", batch, ms, len(STAGES))
         stage_ms += np.array(ms, np.float64)
     wall = time.perf_counter() - t0
     stage_ms /= args.steps
@@ -166,7 +174,7 @@ def main():
     print(f"[bench] plain {plain_mpx:.1f} Mpixel/s", file=sys.stderr, flush=True)
 
     erosion_bytes = steps * BYTES_STEP + erode * BYTES_ERODE + deposit * BYTES_DEPOSIT
-    erosion_s = stage_ms[4] / 1e3
+    erosion_s = stage_ms[STAGES.index("erosion")] / 1e3
     achieved = erosion_bytes / erosion_s / 1e9 if erosion_s > 0 else 0.0
     result = {
         "metric": METRIC, "value": round(batched_mpx, 2), "unit": "Mpixel/s", "n_gpus": 1, "steps": args.steps,
@@ -182,6 +190,8 @@ def main():
             "batched_step_latency_ms": round(ms_per_step, 3),
             "single_graph_mpixel_s": round(pixels / single_wall / 1e6, 2),
             "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES, stage_ms)},
+            "erosion_tuning": {"tile": tuning.ErosionTile, "threads": tuning.ErosionThreads,
+                               "autotuned": not args.no_autotune},
             "erosion_counters": {"steps": steps, "erode_steps": erode, "deposit_steps": deposit, "killed": killed},
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

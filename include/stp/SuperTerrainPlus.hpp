@@ -28,6 +28,7 @@
 #include <utility>
 #include <vector>
 
+#include <stp/STPBiomeProperty.hpp>
 #include <stp/STPSimplexNoise.hpp>
 #include <stp/STPSingleHistogramWrapper.hpp>
 
@@ -138,16 +139,6 @@ namespace SuperTerrainPlus {
 		};
 
 	} // namespace STPEnvironment
-
-	// STPBiomeProperty.hpp:10-27 (demo plugin parameters)
-	struct STPBiomeProperty {
-		float Scale;
-		unsigned int Octave;
-		float Persistence;
-		float Lacunarity;
-		float Depth;
-		float Variation;
-	};
 
 	// RAII over HIP memory (STPSmartDeviceMemory.inl:47-68).
 	namespace STPSmartDeviceMemory {

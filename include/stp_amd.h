@@ -251,6 +251,13 @@ int stp_diversity_simplex_create(const stp_simplex_noise_setting* noise, const s
 int stp_diversity_biomefield_create(const stp_simplex_noise_setting* noise, const stp_biome_property* h_table,
 	uint32_t biome_count, const uint32_t map_size[2], uint32_t radius, uint32_t bin_capacity,
 	stp_diversity_generator** out);
+/* Multi-biome plugin whose blend kernel is compiled at run time with hipRTC (the reference demo's
+ * NVRTC-compiled STPMultiHeightGenerator.cu, SuperDemo+/World/Biomes/STPBiomefieldGenerator.cpp:44-77).
+ * `script` NULL = the built-in script; `options` are extra compiler options. A script that does
+ * not compile gives STP_ERROR_COMPILATION with the compiler log in stp_last_error(). */
+int stp_diversity_runtime_create(const stp_simplex_noise_setting* noise, const stp_biome_property* h_table,
+	uint32_t biome_count, const uint32_t map_size[2], uint32_t radius, uint32_t bin_capacity, const char* script,
+	const char* const* options, uint32_t option_count, stp_diversity_generator** out);
 int stp_diversity_destroy(stp_diversity_generator* generator);
 
 int stp_heightfield_generator_create(const stp_chunk_setting* chunk, const stp_raindrop_setting* raindrop,
@@ -273,6 +280,40 @@ int stp_heightfield_generate_device(stp_heightfield_generator* generator, float*
 	const uint16_t* const* d_biomemap, float offset_x, float offset_y, void* stream);
 int stp_heightfield_erode_device(stp_heightfield_generator* generator, float* const* d_heightfield,
 	uint16_t* const* d_fixed, void* stream);
+
+/* ------------------------------------------------------------------ runtime compilation (hipRTC)
+ * STPDeviceRuntimeBinary::compile (SuperTerrain+/SuperTerrain+/Public/SuperTerrain+/GPGPU/
+ * STPDeviceRuntimeBinary.h:26-127) and STPDeviceRuntimeProgram::link (.../STPDeviceRuntimeProgram.h:26-94).
+ * Compilation needs no GPU; linking and loading a module do. */
+typedef struct stp_rtc_program stp_rtc_program;
+typedef struct stp_rtc_module stp_rtc_module;
+enum { STP_RTC_CODE_OBJECT = 0, STP_RTC_LLVM_BITCODE = 1 };
+
+/* Compiles `source`. Name expressions name kernels, template instantiations and global
+ * variables whose lowered names should be kept; header_names/header_sources are external
+ * headers the source includes by name; library_headers != 0 also offers the library's device
+ * headers (stp/STPSimplexNoise.hpp, stp/STPSingleHistogramWrapper.hpp, stp/STPRandom.hpp,
+ * stp/STPBiomeProperty.hpp). --offload-arch (current device, else gfx950), -O3 and
+ * -ffp-contract=off are added unless given. A failed compilation returns STP_ERROR_COMPILATION
+ * and stp_last_error() holds the compiler log. */
+int stp_rtc_compile(const char* name, const char* source, const char* const* options, uint32_t option_count,
+	const char* const* name_expressions, uint32_t name_count, const char* const* header_names,
+	const char* const* header_sources, uint32_t header_count, int library_headers, stp_rtc_program** out);
+int stp_rtc_program_destroy(stp_rtc_program* program);
+/* Copies the compiler log (NUL-terminated, truncated to `capacity`); *length = full length. */
+int stp_rtc_program_log(const stp_rtc_program* program, char* h_text, uint64_t capacity, uint64_t* length);
+/* Lowered name of a registered name expression; STP_ERROR_INVALID_ARGUMENT if it was not found. */
+int stp_rtc_lowered_name(const stp_rtc_program* program, const char* name_expression, char* h_text,
+	uint64_t capacity, uint64_t* length);
+/* Links programs into a module on the current device: one STP_RTC_CODE_OBJECT program alone, or
+ * STP_RTC_LLVM_BITCODE programs (compiled with -fgpu-rdc) through the hipRTC linker. */
+int stp_rtc_link(stp_rtc_program* const* programs, const int* binary_types, uint32_t count, stp_rtc_module** out);
+int stp_rtc_module_destroy(stp_rtc_module* module);
+int stp_rtc_module_function(stp_rtc_module* module, const char* lowered_name, void** function);
+int stp_rtc_module_global(stp_rtc_module* module, const char* lowered_name, void** d_pointer, uint64_t* bytes);
+/* hipModuleLaunchKernel; `args` holds one pointer per kernel argument. Asynchronous. */
+int stp_rtc_launch(void* function, const uint32_t grid[3], const uint32_t block[3], uint32_t shared_bytes,
+	void* stream, void** args);
 
 /* ------------------------------------------------------------------ batched chunks
  * No reference counterpart (the reference makes one generate()/erode() call per chunk, each ending
@@ -305,8 +346,8 @@ int stp_batch_destroy(stp_heightfield_batch* batch);
  * (asynchronous); 2: plain -- one chunk per launch sequence with a host synchronisation after each
  * chunk, the reference's calling pattern (synchronous). */
 int stp_batch_run(stp_heightfield_batch* batch, int mode, void* stream);
-/* Record HIP events between the 6 stages (biome, vertical filter, horizontal filter, blend,
- * erosion, format) of later runs. */
+/* Record HIP events between the 7 stages (biome, vertical filter, horizontal filter, blend,
+ * erosion drop binning, erosion, format) of later runs. */
 int stp_batch_set_timing(stp_heightfield_batch* batch, int enable);
 /* Stage milliseconds of the last timed run; call after the run's stream has completed. */
 int stp_batch_stage_times(stp_heightfield_batch* batch, float* h_ms, uint32_t capacity);
@@ -315,6 +356,24 @@ int stp_batch_chunk_latencies(stp_heightfield_batch* batch, double* h_ms, uint32
 /* h_out = {pixels, drops, drop steps, erode steps, deposit steps, killed drops} of the last run
  * (synchronises `stream`); fails with STP_ERROR_CAPACITY if a histogram overflowed. */
 int stp_batch_counters(stp_heightfield_batch* batch, uint64_t h_out[6], void* stream);
+/* Launch shape of the erosion stage. ErosionTile > 0: LDS-tiled erosion -- the drops that spawn
+ * in one ErosionTile x ErosionTile tile run on an LDS copy of the tile plus a halo of
+ * H = MaxLifetime + max(r, 2) + 1 cells, and the tiles run in four colour batches (tile
+ * coordinate parities) whose windows are disjoint. Needs MaxLifetime > 0, ErosionTile >= 2 H and
+ * a window that fits in LDS. 0: every drop works on the heightmap in device memory with global
+ * atomics. ErosionThreads: threads of one tile workgroup, 256, 512 or 1024. */
+typedef struct {
+	uint32_t ErosionTile;
+	uint32_t ErosionThreads;
+} stp_batch_tuning;
+
+/* STP_ERROR_INVALID_ARGUMENT if the shape cannot run on this batch. */
+int stp_batch_set_tuning(stp_heightfield_batch* batch, const stp_batch_tuning* tuning);
+int stp_batch_get_tuning(stp_heightfield_batch* batch, stp_batch_tuning* out);
+/* Autotuner: times every candidate shape with eager runs of the whole batch on `stream`
+ * (erosion stage time, `repeat` timed runs after one warm-up each), keeps the fastest and writes
+ * it to `out` (optional). Synchronous. */
+int stp_batch_autotune(stp_heightfield_batch* batch, uint32_t repeat, void* stream, stp_batch_tuning* out);
 /* Copies chunk `index`'s outputs to host memory (each pointer optional): heights and UINT16
  * (side^2), halo biome map ((side + 2 * Radius)^2), per-drop final sediment (drops). */
 int stp_batch_read_chunk(stp_heightfield_batch* batch, uint32_t index, float* h_height, uint16_t* h_fixed,

@@ -71,6 +71,10 @@ class BatchChunk(C.Structure):
     _fields_ = [("Side", C.c_uint32), ("Origin", C.c_int32 * 2)]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("ErosionTile", C.c_uint32), ("ErosionThreads", C.c_uint32)]
+
+
 class Permutation(C.Structure):
     _fields_ = [("Permutation", C.c_void_p), ("Gradient2D", C.c_void_p),
                 ("Gradient2DSize", C.c_uint32)]
@@ -150,6 +154,19 @@ _SIGNATURES = {
     "stp_batch_chunk_latencies": (_I, [_P, _P, _U32]),
     "stp_batch_counters": (_I, [_P, _P, _P]),
     "stp_batch_read_chunk": (_I, [_P, _U32, _P, _P, _P, _P]),
+    "stp_diversity_runtime_create": (_I, [_P, _P, _U32, _P, _U32, _U32, C.c_char_p, _P, _U32, _P]),
+    "stp_rtc_compile": (_I, [C.c_char_p, C.c_char_p, _P, _U32, _P, _U32, _P, _P, _U32, C.c_int, _P]),
+    "stp_rtc_program_destroy": (_I, [_P]),
+    "stp_rtc_program_log": (_I, [_P, C.c_char_p, C.c_uint64, _P]),
+    "stp_rtc_lowered_name": (_I, [_P, C.c_char_p, C.c_char_p, C.c_uint64, _P]),
+    "stp_rtc_link": (_I, [_P, _P, _U32, _P]),
+    "stp_rtc_module_destroy": (_I, [_P]),
+    "stp_rtc_module_function": (_I, [_P, C.c_char_p, _P]),
+    "stp_rtc_module_global": (_I, [_P, C.c_char_p, _P, _P]),
+    "stp_rtc_launch": (_I, [_P, _P, _P, _U32, _P, _P]),
+    "stp_batch_set_tuning": (_I, [_P, _P]),
+    "stp_batch_get_tuning": (_I, [_P, _P]),
+    "stp_batch_autotune": (_I, [_P, _U32, _P, _P]),
 }
 
 _lib = None
@@ -288,3 +305,79 @@ def biome_array(props):
     for i, p in enumerate(props):
         arr[i] = BiomeProperty(*p)
     return arr
+
+
+# ---------------------------------------------------------------- runtime compilation (hipRTC)
+
+def _strings(values):
+    values = [v.encode() if isinstance(v, str) else v for v in values]
+    return (C.c_char_p * max(len(values), 1))(*values), len(values)
+
+
+class RtcProgram:
+    """STPDeviceRuntimeBinary::compile through the C ABI. `headers` maps include names to
+    sources; `library_headers` offers the library's device headers. Raises STPError (status
+    STP_ERROR_COMPILATION, message holding the compiler log) when the source does not compile."""
+
+    def __init__(self, name, source, options=(), names=(), headers=None, library_headers=False):
+        headers = headers or {}
+        opt, nopt = _strings(list(options))
+        nam, nnam = _strings(list(names))
+        hn, nh = _strings(list(headers.keys()))
+        hs, _ = _strings(list(headers.values()))
+        h = C.c_void_p()
+        call("stp_rtc_compile", name.encode(), source.encode(), opt, nopt, nam, nnam, hn, hs, nh,
+             1 if library_headers else 0, C.byref(h))
+        self.h = h.value
+
+    def _text(self, fn, *args):
+        n = C.c_uint64()
+        call(fn, C.c_void_p(self.h), *args, None, 0, C.byref(n))
+        buf = C.create_string_buffer(n.value + 1)
+        call(fn, C.c_void_p(self.h), *args, buf, n.value + 1, C.byref(n))
+        return buf.value.decode()
+
+    def log(self):
+        return self._text("stp_rtc_program_log")
+
+    def lowered(self, expression):
+        return self._text("stp_rtc_lowered_name", expression.encode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().stp_rtc_program_destroy(C.c_void_p(self.h))
+
+
+class RtcModule:
+    """STPDeviceRuntimeProgram::link of compiled programs; kernel and global lookup; launch."""
+
+    CODE_OBJECT, LLVM_BITCODE = 0, 1
+
+    def __init__(self, programs, types):
+        self.programs = list(programs)  # kept alive with the module
+        arr = (C.c_void_p * len(programs))(*[p.h for p in programs])
+        ty = (C.c_int * len(types))(*types)
+        h = C.c_void_p()
+        call("stp_rtc_link", arr, ty, len(programs), C.byref(h))
+        self.h = h.value
+
+    def function(self, lowered):
+        f = C.c_void_p()
+        call("stp_rtc_module_function", C.c_void_p(self.h), lowered.encode(), C.byref(f))
+        return f.value
+
+    def global_(self, lowered):
+        p, n = C.c_void_p(), C.c_uint64()
+        call("stp_rtc_module_global", C.c_void_p(self.h), lowered.encode(), C.byref(p), C.byref(n))
+        return p.value, n.value
+
+    @staticmethod
+    def launch(function, grid, block, args, shared=0, stream=None):
+        """`args`: ctypes values, one per kernel argument."""
+        ptrs = (C.c_void_p * max(len(args), 1))(*[C.cast(C.pointer(a), C.c_void_p) for a in args])
+        call("stp_rtc_launch", C.c_void_p(function), (C.c_uint32 * 3)(*grid), (C.c_uint32 * 3)(*block), shared,
+             stream, ptrs)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().stp_rtc_module_destroy(C.c_void_p(self.h))

@@ -18,6 +18,7 @@
 #include "stp_stage_noise.hpp"
 #include "stp_stage_shf.hpp"
 
+#include <algorithm>
 #include <chrono>
 
 using namespace SuperTerrainPlus;
@@ -129,7 +130,155 @@ namespace SuperTerrainPlus::STPBatch {
 		args.Total = args.MapSize;
 		args.StreamId = c.StreamId;
 		args.Rain.RainDropCount = c.Drops;
-		erodeDropParallel(height + c.Pixel, args, brush, drop, ledger + c.DropBase, drop_info + c.DropBase);
+		erodeDropAnyBrush(GlobalSurface {height + c.Pixel, c.Side}, args, brush, drop, ledger + c.DropBase,
+			drop_info + c.DropBase);
+	}
+
+	// ---- LDS-tiled erosion in colour batches
+	//
+	// A chunk is cut into core tiles of `tile` x `tile` cells. Every drop belongs to the tile its
+	// spawn cell lies in (spawn positions depend only on the seed, the chunk's stream id and the
+	// drop index). One workgroup per tile copies the tile plus a halo of `halo` cells (clipped to
+	// the chunk) from HBM into LDS, runs the tile's drops on that copy with LDS atomics, and
+	// stores the window back. The halo is MaxLifetime + max(r, 2) + 1 cells: a drop moves at most
+	// one cell per step, so no access of a drop can leave the window (the window guard would kill
+	// it otherwise).
+	//
+	// Windows of neighbouring tiles overlap, so tiles run in four colour batches by the parity of
+	// their tile coordinates (the tile-level form of the conflict-free chunk batches of
+	// SuperTerrain+/SuperTerrain+/Private/World/STPWorldPipeline.cpp:339-404). With tile >= 2 halo
+	// the windows of one colour are disjoint: a batch stores its windows with plain coalesced
+	// stores, no atomics, and every later batch reads the terrain the earlier ones eroded.
+
+	constexpr unsigned int MaxTiledTaps = 128u;
+	// tile sides the autotuner tries, largest first (the default is the largest that fits)
+	constexpr unsigned int ErosionTileCandidates[] = {128u, 112u, 96u, 80u, 72u};
+
+	__device__ inline unsigned int dropTile(const DropState& d, const unsigned int side, const unsigned int tile) {
+		const unsigned int tx = (side + tile - 1u) / tile;
+		const unsigned int x = min(static_cast<unsigned int>(d.px) / tile, tx - 1u),
+			y = min(static_cast<unsigned int>(d.py) / tile, tx - 1u);
+		return y * tx + x;
+	}
+
+	__device__ inline ErosionArgs chunkArgs(ErosionArgs a, const Chunk& c) {
+		a.MapSize = make_uint2(c.Side, c.Side);
+		a.NN = make_uint2(1u, 1u);
+		a.Total = a.MapSize;
+		a.StreamId = c.StreamId;
+		a.Rain.RainDropCount = c.Drops;
+		return a;
+	}
+
+	// drop -> rank among the drops of its spawn tile
+	__global__ void __launch_bounds__(ErosionBlock) binCountKERNEL(const uint2* const tiles, const Chunk* const chunks,
+		const ErosionArgs args, const unsigned int tile, unsigned int* const tile_drops, unsigned int* const rank) {
+		const uint2 t = tiles[blockIdx.x];
+		const Chunk c = chunks[t.x];
+		const unsigned int drop = t.y * ErosionBlock + threadIdx.x;
+		if (drop >= c.Drops) {
+			return;
+		}
+		const DropState d = spawn(chunkArgs(args, c), drop);
+		rank[c.DropBase + drop] = atomicAdd(tile_drops + c.TileBase + dropTile(d, c.Side, tile), 1u);
+	}
+
+	// first slot of every tile of a chunk: DropBase + exclusive prefix sum of the tile counts
+	constexpr unsigned int ScanBlock = 256u;
+	__global__ void __launch_bounds__(ScanBlock) binScanKERNEL(const Chunk* const chunks, const unsigned int first,
+		const unsigned int tile, const unsigned int* const tile_drops, unsigned int* const tile_first) {
+		__shared__ unsigned int part[ScanBlock / 64u];
+		const Chunk c = chunks[first + blockIdx.x];
+		const unsigned int tx = (c.Side + tile - 1u) / tile, n = tx * tx;
+		const unsigned int lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+		unsigned int carry = c.DropBase;
+		for (unsigned int base = 0u; base < n; base += ScanBlock) {
+			const unsigned int i = base + threadIdx.x;
+			const unsigned int v = i < n ? tile_drops[c.TileBase + i] : 0u;
+			unsigned int s = v;
+			for (unsigned int o = 1u; o < 64u; o <<= 1) {
+				const unsigned int u = __shfl_up(s, o, 64);
+				if (lane >= o) {
+					s += u;
+				}
+			}
+			if (lane == 63u) {
+				part[wave] = s;
+			}
+			__syncthreads();
+			unsigned int before = carry, total = 0u;
+			for (unsigned int w = 0u; w < ScanBlock / 64u; w++) {
+				before += w < wave ? part[w] : 0u;
+				total += part[w];
+			}
+			if (i < n) {
+				tile_first[c.TileBase + i] = before + s - v;
+			}
+			carry += total;
+			__syncthreads();
+		}
+	}
+
+	// tile-sorted drop list
+	__global__ void __launch_bounds__(ErosionBlock) binScatterKERNEL(const uint2* const tiles, const Chunk* const chunks,
+		const ErosionArgs args, const unsigned int tile, const unsigned int* const tile_first,
+		const unsigned int* const rank, unsigned int* const bin) {
+		const uint2 t = tiles[blockIdx.x];
+		const Chunk c = chunks[t.x];
+		const unsigned int drop = t.y * ErosionBlock + threadIdx.x;
+		if (drop >= c.Drops) {
+			return;
+		}
+		const DropState d = spawn(chunkArgs(args, c), drop);
+		bin[tile_first[c.TileBase + dropTile(d, c.Side, tile)] + rank[c.DropBase + drop]] = drop;
+	}
+
+	template <unsigned int Threads>
+	__global__ void __launch_bounds__(Threads) erodeTileKERNEL(const uint2* const tiles, const Chunk* const chunks,
+		const ErosionArgs args, const unsigned int tile, const unsigned int halo, const int2* const brush_offset,
+		const float* const brush_weight, const unsigned int* const tile_drops, const unsigned int* const tile_first,
+		const unsigned int* const bin, float* const height, float* const ledger, uint2* const drop_info) {
+		extern __shared__ float window[];
+		__shared__ int2 brush[MaxTiledTaps];
+		const uint2 t = tiles[blockIdx.x];
+		const Chunk c = chunks[t.x];
+		const unsigned int side = c.Side, tilesX = (side + tile - 1u) / tile;
+		const unsigned int x0 = (t.y % tilesX) * tile, y0 = (t.y / tilesX) * tile;
+		LocalSurface s;
+		s.Lds = window;
+		s.X0 = x0 > halo ? x0 - halo : 0u;
+		s.Y0 = y0 > halo ? y0 - halo : 0u;
+		s.RW = min(x0 + tile + halo, side) - s.X0;
+		s.RH = min(y0 + tile + halo, side) - s.Y0;
+		float* const map = height + c.Pixel;
+		const unsigned int cells = s.RW * s.RH;
+		for (unsigned int i = threadIdx.x; i < cells; i += Threads) {
+			const unsigned int wy = i / s.RW, wx = i - wy * s.RW;
+			window[i] = map[(s.Y0 + wy) * side + s.X0 + wx];
+		}
+		for (unsigned int k = threadIdx.x; k < args.BrushSize; k += Threads) {
+			const int2 o = brush_offset[k];
+			brush[k] = make_int2(o.y * static_cast<int>(s.RW) + o.x, __float_as_int(brush_weight[k]));
+		}
+		__syncthreads();
+		const ErosionArgs a = chunkArgs(args, c);
+		const unsigned int g = c.TileBase + t.y, first = tile_first[g], n = tile_drops[g];
+		for (unsigned int k = threadIdx.x; k < n; k += Threads) {
+			erodeDropAnyBrush(s, a, brush, bin[first + k], ledger + c.DropBase, drop_info + c.DropBase);
+		}
+		__syncthreads();
+		// no other workgroup of this colour batch touches the window
+		for (unsigned int i = threadIdx.x; i < cells; i += Threads) {
+			const unsigned int wy = i / s.RW, wx = i - wy * s.RW;
+			map[(s.Y0 + wy) * side + s.X0 + wx] = window[i];
+		}
+	}
+
+	__global__ void __launch_bounds__(256) zeroRangeKERNEL(unsigned int* const word, const unsigned int n) {
+		const unsigned int i = blockIdx.x * 256u + threadIdx.x;
+		if (i < n) {
+			word[i] = 0u;
+		}
 	}
 
 	__global__ void zeroKERNEL(unsigned int* const word, const unsigned int n) {
@@ -260,39 +409,6 @@ STPHeightfieldBatch::STPHeightfieldBatch(const STPEnvironment::STPSimplexNoiseSe
 	this->Drops = drops;
 	this->ChunkRecord = upload(rec);
 
-	// tile tables, chunk-major so that one chunk's tiles are a contiguous range (plain mode)
-	std::vector<uint2> tb, tv, th, tl, te;
-	for (unsigned int i = 0u; i < rec.size(); i++) {
-		const Chunk& c = rec[i];
-		STPTileRange range;
-		const unsigned int w = c.Side + 2u * radius;
-		range.Begin[0] = static_cast<unsigned int>(tb.size());
-		for (unsigned int k = 0u; k < (w * w + BiomeTile - 1u) / BiomeTile; k++) tb.push_back(make_uint2(i, k));
-		range.Begin[1] = static_cast<unsigned int>(tv.size());
-		for (unsigned int k = 0u; k < (w + ShfLineBlock - 1u) / ShfLineBlock; k++) tv.push_back(make_uint2(i, k));
-		range.Begin[2] = static_cast<unsigned int>(th.size());
-		for (unsigned int k = 0u; k < (c.Side + ShfLineBlock - 1u) / ShfLineBlock; k++) th.push_back(make_uint2(i, k));
-		range.Begin[3] = static_cast<unsigned int>(tl.size());
-		const unsigned int blend = ((c.Side + BlendX - 1u) / BlendX) * ((c.Side + BlendY * BlendPixel - 1u) / (BlendY * BlendPixel));
-		for (unsigned int k = 0u; k < blend; k++) tl.push_back(make_uint2(i, k));
-		range.Begin[4] = static_cast<unsigned int>(te.size());
-		for (unsigned int k = 0u; k < (c.Drops + ErosionBlock - 1u) / ErosionBlock; k++) te.push_back(make_uint2(i, k));
-		this->Range.push_back(range);
-	}
-	const size_t counts[5] = {tb.size(), tv.size(), th.size(), tl.size(), te.size()};
-	for (unsigned int i = 0u; i < rec.size(); i++) {
-		for (int s = 0; s < 5; s++) {
-			this->Range[i].End[s] = i + 1u < rec.size() ? this->Range[i + 1u].Begin[s] : static_cast<unsigned int>(counts[s]);
-		}
-	}
-	this->TileCount = {static_cast<unsigned int>(tb.size()), static_cast<unsigned int>(tv.size()),
-		static_cast<unsigned int>(th.size()), static_cast<unsigned int>(tl.size()), static_cast<unsigned int>(te.size())};
-	this->Tile[0] = upload(tb);
-	this->Tile[1] = upload(tv);
-	this->Tile[2] = upload(th);
-	this->Tile[3] = upload(tl);
-	this->Tile[4] = upload(te);
-
 	// resident arenas
 	const unsigned int cap = setting.BinCapacity;
 	this->Height = STPSmartDeviceMemory::makeDevice<float>(pixel);
@@ -303,33 +419,220 @@ STPHeightfieldBatch::STPHeightfieldBatch(const STPEnvironment::STPSimplexNoiseSe
 	this->BinCount = STPSmartDeviceMemory::makeDevice<unsigned char>(pixel);
 	this->BinItem = STPSmartDeviceMemory::makeDevice<unsigned short>(pixel * cap);
 	this->BinWeight = STPSmartDeviceMemory::makeDevice<float>(pixel * cap);
+	this->DropRank = STPSmartDeviceMemory::makeDevice<unsigned int>(drops ? drops : 1u);
+	this->DropBin = STPSmartDeviceMemory::makeDevice<unsigned int>(drops ? drops : 1u);
 	this->Ledger = STPSmartDeviceMemory::makeDevice<float>(drops ? drops : 1u);
 	this->DropInfo = STPSmartDeviceMemory::makeDevice<uint2>(drops ? drops : 1u);
 	this->Flag = STPSmartDeviceMemory::makeDevice<unsigned int>(4u);
 	this->Counter = STPSmartDeviceMemory::makeDevice<unsigned long long>(4u);
 	STP_CHECK_HIP(hipMemset(this->Fixed.get(), 0, pixel * sizeof(unsigned short)));
 	STP_CHECK_HIP(hipMemset(this->Height.get(), 0, pixel * sizeof(float)));
+	this->Tuning = this->defaultTuning();
+	this->buildTileTables();
 	for (hipEvent_t& e : this->Event) {
 		STP_CHECK_HIP(hipEventCreate(&e));
 	}
 	STP_CHECK_HIP(hipStreamCreateWithFlags(&this->CaptureStream, hipStreamNonBlocking));
 }
 
-STPHeightfieldBatch::~STPHeightfieldBatch() {
+void STPHeightfieldBatch::releaseGraphs() noexcept {
 	for (int k = 0; k < StageCount; k++) {
 		if (this->StageExec[k]) {
 			(void)hipGraphExecDestroy(this->StageExec[k]);
+			this->StageExec[k] = nullptr;
 		}
 		if (this->StageGraph[k]) {
 			(void)hipGraphDestroy(this->StageGraph[k]);
+			this->StageGraph[k] = nullptr;
 		}
 	}
 	if (this->GraphExec) {
 		(void)hipGraphExecDestroy(this->GraphExec);
+		this->GraphExec = nullptr;
 	}
 	if (this->Graph) {
 		(void)hipGraphDestroy(this->Graph);
+		this->Graph = nullptr;
 	}
+}
+
+namespace {
+	template <unsigned int Threads>
+	void allowLds(const unsigned int bytes) {
+		STP_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&erodeTileKERNEL<Threads>),
+			hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+	}
+} // namespace
+
+ErosionArgs STPHeightfieldBatch::erosionArgs() const {
+	// per-chunk fields (map size, stream id, drop count) are filled in by the kernels
+	ErosionArgs args;
+	args.Rain = this->Rain;
+	args.MapSize = make_uint2(0u, 0u);
+	args.NN = make_uint2(1u, 1u);
+	args.Total = args.MapSize;
+	args.BrushSize = this->BrushSize;
+	args.StreamId = 0u;
+	return args;
+}
+
+unsigned int STPHeightfieldBatch::erosionLdsBytes(const unsigned int tile) const {
+	const unsigned int r = this->Rain.ErosionBrushRadius, life = this->Rain.MaxLifetime;
+	if (tile == 0u || life == 0u || this->BrushSize > MaxTiledTaps) {
+		return 0u;
+	}
+	const unsigned long long halo = static_cast<unsigned long long>(life) + std::max(r, 2u) + 1u;
+	if (tile < 2ull * halo) {
+		return 0u; // the windows of one colour batch would overlap
+	}
+	unsigned long long side = 0ull;
+	for (const Chunk& c : this->Record) {
+		side = std::max(side, std::min<unsigned long long>(c.Side, tile + 2ull * halo));
+	}
+	int device = 0, limit = 0;
+	STP_CHECK_HIP(hipGetDevice(&device));
+	STP_CHECK_HIP(hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+	const unsigned long long bytes = side * side * sizeof(float);
+	const unsigned long long room = static_cast<unsigned long long>(limit) - MaxTiledTaps * sizeof(int2);
+	return bytes <= room ? static_cast<unsigned int>(bytes) : 0u;
+}
+
+STPBatchTuning STPHeightfieldBatch::defaultTuning() const {
+	for (const unsigned int tile : ErosionTileCandidates) {
+		if (this->erosionLdsBytes(tile) != 0u) {
+			return STPBatchTuning {tile, 1024u};
+		}
+	}
+	return STPBatchTuning {0u, ErosionBlock};
+}
+
+void STPHeightfieldBatch::setTuning(const STPBatchTuning& tuning) {
+	const unsigned int th = tuning.ErosionThreads;
+	if (th != 256u && th != 512u && th != 1024u) {
+		STP_THROW(STPInvalidArgument, "erosion threads must be 256, 512 or 1024");
+	}
+	if (tuning.ErosionTile != 0u
+		&& (tuning.ErosionTile < 16u || tuning.ErosionTile > 4096u || this->erosionLdsBytes(tuning.ErosionTile) == 0u)) {
+		STP_THROW(STPInvalidArgument,
+			"tiled erosion needs a tile side in [16, 4096], MaxLifetime > 0 and a window that fits in LDS");
+	}
+	this->releaseGraphs();
+	this->Tuning = tuning;
+	this->buildTileTables();
+}
+
+void STPHeightfieldBatch::buildTileTables() {
+	const unsigned int radius = this->Setting.Radius, tile = this->Tuning.ErosionTile;
+	unsigned int tiles = 0u;
+	for (Chunk& c : this->Record) {
+		c.TileBase = tiles;
+		if (tile != 0u) {
+			const unsigned int tx = (c.Side + tile - 1u) / tile;
+			tiles += tx * tx;
+		}
+	}
+	this->ErosionTiles = tiles;
+	this->ErosionHalo = tile ? this->Rain.MaxLifetime + std::max(this->Rain.ErosionBrushRadius, 2u) + 1u : 0u;
+	this->ErosionLds = this->erosionLdsBytes(tile);
+	this->ChunkRecord = upload(this->Record);
+	this->TileDrops = STPSmartDeviceMemory::makeDevice<unsigned int>(tiles + 1u);
+	this->TileFirst = STPSmartDeviceMemory::makeDevice<unsigned int>(tiles + 1u);
+	if (tile != 0u) {
+		switch (this->Tuning.ErosionThreads) {
+		case 256u:
+			allowLds<256u>(this->ErosionLds);
+			break;
+		case 512u:
+			allowLds<512u>(this->ErosionLds);
+			break;
+		default:
+			allowLds<1024u>(this->ErosionLds);
+			break;
+		}
+	}
+
+	// tile tables, chunk-major so that one chunk's tiles are a contiguous range (plain mode)
+	std::vector<uint2> table[TableCount];
+	this->Range.assign(this->Record.size(), STPTileRange {});
+	for (unsigned int i = 0u; i < this->Record.size(); i++) {
+		const Chunk& c = this->Record[i];
+		STPTileRange& range = this->Range[i];
+		const unsigned int w = c.Side + 2u * radius;
+		const unsigned int blend =
+			((c.Side + BlendX - 1u) / BlendX) * ((c.Side + BlendY * BlendPixel - 1u) / (BlendY * BlendPixel));
+		const unsigned int tx = tile ? (c.Side + tile - 1u) / tile : 0u;
+		const unsigned int n[5] = {(w * w + BiomeTile - 1u) / BiomeTile, (w + ShfLineBlock - 1u) / ShfLineBlock,
+			(c.Side + ShfLineBlock - 1u) / ShfLineBlock, blend, (c.Drops + ErosionBlock - 1u) / ErosionBlock};
+		for (int s = 0; s < 5; s++) {
+			range.Begin[s] = static_cast<unsigned int>(table[s].size());
+			for (unsigned int k = 0u; k < n[s]; k++) {
+				table[s].push_back(make_uint2(i, k));
+			}
+			range.End[s] = static_cast<unsigned int>(table[s].size());
+		}
+		// erosion tiles by colour: (tile x parity) + 2 (tile y parity)
+		for (int colour = 0; colour < 4; colour++) {
+			const int s = 5 + colour;
+			range.Begin[s] = static_cast<unsigned int>(table[s].size());
+			for (unsigned int k = 0u; k < tx * tx; k++) {
+				if (static_cast<int>((k % tx) & 1u) + 2 * static_cast<int>((k / tx) & 1u) == colour) {
+					table[s].push_back(make_uint2(i, k));
+				}
+			}
+			range.End[s] = static_cast<unsigned int>(table[s].size());
+		}
+	}
+	for (int s = 0; s < TableCount; s++) {
+		this->TileCount[s] = static_cast<unsigned int>(table[s].size());
+		this->Tile[s] = upload(table[s]);
+	}
+}
+
+STPBatchTuning STPHeightfieldBatch::autotune(const hipStream_t stream, const unsigned int repeat) {
+	const bool timing = this->Timing;
+	std::vector<STPBatchTuning> candidate;
+	for (const unsigned int tile : ErosionTileCandidates) {
+		if (this->erosionLdsBytes(tile) == 0u) {
+			continue;
+		}
+		for (const unsigned int th : {256u, 512u, 1024u}) {
+			candidate.push_back(STPBatchTuning {tile, th});
+		}
+	}
+	if (candidate.empty()) {
+		candidate.push_back(STPBatchTuning {0u, ErosionBlock});
+	}
+	STPBatchTuning best = candidate.front();
+	float bestMs = 0.0f;
+	this->Timing = true;
+	try {
+		for (const STPBatchTuning& t : candidate) {
+			this->setTuning(t);
+			float ms = 0.0f;
+			for (unsigned int k = 0u; k <= repeat; k++) {
+				this->run(STPRunMode::Eager, stream);
+				STP_CHECK_HIP(hipStreamSynchronize(stream));
+				if (k > 0u) { // the first run warms up
+					const std::vector<float> st = this->stageTimes();
+					ms += st[4] + st[5];
+				}
+			}
+			if (bestMs == 0.0f || ms < bestMs) {
+				bestMs = ms;
+				best = t;
+			}
+		}
+	} catch (...) {
+		this->Timing = timing;
+		throw;
+	}
+	this->Timing = timing;
+	this->setTuning(best);
+	return best;
+}
+
+STPHeightfieldBatch::~STPHeightfieldBatch() {
+	this->releaseGraphs();
 	for (hipEvent_t e : this->Event) {
 		(void)hipEventDestroy(e);
 	}
@@ -385,22 +688,69 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 		break;
 	}
 	case 4: {
+		// tiled erosion: bin the drops by spawn tile
+		const unsigned int tile = this->Tuning.ErosionTile;
 		const auto [t, n] = range(4);
-		if (n) {
-			ErosionArgs args;
-			args.Rain = this->Rain;
-			args.MapSize = make_uint2(0u, 0u);
-			args.NN = make_uint2(1u, 1u);
-			args.Total = args.MapSize;
-			args.BrushSize = this->BrushSize;
-			args.StreamId = 0u;
-			erosionKERNEL<<<n, ErosionBlock, this->BrushSize * sizeof(int2), stream>>>(t, this->ChunkRecord.get(), args,
-				this->BrushOffset.get(), this->BrushWeight.get(), this->Height.get(), this->Ledger.get(),
-				this->DropInfo.get());
+		if (tile == 0u || n == 0u) {
+			// nothing to bin; one trivial launch keeps the stage's graph non-empty
+			zeroRangeKERNEL<<<1, 256, 0, stream>>>(this->TileDrops.get() + this->ErosionTiles, 1u);
+			break;
 		}
+		const ErosionArgs args = this->erosionArgs();
+		const unsigned int tb = this->Record[first].TileBase,
+			te = last < this->Record.size() ? this->Record[last].TileBase : this->ErosionTiles;
+		if (te > tb) {
+			zeroRangeKERNEL<<<(te - tb + 255u) / 256u, 256u, 0, stream>>>(this->TileDrops.get() + tb, te - tb);
+		}
+		binCountKERNEL<<<n, ErosionBlock, 0, stream>>>(t, this->ChunkRecord.get(), args, tile, this->TileDrops.get(),
+			this->DropRank.get());
+		binScanKERNEL<<<last - first, ScanBlock, 0, stream>>>(this->ChunkRecord.get(), first, tile,
+			this->TileDrops.get(), this->TileFirst.get());
+		binScatterKERNEL<<<n, ErosionBlock, 0, stream>>>(t, this->ChunkRecord.get(), args, tile, this->TileFirst.get(),
+			this->DropRank.get(), this->DropBin.get());
 		break;
 	}
 	case 5: {
+		const auto [t, n] = range(4);
+		if (n == 0u) {
+			break;
+		}
+		const ErosionArgs args = this->erosionArgs();
+		const unsigned int tile = this->Tuning.ErosionTile;
+		if (tile == 0u) {
+			erosionKERNEL<<<n, ErosionBlock, this->BrushSize * sizeof(int2), stream>>>(t, this->ChunkRecord.get(), args,
+				this->BrushOffset.get(), this->BrushWeight.get(), this->Height.get(), this->Ledger.get(),
+				this->DropInfo.get());
+			break;
+		}
+		for (int colour = 0; colour < 4; colour++) {
+			const auto tiles = range(5 + colour);
+			const uint2* const tt = tiles.first;
+			const unsigned int nt = tiles.second;
+			if (nt == 0u) {
+				continue;
+			}
+			const auto launch = [&](auto kernel, const unsigned int threads) {
+				kernel<<<nt, threads, this->ErosionLds, stream>>>(tt, this->ChunkRecord.get(), args, tile,
+					this->ErosionHalo, this->BrushOffset.get(), this->BrushWeight.get(), this->TileDrops.get(),
+					this->TileFirst.get(), this->DropBin.get(), this->Height.get(), this->Ledger.get(),
+					this->DropInfo.get());
+			};
+			switch (this->Tuning.ErosionThreads) {
+			case 256u:
+				launch(erodeTileKERNEL<256u>, 256u);
+				break;
+			case 512u:
+				launch(erodeTileKERNEL<512u>, 512u);
+				break;
+			default:
+				launch(erodeTileKERNEL<1024u>, 1024u);
+				break;
+			}
+		}
+		break;
+	}
+	case 6: {
 		const unsigned long long b = this->Record[first].Pixel;
 		const Chunk& l = this->Record[last - 1u];
 		const unsigned long long e = l.Pixel + static_cast<unsigned long long>(l.Side) * l.Side;
@@ -512,8 +862,8 @@ std::vector<float> STPHeightfieldBatch::stageTimes() const {
 	if (!this->LastTimed) {
 		STP_THROW(STPInvalidArgument, "the last run was not timed");
 	}
-	std::vector<float> ms(6);
-	for (int k = 0; k < 6; k++) {
+	std::vector<float> ms(StageCount);
+	for (int k = 0; k < StageCount; k++) {
 		STP_CHECK_HIP(hipEventElapsedTime(&ms[k], this->Event[k], this->Event[k + 1]));
 	}
 	return ms;

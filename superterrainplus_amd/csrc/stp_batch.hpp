@@ -3,6 +3,7 @@
 #ifndef STP_AMD_BATCH_HPP
 #define STP_AMD_BATCH_HPP
 
+#include "stp_stage_erosion.hpp"
 #include "stp_stage_noise.hpp"
 
 #include <array>
@@ -25,6 +26,16 @@ namespace SuperTerrainPlus {
 		int OriginX, OriginY;
 	};
 
+	// Launch shapes of the erosion stage (chosen by STPHeightfieldBatch::autotune or set by hand).
+	struct STPBatchTuning {
+		// Core tile side of the LDS-tiled erosion: each workgroup erodes the drops that spawn in
+		// one tile on an LDS copy of the tile plus a halo the drops cannot leave within their
+		// lifetime (MaxLifetime + max(r, 2) + 1 cells); tiles run in four colour batches, which
+		// needs tile >= 2 halo. 0 = every drop works on the heightmap in HBM with global atomics.
+		unsigned int ErosionTile;
+		unsigned int ErosionThreads; // threads of one tile workgroup: 256, 512 or 1024
+	};
+
 	namespace STPBatch {
 		// Device record of one chunk (square, `Side` pixels).
 		struct Chunk {
@@ -34,8 +45,9 @@ namespace SuperTerrainPlus {
 			unsigned long long Pixel;  // first pixel in the height / fixed / histogram arenas
 			unsigned long long Halo;   // first sample in the halo biome arena ((Side + 2r)^2)
 			unsigned long long Line;   // first cell in the vertical filter arenas ((Side + 2r) * Side)
-			unsigned int DropBase;     // first drop in the ledger / drop-info arenas
+			unsigned int DropBase;     // first drop in the ledger / drop-info / drop-bin arenas
 			unsigned int StreamId;     // Philox counter word of this chunk's drops
+			unsigned int TileBase;     // first erosion tile of the chunk in the tile-bin arenas
 		};
 	}
 
@@ -44,7 +56,8 @@ namespace SuperTerrainPlus {
 		// Graph: one captured hipGraph replay per run. Eager: the same launches without a graph.
 		// Plain: one chunk per launch sequence, the host waits for each chunk (reference pattern).
 		enum class STPRunMode : int { Graph = 0, Eager = 1, Plain = 2 };
-		static constexpr int StageCount = 6; // biome, vertical, horizontal, blend, erosion, format
+		// biome, vertical filter, horizontal filter, blend, erosion binning, erosion, format
+		static constexpr int StageCount = 7;
 
 		STPHeightfieldBatch(const STPEnvironment::STPSimplexNoiseSetting& noise, const std::vector<STPBiomeProperty>& biome,
 			const STPEnvironment::STPRainDropSetting& rain, const STPBatchSetting& setting,
@@ -67,13 +80,27 @@ namespace SuperTerrainPlus {
 		void check(hipStream_t stream);
 		void readChunk(unsigned int i, float* height, unsigned short* fixed, unsigned short* halo, float* ledger) const;
 
+		// Switches the erosion launch shape (throws STPInvalidArgument if it cannot run here: the
+		// tiled erosion needs MaxLifetime > 0 and a window that fits in LDS). Captured graphs are
+		// rebuilt at the next graph run.
+		void setTuning(const STPBatchTuning& tuning);
+		const STPBatchTuning& tuning() const noexcept { return this->Tuning; }
+		// Times every candidate launch shape on this batch (eager runs on `stream`, erosion stage
+		// time) and keeps the fastest. Returns the chosen shape.
+		STPBatchTuning autotune(hipStream_t stream, unsigned int repeat = 2u);
+		// The shape a new batch starts with: the largest tile whose window fits in LDS.
+		STPBatchTuning defaultTuning() const;
+
 		unsigned long long pixels() const noexcept;
 		unsigned long long drops() const noexcept { return this->Drops; }
 		unsigned int chunkCount() const noexcept { return static_cast<unsigned int>(this->Record.size()); }
 
 	private:
+		// tile tables: 0 biome, 1 vertical filter, 2 horizontal filter, 3 blend, 4 drops,
+		// 5..8 erosion tiles of colour 0..3
+		static constexpr int TableCount = 9;
 		struct STPTileRange {
-			unsigned int Begin[5], End[5];
+			unsigned int Begin[TableCount], End[TableCount];
 		};
 
 		STPBatchSetting Setting;
@@ -84,14 +111,19 @@ namespace SuperTerrainPlus {
 		unsigned int VCap = 0u, BrushSize = 0u;
 		std::vector<STPBatch::Chunk> Record;
 		std::vector<STPTileRange> Range;
-		std::array<unsigned int, 5> TileCount {};
+		std::array<unsigned int, TableCount> TileCount {};
+		STPBatchTuning Tuning {};
+		unsigned int ErosionTiles = 0u, ErosionHalo = 0u, ErosionLds = 0u;
 		unsigned long long Pixels = 0ull, Drops = 0ull;
 
 		STPSmartDeviceMemory::STPDevice<STPBatch::Chunk> ChunkRecord;
-		STPSmartDeviceMemory::STPDevice<uint2> Tile[5];
+		STPSmartDeviceMemory::STPDevice<uint2> Tile[TableCount];
 		STPSmartDeviceMemory::STPDevice<int2> BrushOffset;
 		STPSmartDeviceMemory::STPDevice<float> BrushWeight;
 		STPSmartDeviceMemory::STPDevice<float> Height;
+		// tiled erosion: per drop its rank in its spawn tile and the tile-sorted drop list; per
+		// tile its drop count and first slot
+		STPSmartDeviceMemory::STPDevice<unsigned int> DropRank, DropBin, TileDrops, TileFirst;
 		STPSmartDeviceMemory::STPDevice<unsigned short> Fixed, BiomeMap;
 		STPSmartDeviceMemory::STPDevice<unsigned int> VBin;
 		STPSmartDeviceMemory::STPDevice<unsigned char> VCount, BinCount;
@@ -113,6 +145,11 @@ namespace SuperTerrainPlus {
 		void enqueueStage(hipStream_t stream, int stage, unsigned int first, unsigned int last, bool clear_flag);
 		void enqueue(hipStream_t stream, unsigned int first, unsigned int last, bool timed, bool clear_flag);
 		hipGraphExec_t capture(int stage_begin, int stage_end, hipGraph_t& graph);
+		void releaseGraphs() noexcept;
+		void buildTileTables();
+		// Window side bound and LDS bytes of the tiled erosion for a tile side (0 if impossible).
+		unsigned int erosionLdsBytes(unsigned int tile) const;
+		STPStage::ErosionArgs erosionArgs() const;
 	};
 
 } // namespace SuperTerrainPlus

@@ -3,6 +3,8 @@
 #include "stp_batch.hpp"
 #include "stp_internal.hpp"
 
+#include <stp/STPDeviceRuntime.hpp>
+
 #include <stp_amd.h>
 
 #include <cstring>
@@ -147,6 +149,14 @@ struct stp_filter_buffer {
 
 struct stp_diversity_generator {
 	std::unique_ptr<STPDiversityGenerator> Generator;
+};
+
+struct stp_rtc_program {
+	STPDeviceRuntimeBinary::STPCompilationOutput Output;
+};
+
+struct stp_rtc_module {
+	STPDeviceRuntimeProgram::STPSmartModule Module;
 };
 
 struct stp_heightfield_batch {
@@ -500,6 +510,28 @@ int stp_diversity_biomefield_create(const stp_simplex_noise_setting* const noise
 	});
 }
 
+int stp_diversity_runtime_create(const stp_simplex_noise_setting* const noise, const stp_biome_property* const h_table,
+	const uint32_t biome_count, const uint32_t map_size[2], const uint32_t radius, const uint32_t bin_capacity,
+	const char* const script, const char* const* const options, const uint32_t option_count,
+	stp_diversity_generator** const out) {
+	return guarded([&] {
+		require(h_table && map_size && out && (options || option_count == 0u), "null argument");
+		std::vector<STPBiomeProperty> t;
+		for (uint32_t i = 0u; i < biome_count; i++) {
+			t.push_back(toBiome(h_table[i]));
+		}
+		std::vector<std::string> opt;
+		for (uint32_t i = 0u; i < option_count; i++) {
+			require(options[i], "null option");
+			opt.emplace_back(options[i]);
+		}
+		auto g = std::make_unique<stp_diversity_generator>();
+		g->Generator = std::make_unique<STPRuntimeBiomefieldGenerator>(toNoise(noise), std::move(t),
+			make_uint2(map_size[0], map_size[1]), radius, bin_capacity, script ? std::string(script) : std::string(), opt);
+		*out = g.release();
+	});
+}
+
 int stp_diversity_destroy(stp_diversity_generator* const generator) {
 	return guarded([&] { delete generator; });
 }
@@ -560,6 +592,133 @@ int stp_heightfield_erode_device(stp_heightfield_generator* const generator, flo
 	});
 }
 
+namespace {
+	void copyText(const std::string& text, char* const h_text, const uint64_t capacity, uint64_t* const length) {
+		if (length) {
+			*length = text.size();
+		}
+		if (h_text && capacity > 0u) {
+			const size_t n = std::min<size_t>(text.size(), capacity - 1u);
+			std::memcpy(h_text, text.data(), n);
+			h_text[n] = '\0';
+		}
+	}
+} // namespace
+
+int stp_rtc_compile(const char* const name, const char* const source, const char* const* const options,
+	const uint32_t option_count, const char* const* const name_expressions, const uint32_t name_count,
+	const char* const* const header_names, const char* const* const header_sources, const uint32_t header_count,
+	const int library_headers, stp_rtc_program** const out) {
+	return guarded([&] {
+		require(name && source && out, "null argument");
+		require((options || option_count == 0u) && (name_expressions || name_count == 0u)
+				&& ((header_names && header_sources) || header_count == 0u),
+			"null argument array");
+		STPDeviceRuntimeBinary::STPSourceInformation info;
+		for (uint32_t i = 0u; i < option_count; i++) {
+			require(options[i], "null option");
+			info.Option[options[i]];
+		}
+		for (uint32_t i = 0u; i < name_count; i++) {
+			require(name_expressions[i], "null name expression");
+			info.NameExpression[name_expressions[i]];
+		}
+		STPDeviceRuntimeBinary::STPExternalHeaderSource headers;
+		for (uint32_t i = 0u; i < header_count; i++) {
+			require(header_names[i] && header_sources[i], "null header");
+			headers.emplace(header_names[i], header_sources[i]);
+			info.ExternalHeader[header_names[i]];
+		}
+		if (library_headers) {
+			for (const auto& h : STPDeviceRuntimeBinary::libraryHeaders()) {
+				if (headers.emplace(h.first, h.second).second) {
+					info.ExternalHeader[h.first];
+				}
+			}
+		}
+		auto p = std::make_unique<stp_rtc_program>();
+		p->Output = STPDeviceRuntimeBinary::compile(std::string(name), std::string(source), info, headers);
+		*out = p.release();
+	});
+}
+
+int stp_rtc_program_destroy(stp_rtc_program* const program) {
+	return guarded([&] { delete program; });
+}
+
+int stp_rtc_program_log(const stp_rtc_program* const program, char* const h_text, const uint64_t capacity,
+	uint64_t* const length) {
+	return guarded([&] {
+		require(program, "null program");
+		copyText(program->Output.Log, h_text, capacity, length);
+	});
+}
+
+int stp_rtc_lowered_name(const stp_rtc_program* const program, const char* const name_expression, char* const h_text,
+	const uint64_t capacity, uint64_t* const length) {
+	return guarded([&] {
+		require(program && name_expression, "null argument");
+		const auto it = program->Output.LoweredName.find(name_expression);
+		require(it != program->Output.LoweredName.end(), "the name expression has no lowered name");
+		copyText(it->second, h_text, capacity, length);
+	});
+}
+
+int stp_rtc_link(stp_rtc_program* const* const programs, const int* const binary_types, const uint32_t count,
+	stp_rtc_module** const out) {
+	return guarded([&] {
+		require(programs && binary_types && out && count > 0u, "invalid argument");
+		STPDeviceRuntimeProgram::STPLinkerInformation info;
+		for (uint32_t i = 0u; i < count; i++) {
+			require(programs[i], "null program");
+			require(binary_types[i] == STP_RTC_CODE_OBJECT || binary_types[i] == STP_RTC_LLVM_BITCODE, "unknown binary type");
+			info.DataOption.emplace_back(&programs[i]->Output.ProgramObject,
+				binary_types[i] == STP_RTC_CODE_OBJECT ? STPDeviceRuntimeProgram::STPBinaryType::CodeObject
+													   : STPDeviceRuntimeProgram::STPBinaryType::LLVMBitcode,
+				STPDeviceRuntimeProgram::STPLinkerInformation::STPDataJitOption {});
+		}
+		auto m = std::make_unique<stp_rtc_module>();
+		m->Module = STPDeviceRuntimeProgram::link(info);
+		*out = m.release();
+	});
+}
+
+int stp_rtc_module_destroy(stp_rtc_module* const module) {
+	return guarded([&] { delete module; });
+}
+
+int stp_rtc_module_function(stp_rtc_module* const module, const char* const lowered_name, void** const function) {
+	return guarded([&] {
+		require(module && lowered_name && function, "null argument");
+		hipFunction_t f = nullptr;
+		STP_CHECK_HIP(hipModuleGetFunction(&f, module->Module.get(), lowered_name));
+		*function = f;
+	});
+}
+
+int stp_rtc_module_global(stp_rtc_module* const module, const char* const lowered_name, void** const d_pointer,
+	uint64_t* const bytes) {
+	return guarded([&] {
+		require(module && lowered_name && d_pointer, "null argument");
+		hipDeviceptr_t p = nullptr;
+		size_t n = 0u;
+		STP_CHECK_HIP(hipModuleGetGlobal(&p, &n, module->Module.get(), lowered_name));
+		*d_pointer = p;
+		if (bytes) {
+			*bytes = n;
+		}
+	});
+}
+
+int stp_rtc_launch(void* const function, const uint32_t grid[3], const uint32_t block[3], const uint32_t shared_bytes,
+	void* const stream, void** const args) {
+	return guarded([&] {
+		require(function && grid && block, "null argument");
+		STP_CHECK_HIP(hipModuleLaunchKernel(static_cast<hipFunction_t>(function), grid[0], grid[1], grid[2], block[0],
+			block[1], block[2], shared_bytes, toStream(stream), args, nullptr));
+	});
+}
+
 int stp_batch_create(const stp_simplex_noise_setting* const noise, const stp_biome_property* const h_table,
 	const uint32_t biome_count, const stp_raindrop_setting* const raindrop, const stp_batch_setting* const setting,
 	const stp_batch_chunk* const h_chunks, const uint32_t chunk_count, stp_heightfield_batch** const out) {
@@ -608,7 +767,7 @@ int stp_batch_stage_times(stp_heightfield_batch* const batch, float* const h_ms,
 	return guarded([&] {
 		require(batch && h_ms, "null argument");
 		const std::vector<float> t = batch->Batch->stageTimes();
-		require(capacity >= t.size(), "capacity must hold 6 stages");
+		require(capacity >= t.size(), "capacity must hold 7 stages");
 		std::memcpy(h_ms, t.data(), t.size() * sizeof(float));
 	});
 }
@@ -632,6 +791,34 @@ int stp_batch_counters(stp_heightfield_batch* const batch, uint64_t h_out[6], vo
 		h_out[1] = batch->Batch->drops();
 		for (int k = 0; k < 4; k++) {
 			h_out[2 + k] = c[k];
+		}
+	});
+}
+
+int stp_batch_set_tuning(stp_heightfield_batch* const batch, const stp_batch_tuning* const tuning) {
+	return guarded([&] {
+		require(batch && tuning, "null argument");
+		batch->Batch->setTuning(STPBatchTuning {tuning->ErosionTile, tuning->ErosionThreads});
+	});
+}
+
+int stp_batch_get_tuning(stp_heightfield_batch* const batch, stp_batch_tuning* const out) {
+	return guarded([&] {
+		require(batch && out, "null argument");
+		const STPBatchTuning& t = batch->Batch->tuning();
+		out->ErosionTile = t.ErosionTile;
+		out->ErosionThreads = t.ErosionThreads;
+	});
+}
+
+int stp_batch_autotune(stp_heightfield_batch* const batch, const uint32_t repeat, void* const stream,
+	stp_batch_tuning* const out) {
+	return guarded([&] {
+		require(batch && repeat > 0u, "invalid argument");
+		const STPBatchTuning t = batch->Batch->autotune(toStream(stream), repeat);
+		if (out) {
+			out->ErosionTile = t.ErosionTile;
+			out->ErosionThreads = t.ErosionThreads;
 		}
 	});
 }

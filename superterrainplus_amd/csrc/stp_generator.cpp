@@ -208,28 +208,26 @@ unsigned int STPSingleHistogramFilter::STPFilterBuffer::binCapacity() const noex
 	return this->Impl->Capacity;
 }
 
-namespace {
-	// STPSingleHistogramFilter.cpp:874-880, and the window must lie inside the merged map.
-	uint2 checkFilterDomain(const STPNearestNeighbourInformation& nn, const unsigned int radius) {
-		STP_ASSERT_DOMAIN(radius > 0u && (radius & 0x01u) == 0x00u, "radius should be an positive even number");
-		const uint2 start = make_uint2(nn.MapSize.x * (nn.ChunkNearestNeighbour.x / 2u), nn.MapSize.y * (nn.ChunkNearestNeighbour.y / 2u));
-		STP_ASSERT_DOMAIN(radius <= start.x && radius <= start.y, "radius is too large and will overflow nearest neighbour boundary");
-		if (nn.TotalMapSize.x != nn.MapSize.x * nn.ChunkNearestNeighbour.x
-			|| nn.TotalMapSize.y != nn.MapSize.y * nn.ChunkNearestNeighbour.y || nn.MapSize.x == 0u || nn.MapSize.y == 0u) {
-			STP_THROW(STPInvalidArgument, "inconsistent nearest-neighbour information");
-		}
-		// the (map + 2r)^2 window must lie inside the merged map (fails for even neighbourhoods)
-		STP_ASSERT_DOMAIN(static_cast<size_t>(start.x) + nn.MapSize.x + radius <= nn.TotalMapSize.x
-				&& static_cast<size_t>(start.y) + nn.MapSize.y + radius <= nn.TotalMapSize.y,
-			"the filter window leaves the nearest-neighbour map (the neighbourhood must be odd)");
-		return start;
+// STPSingleHistogramFilter.cpp:874-880, and the window must lie inside the merged map.
+uint2 STPHeightfieldKernel::checkFilterDomain(const STPNearestNeighbourInformation& nn, const unsigned int radius) {
+	STP_ASSERT_DOMAIN(radius > 0u && (radius & 0x01u) == 0x00u, "radius should be an positive even number");
+	const uint2 start = make_uint2(nn.MapSize.x * (nn.ChunkNearestNeighbour.x / 2u), nn.MapSize.y * (nn.ChunkNearestNeighbour.y / 2u));
+	STP_ASSERT_DOMAIN(radius <= start.x && radius <= start.y, "radius is too large and will overflow nearest neighbour boundary");
+	if (nn.TotalMapSize.x != nn.MapSize.x * nn.ChunkNearestNeighbour.x
+		|| nn.TotalMapSize.y != nn.MapSize.y * nn.ChunkNearestNeighbour.y || nn.MapSize.x == 0u || nn.MapSize.y == 0u) {
+		STP_THROW(STPInvalidArgument, "inconsistent nearest-neighbour information");
 	}
-} // namespace
+	// the (map + 2r)^2 window must lie inside the merged map (fails for even neighbourhoods)
+	STP_ASSERT_DOMAIN(static_cast<size_t>(start.x) + nn.MapSize.x + radius <= nn.TotalMapSize.x
+			&& static_cast<size_t>(start.y) + nn.MapSize.y + radius <= nn.TotalMapSize.y,
+		"the filter window leaves the nearest-neighbour map (the neighbourhood must be odd)");
+	return start;
+}
 
 STPSingleHistogram STPSingleHistogramFilter::filterDevice(const STPSample_t* const samplemap,
 	const STPNearestNeighbourInformation& nn, STPFilterBuffer& filter_buffer, const unsigned int radius,
 	const hipStream_t stream) {
-	const uint2 start = checkFilterDomain(nn, radius);
+	const uint2 start = STPHeightfieldKernel::checkFilterDomain(nn, radius);
 	STPFilterBuffer::STPImpl& b = *filter_buffer.Impl;
 	const size_t need = STPHeightfieldKernel::filterWorkspaceBytes(nn.MapSize, radius, b.Capacity);
 	if (need > b.WorkspaceBytes) {
@@ -257,7 +255,7 @@ STPSingleHistogram STPSingleHistogramFilter::filterDevice(const STPSample_t* con
 
 STPSingleHistogram STPSingleHistogramFilter::operator()(const STPSample_t* const samplemap,
 	const STPNearestNeighbourInformation& nn, STPFilterBuffer& filter_buffer, const unsigned int radius) {
-	checkFilterDomain(nn, radius);
+	STPHeightfieldKernel::checkFilterDomain(nn, radius);
 	STPFilterBuffer::STPImpl& b = *filter_buffer.Impl;
 	if (!b.OwnStream) {
 		STP_CHECK_HIP(hipStreamCreateWithFlags(&b.OwnStream, hipStreamNonBlocking));
@@ -537,7 +535,7 @@ void STPBiomefieldGenerator::operator()(const STPNearestNeighbourHeightFloatWTex
 		|| heightmap_buffer.NeighbourInfo.MapSize.y != this->MapSize.y) {
 		STP_THROW(STPInvalidArgument, "the map size differs from the size this generator was made for");
 	}
-	const uint2 start = checkFilterDomain(bnn, this->InterpolationRadius);
+	const uint2 start = STPHeightfieldKernel::checkFilterDomain(bnn, this->InterpolationRadius);
 	const auto [pool, stream] = heightmap_buffer.DeviceMemInfo;
 	const STPNearestNeighbourHeightFloatWTextureBuffer::STPMergedBuffer height(heightmap_buffer,
 		STPNearestNeighbourHeightFloatWTextureBuffer::STPMemoryLocation::DeviceMemory);

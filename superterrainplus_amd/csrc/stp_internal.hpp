@@ -106,6 +106,9 @@ namespace SuperTerrainPlus {
 		// Bytes of device workspace filter() needs for the given problem.
 		size_t filterWorkspaceBytes(uint2 map_size, unsigned int radius, unsigned int bin_capacity);
 		STPFilterWorkspace carveFilterWorkspace(void* base, uint2 map_size, unsigned int radius, unsigned int bin_capacity);
+		// Checks the filter domain of a diversity neighbourhood (even radius, window inside the
+		// merged map) and returns the window's first pixel (the centre chunk's origin).
+		uint2 checkFilterDomain(const STPNearestNeighbourInformation& nn, unsigned int radius);
 		// Runs vertical + horizontal passes; writes dense bins and (if jagged) the jagged output.
 		void singleHistogramFilter(const STPSample_t* sample, unsigned int pitch, uint2 start, uint2 map_size,
 			unsigned int radius, unsigned int bin_capacity, bool jagged, const STPFilterWorkspace& ws, hipStream_t stream);

@@ -35,7 +35,7 @@ namespace {
 		__syncthreads();
 		const unsigned int drop = blockIdx.x * blockDim.x + threadIdx.x;
 		if (drop < a.Rain.RainDropCount) {
-			erodeDropParallel(map, a, brush, drop, ledger, drop_info);
+			erodeDropAnyBrush(GlobalSurface {map, a.Total.x}, a, brush, drop, ledger, drop_info);
 		}
 	}
 
@@ -113,12 +113,6 @@ namespace {
 				}
 			}
 		}
-	}
-
-	// STPHeightfieldKernel.cu:112-122: u16 = rint(saturate(h) * 65535), round half to even.
-	__device__ inline unsigned short toFixed(const float h) {
-		const float s = h > 1.0f ? 1.0f : (h > 0.0f ? h : 0.0f);
-		return static_cast<unsigned short>(rintf(s * 65535.0f));
 	}
 
 	__global__ void __launch_bounds__(256) formatKERNEL(const float* const input, STPHeightFixed_t* const output,

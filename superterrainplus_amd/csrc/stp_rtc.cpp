@@ -1,0 +1,396 @@
+// hipRTC runtime compilation and linking, and the runtime-compiled multi-biome diversity
+// generator.
+//
+// Compile: SuperTerrain+/SuperTerrain+/Private/GPGPU/STPDeviceRuntimeBinary.cpp (options, name
+// expressions, external headers, log, lowered names; a failed compilation throws with the log).
+// Link: SuperTerrain+/SuperTerrain+/Private/GPGPU/STPDeviceRuntimeProgram.cpp (per-input options,
+// archives, module options).
+// Plugin: SuperDemo+/World/Biomes/STPBiomefieldGenerator.cpp:44-152 compiles
+// SuperDemo+/Script/STPMultiHeightGenerator.cu at run time, looks its kernel and globals up by
+// lowered name, fills the biome table and launches the kernel after the single histogram filter.
+#include "stp_internal.hpp"
+
+#include <stp/STPDeviceRuntime.hpp>
+
+#include <algorithm>
+#include <cctype>
+#include <cstring>
+
+// The library's device headers, embedded at build time (generated source).
+extern "C" {
+extern const char* const STPLibraryHeaderName[];
+extern const char* const STPLibraryHeaderSource[];
+extern const unsigned int STPLibraryHeaderCount;
+}
+
+using namespace SuperTerrainPlus;
+
+namespace {
+
+	void checkRTC(const hiprtcResult r, const char* what) {
+		if (r != HIPRTC_SUCCESS) {
+			STP_THROW(STPCompilationError, std::string(what) + " -> " + hiprtcGetErrorString(r));
+		}
+	}
+
+	bool isIdentifier(const std::string& n) {
+		if (n.empty() || !(std::isalpha(static_cast<unsigned char>(n[0])) || n[0] == '_')) {
+			return false;
+		}
+		return std::all_of(n.begin(), n.end(), [](const char c) { return std::isalnum(static_cast<unsigned char>(c)) || c == '_'; });
+	}
+
+	bool hasOption(const std::vector<std::string>& opt, const char* prefix) {
+		return std::any_of(opt.begin(), opt.end(), [prefix](const std::string& o) { return o.rfind(prefix, 0) == 0; });
+	}
+
+	// Architecture of the current device, gfx950 when no device is visible (compilation alone
+	// needs no GPU).
+	std::string deviceArchitecture() {
+		int count = 0, device = 0;
+		if (hipGetDeviceCount(&count) != hipSuccess || count == 0 || hipGetDevice(&device) != hipSuccess) {
+			(void)hipGetLastError();
+			return "gfx950";
+		}
+		hipDeviceProp_t prop;
+		if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+			(void)hipGetLastError();
+			return "gfx950";
+		}
+		return prop.gcnArchName;
+	}
+
+} // namespace
+
+// ---------------------------------------------------------------- STPDeviceRuntimeBinary
+
+STPDeviceRuntimeBinary::STPSourceInformation::STPSourceArgument&
+STPDeviceRuntimeBinary::STPSourceInformation::STPSourceArgument::operator[](const std::string& arg) {
+	this->StringArgument.push_back(arg);
+	return *this;
+}
+
+void STPDeviceRuntimeBinary::STPProgramDeleter::operator()(hiprtcProgram program) const {
+	(void)hiprtcDestroyProgram(&program);
+}
+
+STPDeviceRuntimeBinary::STPCompilationOutput STPDeviceRuntimeBinary::compile(std::string&& source_name,
+	const std::string& source_code, const STPSourceInformation& source_info, const STPExternalHeaderSource& external_header) {
+	// external headers named by the source information, in its order
+	std::vector<const char*> headerName, headerSource;
+	for (const std::string& name : source_info.ExternalHeader.StringArgument) {
+		const auto it = external_header.find(name);
+		if (it == external_header.end()) {
+			STP_THROW(STPInvalidArgument, "external header '" + name + "' has no source");
+		}
+		headerName.push_back(it->first.c_str());
+		headerSource.push_back(it->second.c_str());
+	}
+	hiprtcProgram raw = nullptr;
+	checkRTC(hiprtcCreateProgram(&raw, source_code.c_str(), source_name.c_str(), static_cast<int>(headerName.size()),
+				 headerSource.data(), headerName.data()),
+		"hiprtcCreateProgram");
+	STPSmartProgram program(raw);
+
+	const std::vector<std::string>& names = source_info.NameExpression.StringArgument;
+	for (const std::string& n : names) {
+		checkRTC(hiprtcAddNameExpression(program.get(), n.c_str()), "hiprtcAddNameExpression");
+	}
+
+	std::vector<std::string> opt = source_info.Option.StringArgument;
+	if (!hasOption(opt, "--offload-arch") && !hasOption(opt, "--gpu-architecture") && !hasOption(opt, "-arch")) {
+		opt.push_back("--offload-arch=" + deviceArchitecture());
+	}
+	if (!hasOption(opt, "-O")) {
+		opt.push_back("-O3");
+	}
+	if (!hasOption(opt, "-ffp-contract")) {
+		opt.push_back("-ffp-contract=off");
+	}
+	std::vector<const char*> optPtr;
+	for (const std::string& o : opt) {
+		optPtr.push_back(o.c_str());
+	}
+	const hiprtcResult compiled = hiprtcCompileProgram(program.get(), static_cast<int>(optPtr.size()), optPtr.data());
+
+	STPCompilationOutput out;
+	size_t logSize = 0u;
+	checkRTC(hiprtcGetProgramLogSize(program.get(), &logSize), "hiprtcGetProgramLogSize");
+	if (logSize > 1u) {
+		std::string log(logSize, '\0');
+		checkRTC(hiprtcGetProgramLog(program.get(), log.data()), "hiprtcGetProgramLog");
+		log.resize(std::strlen(log.c_str()));
+		out.Log = std::move(log);
+	}
+	if (compiled != HIPRTC_SUCCESS) {
+		STP_THROW(STPCompilationError, "compilation of '" + source_name + "' failed (" + hiprtcGetErrorString(compiled)
+			+ "):\n" + out.Log);
+	}
+	for (const std::string& n : names) {
+		const char* lowered = nullptr;
+		if (hiprtcGetLoweredName(program.get(), n.c_str(), &lowered) == HIPRTC_SUCCESS && lowered) {
+			out.LoweredName.emplace(n, lowered);
+		} else if (isIdentifier(n) && source_code.find(n) != std::string::npos) {
+			// hipRTC does not lower global variables of relocatable (-fgpu-rdc) programs; a
+			// variable at global scope keeps its name in the code object
+			out.LoweredName.emplace(n, n);
+		}
+	}
+	out.ProgramObject.Identifier = std::move(source_name);
+	out.ProgramObject.Program = std::move(program);
+	return out;
+}
+
+STPDeviceRuntimeBinary::STPProgramData STPDeviceRuntimeBinary::readBitcode(hiprtcProgram program) {
+	size_t size = 0u;
+	checkRTC(hiprtcGetBitcodeSize(program, &size), "hiprtcGetBitcodeSize (the program needs -fgpu-rdc)");
+	STPProgramData d(std::make_unique<char[]>(size), size);
+	checkRTC(hiprtcGetBitcode(program, d.first.get()), "hiprtcGetBitcode");
+	return d;
+}
+
+STPDeviceRuntimeBinary::STPProgramData STPDeviceRuntimeBinary::readCodeObject(hiprtcProgram program) {
+	size_t size = 0u;
+	checkRTC(hiprtcGetCodeSize(program, &size), "hiprtcGetCodeSize");
+	if (size == 0u) {
+		STP_THROW(STPCompilationError, "the program has no code object (it was compiled with -fgpu-rdc)");
+	}
+	STPProgramData d(std::make_unique<char[]>(size), size);
+	checkRTC(hiprtcGetCode(program, d.first.get()), "hiprtcGetCode");
+	return d;
+}
+
+const STPDeviceRuntimeBinary::STPExternalHeaderSource& STPDeviceRuntimeBinary::libraryHeaders() {
+	static const STPExternalHeaderSource headers = [] {
+		STPExternalHeaderSource h;
+		for (unsigned int i = 0u; i < STPLibraryHeaderCount; i++) {
+			h.emplace(STPLibraryHeaderName[i], STPLibraryHeaderSource[i]);
+		}
+		return h;
+	}();
+	return headers;
+}
+
+// ---------------------------------------------------------------- STPDeviceRuntimeProgram
+
+STPDeviceRuntimeProgram::STPLinkerInformation::STPDataJitOption&
+STPDeviceRuntimeProgram::STPLinkerInformation::STPDataJitOption::operator()(const hipJitOption flag, void* const value) {
+	this->OptionFlag.push_back(flag);
+	this->OptionValue.push_back(value);
+	return *this;
+}
+
+void STPDeviceRuntimeProgram::STPModuleDeleter::operator()(hipModule_t module) const {
+	(void)hipModuleUnload(module);
+}
+
+namespace {
+	using STPDataJitOption = STPDeviceRuntimeProgram::STPLinkerInformation::STPDataJitOption;
+
+	void checkOption(const STPDataJitOption& o) {
+		if (o.OptionFlag.size() != o.OptionValue.size()) {
+			STP_THROW(STPInvalidArgument, "every JIT option flag needs exactly one value");
+		}
+	}
+
+	struct LinkState {
+		hiprtcLinkState State = nullptr;
+		~LinkState() {
+			if (this->State) {
+				(void)hiprtcLinkDestroy(this->State);
+			}
+		}
+	};
+} // namespace
+
+STPDeviceRuntimeProgram::STPSmartModule STPDeviceRuntimeProgram::link(STPLinkerInformation& linker_info) {
+	checkOption(linker_info.LinkerOption);
+	checkOption(linker_info.ModuleOption);
+	if (linker_info.DataOption.empty()) {
+		STP_THROW(STPInvalidArgument, "nothing to link");
+	}
+	size_t codeObjects = 0u;
+	for (const auto& [binary, type, option] : linker_info.DataOption) {
+		if (!binary || !binary->Program) {
+			STP_THROW(STPInvalidArgument, "a linker input has no compiled program");
+		}
+		checkOption(option);
+		codeObjects += type == STPBinaryType::CodeObject ? 1u : 0u;
+	}
+	const auto load = [&linker_info](const void* image) {
+		hipModule_t module = nullptr;
+		STPDataJitOption& mo = linker_info.ModuleOption;
+		const hipError_t r = hipModuleLoadDataEx(&module, image, static_cast<unsigned int>(mo.OptionFlag.size()),
+			mo.OptionFlag.data(), mo.OptionValue.data());
+		if (r != hipSuccess) {
+			(void)hipGetLastError();
+			STP_THROW(STPCompilationError, std::string("hipModuleLoadDataEx -> ") + hipGetErrorString(r));
+		}
+		return STPSmartModule(module);
+	};
+
+	if (codeObjects > 0u) {
+		// a code object is already linked: it can only be loaded on its own
+		if (codeObjects != 1u || linker_info.DataOption.size() != 1u || !linker_info.ArchiveOption.empty()) {
+			STP_THROW(STPInvalidArgument,
+				"a code object cannot be linked with other inputs; compile every input with -fgpu-rdc for bitcode");
+		}
+		const STPDeviceRuntimeBinary::STPProgramData code =
+			STPDeviceRuntimeBinary::readCodeObject(std::get<0>(linker_info.DataOption.front())->Program.get());
+		return load(code.first.get());
+	}
+
+	LinkState state;
+	STPDataJitOption& lo = linker_info.LinkerOption;
+	checkRTC(hiprtcLinkCreate(static_cast<unsigned int>(lo.OptionFlag.size()), lo.OptionFlag.data(), lo.OptionValue.data(),
+				 &state.State),
+		"hiprtcLinkCreate");
+	std::vector<STPDeviceRuntimeBinary::STPProgramData> bitcode; // alive until the link completes
+	for (auto& [binary, type, option] : linker_info.DataOption) {
+		bitcode.push_back(STPDeviceRuntimeBinary::readBitcode(binary->Program.get()));
+		const STPDeviceRuntimeBinary::STPProgramData& bc = bitcode.back();
+		checkRTC(hiprtcLinkAddData(state.State, HIPRTC_JIT_INPUT_LLVM_BITCODE, bc.first.get(), bc.second,
+					 binary->Identifier.c_str(), static_cast<unsigned int>(option.OptionFlag.size()), option.OptionFlag.data(),
+					 option.OptionValue.data()),
+			"hiprtcLinkAddData");
+	}
+	for (auto& [file, option] : linker_info.ArchiveOption) {
+		checkOption(option);
+		checkRTC(hiprtcLinkAddFile(state.State, HIPRTC_JIT_INPUT_LLVM_ARCHIVES_OF_BUNDLED_BITCODE, file.c_str(),
+					 static_cast<unsigned int>(option.OptionFlag.size()), option.OptionFlag.data(), option.OptionValue.data()),
+			"hiprtcLinkAddFile");
+	}
+	void* image = nullptr;
+	size_t imageSize = 0u;
+	checkRTC(hiprtcLinkComplete(state.State, &image, &imageSize), "hiprtcLinkComplete");
+	return load(image);
+}
+
+// ---------------------------------------------------------------- STPRuntimeBiomefieldGenerator
+
+extern "C" {
+extern const char* const STPLibraryMultiHeightScript;
+}
+
+namespace {
+	constexpr char KernelName[] = "generateMultiBiomeHeightmap";
+	constexpr unsigned int ScriptTileX = 64u, ScriptTileY = 4u;
+
+	struct ScriptGlobal {
+		hipDeviceptr_t Pointer = nullptr;
+		size_t Bytes = 0u;
+	};
+
+	ScriptGlobal scriptGlobal(hipModule_t module, const STPDeviceRuntimeBinary::STPLoweredName& lowered,
+		const std::string& name, const size_t need) {
+		const auto it = lowered.find(name);
+		if (it == lowered.end()) {
+			STP_THROW(STPCompilationError, "the generator script does not define '" + name + "'");
+		}
+		ScriptGlobal g;
+		STP_CHECK_HIP(hipModuleGetGlobal(&g.Pointer, &g.Bytes, module, it->second.c_str()));
+		if (g.Bytes < need) {
+			STP_THROW(STPInvalidArgument, "the script's '" + name + "' holds " + std::to_string(g.Bytes) + " bytes, "
+				+ std::to_string(need) + " are needed");
+		}
+		return g;
+	}
+} // namespace
+
+const char* STPRuntimeBiomefieldGenerator::builtinScript() noexcept {
+	return STPLibraryMultiHeightScript;
+}
+
+STPRuntimeBiomefieldGenerator::STPRuntimeBiomefieldGenerator(const STPEnvironment::STPSimplexNoiseSetting& noise,
+	std::vector<STPBiomeProperty> table, const uint2 map_size, const unsigned int interpolation_radius,
+	const unsigned int bin_capacity, const std::string& script, const std::vector<std::string>& options) :
+	Table(STPAlgorithm::STPPermutationGenerator::generate(noise)), MapSize(map_size),
+	InterpolationRadius(interpolation_radius), BinCapacity(bin_capacity) {
+	if (table.empty() || map_size.x == 0u || map_size.y == 0u) {
+		STP_THROW(STPInvalidArgument, "the runtime generator needs a biome table and a map size");
+	}
+	if (bin_capacity == 0u || bin_capacity > STPHeightfieldKernel::MaxBinCapacity) {
+		STP_THROW(STPInvalidArgument, "histogram bin capacity must be in [1, 64]");
+	}
+	STPDeviceRuntimeBinary::STPSourceInformation info;
+	info.Option["-std=c++17"];
+	for (const std::string& o : options) {
+		info.Option[o];
+	}
+	info.NameExpression[KernelName]["Dimension"]["HalfDimension"]["Permutation"]["BiomeCount"]["BiomeTable"];
+	const STPDeviceRuntimeBinary::STPExternalHeaderSource& headers = STPDeviceRuntimeBinary::libraryHeaders();
+	for (const auto& h : headers) {
+		info.ExternalHeader[h.first];
+	}
+	STPDeviceRuntimeBinary::STPCompilationOutput out = STPDeviceRuntimeBinary::compile("STPMultiHeightGenerator",
+		script.empty() ? std::string(STPLibraryMultiHeightScript) : script, info, headers);
+	this->Log = out.Log;
+
+	STPDeviceRuntimeProgram::STPLinkerInformation link;
+	link.DataOption.emplace_back(&out.ProgramObject, STPDeviceRuntimeProgram::STPBinaryType::CodeObject,
+		STPDeviceRuntimeProgram::STPLinkerInformation::STPDataJitOption {});
+	this->Module = STPDeviceRuntimeProgram::link(link);
+	const auto kernel = out.LoweredName.find(KernelName);
+	if (kernel == out.LoweredName.end()) {
+		STP_THROW(STPCompilationError, std::string("the generator script does not define ") + KernelName);
+	}
+	STP_CHECK_HIP(hipModuleGetFunction(&this->Kernel, this->Module.get(), kernel->second.c_str()));
+
+	// the globals are constant for the generator's lifetime: filled once here
+	const unsigned int count = static_cast<unsigned int>(table.size());
+	const float2 half = make_float2(static_cast<float>(map_size.x) / 2.0f, static_cast<float>(map_size.y) / 2.0f);
+	const STPAlgorithm::STPPermutation perm = this->Table.Permutation;
+	const auto put = [&](const char* name, const void* src, const size_t bytes) {
+		const ScriptGlobal g = scriptGlobal(this->Module.get(), out.LoweredName, name, bytes);
+		STP_CHECK_HIP(hipMemcpyHtoD(g.Pointer, const_cast<void*>(src), bytes));
+	};
+	put("Dimension", &map_size, sizeof(uint2));
+	put("HalfDimension", &half, sizeof(float2));
+	put("Permutation", &perm, sizeof(perm));
+	put("BiomeCount", &count, sizeof(unsigned int));
+	put("BiomeTable", table.data(), table.size() * sizeof(STPBiomeProperty));
+}
+
+void STPRuntimeBiomefieldGenerator::operator()(const STPNearestNeighbourHeightFloatWTextureBuffer& heightmap_buffer,
+	const STPNearestNeighbourSampleRTextureBuffer& biomemap_buffer, const float2 offset) {
+	const STPNearestNeighbourInformation& bnn = biomemap_buffer.NeighbourInfo;
+	if (bnn.MapSize.x != this->MapSize.x || bnn.MapSize.y != this->MapSize.y
+		|| heightmap_buffer.NeighbourInfo.MapSize.x != this->MapSize.x
+		|| heightmap_buffer.NeighbourInfo.MapSize.y != this->MapSize.y) {
+		STP_THROW(STPInvalidArgument, "the map size differs from the size this generator was made for");
+	}
+	const uint2 start = STPHeightfieldKernel::checkFilterDomain(bnn, this->InterpolationRadius);
+	const auto [pool, stream] = heightmap_buffer.DeviceMemInfo;
+	const STPNearestNeighbourHeightFloatWTextureBuffer::STPMergedBuffer height(heightmap_buffer,
+		STPNearestNeighbourHeightFloatWTextureBuffer::STPMemoryLocation::DeviceMemory);
+	const STPNearestNeighbourSampleRTextureBuffer::STPMergedBuffer biome(biomemap_buffer,
+		STPNearestNeighbourSampleRTextureBuffer::STPMemoryLocation::DeviceMemory);
+
+	const size_t bytes =
+		STPHeightfieldKernel::filterWorkspaceBytes(bnn.MapSize, this->InterpolationRadius, this->BinCapacity);
+	void* ws = nullptr;
+	if (pool) {
+		STP_CHECK_HIP(hipMallocFromPoolAsync(&ws, bytes, pool, stream));
+	} else {
+		STP_CHECK_HIP(hipMallocAsync(&ws, bytes, stream));
+	}
+	const STPHeightfieldKernel::STPFilterWorkspace carved =
+		STPHeightfieldKernel::carveFilterWorkspace(ws, bnn.MapSize, this->InterpolationRadius, this->BinCapacity);
+	STPHeightfieldKernel::singleHistogramFilter(biome.getDevice(), bnn.TotalMapSize.x, start, bnn.MapSize,
+		this->InterpolationRadius, this->BinCapacity, true, carved, stream);
+	float* out = height.getDevice();
+	STPAlgorithm::STPSingleHistogram histogram {carved.Bin, carved.Offset};
+	float2 off = offset;
+	void* args[] = {&out, &histogram, &off};
+	const uint2 m = this->MapSize;
+	STP_CHECK_HIP(hipModuleLaunchKernel(this->Kernel, (m.x + ScriptTileX - 1u) / ScriptTileX,
+		(m.y + ScriptTileY - 1u) / ScriptTileY, 1u, ScriptTileX, ScriptTileY, 1u, 0u, stream, args, nullptr));
+	unsigned int overflow = 0u;
+	STP_CHECK_HIP(hipMemcpyAsync(&overflow, carved.Overflow, sizeof(unsigned int), hipMemcpyDeviceToHost, stream));
+	STP_CHECK_HIP(hipFreeAsync(ws, stream));
+	STP_CHECK_HIP(hipStreamSynchronize(stream));
+	if (overflow != 0u) {
+		STP_THROW(STPCapacityError, "a pixel histogram holds more distinct biomes than the bin capacity ("
+			+ std::to_string(this->BinCapacity) + ")");
+	}
+}

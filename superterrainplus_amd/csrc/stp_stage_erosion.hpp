@@ -118,78 +118,154 @@ namespace SuperTerrainPlus::STPStage {
 		return (a.Rain.MaxLifetime == 0u || life < a.Rain.MaxLifetime) && d.volume >= a.Rain.minWaterVolume;
 	}
 
-	// Taps whose heights are fetched together: one memory round trip covers the r = 3 brush.
-	constexpr unsigned int TapGroup = 32u;
-
 	// Cell of a brush tap: the (possibly negative) linear tap offset is added to the cell index
-	// in 32-bit arithmetic. The footprint guard keeps the result inside the map.
+	// in 32-bit arithmetic. The footprint guards keep the result inside the surface.
 	__device__ inline unsigned int tapIndex(const unsigned int map_index, const int tap) {
 		return static_cast<unsigned int>(static_cast<int>(map_index) + tap);
 	}
 
-	// One drop with FP32 atomics (parallel mode). `brush` is the LDS copy (index, weight bits).
-	// The brush taps of one step are distinct cells, so fetching all their heights before
-	// issuing the atomics leaves this drop's own result unchanged (the reference reads and
-	// updates them one by one) while needing one round trip instead of one per tap.
-	__device__ inline void erodeDropParallel(float* const map, const ErosionArgs& a, const int2* const brush,
+	// The height surface a drop reads and updates.
+	// GlobalSurface: the whole map in HBM; reads bypass L1, updates are L2 FP32 atomics.
+	struct GlobalSurface {
+		float* Map;
+		unsigned int W;
+		__device__ unsigned int cell(const unsigned int x, const unsigned int y) const { return y * this->W + x; }
+		__device__ unsigned int stride() const { return this->W; }
+		__device__ float load(const unsigned int i) const { return loadHeight(this->Map + i); }
+		__device__ void add(const unsigned int i, const float v) const { unsafeAtomicAdd(this->Map + i, v); }
+		// the chunk-level guards already keep every access inside the map
+		__device__ bool holds(unsigned int, unsigned int, unsigned int, unsigned int) const { return true; }
+		// all taps of a step in one group: one HBM/L2 round trip
+		static constexpr unsigned int groupOf(const unsigned int taps) { return taps; }
+	};
+
+	// LocalSurface: an LDS window [X0, X0 + RW) x [Y0, Y0 + RH) of the map (map coordinates);
+	// updates are LDS FP32 atomics. A drop whose footprint would leave the window dies.
+	struct LocalSurface {
+		float* Lds;
+		unsigned int X0, Y0, RW, RH;
+		__device__ unsigned int cell(const unsigned int x, const unsigned int y) const {
+			return (y - this->Y0) * this->RW + (x - this->X0);
+		}
+		__device__ unsigned int stride() const { return this->RW; }
+		__device__ float load(const unsigned int i) const { return this->Lds[i]; }
+		__device__ void add(const unsigned int i, const float v) const { atomicAdd(this->Lds + i, v); }
+		__device__ bool holds(const unsigned int x0, const unsigned int y0, const unsigned int x1, const unsigned int y1) const {
+			return x0 >= this->X0 && y0 >= this->Y0 && x1 < this->X0 + this->RW && y1 < this->Y0 + this->RH;
+		}
+		// LDS latency is short: small groups keep registers free for more waves
+		static constexpr unsigned int groupOf(const unsigned int taps) {
+			return taps % 5u == 0u ? 5u : (taps % 3u == 0u ? 3u : taps);
+		}
+	};
+
+	// STPRainDrop.cu:23-51 on a surface.
+	template <class Surface>
+	__device__ inline float3 surfaceGradient(const Surface& s, const float px, const float py) {
+		const unsigned int rx = static_cast<unsigned int>(px), ry = static_cast<unsigned int>(py);
+		const float cx = px - static_cast<float>(rx), cy = py - static_cast<float>(ry);
+		const unsigned int b = s.cell(rx, ry), w = s.stride();
+		const float hx = s.load(b), hy = s.load(b + 1u), hz = s.load(b + w), hw = s.load(b + w + 1u);
+		float3 r;
+		r.x = hx * (1 - cx) * (1 - cy) + hy * cx * (1 - cy) + hz * (1 - cx) * cy + hw * cx * cy;
+		r.y = (hy - hx) * (1 - cy) + (hw - hz) * cy;
+		r.z = (hz - hx) * (1 - cx) + (hw - hy) * cx;
+		return r;
+	}
+
+	// One drop in parallel mode on `s`. `brush` holds (linear tap offset for s.stride(), weight
+	// bits). Taps > 0 is the compile-time brush size (straight-line code: the heights of a group
+	// of taps are fetched, then their updates issued, with one wait per group); Taps == 0 loops
+	// over a.BrushSize taps one by one.
+	// The taps of one step are distinct cells, so fetching all their heights before updating
+	// leaves the drop's own result unchanged (the reference reads and updates them one by one).
+	template <class Surface, unsigned int Taps>
+	__device__ inline void erodeDrop(const Surface& s, const ErosionArgs& a, const int2* const brush,
 		const unsigned int drop, float* const ledger, uint2* const drop_info) {
 		DropState d = spawn(a, drop);
-		const unsigned int W = a.Total.x;
 		float lost = 0.0f;
 		unsigned int steps = 0u, erodeSteps = 0u, depositSteps = 0u, killed = 0u;
-		if (!spawnInside(a, d)) {
+		const unsigned int r = a.Rain.ErosionBrushRadius, lo = r - 1u, reach = r > 2u ? r - 1u : 1u;
+		if (!spawnInside(a, d)
+			|| !s.holds(static_cast<unsigned int>(d.px), static_cast<unsigned int>(d.py),
+				static_cast<unsigned int>(d.px) + 1u, static_cast<unsigned int>(d.py) + 1u)) {
 			d.volume = 0.0f;
 			killed = 1u;
 		}
 		for (unsigned int life = 0u; alive(a, d, life); life++) {
 			steps++;
 			const unsigned int cellX = static_cast<unsigned int>(d.px), cellY = static_cast<unsigned int>(d.py);
-			const unsigned int mapIndex = cellY * W + cellX;
 			const float ox = d.px - static_cast<float>(static_cast<int>(d.px)),
 				oy = d.py - static_cast<float>(static_cast<int>(d.py));
-			float3 hg;
-			if (!moveDrop<true>(a, map, d, hg) || !cellInside(a, cellX, cellY)) {
+			const float3 hg = surfaceGradient(s, d.px, d.py);
+			// STPRainDrop.cu:86-105 (direction, move, kill test)
+			const float inertia = a.Rain.Inertia;
+			d.dx = (-hg.y) * (1 - inertia) + d.dx * inertia;
+			d.dy = (-hg.z) * (1 - inertia) + d.dy * inertia;
+			const float length = sqrtf(d.dx * d.dx + d.dy * d.dy);
+			if (length != 0.0f) {
+				d.dx /= length;
+				d.dy /= length;
+			}
+			d.px += d.dx;
+			d.py += d.dy;
+			const float radius = static_cast<float>(r);
+			const bool inMap = !(d.dx == 0.0f && d.dy == 0.0f) && d.px >= radius
+				&& d.px < 1.0f * static_cast<float>(a.Total.x) - radius && d.py >= radius
+				&& d.py < 1.0f * static_cast<float>(a.Total.y) - radius;
+			if (!inMap || !cellInside(a, cellX, cellY)
+				|| !s.holds(cellX - lo, cellY - lo, cellX + reach, cellY + reach)
+				|| !s.holds(static_cast<unsigned int>(d.px), static_cast<unsigned int>(d.py),
+					static_cast<unsigned int>(d.px) + 1u, static_cast<unsigned int>(d.py) + 1u)) {
 				lost = d.sediment;
 				d.volume = 0.0f;
 				d.sediment = 0.0f;
 				killed = 1u;
 				break;
 			}
-			const float deltaHeight = heightGradient<true>(map, W, d.px, d.py).x - hg.x;
+			const unsigned int mapIndex = s.cell(cellX, cellY), W = s.stride();
+			const float deltaHeight = surfaceGradient(s, d.px, d.py).x - hg.x;
 			const float cap = fmaxf(-deltaHeight * d.speed * d.volume * a.Rain.SedimentCapacityFactor,
 				a.Rain.minSedimentCapacity);
 			if (d.sediment > cap || deltaHeight > 0.0f) {
 				const float amount = (deltaHeight > 0.0f) ? fminf(deltaHeight, d.sediment)
 														  : (d.sediment - cap) * a.Rain.DepositSpeed;
 				d.sediment -= amount;
-				unsafeAtomicAdd(map + mapIndex, amount * (1.0f - ox) * (1.0f - oy));
-				unsafeAtomicAdd(map + mapIndex + 1u, amount * ox * (1.0f - oy));
-				unsafeAtomicAdd(map + mapIndex + W, amount * (1.0f - ox) * oy);
-				unsafeAtomicAdd(map + mapIndex + W + 1u, amount * ox * oy);
+				s.add(mapIndex, amount * (1.0f - ox) * (1.0f - oy));
+				s.add(mapIndex + 1u, amount * ox * (1.0f - oy));
+				s.add(mapIndex + W, amount * (1.0f - ox) * oy);
+				s.add(mapIndex + W + 1u, amount * ox * oy);
 				depositSteps++;
 			} else {
 				const float amount = fminf((cap - d.sediment) * a.Rain.ErodeSpeed, -deltaHeight);
-				for (unsigned int base = 0u; base < a.BrushSize; base += TapGroup) {
-					float h[TapGroup];
+				if constexpr (Taps > 0u) {
+					// taps fetched per group; the group is a compile-time divisor of Taps
+					constexpr unsigned int G = Surface::groupOf(Taps);
+#pragma unroll 1
+					for (unsigned int base = 0u; base < Taps; base += G) {
+						float h[G];
 #pragma unroll
-					for (unsigned int j = 0u; j < TapGroup; j++) {
-						if (base + j < a.BrushSize) {
-							// the tap's cell index is formed in 32 bits first (brush offsets are
-							// negative); only then is it added to the map pointer
-							const unsigned int ei = tapIndex(mapIndex, brush[base + j].x);
-							h[j] = loadHeight(map + ei);
+						for (unsigned int j = 0u; j < G; j++) {
+							h[j] = s.load(tapIndex(mapIndex, brush[base + j].x));
 						}
-					}
 #pragma unroll
-					for (unsigned int j = 0u; j < TapGroup; j++) {
-						if (base + j < a.BrushSize) {
+						for (unsigned int j = 0u; j < G; j++) {
 							const int2 b = brush[base + j];
-							const unsigned int ei = tapIndex(mapIndex, b.x);
 							const float wa = amount * __int_as_float(b.y);
 							const float ds = (h[j] < wa) ? h[j] : wa;
-							unsafeAtomicAdd(map + ei, -ds);
+							s.add(tapIndex(mapIndex, b.x), -ds);
 							d.sediment += ds;
 						}
+					}
+				} else {
+					for (unsigned int j = 0u; j < a.BrushSize; j++) {
+						const int2 b = brush[j];
+						const unsigned int ei = tapIndex(mapIndex, b.x);
+						const float wa = amount * __int_as_float(b.y);
+						const float h = s.load(ei);
+						const float ds = (h < wa) ? h : wa;
+						s.add(ei, -ds);
+						d.sediment += ds;
 					}
 				}
 				erodeSteps++;
@@ -202,6 +278,32 @@ namespace SuperTerrainPlus::STPStage {
 		if (drop_info) {
 			drop_info[drop] = make_uint2(steps | (erodeSteps << 16), depositSteps | (killed << 16));
 		}
+	}
+
+	// Brush sizes with a specialised (straight-line) drop: r = 2, 3, 4.
+	template <class Surface>
+	__device__ inline void erodeDropAnyBrush(const Surface& s, const ErosionArgs& a, const int2* const brush,
+		const unsigned int drop, float* const ledger, uint2* const drop_info) {
+		switch (a.BrushSize) {
+		case 9u:
+			erodeDrop<Surface, 9u>(s, a, brush, drop, ledger, drop_info);
+			break;
+		case 25u:
+			erodeDrop<Surface, 25u>(s, a, brush, drop, ledger, drop_info);
+			break;
+		case 45u:
+			erodeDrop<Surface, 45u>(s, a, brush, drop, ledger, drop_info);
+			break;
+		default:
+			erodeDrop<Surface, 0u>(s, a, brush, drop, ledger, drop_info);
+			break;
+		}
+	}
+
+	// STPHeightfieldKernel.cu:112-122: u16 = rint(saturate(h) * 65535), round half to even.
+	__device__ inline unsigned short toFixed(const float h) {
+		const float s = h > 1.0f ? 1.0f : (h > 0.0f ? h : 0.0f);
+		return static_cast<unsigned short>(rintf(s * 65535.0f));
 	}
 
 } // namespace SuperTerrainPlus::STPStage

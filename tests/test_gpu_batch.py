@@ -49,6 +49,13 @@ class Batch:
         capi.call("stp_batch_read_chunk", C.c_void_p(self.h), i, None, None, None, capi.ptr(out))
         return out[:drops]
 
+    def tuning(self, tile=None, threads=256):
+        t = capi.BatchTuning()
+        if tile is not None:
+            capi.call("stp_batch_set_tuning", C.c_void_p(self.h), C.byref(capi.BatchTuning(tile, threads)))
+        capi.call("stp_batch_get_tuning", C.c_void_p(self.h), C.byref(t))
+        return t.ErosionTile, t.ErosionThreads
+
     def counters(self):
         out = (C.c_uint64 * 6)()
         capi.call("stp_batch_counters", C.c_void_p(self.h), out, None)
@@ -98,7 +105,15 @@ def test_batch_graph_replays_are_stable(gpu):
         assert all((x == y).all() for c0, c1 in zip(first, again) for x, y in zip(c0, c1))
 
 
-def test_batch_erosion_against_serialized_oracle(gpu, oracle):
+# erosion launch shapes: the default LDS-tiled one (128-cell tiles: the 256-cell chunk has four,
+# the others one), the smallest tile the colour batches allow at lifetime 30 and r = 3 (72: the
+# 96- and 128-cell chunks have four tiles, the last ones cut short, windows clipped at the chunk
+# borders), and the global-atomic kernel
+TUNINGS = [None, (72, 256), (0, 256)]
+
+
+@pytest.mark.parametrize("tuning", TUNINGS, ids=["default", "tile72", "global"])
+def test_batch_erosion_against_serialized_oracle(gpu, oracle, tuning):
     """Per chunk: parallel erosion inside the batch vs the serialized FP32 oracle erosion of the
     same pre-erosion heightfield, same seed and stream id (= chunk index).
 
@@ -112,11 +127,15 @@ def test_batch_erosion_against_serialized_oracle(gpu, oracle):
       * RMS(parallel - serialized) <= 1.5 x RMS(serialized - serialized with other drops);
       * height-histogram L1 <= 2 x the same floor + 0.5% of the pixels.
     """
-    log("batch erosion statistics")
+    log(f"batch erosion statistics {tuning}")
     dpp = 0.25
     pre = Batch(0.0)
     pre.run(0)
     b = Batch(dpp)
+    if tuning is not None:
+        assert b.tuning(*tuning) == tuning
+    else:
+        assert b.tuning()[0] > 0
     b.run(0)
     cnt = b.counters()
     drops_total = sum(int(s * s * dpp) for s in SIDES)
@@ -177,9 +196,57 @@ def test_batch_stage_timing(gpu):
     capi.call("stp_batch_set_timing", C.c_void_p(b.h), 1)
     for mode in (0, 1):
         b.run(mode)
-        ms = (C.c_float * 6)()
-        capi.call("stp_batch_stage_times", C.c_void_p(b.h), ms, 6)
+        ms = (C.c_float * 7)()
+        capi.call("stp_batch_stage_times", C.c_void_p(b.h), ms, 7)
         assert all(v >= 0 for v in ms) and sum(ms) > 0
+
+
+def test_batch_tuning(gpu):
+    """Invalid shapes are refused; the autotuner picks a valid candidate and keeps it; switching
+    shapes between graph runs recaptures the graph."""
+    b = Batch(0.25)
+    default = b.tuning()
+    assert default == (128, 1024)
+    # 64 < 2 x halo (30 + 3 + 1): windows of one colour batch would overlap
+    for bad in [(128, 100), (8, 256), (64, 256), (100000, 256)]:
+        with pytest.raises(capi.STPError):
+            b.tuning(*bad)
+    assert b.tuning() == default
+    b.run(0)
+    first = b.read(0)[0]
+    b.tuning(80, 512)
+    b.run(0)
+    assert b.tuning() == (80, 512)
+    out = capi.BatchTuning()
+    capi.call("stp_batch_autotune", C.c_void_p(b.h), 1, None, C.byref(out))
+    assert out.ErosionTile > 0 and b.tuning() == (out.ErosionTile, out.ErosionThreads)
+    b.run(0)
+    capi.synchronize()
+    # both runs erode the same terrain with the same drops: comparable mean heights
+    again = b.read(0)[0]
+    assert np.isfinite(again).all() and abs(again.mean() - first.mean()) < 5e-3
+
+
+def test_batch_tiled_needs_lifetime(gpu):
+    """MaxLifetime = 0 (unlimited drops, the reference's behaviour) cannot be tiled: the batch
+    starts on the global-atomic kernel and refuses tiled shapes."""
+    setting = capi.BatchSetting(SEED, COUNT, CELL, RADIUS, 8, 0.25)
+    chunks = (capi.BatchChunk * 1)(capi.BatchChunk(64, (C.c_int32 * 2)(0, 0)))
+    rain = W.erosion_setting(1)
+    rain.MaxLifetime = 0
+    h = C.c_void_p()
+    capi.call("stp_batch_create", C.byref(NOISE), capi.biome_array(props()), COUNT, C.byref(rain), C.byref(setting),
+              chunks, 1, C.byref(h))
+    try:
+        t = capi.BatchTuning()
+        capi.call("stp_batch_get_tuning", h, C.byref(t))
+        assert t.ErosionTile == 0
+        with pytest.raises(capi.STPError):
+            capi.call("stp_batch_set_tuning", h, C.byref(capi.BatchTuning(64, 256)))
+        capi.call("stp_batch_run", h, 0, None)
+        capi.synchronize()
+    finally:
+        capi.lib().stp_batch_destroy(h)
 
 
 def test_batch_rejects_bad_settings(gpu):
