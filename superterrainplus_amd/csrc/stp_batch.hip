@@ -170,17 +170,47 @@ namespace SuperTerrainPlus::STPBatch {
 		return a;
 	}
 
-	// drop -> rank among the drops of its spawn tile
+	// drop -> rank among the drops of its spawn tile. The workgroup's drops (one chunk) are
+	// counted per tile in LDS first; one global atomic per tile then reserves the workgroup's
+	// ranks, so a counter sees one atomic per workgroup instead of one per drop.
+	constexpr unsigned int BinLds = 4096u;
 	__global__ void __launch_bounds__(ErosionBlock) binCountKERNEL(const uint2* const tiles, const Chunk* const chunks,
 		const ErosionArgs args, const unsigned int tile, unsigned int* const tile_drops, unsigned int* const rank) {
+		__shared__ unsigned int local[BinLds];
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
 		const unsigned int drop = t.y * ErosionBlock + threadIdx.x;
-		if (drop >= c.Drops) {
+		const unsigned int tx = (c.Side + tile - 1u) / tile, ntile = tx * tx;
+		const bool aggregate = ntile <= BinLds; // uniform over the workgroup
+		if (aggregate) {
+			for (unsigned int i = threadIdx.x; i < ntile; i += ErosionBlock) {
+				local[i] = 0u;
+			}
+		}
+		__syncthreads();
+		unsigned int my = 0u, localRank = 0u;
+		if (drop < c.Drops) {
+			my = dropTile(spawn(chunkArgs(args, c), drop), c.Side, tile);
+			if (aggregate) {
+				localRank = atomicAdd(local + my, 1u);
+			} else {
+				rank[c.DropBase + drop] = atomicAdd(tile_drops + c.TileBase + my, 1u);
+			}
+		}
+		if (!aggregate) {
 			return;
 		}
-		const DropState d = spawn(chunkArgs(args, c), drop);
-		rank[c.DropBase + drop] = atomicAdd(tile_drops + c.TileBase + dropTile(d, c.Side, tile), 1u);
+		__syncthreads();
+		for (unsigned int i = threadIdx.x; i < ntile; i += ErosionBlock) {
+			const unsigned int n = local[i];
+			if (n != 0u) {
+				local[i] = atomicAdd(tile_drops + c.TileBase + i, n);
+			}
+		}
+		__syncthreads();
+		if (drop < c.Drops) {
+			rank[c.DropBase + drop] = local[my] + localRank;
+		}
 	}
 
 	// first slot of every tile of a chunk: DropBase + exclusive prefix sum of the tile counts
