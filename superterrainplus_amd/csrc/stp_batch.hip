@@ -45,35 +45,313 @@ namespace SuperTerrainPlus::STPBatch {
 			static_cast<long long>(c.OriginY) - radius + y);
 	}
 
+	// ---- single histogram filter, batched form
+	//
+	// The vertical pass runs one lane per column and the horizontal pass one lane per row
+	// (stp_stage_shf.hpp holds the accumulators and the order rules). Between the two passes the
+	// column histograms are stored TRANSPOSED, [column][slot][row], so that the horizontal pass,
+	// whose lanes are consecutive rows, reads them with coalesced loads. Both passes go through
+	// LDS so that every global access of a wave is a contiguous run: the vertical pass stages
+	// `rows` output rows and writes them column by column; the horizontal pass loads blocks of
+	// `block` columns (entering and leaving) before sliding over them, and stages its dense
+	// histogram output for row-contiguous stores.
+
+	constexpr unsigned int VPrefetch = 16u; // halo rows fetched at once by the vertical pass
+	// columns fetched at once by the register horizontal pass (its registers scale with MaxCap)
+	__host__ __device__ constexpr unsigned int horizontalPrefetch(const unsigned int maxCap) {
+		return maxCap >= 16u ? 2u : (maxCap >= 8u ? 4u : 8u);
+	}
+
+	__host__ __device__ inline unsigned int verticalStageRows(const unsigned int vcap) {
+		const unsigned int r = 12288u / (vcap * 65u);
+		return r < 1u ? 1u : (r > 32u ? 32u : r);
+	}
+
+	__host__ __device__ inline unsigned int verticalLdsWords(const unsigned int vcap) {
+		const unsigned int r = verticalStageRows(vcap);
+		return vcap * ShfLineBlock + vcap * r * 65u + r * 65u;
+	}
+
+	__host__ __device__ inline unsigned int horizontalBlock(const unsigned int vcap, const unsigned int cap) {
+		const unsigned int perColumn = 2u * (vcap + 1u) * ShfLineBlock + (1u + 2u * cap) * 65u, acc = 2u * cap * ShfLineBlock;
+		const unsigned int b = acc < 12288u ? (12288u - acc) / perColumn : 0u;
+		return b < 1u ? 1u : (b > 8u ? 8u : b);
+	}
+
+	// Register accumulator bound of the vertical (up to 16 slots) and horizontal (up to 8: its
+	// prefetched columns also live in registers) passes; 0 = LDS accumulator.
+	__host__ __device__ constexpr unsigned int registerCapacity(const unsigned int cap) {
+		return cap <= 4u ? 4u : (cap <= 8u ? 8u : (cap <= 16u ? 16u : 0u));
+	}
+	__host__ __device__ constexpr unsigned int horizontalRegisterCapacity(const unsigned int cap) {
+		return cap <= 4u ? 4u : (cap <= 8u ? 8u : 0u);
+	}
+
+	// LDS words of horizontalKERNEL<horizontalRegisterCapacity(cap)>
+	__host__ __device__ inline unsigned int horizontalLdsWords(const unsigned int vcap, const unsigned int cap) {
+		if (horizontalRegisterCapacity(cap) > 0u) {
+			return horizontalPrefetch(horizontalRegisterCapacity(cap)) * (1u + 2u * cap) * 65u;
+		}
+		const unsigned int b = horizontalBlock(vcap, cap);
+		return 2u * cap * ShfLineBlock + 2u * b * (vcap + 1u) * ShfLineBlock + b * (1u + 2u * cap) * 65u;
+	}
+
+	// Accumulator of one line: in registers for a capacity up to MaxCap (4, 8, 16), in LDS
+	// (MaxCap = 0) for any other.
+	template <unsigned int MaxCap>
+	struct LineAcc {
+		using Vertical = RegisterLineAcc<MaxCap>;
+		using Horizontal = RegisterLineAcc<MaxCap>;
+		__device__ static Vertical vertical(unsigned int*, const unsigned int cap) { return Vertical(cap); }
+		__device__ static Horizontal horizontal(unsigned int*, const unsigned int cap) { return Horizontal(cap); }
+	};
+	template <>
+	struct LineAcc<0u> {
+		using Vertical = LdsVerticalAcc;
+		using Horizontal = LdsHorizontalAcc;
+		__device__ static Vertical vertical(unsigned int* const lds, const unsigned int cap) { return Vertical(lds, cap); }
+		__device__ static Horizontal horizontal(unsigned int* const lds, const unsigned int cap) {
+			return Horizontal(lds, lds + cap * ShfLineBlock, cap);
+		}
+	};
+
+	// vcount[col * side + y], vbin[(col * vcap + k) * side + y] (per chunk at Line, Line * vcap)
+	template <unsigned int MaxCap>
 	__global__ void __launch_bounds__(ShfLineBlock) verticalKERNEL(const uint2* const tiles, const Chunk* const chunks,
 		const unsigned int radius, const unsigned int vcap, const unsigned short* const biome, unsigned int* const vbin,
 		unsigned char* const vcount, unsigned int* const overflow) {
 		extern __shared__ unsigned int lds[];
+		const unsigned int R = verticalStageRows(vcap);
+		unsigned int* const stage = lds + vcap * ShfLineBlock; // [k][row][65]
+		unsigned int* const stageN = stage + vcap * R * 65u;   // [row][65]
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
-		const unsigned int col = t.y * ShfLineBlock + threadIdx.x, ncols = c.Side + 2u * radius;
-		if (col >= ncols) {
-			return;
+		const unsigned int lane = threadIdx.x, side = c.Side, ncols = side + 2u * radius;
+		const unsigned int col = t.y * ShfLineBlock + lane;
+		const bool active = col < ncols;
+		// halo map (side + 2r)^2 starting at (-r, -r): output row y covers halo rows y .. y + 2r
+		auto acc = LineAcc<MaxCap>::vertical(lds + lane, vcap);
+		// halo rows are fetched VPrefetch at a time (clamped indices, so every load is in bounds
+		// and none waits for another); out-of-range rows are dropped after the load
+		const size_t lastRow = side + 2u * radius - 1u;
+		const unsigned short* const colc = biome + c.Halo + min(col, ncols - 1u);
+		const auto fetch = [&](unsigned int (&v)[VPrefetch], const long long row0) {
+#pragma unroll
+			for (unsigned int i = 0u; i < VPrefetch; i++) {
+				const long long r = row0 + i;
+				v[i] = colc[static_cast<size_t>(r < 0 ? 0 : (r > static_cast<long long>(lastRow) ? lastRow : r)) * ncols];
+			}
+		};
+		for (unsigned int j0 = 0u; j0 <= 2u * radius; j0 += VPrefetch) {
+			unsigned int v[VPrefetch];
+			fetch(v, j0);
+#pragma unroll
+			for (unsigned int i = 0u; i < VPrefetch; i++) {
+				if (active && j0 + i <= 2u * radius) {
+					acc.increment(v[i], 1u);
+				}
+			}
 		}
-		if (shfVerticalLine(biome + c.Halo, ncols, make_uint2(radius, radius), make_uint2(c.Side, c.Side), radius, vcap,
-				lds + threadIdx.x, vbin + c.Line * vcap, vcount + c.Line, col)) {
+		unsigned int* const outBin = vbin + c.Line * vcap;
+		unsigned char* const outN = vcount + c.Line;
+		for (unsigned int y0 = 0u; y0 < side; y0 += R) {
+			const unsigned int rows = min(R, side - y0);
+			for (unsigned int b0 = 0u; b0 < rows; b0 += VPrefetch) {
+				unsigned int enter[VPrefetch], leave[VPrefetch];
+				fetch(enter, static_cast<long long>(y0 + b0) + 2 * static_cast<long long>(radius));
+				fetch(leave, static_cast<long long>(y0 + b0) - 1);
+#pragma unroll
+				for (unsigned int i = 0u; i < VPrefetch; i++) {
+					const unsigned int yy = b0 + i, y = y0 + yy;
+					if (active && yy < rows) {
+						if (y > 0u) {
+							acc.increment(enter[i], 1u);
+							acc.decrement(leave[i], 1u);
+						}
+						stageN[yy * 65u + lane] = acc.size();
+						acc.forEach([&](const unsigned int k, const unsigned int item, const unsigned int count) {
+							stage[(k * R + yy) * 65u + lane] = item | (count << 16);
+						});
+					}
+				}
+			}
+			__syncthreads();
+			// consecutive lanes take consecutive rows of one column
+			for (unsigned int i = lane; i < ShfLineBlock * rows; i += ShfLineBlock) {
+				const unsigned int cc = i / rows, yy = i - cc * rows;
+				const unsigned int cg = t.y * ShfLineBlock + cc;
+				if (cg < ncols) {
+					const unsigned int n = stageN[yy * 65u + cc];
+					const size_t y = static_cast<size_t>(y0) + yy;
+					outN[static_cast<size_t>(cg) * side + y] = static_cast<unsigned char>(n);
+					for (unsigned int k = 0u; k < n; k++) {
+						outBin[(static_cast<size_t>(cg) * vcap + k) * side + y] = stage[(k * R + yy) * 65u + cc];
+					}
+				}
+			}
+			__syncthreads();
+		}
+		if (acc.overflowed()) {
 			atomicOr(overflow, 1u);
 		}
 	}
 
+	// Dense histogram of every pixel: bin_count[p], bin_item/bin_weight[p * cap + slot].
+	// Virtual step s runs from -2r to side - 1: column s + 2r enters, column s - 1 leaves (s >= 1),
+	// pixel s is emitted (s >= 0) -- the same sequence as shfHorizontalLine.
+	template <unsigned int MaxCap>
 	__global__ void __launch_bounds__(ShfLineBlock) horizontalKERNEL(const uint2* const tiles,
 		const Chunk* const chunks, const unsigned int radius, const unsigned int vcap, const unsigned int cap,
 		const unsigned int* const vbin, const unsigned char* const vcount, unsigned char* const bin_count,
 		unsigned short* const bin_item, float* const bin_weight, unsigned int* const overflow) {
 		extern __shared__ unsigned int lds[];
+		// LDS path: [accumulator][entering block][leaving block][output]; register path: [output]
+		constexpr unsigned int HPrefetch = horizontalPrefetch(MaxCap);
+		const unsigned int B = MaxCap > 0u ? HPrefetch : horizontalBlock(vcap, cap);
+		unsigned int* const enterN = lds + 2u * cap * ShfLineBlock; // [col][lane]
+		unsigned int* const enterBin = enterN + B * ShfLineBlock;   // [col][k][lane]
+		unsigned int* const leaveN = enterBin + B * vcap * ShfLineBlock;
+		unsigned int* const leaveBin = leaveN + B * ShfLineBlock;
+		unsigned int* const outN = MaxCap > 0u ? lds : leaveBin + B * vcap * ShfLineBlock; // [px][65]
+		unsigned int* const outItem = outN + B * 65u;                                        // [px][k][65]
+		float* const outW = reinterpret_cast<float*>(outItem + B * cap * 65u);
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
-		const unsigned int row = t.y * ShfLineBlock + threadIdx.x;
-		if (row >= c.Side) {
-			return;
+		const unsigned int lane = threadIdx.x, side = c.Side, ncols = side + 2u * radius;
+		const unsigned int y0 = t.y * ShfLineBlock, y = y0 + lane;
+		const bool active = y < side;
+		const unsigned int* const inBin = vbin + c.Line * vcap;
+		const unsigned char* const inN = vcount + c.Line;
+		auto acc = LineAcc<MaxCap>::horizontal(lds + lane, cap);
+		const int first = -2 * static_cast<int>(radius);
+		// stage one pixel's histogram for the row-contiguous stores
+		const auto emit = [&](const unsigned int j) {
+			unsigned int sum = 0u;
+			acc.forEach([&](unsigned int, unsigned int, const unsigned int count) { sum += count; });
+			const float norm = 1.0f / static_cast<float>(sum);
+			outN[j * 65u + lane] = acc.size();
+			acc.forEach([&](const unsigned int k, const unsigned int item, const unsigned int count) {
+				outItem[(j * cap + k) * 65u + lane] = item;
+				outW[(j * cap + k) * 65u + lane] = static_cast<float>(count) * norm;
+			});
+		};
+		// row-contiguous stores of the staged pixels [xa, sEnd) of the block starting at s0
+		const auto flush = [&](const int s0, const int sEnd) {
+			const int xa = max(s0, 0);
+			if (sEnd > xa) {
+				const unsigned int npx = static_cast<unsigned int>(sEnd - xa), jb = static_cast<unsigned int>(xa - s0);
+				const unsigned int rows = min(ShfLineBlock, side - y0);
+				for (unsigned int i = lane; i < rows * npx; i += ShfLineBlock) {
+					const unsigned int r = i / npx, px = i - r * npx;
+					const size_t p = static_cast<size_t>(y0 + r) * side + static_cast<size_t>(xa) + px;
+					bin_count[c.Pixel + p] = static_cast<unsigned char>(outN[(jb + px) * 65u + r]);
+				}
+				for (unsigned int i = lane; i < rows * npx * cap; i += ShfLineBlock) {
+					const unsigned int r = i / (npx * cap), q = i - r * npx * cap, px = q / cap, k = q - px * cap;
+					const size_t p = static_cast<size_t>(y0 + r) * side + static_cast<size_t>(xa) + px;
+					const unsigned int at = ((jb + px) * cap + k) * 65u + r;
+					bin_item[(c.Pixel + p) * cap + k] = static_cast<unsigned short>(outItem[at]);
+					bin_weight[(c.Pixel + p) * cap + k] = outW[at];
+				}
+			}
+		};
+		if constexpr (MaxCap > 0u) {
+			// register path: HPrefetch columns (counts and every slot) are fetched with clamped,
+			// in-bounds indices before the accumulator slides over them
+			const unsigned int yc = min(y, side - 1u);
+			const auto colIndex = [&](const int col) {
+				return static_cast<size_t>(col < 0 ? 0 : (col >= static_cast<int>(ncols) ? static_cast<int>(ncols) - 1 : col));
+			};
+			for (int s0 = first; s0 < static_cast<int>(side); s0 += static_cast<int>(HPrefetch)) {
+				unsigned int eN[HPrefetch], lN[HPrefetch], eB[HPrefetch][MaxCap], lB[HPrefetch][MaxCap];
+#pragma unroll
+				for (unsigned int j = 0u; j < HPrefetch; j++) {
+					const size_t ce = colIndex(s0 + 2 * static_cast<int>(radius) + static_cast<int>(j)),
+						cl = colIndex(s0 - 1 + static_cast<int>(j));
+					eN[j] = inN[ce * side + yc];
+					lN[j] = inN[cl * side + yc];
+#pragma unroll
+					for (unsigned int k = 0u; k < MaxCap; k++) {
+						eB[j][k] = k < vcap ? inBin[(ce * vcap + k) * side + yc] : 0u;
+						lB[j][k] = k < vcap ? inBin[(cl * vcap + k) * side + yc] : 0u;
+					}
+				}
+				const int sEnd = min(s0 + static_cast<int>(HPrefetch), static_cast<int>(side));
+#pragma unroll
+				for (unsigned int j = 0u; j < HPrefetch; j++) {
+					const int s = s0 + static_cast<int>(j);
+					if (active && s < sEnd) {
+#pragma unroll
+						for (unsigned int k = 0u; k < MaxCap; k++) {
+							if (k < eN[j]) {
+								acc.increment(eB[j][k] & 0xFFFFu, eB[j][k] >> 16);
+							}
+						}
+						if (s >= 1) {
+#pragma unroll
+							for (unsigned int k = 0u; k < MaxCap; k++) {
+								if (k < lN[j]) {
+									acc.decrement(lB[j][k] & 0xFFFFu, lB[j][k] >> 16);
+								}
+							}
+						}
+						if (s >= 0) {
+							emit(j);
+						}
+					}
+				}
+				__syncthreads();
+				flush(s0, sEnd);
+				__syncthreads();
+			}
+		} else {
+			// LDS path (any capacity): blocks of B columns staged in LDS
+			for (int s0 = first; s0 < static_cast<int>(side); s0 += static_cast<int>(B)) {
+				for (unsigned int j = 0u; j < B; j++) {
+					const int ce = s0 + 2 * static_cast<int>(radius) + static_cast<int>(j), cl = s0 - 1 + static_cast<int>(j);
+					if (active && ce < static_cast<int>(ncols)) {
+						const size_t col = static_cast<size_t>(ce);
+						enterN[j * ShfLineBlock + lane] = inN[col * side + y];
+						for (unsigned int k = 0u; k < vcap; k++) {
+							enterBin[(j * vcap + k) * ShfLineBlock + lane] = inBin[(col * vcap + k) * side + y];
+						}
+					}
+					if (active && cl >= 0 && cl < static_cast<int>(ncols)) {
+						const size_t col = static_cast<size_t>(cl);
+						leaveN[j * ShfLineBlock + lane] = inN[col * side + y];
+						for (unsigned int k = 0u; k < vcap; k++) {
+							leaveBin[(j * vcap + k) * ShfLineBlock + lane] = inBin[(col * vcap + k) * side + y];
+						}
+					}
+				}
+				__syncthreads();
+				const int sEnd = min(s0 + static_cast<int>(B), static_cast<int>(side));
+				if (active) {
+					for (int s = s0; s < sEnd; s++) {
+						const unsigned int j = static_cast<unsigned int>(s - s0);
+						const unsigned int ne = enterN[j * ShfLineBlock + lane];
+						for (unsigned int k = 0u; k < ne; k++) {
+							const unsigned int e = enterBin[(j * vcap + k) * ShfLineBlock + lane];
+							acc.increment(e & 0xFFFFu, e >> 16);
+						}
+						if (s >= 1) {
+							const unsigned int nl = leaveN[j * ShfLineBlock + lane];
+							for (unsigned int k = 0u; k < nl; k++) {
+								const unsigned int e = leaveBin[(j * vcap + k) * ShfLineBlock + lane];
+								acc.decrement(e & 0xFFFFu, e >> 16);
+							}
+						}
+						if (s >= 0) {
+							emit(j);
+						}
+					}
+				}
+				__syncthreads();
+				flush(s0, sEnd);
+				__syncthreads();
+			}
 		}
-		if (shfHorizontalLine(make_uint2(c.Side, c.Side), radius, vcap, cap, vbin + c.Line * vcap, vcount + c.Line,
-				lds + threadIdx.x, bin_count + c.Pixel, bin_item + c.Pixel * cap, bin_weight + c.Pixel * cap, row)) {
+		if (acc.overflowed()) {
 			atomicOr(overflow, 1u);
 		}
 	}
@@ -268,7 +546,7 @@ namespace SuperTerrainPlus::STPBatch {
 		const ErosionArgs args, const unsigned int tile, const unsigned int halo, const int2* const brush_offset,
 		const float* const brush_weight, const unsigned int* const tile_drops, const unsigned int* const tile_first,
 		const unsigned int* const bin, float* const height, float* const ledger, uint2* const drop_info) {
-		extern __shared__ float window[];
+		extern __shared__ int window[];
 		__shared__ int2 brush[MaxTiledTaps];
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
@@ -284,7 +562,7 @@ namespace SuperTerrainPlus::STPBatch {
 		const unsigned int cells = s.RW * s.RH;
 		for (unsigned int i = threadIdx.x; i < cells; i += Threads) {
 			const unsigned int wy = i / s.RW, wx = i - wy * s.RW;
-			window[i] = map[(s.Y0 + wy) * side + s.X0 + wx];
+			window[i] = toFixedPoint(map[(s.Y0 + wy) * side + s.X0 + wx]);
 		}
 		for (unsigned int k = threadIdx.x; k < args.BrushSize; k += Threads) {
 			const int2 o = brush_offset[k];
@@ -297,10 +575,16 @@ namespace SuperTerrainPlus::STPBatch {
 			erodeDropAnyBrush(s, a, brush, bin[first + k], ledger + c.DropBase, drop_info + c.DropBase);
 		}
 		__syncthreads();
-		// no other workgroup of this colour batch touches the window
+		// No other workgroup of this colour batch touches the window, so the heights in HBM are
+		// still the ones the window was made from: a cell whose fixed-point value is unchanged
+		// keeps its exact FP32 height.
 		for (unsigned int i = threadIdx.x; i < cells; i += Threads) {
 			const unsigned int wy = i / s.RW, wx = i - wy * s.RW;
-			map[(s.Y0 + wy) * side + s.X0 + wx] = window[i];
+			float* const p = map + (s.Y0 + wy) * side + s.X0 + wx;
+			const int q = window[i];
+			if (q != toFixedPoint(*p)) {
+				*p = fromFixedPoint(q);
+			}
 		}
 	}
 
@@ -457,6 +741,21 @@ STPHeightfieldBatch::STPHeightfieldBatch(const STPEnvironment::STPSimplexNoiseSe
 	this->Counter = STPSmartDeviceMemory::makeDevice<unsigned long long>(4u);
 	STP_CHECK_HIP(hipMemset(this->Fixed.get(), 0, pixel * sizeof(unsigned short)));
 	STP_CHECK_HIP(hipMemset(this->Height.get(), 0, pixel * sizeof(float)));
+	{
+		const int vb = static_cast<int>(verticalLdsWords(this->VCap) * sizeof(unsigned int)),
+			hb = static_cast<int>(horizontalLdsWords(this->VCap, setting.BinCapacity) * sizeof(unsigned int));
+		const void* const vk[] = {reinterpret_cast<const void*>(&verticalKERNEL<0u>),
+			reinterpret_cast<const void*>(&verticalKERNEL<4u>), reinterpret_cast<const void*>(&verticalKERNEL<8u>),
+			reinterpret_cast<const void*>(&verticalKERNEL<16u>)};
+		const void* const hk[] = {reinterpret_cast<const void*>(&horizontalKERNEL<0u>),
+			reinterpret_cast<const void*>(&horizontalKERNEL<4u>), reinterpret_cast<const void*>(&horizontalKERNEL<8u>)};
+		for (const void* k : vk) {
+			STP_CHECK_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, vb));
+		}
+		for (const void* k : hk) {
+			STP_CHECK_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, hb));
+		}
+	}
 	this->Tuning = this->defaultTuning();
 	this->buildTileTables();
 	for (hipEvent_t& e : this->Event) {
@@ -691,20 +990,53 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 		break;
 	}
 	case 1: {
-		const auto [t, n] = range(1);
+		const auto tn = range(1);
+		const uint2* const t = tn.first;
+		const unsigned int n = tn.second;
 		if (n) {
-			verticalKERNEL<<<n, ShfLineBlock, this->VCap * ShfLineBlock * sizeof(unsigned int), stream>>>(t,
-				this->ChunkRecord.get(), radius, this->VCap, this->BiomeMap.get(), this->VBin.get(), this->VCount.get(),
-				this->Flag.get());
+			const auto launch = [&](auto kernel) {
+				kernel<<<n, ShfLineBlock, verticalLdsWords(this->VCap) * sizeof(unsigned int), stream>>>(t,
+					this->ChunkRecord.get(), radius, this->VCap, this->BiomeMap.get(), this->VBin.get(),
+					this->VCount.get(), this->Flag.get());
+			};
+			switch (registerCapacity(this->VCap)) {
+			case 4u:
+				launch(verticalKERNEL<4u>);
+				break;
+			case 8u:
+				launch(verticalKERNEL<8u>);
+				break;
+			case 16u:
+				launch(verticalKERNEL<16u>);
+				break;
+			default:
+				launch(verticalKERNEL<0u>);
+				break;
+			}
 		}
 		break;
 	}
 	case 2: {
-		const auto [t, n] = range(2);
+		const auto tn = range(2);
+		const uint2* const t = tn.first;
+		const unsigned int n = tn.second;
 		if (n) {
-			horizontalKERNEL<<<n, ShfLineBlock, 2u * cap * ShfLineBlock * sizeof(unsigned int), stream>>>(t,
-				this->ChunkRecord.get(), radius, this->VCap, cap, this->VBin.get(), this->VCount.get(), this->BinCount.get(),
-				this->BinItem.get(), this->BinWeight.get(), this->Flag.get());
+			const auto launch = [&](auto kernel) {
+				kernel<<<n, ShfLineBlock, horizontalLdsWords(this->VCap, cap) * sizeof(unsigned int), stream>>>(t,
+					this->ChunkRecord.get(), radius, this->VCap, cap, this->VBin.get(), this->VCount.get(),
+					this->BinCount.get(), this->BinItem.get(), this->BinWeight.get(), this->Flag.get());
+			};
+			switch (horizontalRegisterCapacity(cap)) {
+			case 4u:
+				launch(horizontalKERNEL<4u>);
+				break;
+			case 8u:
+				launch(horizontalKERNEL<8u>);
+				break;
+			default:
+				launch(horizontalKERNEL<0u>);
+				break;
+			}
 		}
 		break;
 	}

@@ -62,6 +62,114 @@ namespace SuperTerrainPlus::STPStage {
 		}
 	};
 
+	// Insertion-ordered bins held in registers, for a compile-time slot bound MaxCap: the same
+	// rules as VerticalAcc / HorizontalAcc (search, append at the end, erase at zero with the later
+	// bins shifting down), with every slot index static so the bins never leave registers. `cap`
+	// (<= MaxCap) is the run-time capacity that decides overflow.
+	template <unsigned int MaxCap>
+	struct RegisterAcc {
+		unsigned int Item[MaxCap], Count[MaxCap];
+		unsigned int n, cap;
+		bool overflow;
+
+		__device__ explicit RegisterAcc(const unsigned int capacity) : n(0u), cap(capacity), overflow(false) {
+#pragma unroll
+			for (unsigned int k = 0u; k < MaxCap; k++) {
+				this->Item[k] = 0xFFFFFFFFu;
+				this->Count[k] = 0u;
+			}
+		}
+		__device__ inline void increment(const unsigned int it, const unsigned int c) {
+			bool found = false;
+#pragma unroll
+			for (unsigned int k = 0u; k < MaxCap; k++) {
+				if (k < this->n && this->Item[k] == it) {
+					this->Count[k] += c;
+					found = true;
+				}
+			}
+			if (found) {
+				return;
+			}
+			if (this->n < this->cap) {
+#pragma unroll
+				for (unsigned int k = 0u; k < MaxCap; k++) {
+					if (k == this->n) {
+						this->Item[k] = it;
+						this->Count[k] = c;
+					}
+				}
+				this->n++;
+			} else {
+				this->overflow = true;
+			}
+		}
+		__device__ inline void decrement(const unsigned int it, const unsigned int c) {
+			unsigned int pos = MaxCap, q = 0u;
+#pragma unroll
+			for (unsigned int k = 0u; k < MaxCap; k++) {
+				if (pos == MaxCap && k < this->n && this->Item[k] == it) {
+					pos = k;
+					q = this->Count[k];
+				}
+			}
+			if (pos == MaxCap) {
+				return;
+			}
+			if (q <= c) {
+#pragma unroll
+				for (unsigned int k = 0u; k + 1u < MaxCap; k++) {
+					if (k >= pos) {
+						this->Item[k] = this->Item[k + 1u];
+						this->Count[k] = this->Count[k + 1u];
+					}
+				}
+				this->n--;
+			} else {
+#pragma unroll
+				for (unsigned int k = 0u; k < MaxCap; k++) {
+					if (k == pos) {
+						this->Count[k] = q - c;
+					}
+				}
+			}
+		}
+		// f(slot, item, count) for every bin, in order
+		template <class F>
+		__device__ inline void forEach(F&& f) const {
+#pragma unroll
+			for (unsigned int k = 0u; k < MaxCap; k++) {
+				if (k < this->n) {
+					f(k, this->Item[k], this->Count[k]);
+				}
+			}
+		}
+	};
+
+	// The LDS accumulators behind the same interface (any capacity).
+	struct LdsVerticalAcc {
+		VerticalAcc a;
+		__device__ LdsVerticalAcc(unsigned int* const slot, const unsigned int capacity) : a {slot, 0u, capacity, false} {}
+		__device__ inline void increment(const unsigned int it, unsigned int) { this->a.increment(it); }
+		__device__ inline void decrement(const unsigned int it, unsigned int) { this->a.decrement(it); }
+		template <class F>
+		__device__ inline void forEach(F&& f) const {
+			for (unsigned int k = 0u; k < this->a.n; k++) {
+				const unsigned int e = this->a.slot[k * ShfLineBlock];
+				f(k, e & 0xFFFFu, e >> 16);
+			}
+		}
+		__device__ unsigned int size() const { return this->a.n; }
+		__device__ bool overflowed() const { return this->a.overflow; }
+	};
+
+	template <unsigned int MaxCap>
+	struct RegisterLineAcc : RegisterAcc<MaxCap> {
+		__device__ explicit RegisterLineAcc(const unsigned int capacity) : RegisterAcc<MaxCap>(capacity) {}
+		__device__ unsigned int size() const { return this->n; }
+		__device__ bool overflowed() const { return this->overflow; }
+	};
+
 	// Vertical pass over column `c` (0 <= c < map.x + 2r) of the window whose central map starts at
 	// `start` in a sample map with row pitch `pitch`. Writes vcount[y][c] and vbin[y][slot][c].
 	// Returns true when a histogram needed more than `vcap` bins.
@@ -127,6 +235,22 @@ namespace SuperTerrainPlus::STPStage {
 				}
 			}
 		}
+	};
+
+	struct LdsHorizontalAcc {
+		HorizontalAcc a;
+		__device__ LdsHorizontalAcc(unsigned int* const item, unsigned int* const count, const unsigned int capacity) :
+			a {item, count, 0u, capacity, false} {}
+		__device__ inline void increment(const unsigned int it, const unsigned int c) { this->a.increment(it, c); }
+		__device__ inline void decrement(const unsigned int it, const unsigned int c) { this->a.decrement(it, c); }
+		template <class F>
+		__device__ inline void forEach(F&& f) const {
+			for (unsigned int k = 0u; k < this->a.n; k++) {
+				f(k, this->a.item[k * ShfLineBlock], this->a.count[k * ShfLineBlock]);
+			}
+		}
+		__device__ unsigned int size() const { return this->a.n; }
+		__device__ bool overflowed() const { return this->a.overflow; }
 	};
 
 	// Horizontal pass over row `y` (0 <= y < map.y) of the vertical output. Writes the dense

@@ -22,9 +22,9 @@ def props():
 
 
 class Batch:
-    def __init__(self, dpp, sides=SIDES, origins=ORIGINS, radius=RADIUS):
+    def __init__(self, dpp, sides=SIDES, origins=ORIGINS, radius=RADIUS, cap=8):
         self.sides, self.origins, self.radius = sides, origins, radius
-        setting = capi.BatchSetting(SEED, COUNT, CELL, radius, 8, dpp)
+        setting = capi.BatchSetting(SEED, COUNT, CELL, radius, cap, dpp)
         chunks = (capi.BatchChunk * len(sides))(*[capi.BatchChunk(s, (C.c_int32 * 2)(*o))
                                                    for s, o in zip(sides, origins)])
         h = C.c_void_p()
@@ -75,9 +75,12 @@ def oracle_chunk(oracle, i):
     return halo, h
 
 
-def test_batch_without_erosion_bit_exact(gpu, oracle):
-    log("batch no erosion")
-    b = Batch(0.0)
+# histogram capacities: 8 keeps both filter passes' accumulators in registers, 16 keeps the
+# vertical one in registers and the horizontal one in LDS, 33 puts both in LDS
+@pytest.mark.parametrize("cap", [8, 16, 33])
+def test_batch_without_erosion_bit_exact(gpu, oracle, cap):
+    log(f"batch no erosion cap={cap}")
+    b = Batch(0.0, cap=cap)
     results = {}
     for mode in (0, 1, 2):
         b.run(mode)
