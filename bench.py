@@ -35,6 +35,9 @@ STAGES = ["biome", "filter_vertical", "filter_horizontal", "blend", "erosion_bin
 # algorithmic bytes of one drop step (r = 3 brush, 25 taps): two 2x2 bilinear height reads;
 # an erode step adds 25 x (4 B read + 8 B atomic read-modify-write); a deposit step 4 x 8 B.
 BYTES_STEP, BYTES_ERODE, BYTES_DEPOSIT = 32, 25 * 12, 4 * 8
+# HBM bytes per launch of the tiled erosion kernel (tile 128, 1024 threads) on this workload, from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a kernel of known size (profiles/erosion_hbm_traffic.md)
+TILED_EROSION_TRAFFIC = 492.7e6
 
 
 def workload():
@@ -160,20 +163,26 @@ def main():
     batched_mpx = pixels / (wall / args.steps) / 1e6
     print(f"[bench] batched {batched_mpx:.1f} Mpixel/s ({ms_per_step:.2f} ms/step)", file=sys.stderr, flush=True)
 
-    # plain: one chunk per launch sequence + host sync per chunk
-    lat = []
-    t0 = time.perf_counter()
-    for _ in range(args.plain_steps):
-        capi.call("stp_batch_run", batch, 2, stream)
-        arr = (C.c_double * len(sides))()
-        capi.call("stp_batch_chunk_latencies", batch, arr, len(sides))
-        lat.extend(arr)
-    plain_wall = (time.perf_counter() - t0) / args.plain_steps
-    plain_mpx = pixels / plain_wall / 1e6
-    lat = np.array(lat)
-    print(f"[bench] plain {plain_mpx:.1f} Mpixel/s", file=sys.stderr, flush=True)
+    # plain: one chunk per launch sequence + host sync per chunk (reported beside the batched
+    # value; --plain-steps 0 leaves it out, e.g. for a kernel profile of the batched launches only)
+    plain_mpx, lat = None, np.zeros(0)
+    if args.plain_steps > 0:
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            capi.call("stp_batch_run", batch, 2, stream)
+            arr = (C.c_double * len(sides))()
+            capi.call("stp_batch_chunk_latencies", batch, arr, len(sides))
+            lat.extend(arr)
+        plain_wall = (time.perf_counter() - t0) / args.plain_steps
+        plain_mpx = pixels / plain_wall / 1e6
+        lat = np.array(lat)
+        print(f"[bench] plain {plain_mpx:.1f} Mpixel/s", file=sys.stderr, flush=True)
 
     erosion_bytes = steps * BYTES_STEP + erode * BYTES_ERODE + deposit * BYTES_DEPOSIT
+    # the tiled erosion launches once per colour batch; the global-atomic kernel once
+    kernel = f"erodeTileKERNEL<{tuning.ErosionThreads}>" if tuning.ErosionTile else "erosionKERNEL"
+    launches = 4 if tuning.ErosionTile else 1
     erosion_s = stage_ms[STAGES.index("erosion")] / 1e3
     achieved = erosion_bytes / erosion_s / 1e9 if erosion_s > 0 else 0.0
     result = {
@@ -184,9 +193,10 @@ def main():
             "workload": "batched: 256 chunks, sides {128,256,512,1024} p={.4,.3,.2,.1} seed 11, 4-biome Voronoi "
                         "(seed 7, cell 256) blend r=16, 8-octave simplex, 0.25 drops/px parallel erosion, UINT16",
             "chunks": len(sides), "mpixel": round(pixels / 1e6, 3), "drops": drops,
-            "plain_mpixel_s": round(plain_mpx, 2), "batched_vs_plain": round(batched_mpx / plain_mpx, 2),
+            "plain_mpixel_s": round(plain_mpx, 2) if plain_mpx else None,
+            "batched_vs_plain": round(batched_mpx / plain_mpx, 2) if plain_mpx else None,
             "plain_chunk_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
-                                       "p99": round(float(np.percentile(lat, 99)), 3)},
+                                       "p99": round(float(np.percentile(lat, 99)), 3)} if len(lat) else None,
             "batched_step_latency_ms": round(ms_per_step, 3),
             "single_graph_mpixel_s": round(pixels / single_wall / 1e6, 2),
             "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES, stage_ms)},
@@ -195,8 +205,13 @@ def main():
             "erosion_counters": {"steps": steps, "erode_steps": erode, "deposit_steps": deposit, "killed": killed},
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "erosionKERNEL", "algorithmic_bytes": erosion_bytes},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TILED_EROSION_TRAFFIC if (tuning.ErosionTile, tuning.ErosionThreads) == (128, 1024) else None,
+                     "kernel": kernel, "launches_per_step": launches,
+                     "launch_ms_avg": round(stage_ms[STAGES.index("erosion")] / launches, 4),
+                     "algorithmic_bytes_per_step": erosion_bytes,
+                     "note": "algorithmic bytes = the drop steps' height reads and updates (32 B per step, "
+                             "+300 B per erode step, +32 B per deposit step); the LDS-tiled kernel serves "
+                             "them from LDS, so they can exceed the HBM peak"},
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
