@@ -38,6 +38,8 @@ BYTES_STEP, BYTES_ERODE, BYTES_DEPOSIT = 32, 25 * 12, 4 * 8
 # HBM bytes per launch of the tiled erosion kernel (tile 128, 1024 threads) on this workload, from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a kernel of known size (profiles/erosion_hbm_traffic.md)
 TILED_EROSION_TRAFFIC = 492.7e6
+# serving loop: two engines of 8 Mpx / 512 chunks each
+SERVER_BATCH_PIXELS, SERVER_BATCH_CHUNKS, SERVER_ENGINES = 8 << 20, 512, 2
 
 
 def workload():
@@ -99,6 +101,48 @@ def cpu_baseline(cfg, sides, origins, props, seconds, threads):
                       f"(noise, filter, blend, serialized erosion, format), {wall:.1f} s wall"}
 
 
+def served(seconds_scale=1.0):
+    """Camera-walk request stream replayed in real time at each rate on a fresh server; a short
+    warm-up trace far from the walk comes first so that the timed replay finds warm engines."""
+    from superterrainplus_amd import capi, workloads as W
+    sv = W.served()
+    cfg = sv.batch
+    noise = capi.SimplexNoiseSetting(cfg.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, cfg.engine)
+    props = [(p[0], cfg.fractal.octave, p[2], p[3], p[4], p[5]) for p in cfg.table()]
+    batch = capi.BatchSetting(cfg.biome_seed, cfg.biome_count, cfg.cell, cfg.radius, cfg.biome_count,
+                              cfg.drops_per_pixel)
+    setting = capi.ServerSetting(sv.memory_budget, SERVER_BATCH_PIXELS, SERVER_BATCH_CHUNKS, 2048, SERVER_ENGINES,
+                                 sv.world)
+    out = {}
+    for rate in sv.rates:
+        h = C.c_void_p()
+        capi.call("stp_server_create", C.byref(noise), capi.biome_array(props), len(props),
+                  C.byref(W.erosion_setting(1, seed=cfg.erosion_seed)), C.byref(batch), C.byref(setting), C.byref(h))
+        try:
+            def replay(req, arr):
+                r = (capi.ChunkRequest * len(req))(*[capi.ChunkRequest(x, y, side) for x, y, side in req])
+                a = (C.c_double * len(req))(*arr)
+                lat = (C.c_double * len(req))()
+                st = capi.ServerStats()
+                capi.call("stp_server_replay", h, r, a, len(req), sv.deadline_ms, lat, C.byref(st))
+                return st
+            wreq, warr = sv.trace(rate, count=min(200, int(rate * sv.duration_s)))
+            replay([(x + 4096, y, side) for x, y, side in wreq], warr)
+            req, arr = sv.trace(rate, count=int(rate * sv.duration_s * seconds_scale))
+            st = replay(req, arr)
+        finally:
+            capi.lib().stp_server_destroy(h)
+        out[str(rate)] = {"requests": int(st.Requests), "p50_ms": round(st.P50, 3), "p99_ms": round(st.P99, 3),
+                          "p99_9_ms": round(st.P999, 3), "max_ms": round(st.Max, 3),
+                          "mpixel_s": round(st.MpixelPerSecond, 1),
+                          "deadline_miss": round(st.DeadlineMissFraction, 4), "batches": int(st.Batches),
+                          "chunks_per_batch": round(st.MeanBatchChunks, 1), "cache_hits": int(st.CacheHits),
+                          "evictions": int(st.Evictions)}
+        print(f"[bench] served {rate}/s: p50 {st.P50:.2f} p99 {st.P99:.2f} p99.9 {st.P999:.2f} ms, "
+              f"{st.MpixelPerSecond:.0f} Mpixel/s, {st.Batches} batches", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +152,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default erosion launch shape")
+    ap.add_argument("--no-served", action="store_true", help="leave out the camera-walk serving replays")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -213,6 +258,12 @@ def main():
                              "+300 B per erode step, +32 B per deposit step); the LDS-tiled kernel serves "
                              "them from LDS, so they can exceed the HBM peak"},
     }
+    if not args.no_served:
+        result["config"]["served"] = {
+            "workload": "camera random walk 0.5 chunk / 16 ms (seed 3), view radius 8, LOD 64..2048 px by distance, "
+                        "Poisson arrivals, 16 ms deadline, whole path per chunk as the batched workload, "
+                        "2 engines x 8 Mpx, 200 GB budget with LRU",
+            "rates": served()}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(cfg, sides, origins, props, args.cpu_seconds, threads)

@@ -332,7 +332,8 @@ typedef struct {
 
 typedef struct {
 	uint32_t Side;         /* square chunk side in pixels */
-	int32_t Origin[2];     /* world pixel of the chunk's first pixel */
+	int32_t Origin[2];     /* world coordinate of the chunk's first pixel */
+	uint32_t Step;         /* world units per pixel, a power of two (level of detail); 0 means 1 */
 } stp_batch_chunk;
 
 typedef struct stp_heightfield_batch stp_heightfield_batch;
@@ -378,6 +379,64 @@ int stp_batch_autotune(stp_heightfield_batch* batch, uint32_t repeat, void* stre
  * (side^2), halo biome map ((side + 2 * Radius)^2), per-drop final sediment (drops). */
 int stp_batch_read_chunk(stp_heightfield_batch* batch, uint32_t index, float* h_height, uint16_t* h_fixed,
 	uint16_t* h_biome_halo, float* h_ledger);
+
+/* ------------------------------------------------------------------ serving loop
+ * No reference counterpart (nearest: the world pipeline's asynchronous chunk loader,
+ * SuperTerrain+/SuperTerrain+/Private/World/STPWorldPipeline.cpp:1504-1633). Chunk requests arrive
+ * one by one; `Engines` batches run concurrently on their own streams; requests that arrive while
+ * batches run join the next batch (continuous batching); results stay resident in device memory
+ * within MemoryBudget, least recently used evicted first. A chunk's result does not depend on the
+ * batch it runs in (its drops use a Philox stream word made from its coordinate and side). */
+typedef struct {
+	uint64_t MemoryBudget;   /* device bytes for the batch engines plus the resident chunks */
+	uint64_t BatchPixels;    /* pixel capacity of one batch (>= MaxSide^2) */
+	uint32_t BatchChunks;    /* chunk capacity of one batch (1..65535) */
+	uint32_t MaxSide;        /* largest chunk side, a power of two */
+	uint32_t Engines;        /* batches in flight (1..8) */
+	uint32_t ChunkWorldSize; /* world units of one grid chunk, a power of two >= MaxSide */
+} stp_server_setting;
+
+/* Grid chunk (X, Y) at Side pixels per side (a power of two): world origin (X, Y) * ChunkWorldSize,
+ * ChunkWorldSize / Side world units per pixel. */
+typedef struct {
+	int32_t X, Y;
+	uint32_t Side;
+} stp_chunk_request;
+
+typedef struct {
+	double P50, P99, P999, Mean, Max;  /* request latency, milliseconds */
+	double MpixelPerSecond;            /* completed pixels / replay wall time */
+	double DeadlineMissFraction;
+	uint64_t Requests, CacheHits, Batches, Evictions;
+	double MeanBatchChunks;
+	uint64_t PeakResidentBytes;
+} stp_server_stats;
+
+typedef struct stp_heightfield_server stp_heightfield_server;
+
+/* raindrop->RainDropCount is ignored (per-chunk counts come from batch->DropsPerPixel). */
+int stp_server_create(const stp_simplex_noise_setting* noise, const stp_biome_property* h_table, uint32_t biome_count,
+	const stp_raindrop_setting* raindrop, const stp_batch_setting* batch, const stp_server_setting* setting,
+	stp_heightfield_server** out);
+int stp_server_destroy(stp_heightfield_server* server);
+/* Queues a request (arrival = now) and returns its ticket. */
+int stp_server_submit(stp_heightfield_server* server, const stp_chunk_request* request, uint64_t* ticket);
+/* One scheduler pass (never waits for the GPU), then returns up to `capacity` completed tickets
+ * with their latency in milliseconds; *count = how many were written. */
+int stp_server_poll(stp_heightfield_server* server, uint64_t* h_ticket, double* h_latency_ms, uint32_t capacity,
+	uint32_t* count);
+/* Waits until every queued request has completed (the completions stay available to poll). */
+int stp_server_flush(stp_heightfield_server* server);
+/* Copies a resident chunk to host memory (either pointer may be NULL); STP_ERROR_INVALID_ARGUMENT if
+ * it is not resident. Synchronous. */
+int stp_server_read(stp_heightfield_server* server, const stp_chunk_request* request, float* h_height,
+	uint16_t* h_fixed);
+/* Device bytes held by the batch engines and by the resident chunks. */
+int stp_server_memory(stp_heightfield_server* server, uint64_t* engine_bytes, uint64_t* resident_bytes);
+/* Replays a trace in real time: request i is submitted h_arrival_s[i] seconds after the start
+ * (non-decreasing). h_latency_ms[i] receives each request's latency. Synchronous. */
+int stp_server_replay(stp_heightfield_server* server, const stp_chunk_request* h_requests, const double* h_arrival_s,
+	uint32_t count, double deadline_ms, double* h_latency_ms, stp_server_stats* out);
 
 #ifdef __cplusplus
 }

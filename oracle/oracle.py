@@ -94,10 +94,10 @@ def lib():
             getattr(L, n).restype = None
             getattr(L, n).argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(BiomeProperty),
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
-                                      C.c_float, C.c_float, C.c_void_p]
+                                      C.c_float, C.c_float, C.c_uint32, C.c_void_p]
         L.orc_voronoi_map.restype = None
         L.orc_voronoi_map.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.c_int64,
-                                      C.c_uint32, C.c_uint32, C.c_void_p]
+                                      C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
         L.orc_chunk_world_coord.restype = None
         L.orc_chunk_world_coord.argtypes = [C.c_double, C.c_double, C.c_uint32, C.c_uint32,
                                             C.c_double, C.c_double, C.c_void_p]
@@ -247,20 +247,21 @@ def biome_table(props):
     return arr
 
 
-def multi_biome(perm, grad, props, item, weight, offset, map_size, offset_xy, f64=False):
+def multi_biome(perm, grad, props, item, weight, offset, map_size, offset_xy, f64=False, step=1):
+    """`step`: world units per pixel (pixel x samples x * step; 1 = the reference's sampling)."""
     mx, my = map_size
     out = np.zeros((my, mx), np.float64 if f64 else np.float32)
     tbl = biome_table(props)
     fn = lib().orc_multi_biome_f64 if f64 else lib().orc_multi_biome_f32
     w = np.ascontiguousarray(weight, np.float64 if f64 else np.float32)
     fn(perm.ctypes.data, grad.ctypes.data, grad.size // 2, tbl, item.ctypes.data, w.ctypes.data,
-       offset.ctypes.data, mx, my, offset_xy[0], offset_xy[1], out.ctypes.data)
+       offset.ctypes.data, mx, my, offset_xy[0], offset_xy[1], step, out.ctypes.data)
     return out
 
 
-def voronoi_map(seed, biome_count, cell, origin, W, H):
+def voronoi_map(seed, biome_count, cell, origin, W, H, step=1):
     out = np.zeros((H, W), np.uint16)
-    lib().orc_voronoi_map(seed, biome_count, cell, origin[0], origin[1], W, H, out.ctypes.data)
+    lib().orc_voronoi_map(seed, biome_count, cell, origin[0], origin[1], W, H, step, out.ctypes.data)
     return out
 
 

@@ -517,9 +517,10 @@ struct BiomeProperty {  // SuperDemo+/World/Biomes/STPBiomeProperty.hpp:10-27
 template <class Real>
 void multiBiomeHeightmap(const Permutation<Real>& p, const BiomeProperty* table, const uint16_t* binItem,
 	const Real* binWeight, const uint32_t* offset, const uint32_t mapX, const uint32_t mapY, const float offX,
-	const float offY, Real* out) {
-	const Real halfX = static_cast<Real>(static_cast<float>(mapX) / 2.0f),
-		halfY = static_cast<Real>(static_cast<float>(mapY) / 2.0f);
+	const float offY, const uint32_t step, Real* out) {
+	// `step`: world units per pixel (1 = the reference's sampling); pixel x samples x * step
+	const Real halfX = static_cast<Real>(static_cast<float>(mapX * step) / 2.0f),
+		halfY = static_cast<Real>(static_cast<float>(mapY * step) / 2.0f);
 	for (uint32_t y = 0u; y < mapY; y++) {
 		for (uint32_t x = 0u; x < mapX; x++) {
 			const uint32_t idx = x + y * mapX;
@@ -537,7 +538,8 @@ void multiBiomeHeightmap(const Permutation<Real>& p, const BiomeProperty* table,
 				f.HalfY = halfY;
 				f.Amplitude = Real(1);
 				f.Frequency = Real(1);
-				const Real n = simplex2DFractal<Real>(p, Real(1) * static_cast<Real>(x), Real(1) * static_cast<Real>(y), f)
+				const Real n = simplex2DFractal<Real>(p, Real(1) * static_cast<Real>(x * step),
+					Real(1) * static_cast<Real>(y * step), f)
 					* static_cast<Real>(bp.Variation) + static_cast<Real>(bp.Depth);
 				h += binWeight[b] * n;
 			}
@@ -800,23 +802,25 @@ struct orc_biome_property {
 
 void orc_multi_biome_f32(const uint8_t* perm, const float* grad, const uint32_t n, const orc_biome_property* table,
 	const uint16_t* item, const float* weight, const uint32_t* offset, const uint32_t mapX, const uint32_t mapY,
-	const float offX, const float offY, float* out) {
+	const float offX, const float offY, const uint32_t step, float* out) {
 	multiBiomeHeightmap<float>(Permutation<float> {perm, grad, n}, reinterpret_cast<const BiomeProperty*>(table), item,
-		weight, offset, mapX, mapY, offX, offY, out);
+		weight, offset, mapX, mapY, offX, offY, step, out);
 }
 
 void orc_multi_biome_f64(const uint8_t* perm, const double* grad, const uint32_t n, const orc_biome_property* table,
 	const uint16_t* item, const double* weight, const uint32_t* offset, const uint32_t mapX, const uint32_t mapY,
-	const float offX, const float offY, double* out) {
+	const float offX, const float offY, const uint32_t step, double* out) {
 	multiBiomeHeightmap<double>(Permutation<double> {perm, grad, n}, reinterpret_cast<const BiomeProperty*>(table),
-		item, weight, offset, mapX, mapY, offX, offY, out);
+		item, weight, offset, mapX, mapY, offX, offY, step, out);
 }
 
 void orc_voronoi_map(const uint64_t seed, const uint32_t biomeCount, const uint32_t cellSize, const int64_t originX,
-	const int64_t originY, const uint32_t W, const uint32_t H, uint16_t* out) {
+	const int64_t originY, const uint32_t W, const uint32_t H, const uint32_t step, uint16_t* out) {
+	// pixel (x, y) samples world (origin + x * step, origin + y * step)
 	for (uint32_t y = 0u; y < H; y++) {
 		for (uint32_t x = 0u; x < W; x++) {
-			out[x + static_cast<uint64_t>(y) * W] = voronoiBiome(seed, biomeCount, cellSize, originX + x, originY + y);
+			out[x + static_cast<uint64_t>(y) * W] = voronoiBiome(seed, biomeCount, cellSize,
+				originX + static_cast<int64_t>(x) * step, originY + static_cast<int64_t>(y) * step);
 		}
 	}
 }

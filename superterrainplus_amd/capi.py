@@ -68,7 +68,23 @@ class BatchSetting(C.Structure):
 
 
 class BatchChunk(C.Structure):
-    _fields_ = [("Side", C.c_uint32), ("Origin", C.c_int32 * 2)]
+    _fields_ = [("Side", C.c_uint32), ("Origin", C.c_int32 * 2), ("Step", C.c_uint32)]
+
+
+class ServerSetting(C.Structure):
+    _fields_ = [("MemoryBudget", C.c_uint64), ("BatchPixels", C.c_uint64), ("BatchChunks", C.c_uint32),
+                ("MaxSide", C.c_uint32), ("Engines", C.c_uint32), ("ChunkWorldSize", C.c_uint32)]
+
+
+class ChunkRequest(C.Structure):
+    _fields_ = [("X", C.c_int32), ("Y", C.c_int32), ("Side", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("P50", C.c_double), ("P99", C.c_double), ("P999", C.c_double), ("Mean", C.c_double),
+                ("Max", C.c_double), ("MpixelPerSecond", C.c_double), ("DeadlineMissFraction", C.c_double),
+                ("Requests", C.c_uint64), ("CacheHits", C.c_uint64), ("Batches", C.c_uint64),
+                ("Evictions", C.c_uint64), ("MeanBatchChunks", C.c_double), ("PeakResidentBytes", C.c_uint64)]
 
 
 class BatchTuning(C.Structure):
@@ -164,6 +180,14 @@ _SIGNATURES = {
     "stp_rtc_module_function": (_I, [_P, C.c_char_p, _P]),
     "stp_rtc_module_global": (_I, [_P, C.c_char_p, _P, _P]),
     "stp_rtc_launch": (_I, [_P, _P, _P, _U32, _P, _P]),
+    "stp_server_create": (_I, [_P, _P, _U32, _P, _P, _P, _P]),
+    "stp_server_destroy": (_I, [_P]),
+    "stp_server_submit": (_I, [_P, _P, _P]),
+    "stp_server_poll": (_I, [_P, _P, _P, _U32, _P]),
+    "stp_server_flush": (_I, [_P]),
+    "stp_server_read": (_I, [_P, _P, _P, _P]),
+    "stp_server_replay": (_I, [_P, _P, _P, _U32, C.c_double, _P, _P]),
+    "stp_server_memory": (_I, [_P, _P, _P]),
     "stp_batch_set_tuning": (_I, [_P, _P]),
     "stp_batch_get_tuning": (_I, [_P, _P]),
     "stp_batch_autotune": (_I, [_P, _U32, _P, _P]),

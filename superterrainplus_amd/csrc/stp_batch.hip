@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 
 using namespace SuperTerrainPlus;
 using namespace SuperTerrainPlus::STPStage;
@@ -41,8 +42,11 @@ namespace SuperTerrainPlus::STPBatch {
 			return;
 		}
 		const unsigned int x = p % w, y = p / w;
-		biome[c.Halo + p] = voronoiBiome(seed, count, cell, static_cast<long long>(c.OriginX) - radius + x,
-			static_cast<long long>(c.OriginY) - radius + y);
+		// halo pixel (x, y) is chunk pixel (x - r, y - r), at world Origin + pixel * Step
+		const long long step = c.Step;
+		biome[c.Halo + p] = voronoiBiome(seed, count, cell,
+			static_cast<long long>(c.OriginX) + (static_cast<long long>(x) - static_cast<long long>(radius)) * step,
+			static_cast<long long>(c.OriginY) + (static_cast<long long>(y) - static_cast<long long>(radius)) * step);
 	}
 
 	// ---- single histogram filter, batched form
@@ -375,15 +379,15 @@ namespace SuperTerrainPlus::STPBatch {
 		if (x >= c.Side) {
 			return;
 		}
-		const float half = static_cast<float>(c.Side) / 2.0f;
-		// pixel x of the chunk samples world coordinate Origin + x
+		const float half = static_cast<float>(c.Side * c.Step) / 2.0f;
+		// pixel x of the chunk samples world coordinate Origin + x * Step
 		const float2 offset = make_float2(static_cast<float>(c.OriginX) + half, static_cast<float>(c.OriginY) + half);
 		for (unsigned int k = 0u; k < BlendPixel; k++) {
 			const unsigned int y = y0 + k;
 			if (y < c.Side) {
 				const unsigned long long i = c.Pixel + static_cast<unsigned long long>(y) * c.Side + x;
-				height[i] = blendPixel(table, prop, biome_count, bin_count[i], bin_item + i * cap, bin_weight + i * cap, x, y,
-					offset, make_float2(half, half));
+				height[i] = blendPixel(table, prop, biome_count, bin_count[i], bin_item + i * cap, bin_weight + i * cap,
+					x * c.Step, y * c.Step, offset, make_float2(half, half));
 			}
 		}
 	}
@@ -588,6 +592,19 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 	}
 
+	// chunk blockIdx.y's outputs -> out[chunk] (heights), out[chunks + chunk] (UINT16)
+	__global__ void __launch_bounds__(256) copyOutKERNEL(const Chunk* const chunks, const unsigned int count,
+		const float* const height, const unsigned short* const fixed, void* const* const out) {
+		const Chunk c = chunks[blockIdx.y];
+		float* const h = static_cast<float*>(out[blockIdx.y]);
+		unsigned short* const f = static_cast<unsigned short*>(out[count + blockIdx.y]);
+		const unsigned long long n = static_cast<unsigned long long>(c.Side) * c.Side;
+		for (unsigned long long i = blockIdx.x * 256ull + threadIdx.x; i < n; i += 256ull * gridDim.x) {
+			h[i] = height[c.Pixel + i];
+			f[i] = fixed[c.Pixel + i];
+		}
+	}
+
 	__global__ void __launch_bounds__(256) zeroRangeKERNEL(unsigned int* const word, const unsigned int n) {
 		const unsigned int i = blockIdx.x * 256u + threadIdx.x;
 		if (i < n) {
@@ -645,12 +662,39 @@ namespace {
 	}
 } // namespace
 
+STPBatchCapacity STPHeightfieldBatch::capacityOf(const std::vector<STPBatchChunk>& chunks, const float drops_per_pixel) {
+	STPBatchCapacity cap {0ull, 0ull, static_cast<unsigned int>(chunks.size()), 0u};
+	for (const STPBatchChunk& c : chunks) {
+		const unsigned long long px = static_cast<unsigned long long>(c.Side) * c.Side;
+		cap.Pixels += (px + 3ull) & ~3ull;
+		cap.Drops += static_cast<unsigned long long>(static_cast<double>(px) * drops_per_pixel);
+		cap.MaxSide = std::max(cap.MaxSide, c.Side);
+	}
+	return cap;
+}
+
 STPHeightfieldBatch::STPHeightfieldBatch(const STPEnvironment::STPSimplexNoiseSetting& noise,
 	const std::vector<STPBiomeProperty>& biome, const STPEnvironment::STPRainDropSetting& rain,
 	const STPBatchSetting& setting, const std::vector<STPBatchChunk>& chunks) :
-	Setting(setting), Permutation(STPAlgorithm::STPPermutationGenerator::generate(noise)),
+	STPHeightfieldBatch(noise, biome, rain, setting, capacityOf(chunks, setting.DropsPerPixel)) {
+	if (chunks.empty()) {
+		STP_THROW(STPInvalidArgument, "empty chunk list");
+	}
+	this->bind(chunks, {}, nullptr);
+	STP_CHECK_HIP(hipStreamSynchronize(nullptr));
+}
+
+STPHeightfieldBatch::STPHeightfieldBatch(const STPEnvironment::STPSimplexNoiseSetting& noise,
+	const std::vector<STPBiomeProperty>& biome, const STPEnvironment::STPRainDropSetting& rain,
+	const STPBatchSetting& setting, const STPBatchCapacity& capacity) :
+	Setting(setting), Capacity(capacity), Permutation(STPAlgorithm::STPPermutationGenerator::generate(noise)),
 	Biome(packBiomeTable(biome.data(), static_cast<unsigned int>(biome.size()))),
 	BiomeCount(static_cast<unsigned int>(biome.size())) {
+	this->init(rain);
+}
+
+void STPHeightfieldBatch::init(const STPEnvironment::STPRainDropSetting& rain) {
+	const STPBatchSetting& setting = this->Setting;
 	STPEnvironment::STPRainDropSetting r = rain;
 	r.RainDropCount = 1u; // per chunk counts come from DropsPerPixel
 	r.validate();
@@ -662,8 +706,12 @@ STPHeightfieldBatch::STPHeightfieldBatch(const STPEnvironment::STPSimplexNoiseSe
 	if (setting.BiomeCount == 0u || setting.BiomeCell == 0u || !(setting.DropsPerPixel >= 0.0f)) {
 		STP_THROW(STPInvalidArgument, "invalid batch setting");
 	}
-	if (chunks.empty()) {
-		STP_THROW(STPInvalidArgument, "empty chunk list");
+	const STPBatchCapacity& cap = this->Capacity;
+	if (cap.Chunks == 0u || cap.Pixels == 0ull || cap.MaxSide < 2u * r.ErosionBrushRadius + 2u || cap.MaxSide > 16384u) {
+		STP_THROW(STPInvalidArgument, "invalid batch capacity");
+	}
+	if (cap.Drops >= 0xFFFFFFFFull) {
+		STP_THROW(STPInvalidArgument, "too many drops in one batch");
 	}
 	this->VCap = std::min(setting.BinCapacity, 2u * radius + 1u);
 	this->Rain.Seed = r.Seed;
@@ -689,54 +737,25 @@ STPHeightfieldBatch::STPHeightfieldBatch(const STPEnvironment::STPSimplexNoiseSe
 	this->BrushOffset = upload(offset);
 	this->BrushWeight = upload(weight);
 
-	// chunk records and arena offsets
-	std::vector<Chunk> rec;
-	unsigned long long pixel = 0ull, halo = 0ull, line = 0ull, drops = 0ull;
-	for (size_t i = 0u; i < chunks.size(); i++) {
-		const STPBatchChunk& in = chunks[i];
-		if (in.Side < 2u * r.ErosionBrushRadius + 2u || in.Side > 16384u) {
-			STP_THROW(STPInvalidArgument, "chunk side must be in [2r + 2, 16384]");
-		}
-		Chunk c;
-		c.Side = in.Side;
-		c.OriginX = in.OriginX;
-		c.OriginY = in.OriginY;
-		const unsigned long long px = static_cast<unsigned long long>(in.Side) * in.Side;
-		c.Drops = static_cast<unsigned int>(static_cast<double>(px) * setting.DropsPerPixel);
-		c.Pixel = pixel;
-		c.Halo = halo;
-		c.Line = line;
-		c.DropBase = static_cast<unsigned int>(drops);
-		c.StreamId = static_cast<unsigned int>(i);
-		rec.push_back(c);
-		// keep every chunk's first pixel 16-byte aligned in the float arena
-		pixel += (px + 3ull) & ~3ull;
-		halo += static_cast<unsigned long long>(in.Side + 2u * radius) * (in.Side + 2u * radius);
-		line += static_cast<unsigned long long>(in.Side + 2u * radius) * in.Side;
-		drops += c.Drops;
-	}
-	if (drops >= 0xFFFFFFFFull) {
-		STP_THROW(STPInvalidArgument, "too many drops in one batch");
-	}
-	this->Record = rec;
-	this->Pixels = pixel;
-	this->Drops = drops;
-	this->ChunkRecord = upload(rec);
-
-	// resident arenas
-	const unsigned int cap = setting.BinCapacity;
+	// resident arenas, sized by the capacity: (side + 2r)^2 <= side^2 + 4r side + 4r^2 per chunk
+	const unsigned long long pixel = cap.Pixels, drops = std::max<unsigned long long>(cap.Drops, 1ull);
+	const unsigned long long r2 = 2ull * radius;
+	const unsigned long long sideSum = std::min<unsigned long long>(pixel, static_cast<unsigned long long>(cap.Chunks) * cap.MaxSide);
+	const unsigned long long halo = pixel + 2ull * r2 * sideSum + r2 * r2 * cap.Chunks;
+	const unsigned long long line = pixel + r2 * sideSum;
+	const unsigned int binCap = setting.BinCapacity;
 	this->Height = STPSmartDeviceMemory::makeDevice<float>(pixel);
 	this->Fixed = STPSmartDeviceMemory::makeDevice<unsigned short>(pixel);
 	this->BiomeMap = STPSmartDeviceMemory::makeDevice<unsigned short>(halo);
 	this->VBin = STPSmartDeviceMemory::makeDevice<unsigned int>(line * this->VCap);
 	this->VCount = STPSmartDeviceMemory::makeDevice<unsigned char>(line);
 	this->BinCount = STPSmartDeviceMemory::makeDevice<unsigned char>(pixel);
-	this->BinItem = STPSmartDeviceMemory::makeDevice<unsigned short>(pixel * cap);
-	this->BinWeight = STPSmartDeviceMemory::makeDevice<float>(pixel * cap);
-	this->DropRank = STPSmartDeviceMemory::makeDevice<unsigned int>(drops ? drops : 1u);
-	this->DropBin = STPSmartDeviceMemory::makeDevice<unsigned int>(drops ? drops : 1u);
-	this->Ledger = STPSmartDeviceMemory::makeDevice<float>(drops ? drops : 1u);
-	this->DropInfo = STPSmartDeviceMemory::makeDevice<uint2>(drops ? drops : 1u);
+	this->BinItem = STPSmartDeviceMemory::makeDevice<unsigned short>(pixel * binCap);
+	this->BinWeight = STPSmartDeviceMemory::makeDevice<float>(pixel * binCap);
+	this->DropRank = STPSmartDeviceMemory::makeDevice<unsigned int>(drops);
+	this->DropBin = STPSmartDeviceMemory::makeDevice<unsigned int>(drops);
+	this->Ledger = STPSmartDeviceMemory::makeDevice<float>(drops);
+	this->DropInfo = STPSmartDeviceMemory::makeDevice<uint2>(drops);
 	this->Flag = STPSmartDeviceMemory::makeDevice<unsigned int>(4u);
 	this->Counter = STPSmartDeviceMemory::makeDevice<unsigned long long>(4u);
 	STP_CHECK_HIP(hipMemset(this->Fixed.get(), 0, pixel * sizeof(unsigned short)));
@@ -757,11 +776,110 @@ STPHeightfieldBatch::STPHeightfieldBatch(const STPEnvironment::STPSimplexNoiseSe
 		}
 	}
 	this->Tuning = this->defaultTuning();
-	this->buildTileTables();
+	this->ErosionHalo = this->Tuning.ErosionTile ? this->Rain.MaxLifetime + std::max(this->Rain.ErosionBrushRadius, 2u) + 1u : 0u;
+	this->ErosionLds = this->erosionLdsBytes(this->Tuning.ErosionTile);
 	for (hipEvent_t& e : this->Event) {
 		STP_CHECK_HIP(hipEventCreate(&e));
 	}
+	STP_CHECK_HIP(hipEventCreateWithFlags(&this->TableCopied, hipEventDisableTiming));
+	STP_CHECK_HIP(hipEventCreateWithFlags(&this->OutCopied, hipEventDisableTiming));
 	STP_CHECK_HIP(hipStreamCreateWithFlags(&this->CaptureStream, hipStreamNonBlocking));
+}
+
+void STPHeightfieldBatch::bind(const std::vector<STPBatchChunk>& chunks, const std::vector<unsigned int>& stream_id,
+	const hipStream_t stream) {
+	if (chunks.empty() || chunks.size() > this->Capacity.Chunks) {
+		STP_THROW(STPInvalidArgument, "the chunk list is empty or exceeds the batch capacity");
+	}
+	if (!stream_id.empty() && stream_id.size() != chunks.size()) {
+		STP_THROW(STPInvalidArgument, "one stream id per chunk");
+	}
+	const unsigned int radius = this->Setting.Radius;
+	std::vector<Chunk> rec;
+	rec.reserve(chunks.size());
+	unsigned long long pixel = 0ull, halo = 0ull, line = 0ull, drops = 0ull;
+	for (size_t i = 0u; i < chunks.size(); i++) {
+		const STPBatchChunk& in = chunks[i];
+		if (in.Side < 2u * this->Rain.ErosionBrushRadius + 2u || in.Side > this->Capacity.MaxSide) {
+			STP_THROW(STPInvalidArgument, "chunk side must be in [2r + 2, the capacity's largest side]");
+		}
+		const unsigned int step = in.Step == 0u ? 1u : in.Step;
+		if ((step & (step - 1u)) != 0u || step > 65536u) {
+			STP_THROW(STPInvalidArgument, "chunk step must be a power of two up to 65536");
+		}
+		Chunk c;
+		c.Side = in.Side;
+		c.OriginX = in.OriginX;
+		c.OriginY = in.OriginY;
+		const unsigned long long px = static_cast<unsigned long long>(in.Side) * in.Side;
+		c.Drops = static_cast<unsigned int>(static_cast<double>(px) * this->Setting.DropsPerPixel);
+		c.Pixel = pixel;
+		c.Halo = halo;
+		c.Line = line;
+		c.DropBase = static_cast<unsigned int>(drops);
+		c.StreamId = stream_id.empty() ? static_cast<unsigned int>(i) : stream_id[i];
+		c.TileBase = 0u;
+		c.Step = step;
+		rec.push_back(c);
+		// keep every chunk's first pixel 16-byte aligned in the float arena
+		pixel += (px + 3ull) & ~3ull;
+		halo += static_cast<unsigned long long>(in.Side + 2u * radius) * (in.Side + 2u * radius);
+		line += static_cast<unsigned long long>(in.Side + 2u * radius) * in.Side;
+		drops += c.Drops;
+	}
+	if (pixel > this->Capacity.Pixels || drops > std::max<unsigned long long>(this->Capacity.Drops, 1ull)) {
+		STP_THROW(STPInvalidArgument, "the chunk list exceeds the batch capacity");
+	}
+	this->releaseGraphs();
+	this->Record = std::move(rec);
+	this->Pixels = pixel;
+	this->Drops = drops;
+	this->buildTileTables(stream);
+}
+
+void STPHeightfieldBatch::copyOut(const std::vector<float*>& height, const std::vector<unsigned short*>& fixed,
+	const hipStream_t stream) {
+	const size_t n = this->Record.size();
+	if (height.size() != n || fixed.size() != n || n > 65535u) {
+		STP_THROW(STPInvalidArgument, "one height and one UINT16 destination per bound chunk (at most 65535)");
+	}
+	for (size_t i = 0u; i < n; i++) {
+		if (!height[i] || !fixed[i]) {
+			STP_THROW(STPInvalidArgument, "null copy destination");
+		}
+	}
+	if (2u * n > this->OutCapacity) {
+		STP_CHECK_HIP(hipStreamSynchronize(stream));
+		this->OutCapacity = 2u * n + n / 2u + 2u;
+		this->OutTable = STPSmartDeviceMemory::makeDevice<void*>(this->OutCapacity);
+		this->OutStaging = STPSmartDeviceMemory::makeHost<void*>(this->OutCapacity);
+	}
+	// the staging buffer may still feed the previous copy
+	STP_CHECK_HIP(hipEventSynchronize(this->OutCopied));
+	for (size_t i = 0u; i < n; i++) {
+		this->OutStaging[i] = height[i];
+		this->OutStaging[n + i] = fixed[i];
+	}
+	STP_CHECK_HIP(hipMemcpyAsync(this->OutTable.get(), this->OutStaging.get(), 2u * n * sizeof(void*),
+		hipMemcpyHostToDevice, stream));
+	STP_CHECK_HIP(hipEventRecord(this->OutCopied, stream));
+	copyOutKERNEL<<<dim3(64u, static_cast<unsigned int>(n)), 256u, 0, stream>>>(this->ChunkRecord,
+		static_cast<unsigned int>(n), this->Height.get(), this->Fixed.get(), this->OutTable.get());
+	STP_CHECK_HIP(hipGetLastError());
+}
+
+float* STPHeightfieldBatch::heightOf(const unsigned int i) const {
+	if (i >= this->Record.size()) {
+		STP_THROW(STPInvalidArgument, "chunk index out of range");
+	}
+	return this->Height.get() + this->Record[i].Pixel;
+}
+
+unsigned short* STPHeightfieldBatch::fixedOf(const unsigned int i) const {
+	if (i >= this->Record.size()) {
+		STP_THROW(STPInvalidArgument, "chunk index out of range");
+	}
+	return this->Fixed.get() + this->Record[i].Pixel;
 }
 
 void STPHeightfieldBatch::releaseGraphs() noexcept {
@@ -814,10 +932,7 @@ unsigned int STPHeightfieldBatch::erosionLdsBytes(const unsigned int tile) const
 	if (tile < 2ull * halo) {
 		return 0u; // the windows of one colour batch would overlap
 	}
-	unsigned long long side = 0ull;
-	for (const Chunk& c : this->Record) {
-		side = std::max(side, std::min<unsigned long long>(c.Side, tile + 2ull * halo));
-	}
+	const unsigned long long side = std::min<unsigned long long>(this->Capacity.MaxSide, tile + 2ull * halo);
 	int device = 0, limit = 0;
 	STP_CHECK_HIP(hipGetDevice(&device));
 	STP_CHECK_HIP(hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -847,10 +962,15 @@ void STPHeightfieldBatch::setTuning(const STPBatchTuning& tuning) {
 	}
 	this->releaseGraphs();
 	this->Tuning = tuning;
-	this->buildTileTables();
+	this->ErosionHalo = tuning.ErosionTile ? this->Rain.MaxLifetime + std::max(this->Rain.ErosionBrushRadius, 2u) + 1u : 0u;
+	this->ErosionLds = this->erosionLdsBytes(tuning.ErosionTile);
+	if (!this->Record.empty()) {
+		this->buildTileTables(nullptr);
+		STP_CHECK_HIP(hipStreamSynchronize(nullptr));
+	}
 }
 
-void STPHeightfieldBatch::buildTileTables() {
+void STPHeightfieldBatch::buildTileTables(const hipStream_t stream) {
 	const unsigned int radius = this->Setting.Radius, tile = this->Tuning.ErosionTile;
 	unsigned int tiles = 0u;
 	for (Chunk& c : this->Record) {
@@ -861,11 +981,6 @@ void STPHeightfieldBatch::buildTileTables() {
 		}
 	}
 	this->ErosionTiles = tiles;
-	this->ErosionHalo = tile ? this->Rain.MaxLifetime + std::max(this->Rain.ErosionBrushRadius, 2u) + 1u : 0u;
-	this->ErosionLds = this->erosionLdsBytes(tile);
-	this->ChunkRecord = upload(this->Record);
-	this->TileDrops = STPSmartDeviceMemory::makeDevice<unsigned int>(tiles + 1u);
-	this->TileFirst = STPSmartDeviceMemory::makeDevice<unsigned int>(tiles + 1u);
 	if (tile != 0u) {
 		switch (this->Tuning.ErosionThreads) {
 		case 256u:
@@ -911,10 +1026,43 @@ void STPHeightfieldBatch::buildTileTables() {
 			range.End[s] = static_cast<unsigned int>(table[s].size());
 		}
 	}
+
+	// one arena: [chunk records][table 0] ... [table 8], 256-byte aligned parts
+	const auto align = [](const size_t b) { return (b + 255u) & ~static_cast<size_t>(255u); };
+	size_t offset[TableCount + 1];
+	size_t bytes = align(this->Record.size() * sizeof(Chunk));
 	for (int s = 0; s < TableCount; s++) {
-		this->TileCount[s] = static_cast<unsigned int>(table[s].size());
-		this->Tile[s] = upload(table[s]);
+		offset[s] = bytes;
+		bytes += align(std::max<size_t>(table[s].size(), 1u) * sizeof(uint2));
 	}
+	if (bytes > this->TableBytes || tiles + 1u > this->TileCapacity) {
+		// growing: the previous tables may still be read by work on `stream`
+		STP_CHECK_HIP(hipStreamSynchronize(stream));
+		if (bytes > this->TableBytes) {
+			this->TableBytes = bytes + bytes / 4u;
+			this->TableArena = STPSmartDeviceMemory::makeDevice<unsigned char>(this->TableBytes);
+			this->TableStaging = STPSmartDeviceMemory::makeHost<unsigned char>(this->TableBytes);
+		}
+		if (tiles + 1u > this->TileCapacity) {
+			this->TileCapacity = tiles + 1u + (tiles + 1u) / 4u;
+			this->TileDrops = STPSmartDeviceMemory::makeDevice<unsigned int>(this->TileCapacity);
+			this->TileFirst = STPSmartDeviceMemory::makeDevice<unsigned int>(this->TileCapacity);
+		}
+	}
+	// the staging buffer may still feed the previous binding's copy
+	STP_CHECK_HIP(hipEventSynchronize(this->TableCopied));
+	unsigned char* const host = this->TableStaging.get();
+	std::memcpy(host, this->Record.data(), this->Record.size() * sizeof(Chunk));
+	this->ChunkRecord = reinterpret_cast<Chunk*>(this->TableArena.get());
+	for (int s = 0; s < TableCount; s++) {
+		if (!table[s].empty()) {
+			std::memcpy(host + offset[s], table[s].data(), table[s].size() * sizeof(uint2));
+		}
+		this->TileCount[s] = static_cast<unsigned int>(table[s].size());
+		this->Tile[s] = reinterpret_cast<uint2*>(this->TableArena.get() + offset[s]);
+	}
+	STP_CHECK_HIP(hipMemcpyAsync(this->TableArena.get(), host, bytes, hipMemcpyHostToDevice, stream));
+	STP_CHECK_HIP(hipEventRecord(this->TableCopied, stream));
 }
 
 STPBatchTuning STPHeightfieldBatch::autotune(const hipStream_t stream, const unsigned int repeat) {
@@ -965,6 +1113,12 @@ STPHeightfieldBatch::~STPHeightfieldBatch() {
 	for (hipEvent_t e : this->Event) {
 		(void)hipEventDestroy(e);
 	}
+	if (this->TableCopied) {
+		(void)hipEventDestroy(this->TableCopied);
+	}
+	if (this->OutCopied) {
+		(void)hipEventDestroy(this->OutCopied);
+	}
 	if (this->CaptureStream) {
 		(void)hipStreamDestroy(this->CaptureStream);
 	}
@@ -975,7 +1129,7 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 	const unsigned int radius = this->Setting.Radius, cap = this->Setting.BinCapacity;
 	const auto range = [&](const int s) {
 		const unsigned int b = this->Range[first].Begin[s], e = this->Range[last - 1u].End[s];
-		return std::make_pair(this->Tile[s].get() + b, e - b);
+		return std::make_pair(this->Tile[s] + b, e - b);
 	};
 	switch (stage) {
 	case 0: {
@@ -984,7 +1138,7 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 		}
 		const auto [t, n] = range(0);
 		if (n) {
-			biomeKERNEL<<<n, BiomeTile, 0, stream>>>(t, this->ChunkRecord.get(), this->Setting.BiomeSeed,
+			biomeKERNEL<<<n, BiomeTile, 0, stream>>>(t, this->ChunkRecord, this->Setting.BiomeSeed,
 				this->Setting.BiomeCount, this->Setting.BiomeCell, radius, this->BiomeMap.get());
 		}
 		break;
@@ -996,7 +1150,7 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 		if (n) {
 			const auto launch = [&](auto kernel) {
 				kernel<<<n, ShfLineBlock, verticalLdsWords(this->VCap) * sizeof(unsigned int), stream>>>(t,
-					this->ChunkRecord.get(), radius, this->VCap, this->BiomeMap.get(), this->VBin.get(),
+					this->ChunkRecord, radius, this->VCap, this->BiomeMap.get(), this->VBin.get(),
 					this->VCount.get(), this->Flag.get());
 			};
 			switch (registerCapacity(this->VCap)) {
@@ -1023,7 +1177,7 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 		if (n) {
 			const auto launch = [&](auto kernel) {
 				kernel<<<n, ShfLineBlock, horizontalLdsWords(this->VCap, cap) * sizeof(unsigned int), stream>>>(t,
-					this->ChunkRecord.get(), radius, this->VCap, cap, this->VBin.get(), this->VCount.get(),
+					this->ChunkRecord, radius, this->VCap, cap, this->VBin.get(), this->VCount.get(),
 					this->BinCount.get(), this->BinItem.get(), this->BinWeight.get(), this->Flag.get());
 			};
 			switch (horizontalRegisterCapacity(cap)) {
@@ -1043,7 +1197,7 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 	case 3: {
 		const auto [t, n] = range(3);
 		if (n) {
-			blendKERNEL<<<n, dim3(BlendX, BlendY), 0, stream>>>(t, this->ChunkRecord.get(), this->Permutation.Permutation,
+			blendKERNEL<<<n, dim3(BlendX, BlendY), 0, stream>>>(t, this->ChunkRecord, this->Permutation.Permutation,
 				this->Biome, this->BiomeCount, cap, this->BinCount.get(), this->BinItem.get(), this->BinWeight.get(),
 				this->Height.get());
 		}
@@ -1064,11 +1218,11 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 		if (te > tb) {
 			zeroRangeKERNEL<<<(te - tb + 255u) / 256u, 256u, 0, stream>>>(this->TileDrops.get() + tb, te - tb);
 		}
-		binCountKERNEL<<<n, ErosionBlock, 0, stream>>>(t, this->ChunkRecord.get(), args, tile, this->TileDrops.get(),
+		binCountKERNEL<<<n, ErosionBlock, 0, stream>>>(t, this->ChunkRecord, args, tile, this->TileDrops.get(),
 			this->DropRank.get());
-		binScanKERNEL<<<last - first, ScanBlock, 0, stream>>>(this->ChunkRecord.get(), first, tile,
+		binScanKERNEL<<<last - first, ScanBlock, 0, stream>>>(this->ChunkRecord, first, tile,
 			this->TileDrops.get(), this->TileFirst.get());
-		binScatterKERNEL<<<n, ErosionBlock, 0, stream>>>(t, this->ChunkRecord.get(), args, tile, this->TileFirst.get(),
+		binScatterKERNEL<<<n, ErosionBlock, 0, stream>>>(t, this->ChunkRecord, args, tile, this->TileFirst.get(),
 			this->DropRank.get(), this->DropBin.get());
 		break;
 	}
@@ -1080,7 +1234,7 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 		const ErosionArgs args = this->erosionArgs();
 		const unsigned int tile = this->Tuning.ErosionTile;
 		if (tile == 0u) {
-			erosionKERNEL<<<n, ErosionBlock, this->BrushSize * sizeof(int2), stream>>>(t, this->ChunkRecord.get(), args,
+			erosionKERNEL<<<n, ErosionBlock, this->BrushSize * sizeof(int2), stream>>>(t, this->ChunkRecord, args,
 				this->BrushOffset.get(), this->BrushWeight.get(), this->Height.get(), this->Ledger.get(),
 				this->DropInfo.get());
 			break;
@@ -1093,7 +1247,7 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 				continue;
 			}
 			const auto launch = [&](auto kernel, const unsigned int threads) {
-				kernel<<<nt, threads, this->ErosionLds, stream>>>(tt, this->ChunkRecord.get(), args, tile,
+				kernel<<<nt, threads, this->ErosionLds, stream>>>(tt, this->ChunkRecord, args, tile,
 					this->ErosionHalo, this->BrushOffset.get(), this->BrushWeight.get(), this->TileDrops.get(),
 					this->TileFirst.get(), this->DropBin.get(), this->Height.get(), this->Ledger.get(),
 					this->DropInfo.get());

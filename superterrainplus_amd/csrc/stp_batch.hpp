@@ -24,6 +24,15 @@ namespace SuperTerrainPlus {
 	struct STPBatchChunk {
 		unsigned int Side; // square chunk, pixels per side
 		int OriginX, OriginY;
+		unsigned int Step = 1u; // world units per pixel (level of detail); 0 is read as 1
+	};
+
+	// Arena sizes of a batch that is bound to different chunk lists over its lifetime.
+	struct STPBatchCapacity {
+		unsigned long long Pixels; // sum of side^2 (each chunk rounded up to 4 pixels)
+		unsigned long long Drops;
+		unsigned int Chunks;
+		unsigned int MaxSide;
 	};
 
 	// Launch shapes of the erosion stage (chosen by STPHeightfieldBatch::autotune or set by hand).
@@ -48,6 +57,7 @@ namespace SuperTerrainPlus {
 			unsigned int DropBase;     // first drop in the ledger / drop-info / drop-bin arenas
 			unsigned int StreamId;     // Philox counter word of this chunk's drops
 			unsigned int TileBase;     // first erosion tile of the chunk in the tile-bin arenas
+			unsigned int Step;         // world units per pixel
 		};
 	}
 
@@ -59,9 +69,27 @@ namespace SuperTerrainPlus {
 		// biome, vertical filter, horizontal filter, blend, erosion binning, erosion, format
 		static constexpr int StageCount = 7;
 
+		// A batch sized for and bound to `chunks` (chunk i's drops use stream id i).
 		STPHeightfieldBatch(const STPEnvironment::STPSimplexNoiseSetting& noise, const std::vector<STPBiomeProperty>& biome,
 			const STPEnvironment::STPRainDropSetting& rain, const STPBatchSetting& setting,
 			const std::vector<STPBatchChunk>& chunks);
+		// An unbound batch with arenas of the given capacity; bind() a chunk list before running.
+		STPHeightfieldBatch(const STPEnvironment::STPSimplexNoiseSetting& noise, const std::vector<STPBiomeProperty>& biome,
+			const STPEnvironment::STPRainDropSetting& rain, const STPBatchSetting& setting,
+			const STPBatchCapacity& capacity);
+
+		// Binds a chunk list within the capacity (stream_id: the Philox stream word of each chunk's
+		// drops; empty = chunk index). The chunk and tile tables are copied asynchronously on
+		// `stream`; the batch must not be running.
+		void bind(const std::vector<STPBatchChunk>& chunks, const std::vector<unsigned int>& stream_id, hipStream_t stream);
+		static STPBatchCapacity capacityOf(const std::vector<STPBatchChunk>& chunks, float drops_per_pixel);
+		const STPBatchCapacity& capacity() const noexcept { return this->Capacity; }
+		// Device addresses of chunk i's outputs (valid until the next bind).
+		float* heightOf(unsigned int i) const;
+		unsigned short* fixedOf(unsigned int i) const;
+		// Enqueues copies of every bound chunk's outputs to height[i] / fixed[i] (device memory, one
+		// pointer per chunk) on `stream`, in one launch.
+		void copyOut(const std::vector<float*>& height, const std::vector<unsigned short*>& fixed, hipStream_t stream);
 		STPHeightfieldBatch(const STPHeightfieldBatch&) = delete;
 		STPHeightfieldBatch& operator=(const STPHeightfieldBatch&) = delete;
 		~STPHeightfieldBatch();
@@ -104,6 +132,7 @@ namespace SuperTerrainPlus {
 		};
 
 		STPBatchSetting Setting;
+		STPBatchCapacity Capacity {};
 		STPAlgorithm::STPPermutationGenerator::STPPermutationTable Permutation;
 		STPStage::BiomeTable Biome;
 		unsigned int BiomeCount;
@@ -116,8 +145,17 @@ namespace SuperTerrainPlus {
 		unsigned int ErosionTiles = 0u, ErosionHalo = 0u, ErosionLds = 0u;
 		unsigned long long Pixels = 0ull, Drops = 0ull;
 
-		STPSmartDeviceMemory::STPDevice<STPBatch::Chunk> ChunkRecord;
-		STPSmartDeviceMemory::STPDevice<uint2> Tile[TableCount];
+		// chunk records and tile tables, packed in one device arena (filled from pinned staging)
+		STPSmartDeviceMemory::STPDevice<unsigned char> TableArena;
+		STPSmartDeviceMemory::STPHost<unsigned char> TableStaging;
+		size_t TableBytes = 0u;
+		STPBatch::Chunk* ChunkRecord = nullptr;
+		uint2* Tile[TableCount] {};
+		unsigned int TileCapacity = 0u;
+		// destination pointers of copyOut (device) and their pinned staging
+		STPSmartDeviceMemory::STPDevice<void*> OutTable;
+		STPSmartDeviceMemory::STPHost<void*> OutStaging;
+		size_t OutCapacity = 0u;
 		STPSmartDeviceMemory::STPDevice<int2> BrushOffset;
 		STPSmartDeviceMemory::STPDevice<float> BrushWeight;
 		STPSmartDeviceMemory::STPDevice<float> Height;
@@ -140,13 +178,15 @@ namespace SuperTerrainPlus {
 		hipGraphExec_t StageExec[StageCount] {};
 		bool Timing = false, LastTimed = false;
 		hipEvent_t Event[StageCount + 1] {};
+		hipEvent_t TableCopied = nullptr, OutCopied = nullptr;
 		std::vector<double> Latency;
 
 		void enqueueStage(hipStream_t stream, int stage, unsigned int first, unsigned int last, bool clear_flag);
 		void enqueue(hipStream_t stream, unsigned int first, unsigned int last, bool timed, bool clear_flag);
 		hipGraphExec_t capture(int stage_begin, int stage_end, hipGraph_t& graph);
 		void releaseGraphs() noexcept;
-		void buildTileTables();
+		void init(const STPEnvironment::STPRainDropSetting& rain);
+		void buildTileTables(hipStream_t stream);
 		// Window side bound and LDS bytes of the tiled erosion for a tile side (0 if impossible).
 		unsigned int erosionLdsBytes(unsigned int tile) const;
 		STPStage::ErosionArgs erosionArgs() const;

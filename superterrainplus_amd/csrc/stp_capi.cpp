@@ -1,12 +1,14 @@
 // extern "C" boundary (include/stp_amd.h): converts exceptions into status codes plus a
 // thread-local message, and plain C structs into the C++ API types.
 #include "stp_batch.hpp"
+#include "stp_server.hpp"
 #include "stp_internal.hpp"
 
 #include <stp/STPDeviceRuntime.hpp>
 
 #include <stp_amd.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -157,6 +159,11 @@ struct stp_rtc_program {
 
 struct stp_rtc_module {
 	STPDeviceRuntimeProgram::STPSmartModule Module;
+};
+
+struct stp_heightfield_server {
+	std::unique_ptr<STPHeightfieldServer> Server;
+	std::vector<std::pair<unsigned long long, double>> Done; // completed, not yet returned by poll
 };
 
 struct stp_heightfield_batch {
@@ -737,7 +744,7 @@ int stp_batch_create(const stp_simplex_noise_setting* const noise, const stp_bio
 		bs.DropsPerPixel = setting->DropsPerPixel;
 		std::vector<STPBatchChunk> c(chunk_count);
 		for (uint32_t i = 0u; i < chunk_count; i++) {
-			c[i] = STPBatchChunk {h_chunks[i].Side, h_chunks[i].Origin[0], h_chunks[i].Origin[1]};
+			c[i] = STPBatchChunk {h_chunks[i].Side, h_chunks[i].Origin[0], h_chunks[i].Origin[1], h_chunks[i].Step};
 		}
 		auto b = std::make_unique<stp_heightfield_batch>();
 		b->Batch = std::make_unique<STPHeightfieldBatch>(toNoise(noise), t, toRain(raindrop), bs, c);
@@ -828,6 +835,125 @@ int stp_batch_read_chunk(stp_heightfield_batch* const batch, const uint32_t inde
 	return guarded([&] {
 		require(batch, "null batch");
 		batch->Batch->readChunk(index, h_height, h_fixed, h_biome_halo, h_ledger);
+	});
+}
+
+int stp_server_create(const stp_simplex_noise_setting* const noise, const stp_biome_property* const h_table,
+	const uint32_t biome_count, const stp_raindrop_setting* const raindrop, const stp_batch_setting* const batch,
+	const stp_server_setting* const setting, stp_heightfield_server** const out) {
+	return guarded([&] {
+		require(h_table && batch && setting && out, "null argument");
+		std::vector<STPBiomeProperty> t;
+		for (uint32_t i = 0u; i < biome_count; i++) {
+			t.push_back(toBiome(h_table[i]));
+		}
+		STPBatchSetting bs;
+		bs.BiomeSeed = batch->BiomeSeed;
+		bs.BiomeCount = batch->BiomeCount;
+		bs.BiomeCell = batch->BiomeCell;
+		bs.Radius = batch->Radius;
+		bs.BinCapacity = batch->BinCapacity;
+		bs.DropsPerPixel = batch->DropsPerPixel;
+		const STPServerSetting ss {setting->MemoryBudget, setting->BatchPixels, setting->BatchChunks, setting->MaxSide,
+			setting->Engines, setting->ChunkWorldSize};
+		auto s = std::make_unique<stp_heightfield_server>();
+		s->Server = std::make_unique<STPHeightfieldServer>(toNoise(noise), t, toRain(raindrop), bs, ss);
+		*out = s.release();
+	});
+}
+
+int stp_server_destroy(stp_heightfield_server* const server) {
+	return guarded([&] { delete server; });
+}
+
+int stp_server_submit(stp_heightfield_server* const server, const stp_chunk_request* const request,
+	uint64_t* const ticket) {
+	return guarded([&] {
+		require(server && request, "null argument");
+		const unsigned long long t = server->Server->submit(STPChunkRequest {request->X, request->Y, request->Side});
+		if (ticket) {
+			*ticket = t;
+		}
+	});
+}
+
+int stp_server_poll(stp_heightfield_server* const server, uint64_t* const h_ticket, double* const h_latency_ms,
+	const uint32_t capacity, uint32_t* const count) {
+	return guarded([&] {
+		require(server && count && (capacity == 0u || (h_ticket && h_latency_ms)), "null argument");
+		server->Server->poll();
+		for (const auto& d : server->Server->drain()) {
+			server->Done.push_back(d);
+		}
+		const uint32_t n = static_cast<uint32_t>(std::min<size_t>(capacity, server->Done.size()));
+		for (uint32_t i = 0u; i < n; i++) {
+			h_ticket[i] = server->Done[i].first;
+			h_latency_ms[i] = server->Done[i].second;
+		}
+		server->Done.erase(server->Done.begin(), server->Done.begin() + n);
+		*count = n;
+	});
+}
+
+int stp_server_flush(stp_heightfield_server* const server) {
+	return guarded([&] {
+		require(server, "null server");
+		server->Server->flush();
+		for (const auto& d : server->Server->drain()) {
+			server->Done.push_back(d);
+		}
+	});
+}
+
+int stp_server_read(stp_heightfield_server* const server, const stp_chunk_request* const request,
+	float* const h_height, uint16_t* const h_fixed) {
+	return guarded([&] {
+		require(server && request, "null argument");
+		require(server->Server->read(STPChunkRequest {request->X, request->Y, request->Side}, h_height, h_fixed),
+			"the chunk is not resident");
+	});
+}
+
+int stp_server_memory(stp_heightfield_server* const server, uint64_t* const engine_bytes,
+	uint64_t* const resident_bytes) {
+	return guarded([&] {
+		require(server, "null server");
+		if (engine_bytes) {
+			*engine_bytes = server->Server->engineBytes();
+		}
+		if (resident_bytes) {
+			*resident_bytes = server->Server->residentBytes();
+		}
+	});
+}
+
+int stp_server_replay(stp_heightfield_server* const server, const stp_chunk_request* const h_requests,
+	const double* const h_arrival_s, const uint32_t count, const double deadline_ms, double* const h_latency_ms,
+	stp_server_stats* const out) {
+	return guarded([&] {
+		require(server && (count == 0u || (h_requests && h_arrival_s && h_latency_ms)), "null argument");
+		std::vector<STPChunkRequest> req(count);
+		for (uint32_t i = 0u; i < count; i++) {
+			req[i] = STPChunkRequest {h_requests[i].X, h_requests[i].Y, h_requests[i].Side};
+		}
+		std::vector<double> arrival(h_arrival_s, h_arrival_s + count), latency;
+		const STPServerStats st = server->Server->replay(req, arrival, deadline_ms, latency);
+		std::copy(latency.begin(), latency.end(), h_latency_ms);
+		if (out) {
+			out->P50 = st.P50;
+			out->P99 = st.P99;
+			out->P999 = st.P999;
+			out->Mean = st.Mean;
+			out->Max = st.Max;
+			out->MpixelPerSecond = st.MpixelPerSecond;
+			out->DeadlineMissFraction = st.DeadlineMissFraction;
+			out->Requests = st.Requests;
+			out->CacheHits = st.CacheHits;
+			out->Batches = st.Batches;
+			out->Evictions = st.Evictions;
+			out->MeanBatchChunks = st.MeanBatchChunks;
+			out->PeakResidentBytes = st.PeakResidentBytes;
+		}
 	});
 }
 

@@ -179,3 +179,65 @@ class Batched:
 
 def batched():
     return Batched()
+
+
+@dataclass
+class Served:
+    """Camera-driven request stream: a random walk over the chunk grid, 0.5 chunk per 16 ms
+    frame (seed 3); every frame requests the chunks within a view radius of 8 chunks whose level
+    of detail changed or that were never requested; the level of detail follows the distance
+    from the camera (2048 px per side next to it down to 64 px at the edge of the view). The
+    requests keep the walk's order and arrive as a Poisson process at `rate` per second. The
+    chunk grid's world size is 2048, so a chunk of side s samples every 2048 / s world units."""
+    seed: int = 3
+    step: float = 0.5
+    frame_ms: float = 16.0
+    view_radius: float = 8.0
+    rates: tuple = (200, 1000, 4000)
+    duration_s: float = 2.0
+    deadline_ms: float = 16.0
+    world: int = 2048
+    memory_budget: int = 200 << 30
+    batch: Batched = field(default_factory=Batched)
+
+    @staticmethod
+    def side_at(distance):
+        for limit, side in ((1.5, 2048), (2.5, 1024), (3.5, 512), (5.0, 256), (6.5, 128)):
+            if distance < limit:
+                return side
+        return 64
+
+    def walk(self, count):
+        """The first `count` requests of the walk: list of (x, y, side)."""
+        rng = np.random.default_rng(self.seed)
+        cam = np.zeros(2)
+        have = {}
+        out = []
+        r = int(np.ceil(self.view_radius))
+        while len(out) < count:
+            cx, cy = int(np.floor(cam[0])), int(np.floor(cam[1]))
+            view = []
+            for y in range(cy - r - 1, cy + r + 2):
+                for x in range(cx - r - 1, cx + r + 2):
+                    d = float(np.hypot(x + 0.5 - cam[0], y + 0.5 - cam[1]))
+                    if d <= self.view_radius:
+                        view.append((d, x, y))
+            for d, x, y in sorted(view):
+                side = self.side_at(d)
+                if have.get((x, y)) != side:
+                    have[(x, y)] = side
+                    out.append((x, y, side))
+            a = rng.uniform(0.0, 2.0 * np.pi)
+            cam = cam + self.step * np.array([np.cos(a), np.sin(a)])
+        return out[:count]
+
+    def trace(self, rate, count=None):
+        """(requests, arrival seconds) at `rate` requests per second."""
+        n = int(count if count is not None else rate * self.duration_s)
+        req = self.walk(n)
+        gaps = np.random.default_rng(self.seed + int(rate)).exponential(1.0 / rate, n)
+        return req, np.cumsum(gaps) - gaps[0]
+
+
+def served():
+    return Served()

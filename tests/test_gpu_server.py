@@ -1,0 +1,197 @@
+"""Serving loop: results of a chunk are the same alone and in a mix of other requests (bit-exact
+without erosion, and equal to the oracle at every level of detail; within the parallel-erosion
+tolerance with erosion), cache hits, requests for chunks in production, least-recently-used
+eviction under a small memory budget, and a real-time replay of the camera-walk trace."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from superterrainplus_amd import capi, workloads as W
+
+from helpers import log, oracle_rain
+
+pytestmark = pytest.mark.gpu
+
+NOISE = capi.SimplexNoiseSetting(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+RADIUS, COUNT, CELL, SEED, WORLD = 8, 4, 96, 7, 1024
+
+
+def props():
+    return [(p[0], 8, p[2], p[3], p[4], p[5]) for p in W.biome_table(COUNT, SEED)]
+
+
+class Server:
+    def __init__(self, dpp, budget=8 << 30, batch_pixels=1 << 21, engines=2, max_side=1024):
+        batch = capi.BatchSetting(SEED, COUNT, CELL, RADIUS, 8, dpp)
+        setting = capi.ServerSetting(budget, batch_pixels, 64, max_side, engines, WORLD)
+        h = C.c_void_p()
+        capi.call("stp_server_create", C.byref(NOISE), capi.biome_array(props()), COUNT,
+                  C.byref(W.erosion_setting(1)), C.byref(batch), C.byref(setting), C.byref(h))
+        self.h = h.value
+
+    def submit(self, x, y, side):
+        t = C.c_uint64()
+        capi.call("stp_server_submit", C.c_void_p(self.h), C.byref(capi.ChunkRequest(x, y, side)), C.byref(t))
+        return t.value
+
+    def flush(self):
+        capi.call("stp_server_flush", C.c_void_p(self.h))
+
+    def poll(self, capacity=4096):
+        tickets, lat, n = (C.c_uint64 * capacity)(), (C.c_double * capacity)(), C.c_uint32()
+        capi.call("stp_server_poll", C.c_void_p(self.h), tickets, lat, capacity, C.byref(n))
+        return {tickets[i]: lat[i] for i in range(n.value)}
+
+    def read(self, x, y, side):
+        h = np.zeros((side, side), np.float32)
+        f = np.zeros((side, side), np.uint16)
+        capi.call("stp_server_read", C.c_void_p(self.h), C.byref(capi.ChunkRequest(x, y, side)), capi.ptr(h),
+                  capi.ptr(f))
+        return h, f
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            capi.lib().stp_server_destroy(C.c_void_p(self.h))
+
+
+MIX = [(0, 0, 256), (1, 0, 128), (-1, 2, 512), (3, -3, 64), (0, 1, 1024), (5, 5, 256), (-2, -1, 128)]
+
+
+def oracle_chunk(oracle, x, y, side):
+    """Pre-erosion heights of grid chunk (x, y) at `side` pixels: world origin (x, y) * WORLD,
+    WORLD / side world units per pixel."""
+    step, r = WORLD // side, RADIUS
+    o = (x * WORLD, y * WORLD)
+    halo = oracle.voronoi_map(SEED, COUNT, CELL, (o[0] - r * step, o[1] - r * step), side + 2 * r, side + 2 * r,
+                              step=step)
+    item, weight, off = oracle.shf_window(halo, (r, r), (side, side), r)
+    perm, g32, _ = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+    half = side * step / 2.0
+    return oracle.multi_biome(perm, g32, props(), item, weight, off, (side, side), (o[0] + half, o[1] + half),
+                              step=step)
+
+
+def test_server_alone_and_in_mix_bit_exact(gpu, oracle):
+    """Without erosion every chunk is bit-exact: alone, in a mix, and against the oracle at its
+    level of detail."""
+    log("server alone vs mix")
+    alone = {}
+    for x, y, side in MIX:
+        s = Server(0.0)
+        t = s.submit(x, y, side)
+        s.flush()
+        assert t in s.poll()
+        alone[(x, y, side)] = s.read(x, y, side)
+        del s
+    s = Server(0.0, batch_pixels=1 << 20)  # several batches, two engines in flight
+    tickets = [s.submit(*m) for m in MIX]
+    s.flush()
+    assert set(tickets) <= set(s.poll())
+    for x, y, side in MIX:
+        h, f = s.read(x, y, side)
+        assert h.tobytes() == alone[(x, y, side)][0].tobytes(), (x, y, side)
+        assert (f == alone[(x, y, side)][1]).all()
+        assert h.tobytes() == oracle_chunk(oracle, x, y, side).tobytes(), (x, y, side)
+        assert (f == oracle.format_heightmap(h)).all()
+
+
+def test_server_erosion_alone_and_in_mix(gpu, oracle):
+    """With erosion a chunk's drops are fixed by its coordinate and level of detail, so alone and
+    in a mix it erodes with the same drops; the result agrees within the parallel-erosion yardstick
+    (RMS <= 1.5 x the RMS between two serialized runs with different drops)."""
+    log("server erosion alone vs mix")
+    dpp = 0.25
+    setting = W.erosion_setting(1)
+    for x, y, side in [(0, 0, 256), (-2, -1, 128)]:
+        s = Server(dpp)
+        s.submit(x, y, side)
+        s.flush()
+        alone = s.read(x, y, side)[0]
+        del s
+        s = Server(dpp, batch_pixels=1 << 20)
+        for m in MIX:
+            s.submit(*m)
+        s.flush()
+        mix = s.read(x, y, side)[0]
+        del s
+        before = oracle_chunk(oracle, x, y, side)
+        drops = int(side * side * dpp)
+        sa, sb = before.copy(), before.copy()
+        oracle.erode(sa, (1, 1), (side, side), oracle_rain(oracle, setting), drops, setting.Seed, stream_id=1)
+        oracle.erode(sb, (1, 1), (side, side), oracle_rain(oracle, setting), drops, setting.Seed, stream_id=2)
+        floor = np.sqrt(np.mean((sa.astype(np.float64) - sb) ** 2))
+        assert np.isfinite(mix).all() and np.abs(mix - before).max() > 0.0
+        assert np.sqrt(np.mean((alone.astype(np.float64) - mix) ** 2)) <= 1.5 * floor, (x, y, side)
+
+
+def test_server_cache_and_in_flight(gpu):
+    log("server cache")
+    s = Server(0.0)
+    a = s.submit(0, 0, 256)
+    b = s.submit(0, 0, 256)  # the same chunk while it is queued / in production
+    s.flush()
+    done = s.poll()
+    assert a in done and b in done
+    c = s.submit(0, 0, 256)  # resident: a cache hit
+    s.flush()
+    assert c in s.poll()
+    assert np.isfinite(s.read(0, 0, 256)[0]).all()
+
+
+def test_server_lru_eviction(gpu):
+    """A budget of the engines plus three 1024^2 chunks (6 MiB each): the least recently used
+    chunks are evicted, an evicted chunk is recomputed identically on request."""
+    log("server eviction")
+    s = Server(0.0)
+    engine, resident = C.c_uint64(), C.c_uint64()
+    capi.call("stp_server_memory", C.c_void_p(s.h), C.byref(engine), C.byref(resident))
+    assert engine.value > 0 and resident.value == 0
+    del s
+    s = Server(0.0, budget=engine.value + 3 * (6 << 20) + (4 << 20))
+    first = None
+    for k in range(6):
+        s.submit(k, 0, 1024)
+        s.flush()
+        if k == 0:
+            first = s.read(0, 0, 1024)[0]
+    s.poll()
+    capi.call("stp_server_memory", C.c_void_p(s.h), C.byref(engine), C.byref(resident))
+    assert resident.value <= 4 * (6 << 20)
+    with pytest.raises(capi.STPError):
+        s.read(0, 0, 1024)  # evicted
+    assert np.isfinite(s.read(5, 0, 1024)[0]).all()
+    s.submit(0, 0, 1024)
+    s.flush()
+    assert s.read(0, 0, 1024)[0].tobytes() == first.tobytes()
+
+
+def test_server_replay_camera_walk(gpu):
+    """A short real-time replay of the camera-walk trace: every request completes, batches hold
+    several chunks, repeated levels of detail are cache hits."""
+    log("server replay")
+    sv = W.served()
+    req, arr = sv.trace(1000, count=300)
+    s = Server(0.25, batch_pixels=1 << 23, max_side=1024)
+    req = [(x, y, min(side, 1024)) for x, y, side in req]
+    r = (capi.ChunkRequest * len(req))(*[capi.ChunkRequest(x, y, side) for x, y, side in req])
+    a = (C.c_double * len(req))(*arr)
+    lat = (C.c_double * len(req))()
+    stats = capi.ServerStats()
+    capi.call("stp_server_replay", C.c_void_p(s.h), r, a, len(req), sv.deadline_ms, lat, C.byref(stats))
+    lat = np.array(lat)
+    assert stats.Requests == len(req) and (lat >= 0).all()
+    assert stats.Batches >= 1 and stats.MeanBatchChunks >= 1.0
+    assert stats.P50 <= stats.P99 <= stats.P999 <= stats.Max + 1e-9
+    assert stats.MpixelPerSecond > 0
+    log(f"server replay p50 {stats.P50:.3f} p99 {stats.P99:.3f} ms, {stats.MpixelPerSecond:.1f} Mpx/s, "
+        f"{stats.Batches} batches, {stats.MeanBatchChunks:.1f} chunks/batch")
+
+
+def test_server_rejects_bad_requests(gpu):
+    s = Server(0.0)
+    for bad in [(0, 0, 100), (0, 0, 2048), (0, 0, 8), (1 << 22, 0, 256)]:
+        with pytest.raises(capi.STPError):
+            s.submit(*bad)
+    with pytest.raises(capi.STPError):
+        s.read(9, 9, 256)
