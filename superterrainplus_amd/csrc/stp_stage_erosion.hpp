@@ -125,9 +125,10 @@ namespace SuperTerrainPlus::STPStage {
 	}
 
 	// The height surface a drop reads and updates. `load` gives the FP32 height; `Raw` is the
-	// stored form, `raw` reads it, `take` removes min(height, amount) from a cell with one atomic
-	// and returns the amount removed (the reference's erode update, STPRainDrop.cu:150-155);
-	// `add` adds (deposit).
+	// stored form, `raw` reads it. `take` removes min(height, amount) from a cell with one atomic
+	// (the reference's erode update, STPRainDrop.cu:150-155); the brush weights it receives amounts
+	// with are pre-multiplied by WeightScale, and the sum of what it returns converts to height units
+	// with `height`. `add` adds (deposit).
 	//
 	// GlobalSurface: the whole map in HBM; reads bypass L1, updates are L2 FP32 atomics.
 	struct GlobalSurface {
@@ -138,11 +139,13 @@ namespace SuperTerrainPlus::STPStage {
 		__device__ unsigned int stride() const { return this->W; }
 		__device__ float load(const unsigned int i) const { return loadHeight(this->Map + i); }
 		__device__ Raw raw(const unsigned int i) const { return loadHeight(this->Map + i); }
+		static constexpr float WeightScale = 1.0f;
 		__device__ float take(const unsigned int i, const Raw h, const float amount) const {
 			const float ds = (h < amount) ? h : amount;
 			unsafeAtomicAdd(this->Map + i, -ds);
 			return ds;
 		}
+		__device__ static float height(const float taken) { return taken; }
 		__device__ void add(const unsigned int i, const float v) const { unsafeAtomicAdd(this->Map + i, v); }
 		// the chunk-level guards already keep every access inside the map
 		__device__ bool holds(unsigned int, unsigned int, unsigned int, unsigned int) const { return true; }
@@ -172,12 +175,15 @@ namespace SuperTerrainPlus::STPStage {
 		__device__ unsigned int stride() const { return this->RW; }
 		__device__ float load(const unsigned int i) const { return fromFixedPoint(this->Lds[i]); }
 		__device__ Raw raw(const unsigned int i) const { return this->Lds[i]; }
+		// weights arrive pre-scaled to fixed point, so `amount` is already in fixed-point units
+		static constexpr float WeightScale = FixedScale;
 		__device__ float take(const unsigned int i, const Raw h, const float amount) const {
-			const int want = toFixedPoint(amount);
+			const int want = __float2int_rn(amount);
 			const int dq = (h < want) ? h : want;
-			atomicAdd(this->Lds + i, -dq);
-			return fromFixedPoint(dq);
+			atomicSub(this->Lds + i, dq);
+			return static_cast<float>(dq);
 		}
+		__device__ static float height(const float taken) { return taken * FixedStep; }
 		__device__ void add(const unsigned int i, const float v) const { atomicAdd(this->Lds + i, toFixedPoint(v)); }
 		__device__ bool holds(const unsigned int x0, const unsigned int y0, const unsigned int x1, const unsigned int y1) const {
 			return x0 >= this->X0 && y0 >= this->Y0 && x1 < this->X0 + this->RW && y1 < this->Y0 + this->RH;
@@ -231,10 +237,13 @@ namespace SuperTerrainPlus::STPStage {
 			const float inertia = a.Rain.Inertia;
 			d.dx = (-hg.y) * (1 - inertia) + d.dx * inertia;
 			d.dy = (-hg.z) * (1 - inertia) + d.dy * inertia;
-			const float length = sqrtf(d.dx * d.dx + d.dy * d.dy);
-			if (length != 0.0f) {
-				d.dx /= length;
-				d.dy /= length;
+			// parallel mode normalises with the hardware reciprocal square root (1 ulp): its result
+			// is judged statistically; the serialized kernel keeps the reference's division
+			const float length2 = d.dx * d.dx + d.dy * d.dy;
+			if (length2 != 0.0f) {
+				const float inv = __builtin_amdgcn_rsqf(length2);
+				d.dx *= inv;
+				d.dy *= inv;
 			}
 			d.px += d.dx;
 			d.py += d.dy;
@@ -270,6 +279,7 @@ namespace SuperTerrainPlus::STPStage {
 				if constexpr (Taps > 0u) {
 					// taps fetched per group; the group is a compile-time divisor of Taps
 					constexpr unsigned int G = Surface::groupOf(Taps);
+					float taken = 0.0f;
 #pragma unroll 1
 					for (unsigned int base = 0u; base < Taps; base += G) {
 						typename Surface::Raw h[G];
@@ -280,15 +290,18 @@ namespace SuperTerrainPlus::STPStage {
 #pragma unroll
 						for (unsigned int j = 0u; j < G; j++) {
 							const int2 b = brush[base + j];
-							d.sediment += s.take(tapIndex(mapIndex, b.x), h[j], amount * __int_as_float(b.y));
+							taken += s.take(tapIndex(mapIndex, b.x), h[j], amount * __int_as_float(b.y));
 						}
 					}
+					d.sediment += Surface::height(taken);
 				} else {
+					float taken = 0.0f;
 					for (unsigned int j = 0u; j < a.BrushSize; j++) {
 						const int2 b = brush[j];
 						const unsigned int ei = tapIndex(mapIndex, b.x);
-						d.sediment += s.take(ei, s.raw(ei), amount * __int_as_float(b.y));
+						taken += s.take(ei, s.raw(ei), amount * __int_as_float(b.y));
 					}
+					d.sediment += Surface::height(taken);
 				}
 				erodeSteps++;
 			}
