@@ -61,6 +61,7 @@ namespace SuperTerrainPlus::STPBatch {
 	// histogram output for row-contiguous stores.
 
 	constexpr unsigned int VPrefetch = 16u; // halo rows fetched at once by the vertical pass
+	constexpr unsigned int HItemPrefetch = 8u; // columns fetched at once by the biome-indexed horizontal pass
 	// columns fetched at once by the register horizontal pass (its registers scale with MaxCap)
 	__host__ __device__ constexpr unsigned int horizontalPrefetch(const unsigned int maxCap) {
 		return maxCap >= 16u ? 2u : (maxCap >= 8u ? 4u : 8u);
@@ -360,6 +361,300 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 	}
 
+	// ---- single histogram filter for maps of at most MaxB biomes (ids < MaxB)
+	//
+	// The same filter, stated without the accumulator. For one line and one biome, call an
+	// occurrence a position whose value (vertical pass) or column histogram (horizontal pass)
+	// holds the biome, and a run start an occurrence with no occurrence in the 2r + 1 positions
+	// before it. The accumulator of the reference (STPSingleHistogramFilter.cpp:400-450) then
+	// holds, for the window ending at position e, exactly the biomes with an occurrence in the
+	// window, counted over it, in the order of (their latest run start <= e, their slot in that
+	// run start's column histogram): a biome is appended when its run's first occurrence enters,
+	// stays while the run lasts, and erasures keep the relative order of the others. So one lane
+	// keeps, per biome, the window count, the last occurrence and the latest run-start key, all in
+	// registers, and sorts the present biomes by key when it emits -- O(MaxB) register work per
+	// step instead of searching and shifting bins.
+
+	// ascending sorting network over (key, value) pairs, every index static
+	template <unsigned int N>
+	__device__ inline void sortPairs(unsigned int (&key)[N], unsigned int (&val)[N]) {
+		const auto swap = [&](const unsigned int a, const unsigned int b) {
+			const bool x = key[b] < key[a];
+			const unsigned int ka = x ? key[b] : key[a], kb = x ? key[a] : key[b];
+			const unsigned int va = x ? val[b] : val[a], vb = x ? val[a] : val[b];
+			key[a] = ka;
+			key[b] = kb;
+			val[a] = va;
+			val[b] = vb;
+		};
+		// odd-even transposition: N rounds of adjacent compare-exchanges
+#pragma unroll
+		for (unsigned int round = 0u; round < N; round++) {
+#pragma unroll
+			for (unsigned int a = round & 1u; a + 1u < N; a += 2u) {
+				swap(a, a + 1u);
+			}
+		}
+	}
+
+	constexpr unsigned int NoRun = 0xFFFFFFFFu;
+	constexpr unsigned int ItemStageRows = 16u;
+
+	// Per-biome window state of one line.
+	template <unsigned int MaxB>
+	struct ItemLine {
+		unsigned int Count[MaxB], Key[MaxB];
+		int Last[MaxB];
+		__device__ ItemLine() {
+#pragma unroll
+			for (unsigned int i = 0u; i < MaxB; i++) {
+				this->Count[i] = 0u;
+				this->Key[i] = NoRun;
+				this->Last[i] = -0x40000000;
+			}
+		}
+		// biome `it` occurs at position `pos` with count `c`, as slot `slot` of that position
+		__device__ inline void enter(const unsigned int it, const int pos, const unsigned int c, const unsigned int slot,
+			const unsigned int radius) {
+#pragma unroll
+			for (unsigned int i = 0u; i < MaxB; i++) {
+				if (i == it) {
+					if (this->Last[i] < pos - 2 * static_cast<int>(radius) - 1) {
+						this->Key[i] = static_cast<unsigned int>(pos) * 64u + slot;
+					}
+					this->Last[i] = pos;
+					this->Count[i] += c;
+				}
+			}
+		}
+		__device__ inline void leave(const unsigned int it, const unsigned int c) {
+#pragma unroll
+			for (unsigned int i = 0u; i < MaxB; i++) {
+				if (i == it) {
+					this->Count[i] -= c;
+				}
+			}
+		}
+		// present biomes in order: item[k], count[k] for k < returned n
+		__device__ inline unsigned int order(unsigned int (&item)[MaxB], unsigned int (&count)[MaxB]) const {
+			unsigned int key[MaxB], n = 0u;
+#pragma unroll
+			for (unsigned int i = 0u; i < MaxB; i++) {
+				const bool present = this->Count[i] > 0u;
+				key[i] = present ? this->Key[i] : NoRun;
+				item[i] = i;
+				n += present ? 1u : 0u;
+			}
+			sortPairs(key, item);
+#pragma unroll
+			for (unsigned int k = 0u; k < MaxB; k++) {
+				unsigned int c = 0u;
+#pragma unroll
+				for (unsigned int i = 0u; i < MaxB; i++) {
+					c = item[k] == i ? this->Count[i] : c;
+				}
+				count[k] = c;
+			}
+			return n;
+		}
+	};
+
+	// vertical pass, vcount[col * side + y], vbin[(col * vcap + k) * side + y] as verticalKERNEL
+	template <unsigned int MaxB>
+	__global__ void __launch_bounds__(ShfLineBlock) verticalItemKERNEL(const uint2* const tiles, const Chunk* const chunks,
+		const unsigned int radius, const unsigned int vcap, const unsigned short* const biome, unsigned int* const vbin,
+		unsigned char* const vcount, unsigned int* const overflow) {
+		extern __shared__ unsigned int lds[];
+		constexpr unsigned int R = ItemStageRows;
+		unsigned int* const stage = lds;                      // [k][row][65]
+		unsigned int* const stageN = stage + vcap * R * 65u;  // [row][65]
+		const uint2 t = tiles[blockIdx.x];
+		const Chunk c = chunks[t.x];
+		const unsigned int lane = threadIdx.x, side = c.Side, ncols = side + 2u * radius;
+		const unsigned int col = t.y * ShfLineBlock + lane;
+		const bool active = col < ncols;
+		const size_t lastRow = side + 2u * radius - 1u;
+		const unsigned short* const colc = biome + c.Halo + min(col, ncols - 1u);
+		const auto fetch = [&](unsigned int (&v)[VPrefetch], const long long row0) {
+#pragma unroll
+			for (unsigned int i = 0u; i < VPrefetch; i++) {
+				const long long r = row0 + i;
+				v[i] = colc[static_cast<size_t>(r < 0 ? 0 : (r > static_cast<long long>(lastRow) ? lastRow : r)) * ncols];
+			}
+		};
+		ItemLine<MaxB> line;
+		bool over = false;
+		for (unsigned int j0 = 0u; j0 <= 2u * radius; j0 += VPrefetch) {
+			unsigned int v[VPrefetch];
+			fetch(v, j0);
+#pragma unroll
+			for (unsigned int i = 0u; i < VPrefetch; i++) {
+				if (active && j0 + i <= 2u * radius) {
+					line.enter(v[i], static_cast<int>(j0 + i), 1u, 0u, radius);
+				}
+			}
+		}
+		unsigned int* const outBin = vbin + c.Line * vcap;
+		unsigned char* const outN = vcount + c.Line;
+		for (unsigned int y0 = 0u; y0 < side; y0 += R) {
+			const unsigned int rows = min(R, side - y0);
+			unsigned int enter[VPrefetch], leave[VPrefetch];
+			fetch(enter, static_cast<long long>(y0) + 2 * static_cast<long long>(radius));
+			fetch(leave, static_cast<long long>(y0) - 1);
+#pragma unroll
+			for (unsigned int i = 0u; i < VPrefetch; i++) {
+				const unsigned int y = y0 + i;
+				if (active && i < rows) {
+					if (y > 0u) {
+						line.enter(enter[i], static_cast<int>(y + 2u * radius), 1u, 0u, radius);
+						line.leave(leave[i], 1u);
+					}
+					unsigned int item[MaxB], count[MaxB];
+					const unsigned int n = line.order(item, count);
+					over = over || n > vcap;
+					stageN[i * 65u + lane] = min(n, vcap);
+#pragma unroll
+					for (unsigned int k = 0u; k < MaxB; k++) {
+						if (k < n && k < vcap) {
+							stage[(k * R + i) * 65u + lane] = item[k] | (count[k] << 16);
+						}
+					}
+				}
+			}
+			__syncthreads();
+			// consecutive lanes take consecutive rows of one column
+			for (unsigned int i = lane; i < ShfLineBlock * R; i += ShfLineBlock) {
+				const unsigned int cc = i / R, yy = i % R;
+				const unsigned int cg = t.y * ShfLineBlock + cc;
+				if (cg < ncols && yy < rows) {
+					const unsigned int n = stageN[yy * 65u + cc];
+					const size_t y = static_cast<size_t>(y0) + yy;
+					outN[static_cast<size_t>(cg) * side + y] = static_cast<unsigned char>(n);
+#pragma unroll
+					for (unsigned int k = 0u; k < MaxB; k++) {
+						if (k < n) {
+							outBin[(static_cast<size_t>(cg) * vcap + k) * side + y] = stage[(k * R + yy) * 65u + cc];
+						}
+					}
+				}
+			}
+			__syncthreads();
+		}
+		if (over) {
+			atomicOr(overflow, 1u);
+		}
+	}
+
+	// horizontal pass, dense output as horizontalKERNEL
+	template <unsigned int MaxB>
+	__global__ void __launch_bounds__(ShfLineBlock) horizontalItemKERNEL(const uint2* const tiles,
+		const Chunk* const chunks, const unsigned int radius, const unsigned int vcap, const unsigned int cap,
+		const unsigned int* const vbin, const unsigned char* const vcount, unsigned char* const bin_count,
+		unsigned short* const bin_item, float* const bin_weight, unsigned int* const overflow) {
+		extern __shared__ unsigned int lds[];
+		constexpr unsigned int P = HItemPrefetch;
+		unsigned int* const outN = lds;                // [px][65]
+		unsigned int* const outItem = outN + P * 65u;  // [px][k][65]
+		float* const outW = reinterpret_cast<float*>(outItem + P * cap * 65u);
+		const uint2 t = tiles[blockIdx.x];
+		const Chunk c = chunks[t.x];
+		const unsigned int lane = threadIdx.x, side = c.Side, ncols = side + 2u * radius;
+		const unsigned int y0 = t.y * ShfLineBlock, y = y0 + lane;
+		const bool active = y < side;
+		const unsigned int* const inBin = vbin + c.Line * vcap;
+		const unsigned char* const inN = vcount + c.Line;
+		const unsigned int yc = min(y, side - 1u);
+		const auto colIndex = [&](const int col) {
+			return static_cast<size_t>(col < 0 ? 0 : (col >= static_cast<int>(ncols) ? static_cast<int>(ncols) - 1 : col));
+		};
+		ItemLine<MaxB> line;
+		bool over = false;
+		const int first = -2 * static_cast<int>(radius);
+		for (int s0 = first; s0 < static_cast<int>(side); s0 += static_cast<int>(P)) {
+			// column histograms of the block: entering s0 + 2r + j, leaving s0 - 1 + j
+			unsigned int eN[P], lN[P], eB[P][MaxB], lB[P][MaxB];
+#pragma unroll
+			for (unsigned int j = 0u; j < P; j++) {
+				const size_t ce = colIndex(s0 + 2 * static_cast<int>(radius) + static_cast<int>(j)),
+					cl = colIndex(s0 - 1 + static_cast<int>(j));
+				eN[j] = inN[ce * side + yc];
+				lN[j] = inN[cl * side + yc];
+#pragma unroll
+				for (unsigned int k = 0u; k < MaxB; k++) {
+					eB[j][k] = k < vcap ? inBin[(ce * vcap + k) * side + yc] : 0u;
+					lB[j][k] = k < vcap ? inBin[(cl * vcap + k) * side + yc] : 0u;
+				}
+			}
+			const int sEnd = min(s0 + static_cast<int>(P), static_cast<int>(side));
+#pragma unroll
+			for (unsigned int j = 0u; j < P; j++) {
+				const int s = s0 + static_cast<int>(j);
+				if (active && s < sEnd) {
+					const int ce = s + 2 * static_cast<int>(radius);
+#pragma unroll
+					for (unsigned int k = 0u; k < MaxB; k++) {
+						if (k < eN[j]) {
+							line.enter(eB[j][k] & 0xFFFFu, ce, eB[j][k] >> 16, k, radius);
+						}
+					}
+					if (s >= 1) {
+#pragma unroll
+						for (unsigned int k = 0u; k < MaxB; k++) {
+							if (k < lN[j]) {
+								line.leave(lB[j][k] & 0xFFFFu, lB[j][k] >> 16);
+							}
+						}
+					}
+					if (s >= 0) {
+						unsigned int item[MaxB], count[MaxB];
+						const unsigned int n = line.order(item, count);
+						over = over || n > cap;
+						unsigned int sum = 0u;
+#pragma unroll
+						for (unsigned int k = 0u; k < MaxB; k++) {
+							sum += k < n ? count[k] : 0u;
+						}
+						const float norm = 1.0f / static_cast<float>(sum);
+						outN[j * 65u + lane] = min(n, cap);
+#pragma unroll
+						for (unsigned int k = 0u; k < MaxB; k++) {
+							if (k < n && k < cap) {
+								outItem[(j * cap + k) * 65u + lane] = item[k];
+								outW[(j * cap + k) * 65u + lane] = static_cast<float>(count[k]) * norm;
+							}
+						}
+					}
+				}
+			}
+			__syncthreads();
+			// row-contiguous stores of the staged pixels [xa, sEnd)
+			const int xa = max(s0, 0);
+			if (sEnd > xa) {
+				const unsigned int npx = static_cast<unsigned int>(sEnd - xa), jb = static_cast<unsigned int>(xa - s0);
+				const unsigned int rows = min(ShfLineBlock, side - y0), width = npx * cap;
+				// lane -> (pixel, slot) of one row; rows advance by the lanes that fit
+				const unsigned int perRow = width, rowsPerPass = max(ShfLineBlock / perRow, 1u);
+				const unsigned int lr = lane / perRow, lq = lane - lr * perRow, px = lq / cap, k = lq - px * cap;
+				for (unsigned int r0 = 0u; r0 < rows; r0 += rowsPerPass) {
+					const unsigned int r = r0 + lr;
+					if (lr < rowsPerPass && r < rows && lq < width) {
+						const size_t p = static_cast<size_t>(y0 + r) * side + static_cast<size_t>(xa) + px;
+						const unsigned int at = ((jb + px) * cap + k) * 65u + r;
+						bin_item[(c.Pixel + p) * cap + k] = static_cast<unsigned short>(outItem[at]);
+						bin_weight[(c.Pixel + p) * cap + k] = outW[at];
+						if (k == 0u) {
+							bin_count[c.Pixel + p] = static_cast<unsigned char>(outN[(jb + px) * 65u + r]);
+						}
+					}
+				}
+			}
+			__syncthreads();
+		}
+		if (over) {
+			atomicOr(overflow, 1u);
+		}
+	}
+
 	__global__ void __launch_bounds__(BlendX * BlendY) blendKERNEL(const uint2* const tiles, const Chunk* const chunks,
 		const STPAlgorithm::STPPermutation perm, const BiomeTable biomes, const unsigned int biome_count,
 		const unsigned int cap, const unsigned char* const bin_count, const unsigned short* const bin_item,
@@ -570,7 +865,7 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 		for (unsigned int k = threadIdx.x; k < args.BrushSize; k += Threads) {
 			const int2 o = brush_offset[k];
-			brush[k] = make_int2(o.y * static_cast<int>(s.RW) + o.x, __float_as_int(brush_weight[k]));
+			brush[k] = make_int2(o.y * static_cast<int>(s.RW) + o.x, __float_as_int(brush_weight[k] * LocalSurface::WeightScale));
 		}
 		__syncthreads();
 		const ErosionArgs a = chunkArgs(args, c);
