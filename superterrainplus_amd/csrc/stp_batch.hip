@@ -1206,6 +1206,11 @@ namespace {
 	}
 } // namespace
 
+bool STPHeightfieldBatch::itemPath() const noexcept {
+	// biome ids come from the synthetic biome map, so they are < BiomeCount
+	return this->Setting.BiomeCount <= 8u && this->Setting.BinCapacity <= 8u;
+}
+
 ErosionArgs STPHeightfieldBatch::erosionArgs() const {
 	// per-chunk fields (map size, stream id, drop count) are filled in by the kernels
 	ErosionArgs args;
@@ -1448,6 +1453,17 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 					this->ChunkRecord, radius, this->VCap, this->BiomeMap.get(), this->VBin.get(),
 					this->VCount.get(), this->Flag.get());
 			};
+			if (this->itemPath()) {
+				const unsigned int words = (this->VCap * ItemStageRows + ItemStageRows) * 65u;
+				if (this->Setting.BiomeCount <= 4u) {
+					verticalItemKERNEL<4u><<<n, ShfLineBlock, words * sizeof(unsigned int), stream>>>(t, this->ChunkRecord,
+						radius, this->VCap, this->BiomeMap.get(), this->VBin.get(), this->VCount.get(), this->Flag.get());
+				} else {
+					verticalItemKERNEL<8u><<<n, ShfLineBlock, words * sizeof(unsigned int), stream>>>(t, this->ChunkRecord,
+						radius, this->VCap, this->BiomeMap.get(), this->VBin.get(), this->VCount.get(), this->Flag.get());
+				}
+				break;
+			}
 			switch (registerCapacity(this->VCap)) {
 			case 4u:
 				launch(verticalKERNEL<4u>);
@@ -1475,6 +1491,19 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 					this->ChunkRecord, radius, this->VCap, cap, this->VBin.get(), this->VCount.get(),
 					this->BinCount.get(), this->BinItem.get(), this->BinWeight.get(), this->Flag.get());
 			};
+			if (this->itemPath()) {
+				const unsigned int words = HItemPrefetch * (1u + 2u * cap) * 65u;
+				if (this->Setting.BiomeCount <= 4u) {
+					horizontalItemKERNEL<4u><<<n, ShfLineBlock, words * sizeof(unsigned int), stream>>>(t, this->ChunkRecord,
+						radius, this->VCap, cap, this->VBin.get(), this->VCount.get(), this->BinCount.get(),
+						this->BinItem.get(), this->BinWeight.get(), this->Flag.get());
+				} else {
+					horizontalItemKERNEL<8u><<<n, ShfLineBlock, words * sizeof(unsigned int), stream>>>(t, this->ChunkRecord,
+						radius, this->VCap, cap, this->VBin.get(), this->VCount.get(), this->BinCount.get(),
+						this->BinItem.get(), this->BinWeight.get(), this->Flag.get());
+				}
+				break;
+			}
 			switch (horizontalRegisterCapacity(cap)) {
 			case 4u:
 				launch(horizontalKERNEL<4u>);

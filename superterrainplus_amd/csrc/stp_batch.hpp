@@ -190,6 +190,8 @@ namespace SuperTerrainPlus {
 		// Window side bound and LDS bytes of the tiled erosion for a tile side (0 if impossible).
 		unsigned int erosionLdsBytes(unsigned int tile) const;
 		STPStage::ErosionArgs erosionArgs() const;
+		// the filter passes with per-biome registers (at most 8 biomes and 8 bins per pixel)
+		bool itemPath() const noexcept;
 	};
 
 } // namespace SuperTerrainPlus
