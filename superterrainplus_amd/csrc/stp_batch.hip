@@ -31,22 +31,70 @@ namespace SuperTerrainPlus::STPBatch {
 	constexpr unsigned int BlendX = 64u, BlendY = 4u, BlendPixel = 2u;
 	constexpr unsigned int ErosionBlock = 256u;
 
+	// Synthetic biome map over a 16 x 16 tile of the halo map: the Voronoi sites of every grid cell
+	// the tile's samples look at (their cells +- 1) are drawn once into LDS, then each sample takes
+	// the nearest of its 3 x 3 cells in the order and with the tie rule of voronoiBiome. Distances
+	// are exact integers, so the result is the same as voronoiBiome per sample.
+	constexpr unsigned int BiomeSide = 16u, BiomeCells = 1024u;
 	__global__ void __launch_bounds__(BiomeTile) biomeKERNEL(const uint2* const tiles, const Chunk* const chunks,
 		const unsigned long long seed, const unsigned int count, const unsigned int cell, const unsigned int radius,
 		unsigned short* const biome) {
+		__shared__ int siteX[BiomeCells], siteY[BiomeCells];
+		__shared__ unsigned short siteBiome[BiomeCells];
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
-		const unsigned int w = c.Side + 2u * radius;
-		const unsigned int p = t.y * BiomeTile + threadIdx.x;
-		if (p >= w * w) {
-			return;
-		}
-		const unsigned int x = p % w, y = p / w;
+		const unsigned int w = c.Side + 2u * radius, tilesX = (w + BiomeSide - 1u) / BiomeSide;
+		const unsigned int x0 = (t.y % tilesX) * BiomeSide, y0 = (t.y / tilesX) * BiomeSide;
+		const unsigned int x = x0 + threadIdx.x % BiomeSide, y = y0 + threadIdx.x / BiomeSide;
 		// halo pixel (x, y) is chunk pixel (x - r, y - r), at world Origin + pixel * Step
 		const long long step = c.Step;
-		biome[c.Halo + p] = voronoiBiome(seed, count, cell,
-			static_cast<long long>(c.OriginX) + (static_cast<long long>(x) - static_cast<long long>(radius)) * step,
-			static_cast<long long>(c.OriginY) + (static_cast<long long>(y) - static_cast<long long>(radius)) * step);
+		const long long wx0 = static_cast<long long>(c.OriginX) + (static_cast<long long>(x0) - radius) * step,
+			wy0 = static_cast<long long>(c.OriginY) + (static_cast<long long>(y0) - radius) * step;
+		const long long span = static_cast<long long>(BiomeSide - 1u) * step;
+		const long long cx0 = floorDiv(wx0, cell) - 1, cy0 = floorDiv(wy0, cell) - 1;
+		const long long nx = floorDiv(wx0 + span, cell) + 1 - cx0 + 1, ny = floorDiv(wy0 + span, cell) + 1 - cy0 + 1;
+		const long long wx = wx0 + static_cast<long long>(threadIdx.x % BiomeSide) * step,
+			wy = wy0 + static_cast<long long>(threadIdx.x / BiomeSide) * step;
+		const bool inside = x < w && y < w;
+		if (nx * ny > static_cast<long long>(BiomeCells) || cell > (1u << 20)) {
+			// very coarse levels of detail (or cells too large for 32-bit relative sites): sample by sample
+			if (inside) {
+				biome[c.Halo + static_cast<size_t>(y) * w + x] = voronoiBiome(seed, count, cell, wx, wy);
+			}
+			return;
+		}
+		const unsigned int cells = static_cast<unsigned int>(nx * ny);
+		for (unsigned int k = threadIdx.x; k < cells; k += BiomeTile) {
+			const long long ncx = cx0 + static_cast<long long>(k % static_cast<unsigned int>(nx)),
+				ncy = cy0 + static_cast<long long>(k / static_cast<unsigned int>(nx));
+			const STPRandom::Philox4 r = STPRandom::philox4x32_10(static_cast<unsigned int>(ncx),
+				static_cast<unsigned int>(ncy), 0x564F524Fu, 0u, static_cast<unsigned int>(seed),
+				static_cast<unsigned int>(seed >> 32));
+			// site relative to the tile's first world sample
+			siteX[k] = static_cast<int>(ncx * cell + (r.v[0] % cell) - wx0);
+			siteY[k] = static_cast<int>(ncy * cell + (r.v[1] % cell) - wy0);
+			siteBiome[k] = static_cast<unsigned short>(r.v[2] % count);
+		}
+		__syncthreads();
+		if (!inside) {
+			return;
+		}
+		const long long px = wx - wx0, py = wy - wy0;
+		const long long lx = floorDiv(wx, cell) - cx0, ly = floorDiv(wy, cell) - cy0;
+		long long best = 0x7FFFFFFFFFFFFFFFll;
+		unsigned short b = 0u;
+		for (int dy = -1; dy <= 1; dy++) {
+			for (int dx = -1; dx <= 1; dx++) {
+				const unsigned int k = static_cast<unsigned int>((ly + dy) * nx + (lx + dx));
+				const long long ex = px - siteX[k], ey = py - siteY[k];
+				const long long d = ex * ex + ey * ey;
+				if (d < best) {
+					best = d;
+					b = siteBiome[k];
+				}
+			}
+		}
+		biome[c.Halo + static_cast<size_t>(y) * w + x] = b;
 	}
 
 	// ---- single histogram filter, batched form
@@ -1305,7 +1353,8 @@ void STPHeightfieldBatch::buildTileTables(const hipStream_t stream) {
 		const unsigned int blend =
 			((c.Side + BlendX - 1u) / BlendX) * ((c.Side + BlendY * BlendPixel - 1u) / (BlendY * BlendPixel));
 		const unsigned int tx = tile ? (c.Side + tile - 1u) / tile : 0u;
-		const unsigned int n[5] = {(w * w + BiomeTile - 1u) / BiomeTile, (w + ShfLineBlock - 1u) / ShfLineBlock,
+		const unsigned int bt = (w + BiomeSide - 1u) / BiomeSide;
+		const unsigned int n[5] = {bt * bt, (w + ShfLineBlock - 1u) / ShfLineBlock,
 			(c.Side + ShfLineBlock - 1u) / ShfLineBlock, blend, (c.Drops + ErosionBlock - 1u) / ErosionBlock};
 		for (int s = 0; s < 5; s++) {
 			range.Begin[s] = static_cast<unsigned int>(table[s].size());
