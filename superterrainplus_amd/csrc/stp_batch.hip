@@ -1079,6 +1079,13 @@ void STPHeightfieldBatch::init(const STPEnvironment::STPRainDropSetting& rain) {
 	this->BrushSize = static_cast<unsigned int>(offset.size());
 	this->BrushOffset = upload(offset);
 	this->BrushWeight = upload(weight);
+	for (unsigned int k = 0u; shapedRadius(this->BrushSize) != 0 && k < this->BrushSize; k++) {
+		const int d2 = offset[k].x * offset[k].x + offset[k].y * offset[k].y;
+		const float w = weight[k] * STPStage::FixedScale;
+		if (w != shapedWeight(static_cast<int>(r.ErosionBrushRadius), d2)) {
+			STP_THROW(STPNumericDomainError, "erosion brush weight differs from the tiled kernel's table");
+		}
+	}
 
 	// resident arenas, sized by the capacity: (side + 2r)^2 <= side^2 + 4r side + 4r^2 per chunk
 	const unsigned long long pixel = cap.Pixels, drops = std::max<unsigned long long>(cap.Drops, 1ull);

@@ -124,6 +124,15 @@ namespace SuperTerrainPlus::STPStage {
 		return static_cast<unsigned int>(static_cast<int>(map_index) + tap);
 	}
 
+	// Erode update of all taps of a step on a surface with per-tap `take`; returns the sediment
+	// taken in height units. Taps > 0: straight-line code, the heights of a group of taps are
+	// fetched, then their updates issued, with one wait per group. The taps of one step are
+	// distinct cells, so fetching a group's heights before updating leaves the drop's own result
+	// unchanged (the reference reads and updates them one by one). Taps == 0 loops over `count`.
+	template <class Surface, unsigned int Taps>
+	__device__ inline float erodeGrouped(const Surface& s, const unsigned int map_index, const float amount,
+		const int2* const brush, const unsigned int count);
+
 	// The height surface a drop reads and updates. `load` gives the FP32 height; `Raw` is the
 	// stored form, `raw` reads it. `take` removes min(height, amount) from a cell with one atomic
 	// (the reference's erode update, STPRainDrop.cu:150-155); the brush weights it receives amounts
@@ -151,6 +160,11 @@ namespace SuperTerrainPlus::STPStage {
 		__device__ bool holds(unsigned int, unsigned int, unsigned int, unsigned int) const { return true; }
 		// all taps of a step in one group: one HBM/L2 round trip
 		static constexpr unsigned int groupOf(const unsigned int taps) { return taps; }
+		template <unsigned int Taps>
+		__device__ float erode(const unsigned int i, const float amount, const int2* const brush,
+			const unsigned int count) const {
+			return erodeGrouped<GlobalSurface, Taps>(*this, i, amount, brush, count);
+		}
 	};
 
 	// LocalSurface: an LDS window [X0, X0 + RW) x [Y0, Y0 + RH) of the map (map coordinates),
@@ -164,6 +178,110 @@ namespace SuperTerrainPlus::STPStage {
 
 	__device__ inline int toFixedPoint(const float h) { return __float2int_rn(h * FixedScale); }
 	__device__ inline float fromFixedPoint(const int q) { return static_cast<float>(q) * FixedStep; }
+
+	// The brush generator's weights (STPHeightfieldGenerator.cpp:44-99: 1 - |d| / r in double,
+	// stored as float, times float(1 / sum) in double) of the radius 2, 3 and 4 brushes, multiplied
+	// by FixedScale (exact: a power of two), by squared tap distance. As literals they cost no
+	// register and no load; the batch engine checks them bit for bit against the generator when
+	// it is built. 0 = not a tap.
+	constexpr float shapedWeight(const int R, const int d2) {
+		switch (R * 64 + d2) {
+		case 2 * 64 + 0:
+			return 0x1.eaf122p+25f;
+		case 2 * 64 + 1:
+			return 0x1.eaf122p+24f;
+		case 2 * 64 + 2:
+			return 0x1.1f964ep+24f;
+		case 3 * 64 + 0:
+			return 0x1.b4a8fp+24f;
+		case 3 * 64 + 1:
+			return 0x1.231b4cp+24f;
+		case 3 * 64 + 2:
+			return 0x1.cda208p+23f;
+		case 3 * 64 + 4:
+			return 0x1.231b4cp+23f;
+		case 3 * 64 + 5:
+			return 0x1.bcc574p+22f;
+		case 3 * 64 + 8:
+			return 0x1.8f9168p+20f;
+		case 4 * 64 + 0:
+			return 0x1.e9165ap+23f;
+		case 4 * 64 + 1:
+			return 0x1.6ed0c4p+23f;
+		case 4 * 64 + 2:
+			return 0x1.3c2b38p+23f;
+		case 4 * 64 + 4:
+			return 0x1.e9165ap+22f;
+		case 4 * 64 + 5:
+			return 0x1.af5bbep+22f;
+		case 4 * 64 + 8:
+			return 0x1.1e803p+22f;
+		case 4 * 64 + 9:
+			return 0x1.e9165ap+21f;
+		case 4 * 64 + 10:
+			return 0x1.99b82ap+21f;
+		case 4 * 64 + 13:
+			return 0x1.81d6fap+20f;
+		default:
+			return 0.0f;
+		}
+	}
+
+	// The brush of radius R as the generator lists it (STPHeightfieldGenerator.cpp:44-99): rows
+	// dy = -(R-1) .. R-1 in order, each the contiguous run dx = -h .. h with dx^2 + dy^2 < R^2.
+	constexpr int brushRowHalf(const int R, const int dy) {
+		int h = 0;
+		while ((h + 1) * (h + 1) + dy * dy < R * R) {
+			h++;
+		}
+		return h;
+	}
+	// brush radius of a tap count with a shaped (compile-time) form; 0 = none
+	constexpr int shapedRadius(const unsigned int taps) {
+		return taps == 9u ? 2 : (taps == 25u ? 3 : (taps == 45u ? 4 : 0));
+	}
+
+	// Erode taps of rows Dy .. R-1 around `cell` (LDS, row stride rw); returns the fixed-point
+	// sediment taken. Per tap: want = round(amount * w) and one returning atomic subtraction; the
+	// old height it returns gives min(old, want), and a tap with old < want gets the excess back, so
+	// the cell ends at old - min(old, want), the reference's erode update. Within a row the taps
+	// are consecutive words (one address, immediate offsets); a row is settled before the next is
+	// issued, which bounds the live registers to one row.
+	template <int R, int Dy>
+	__device__ inline int shapedTake(int* const cell, const int rw, const float amount) {
+		constexpr int H = brushRowHalf(R, Dy), N = 2 * H + 1;
+		// the row's first tap; opaque to the optimiser, so that it keeps one address per row with
+		// non-negative immediate offsets instead of one address per tap left of the centre
+		int first = Dy * rw - H;
+		asm("" : "+s"(first));
+		int* const p = cell + first;
+		int want[N], old[N];
+#pragma unroll
+		for (int j = 0; j < N; j++) {
+			want[j] = static_cast<int>(fmaf(amount, shapedWeight(R, (j - H) * (j - H) + Dy * Dy), 0.5f));
+			old[j] = atomicSub(p + j, want[j]);
+		}
+		int sum = 0;
+		bool refund = false;
+#pragma unroll
+		for (int j = 0; j < N; j++) {
+			sum += want[j];
+			refund |= old[j] < want[j];
+		}
+		if (refund) {
+#pragma unroll
+			for (int j = 0; j < N; j++) {
+				if (old[j] < want[j]) {
+					atomicAdd(p + j, want[j] - old[j]);
+					sum -= want[j] - old[j];
+				}
+			}
+		}
+		if constexpr (Dy < R - 1) {
+			sum += shapedTake<R, Dy + 1>(cell, rw, amount);
+		}
+		return sum;
+	}
 
 	struct LocalSurface {
 		typedef int Raw;
@@ -188,11 +306,53 @@ namespace SuperTerrainPlus::STPStage {
 		__device__ bool holds(const unsigned int x0, const unsigned int y0, const unsigned int x1, const unsigned int y1) const {
 			return x0 >= this->X0 && y0 >= this->Y0 && x1 < this->X0 + this->RW && y1 < this->Y0 + this->RH;
 		}
-		// LDS latency is short: small groups keep registers free for more waves
-		static constexpr unsigned int groupOf(const unsigned int taps) {
-			return taps % 5u == 0u ? 5u : (taps % 3u == 0u ? 3u : taps);
+		static constexpr unsigned int groupOf(const unsigned int taps) { return taps; }
+		// The 9/25/45-tap brushes go through shapedTake. Concurrent drops can see a cell's height
+		// transiently lowered by a refunded excess, a race of the same kind as the reference's
+		// read-then-update. `amount` is held below 8 (the window's height range) so the fixed-point
+		// tap sum stays below 2^31.
+		template <unsigned int Taps>
+		__device__ float erode(const unsigned int i, const float amount, const int2* const brush,
+			const unsigned int count) const {
+			constexpr int R = shapedRadius(Taps);
+			if constexpr (R == 0) {
+				return erodeGrouped<LocalSurface, Taps>(*this, i, amount, brush, count);
+			} else {
+				const int sum = shapedTake<R, 1 - R>(this->Lds + i, static_cast<int>(this->RW), fminf(amount, 7.5f));
+				return static_cast<float>(sum) * FixedStep;
+			}
 		}
 	};
+
+	template <class Surface, unsigned int Taps>
+	__device__ inline float erodeGrouped(const Surface& s, const unsigned int map_index, const float amount,
+		const int2* const brush, const unsigned int count) {
+		float taken = 0.0f;
+		if constexpr (Taps > 0u) {
+			// the group is a compile-time divisor of Taps
+			constexpr unsigned int G = Surface::groupOf(Taps);
+#pragma unroll 1
+			for (unsigned int base = 0u; base < Taps; base += G) {
+				typename Surface::Raw h[G];
+#pragma unroll
+				for (unsigned int j = 0u; j < G; j++) {
+					h[j] = s.raw(tapIndex(map_index, brush[base + j].x));
+				}
+#pragma unroll
+				for (unsigned int j = 0u; j < G; j++) {
+					const int2 b = brush[base + j];
+					taken += s.take(tapIndex(map_index, b.x), h[j], amount * __int_as_float(b.y));
+				}
+			}
+		} else {
+			for (unsigned int j = 0u; j < count; j++) {
+				const int2 b = brush[j];
+				const unsigned int ei = tapIndex(map_index, b.x);
+				taken += s.take(ei, s.raw(ei), amount * __int_as_float(b.y));
+			}
+		}
+		return Surface::height(taken);
+	}
 
 	// STPRainDrop.cu:23-51 on a surface.
 	template <class Surface>
@@ -209,11 +369,8 @@ namespace SuperTerrainPlus::STPStage {
 	}
 
 	// One drop in parallel mode on `s`. `brush` holds (linear tap offset for s.stride(), weight
-	// bits). Taps > 0 is the compile-time brush size (straight-line code: the heights of a group
-	// of taps are fetched, then their updates issued, with one wait per group); Taps == 0 loops
-	// over a.BrushSize taps one by one.
-	// The taps of one step are distinct cells, so fetching all their heights before updating
-	// leaves the drop's own result unchanged (the reference reads and updates them one by one).
+	// bits). Taps > 0 is the compile-time brush size, Taps == 0 a loop over a.BrushSize taps; the
+	// surface's `erode` does a step's erode update.
 	template <class Surface, unsigned int Taps>
 	__device__ inline void erodeDrop(const Surface& s, const ErosionArgs& a, const int2* const brush,
 		const unsigned int drop, float* const ledger, uint2* const drop_info) {
@@ -276,33 +433,7 @@ namespace SuperTerrainPlus::STPStage {
 				depositSteps++;
 			} else {
 				const float amount = fminf((cap - d.sediment) * a.Rain.ErodeSpeed, -deltaHeight);
-				if constexpr (Taps > 0u) {
-					// taps fetched per group; the group is a compile-time divisor of Taps
-					constexpr unsigned int G = Surface::groupOf(Taps);
-					float taken = 0.0f;
-#pragma unroll 1
-					for (unsigned int base = 0u; base < Taps; base += G) {
-						typename Surface::Raw h[G];
-#pragma unroll
-						for (unsigned int j = 0u; j < G; j++) {
-							h[j] = s.raw(tapIndex(mapIndex, brush[base + j].x));
-						}
-#pragma unroll
-						for (unsigned int j = 0u; j < G; j++) {
-							const int2 b = brush[base + j];
-							taken += s.take(tapIndex(mapIndex, b.x), h[j], amount * __int_as_float(b.y));
-						}
-					}
-					d.sediment += Surface::height(taken);
-				} else {
-					float taken = 0.0f;
-					for (unsigned int j = 0u; j < a.BrushSize; j++) {
-						const int2 b = brush[j];
-						const unsigned int ei = tapIndex(mapIndex, b.x);
-						taken += s.take(ei, s.raw(ei), amount * __int_as_float(b.y));
-					}
-					d.sediment += Surface::height(taken);
-				}
+				d.sediment += s.template erode<Taps>(mapIndex, amount, brush, a.BrushSize);
 				erodeSteps++;
 			}
 			advance(a, d, deltaHeight);
