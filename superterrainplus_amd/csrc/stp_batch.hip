@@ -109,6 +109,11 @@ namespace SuperTerrainPlus::STPBatch {
 	// histogram output for row-contiguous stores.
 
 	constexpr unsigned int VPrefetch = 16u; // halo rows fetched at once by the vertical pass
+	// Filter line segments: a binding whose lines number fewer than SegmentTarget (a few large
+	// chunks) splits them into up to MaxSegments segments of at least MinSegment outputs, so
+	// that about SegmentTarget segments keep the GPU busy; fuller bindings keep whole lines.
+	constexpr unsigned long long SegmentTarget = 131072ull;
+	constexpr unsigned int MaxSegments = 32u, MinSegment = 64u;
 	constexpr unsigned int HItemPrefetch = 8u; // columns fetched at once by the biome-indexed horizontal pass
 	// columns fetched at once by the register horizontal pass (its registers scale with MaxCap)
 	__host__ __device__ constexpr unsigned int horizontalPrefetch(const unsigned int maxCap) {
@@ -461,6 +466,9 @@ namespace SuperTerrainPlus::STPBatch {
 				this->Last[i] = -0x40000000;
 			}
 		}
+		// the state carried into segment `seg` of `line` (left there by segmentCarryKERNEL)
+		__device__ inline void resume(const unsigned int* const state, const Chunk& c, const unsigned int lines,
+			const unsigned int seg, const unsigned int line);
 		// biome `it` occurs at position `pos` with count `c`, as slot `slot` of that position
 		__device__ inline void enter(const unsigned int it, const int pos, const unsigned int c, const unsigned int slot,
 			const unsigned int radius) {
@@ -507,11 +515,210 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 	};
 
+	// ---- line segments
+	//
+	// A filter line of a large chunk can be split into segments of outputs that run in parallel.
+	// The state an unsplit line holds before position e is, per biome, its last occurrence and
+	// its latest run-start key (the window counts restart from the segment's own first window).
+	// A block of positions changes that state as its run summary says: the key of the biome's
+	// first occurrence in the block, the key of its latest run start after that first occurrence
+	// (the gap before it lies inside the block) and its last occurrence. So a summary pass over the
+	// blocks, a fold of each line's summaries in order (segmentCarryKERNEL) and a resumed filter
+	// per segment give the unsplit line's result exactly.
+
+	// outputs [x, y) of segment `seg` when `outputs` outputs are split into `segs`
+	__device__ inline uint2 segmentRange(const unsigned int outputs, const unsigned int segs, const unsigned int seg) {
+		const unsigned int len = (outputs + segs - 1u) / segs, a = min(seg * len, outputs);
+		return make_uint2(a, min(a + len, outputs));
+	}
+
+	// word of (segment, field, biome, line) in a chunk's part of the segment-state arena; fields
+	// 0, 1, 2 hold the summary (first key, inner key, last occurrence), then 0, 1 the carried state
+	// (last occurrence, key)
+	template <unsigned int MaxB>
+	__device__ inline size_t segmentWord(const Chunk& c, const unsigned int lines, const unsigned int seg,
+		const unsigned int field, const unsigned int b, const unsigned int line) {
+		return static_cast<size_t>(c.SegLine) * 3u * MaxB + ((static_cast<size_t>(seg) * 3u + field) * MaxB + b) * lines
+			+ line;
+	}
+
+	template <unsigned int MaxB>
+	struct RunSummary {
+		unsigned int First[MaxB], Inner[MaxB];
+		int Last[MaxB];
+		__device__ RunSummary() {
+#pragma unroll
+			for (unsigned int i = 0u; i < MaxB; i++) {
+				this->First[i] = NoRun;
+				this->Inner[i] = NoRun;
+				this->Last[i] = -0x40000000;
+			}
+		}
+		__device__ inline void occur(const unsigned int it, const int pos, const unsigned int slot, const unsigned int radius) {
+#pragma unroll
+			for (unsigned int i = 0u; i < MaxB; i++) {
+				if (i == it) {
+					const unsigned int key = static_cast<unsigned int>(pos) * 64u + slot;
+					if (this->First[i] == NoRun) {
+						this->First[i] = key;
+					} else if (this->Last[i] < pos - 2 * static_cast<int>(radius) - 1) {
+						this->Inner[i] = key;
+					}
+					this->Last[i] = pos;
+				}
+			}
+		}
+		__device__ inline void store(unsigned int* const state, const Chunk& c, const unsigned int lines,
+			const unsigned int seg, const unsigned int line) const {
+#pragma unroll
+			for (unsigned int i = 0u; i < MaxB; i++) {
+				state[segmentWord<MaxB>(c, lines, seg, 0u, i, line)] = this->First[i];
+				state[segmentWord<MaxB>(c, lines, seg, 1u, i, line)] = this->Inner[i];
+				state[segmentWord<MaxB>(c, lines, seg, 2u, i, line)] = static_cast<unsigned int>(this->Last[i]);
+			}
+		}
+	};
+
+	template <unsigned int MaxB>
+	__device__ inline void ItemLine<MaxB>::resume(const unsigned int* const state, const Chunk& c,
+		const unsigned int lines, const unsigned int seg, const unsigned int line) {
+#pragma unroll
+		for (unsigned int i = 0u; i < MaxB; i++) {
+			this->Last[i] = static_cast<int>(state[segmentWord<MaxB>(c, lines, seg, 0u, i, line)]);
+			this->Key[i] = state[segmentWord<MaxB>(c, lines, seg, 1u, i, line)];
+		}
+	}
+
+	// Folds every line's segment summaries in order and leaves in each segment's slot the state
+	// carried into it. `extra` = 2r for the vertical pass (Side + 2r lines), 0 for the horizontal.
+	// The first segment's line blocks do the work; the other blocks of the table return at once.
+	template <unsigned int MaxB>
+	__global__ void __launch_bounds__(ShfLineBlock) segmentCarryKERNEL(const uint2* const tiles, const Chunk* const chunks,
+		const unsigned int radius, const unsigned int extra, unsigned int* const state) {
+		const uint2 t = tiles[blockIdx.x];
+		const Chunk c = chunks[t.x];
+		const unsigned int lines = c.Side + extra, nb = (lines + ShfLineBlock - 1u) / ShfLineBlock;
+		const unsigned int line = t.y * ShfLineBlock + threadIdx.x;
+		if (t.y >= nb || c.Segs < 2u || line >= lines) {
+			return;
+		}
+		int last[MaxB];
+		unsigned int key[MaxB];
+#pragma unroll
+		for (unsigned int i = 0u; i < MaxB; i++) {
+			last[i] = -0x40000000;
+			key[i] = NoRun;
+		}
+		for (unsigned int g = 0u; g < c.Segs; g++) {
+#pragma unroll
+			for (unsigned int i = 0u; i < MaxB; i++) {
+				const size_t w0 = segmentWord<MaxB>(c, lines, g, 0u, i, line), w1 = segmentWord<MaxB>(c, lines, g, 1u, i, line);
+				unsigned int first = NoRun, inner = NoRun, slast = 0u;
+				if (g + 1u < c.Segs) {
+					first = state[w0];
+					inner = state[w1];
+					slast = state[segmentWord<MaxB>(c, lines, g, 2u, i, line)];
+				}
+				state[w0] = static_cast<unsigned int>(last[i]);
+				state[w1] = key[i];
+				if (first != NoRun) {
+					if (inner != NoRun) {
+						key[i] = inner;
+					} else if (last[i] < static_cast<int>(first >> 6) - 2 * static_cast<int>(radius) - 1) {
+						key[i] = first;
+					}
+					last[i] = static_cast<int>(slast);
+				}
+			}
+		}
+	}
+
+	// run summaries of the vertical pass's segments (all but each line's last)
+	template <unsigned int MaxB>
+	__global__ void __launch_bounds__(ShfLineBlock) verticalSummaryKERNEL(const uint2* const tiles,
+		const Chunk* const chunks, const unsigned int radius, const unsigned short* const biome,
+		unsigned int* const state) {
+		const uint2 t = tiles[blockIdx.x];
+		const Chunk c = chunks[t.x];
+		const unsigned int side = c.Side, ncols = side + 2u * radius, nb = (ncols + ShfLineBlock - 1u) / ShfLineBlock;
+		const unsigned int seg = t.y / nb, col = (t.y % nb) * ShfLineBlock + threadIdx.x;
+		if (seg + 1u >= c.Segs) {
+			return;
+		}
+		const uint2 yr = segmentRange(side, c.Segs, seg);
+		const bool active = col < ncols;
+		const unsigned short* const colc = biome + c.Halo + min(col, ncols - 1u);
+		RunSummary<MaxB> sum;
+		for (unsigned int y0 = yr.x; y0 < yr.y; y0 += VPrefetch) {
+			unsigned int v[VPrefetch];
+#pragma unroll
+			for (unsigned int i = 0u; i < VPrefetch; i++) {
+				v[i] = colc[static_cast<size_t>(min(y0 + i, yr.y - 1u)) * ncols];
+			}
+#pragma unroll
+			for (unsigned int i = 0u; i < VPrefetch; i++) {
+				if (active && y0 + i < yr.y) {
+					sum.occur(v[i], static_cast<int>(y0 + i), 0u, radius);
+				}
+			}
+		}
+		if (active) {
+			sum.store(state, c, ncols, seg, col);
+		}
+	}
+
+	// run summaries of the horizontal pass's segments (all but each line's last)
+	template <unsigned int MaxB>
+	__global__ void __launch_bounds__(ShfLineBlock) horizontalSummaryKERNEL(const uint2* const tiles,
+		const Chunk* const chunks, const unsigned int radius, const unsigned int vcap, const unsigned int* const vbin,
+		const unsigned char* const vcount, unsigned int* const state) {
+		constexpr unsigned int P = HItemPrefetch;
+		const uint2 t = tiles[blockIdx.x];
+		const Chunk c = chunks[t.x];
+		const unsigned int side = c.Side, nb = (side + ShfLineBlock - 1u) / ShfLineBlock;
+		const unsigned int seg = t.y / nb, y = (t.y % nb) * ShfLineBlock + threadIdx.x;
+		if (seg + 1u >= c.Segs) {
+			return;
+		}
+		const uint2 xr = segmentRange(side, c.Segs, seg);
+		const bool active = y < side;
+		const unsigned int yc = min(y, side - 1u);
+		const unsigned int* const inBin = vbin + c.Line * vcap;
+		const unsigned char* const inN = vcount + c.Line;
+		RunSummary<MaxB> sum;
+		for (unsigned int x0 = xr.x; x0 < xr.y; x0 += P) {
+			unsigned int n[P], b[P][MaxB];
+#pragma unroll
+			for (unsigned int j = 0u; j < P; j++) {
+				const size_t col = min(x0 + j, xr.y - 1u);
+				n[j] = inN[col * side + yc];
+#pragma unroll
+				for (unsigned int k = 0u; k < MaxB; k++) {
+					b[j][k] = k < vcap ? inBin[(col * vcap + k) * side + yc] : 0u;
+				}
+			}
+#pragma unroll
+			for (unsigned int j = 0u; j < P; j++) {
+				if (active && x0 + j < xr.y) {
+#pragma unroll
+					for (unsigned int k = 0u; k < MaxB; k++) {
+						if (k < n[j]) {
+							sum.occur(b[j][k] & 0xFFFFu, static_cast<int>(x0 + j), k, radius);
+						}
+					}
+				}
+			}
+		}
+		if (active) {
+			sum.store(state, c, side, seg, y);
+		}
+	}
+
 	// vertical pass, vcount[col * side + y], vbin[(col * vcap + k) * side + y] as verticalKERNEL
 	template <unsigned int MaxB>
 	__global__ void __launch_bounds__(ShfLineBlock) verticalItemKERNEL(const uint2* const tiles, const Chunk* const chunks,
 		const unsigned int radius, const unsigned int vcap, const unsigned short* const biome, unsigned int* const vbin,
-		unsigned char* const vcount, unsigned int* const overflow) {
+		unsigned char* const vcount, const unsigned int* const state, unsigned int* const overflow) {
 		extern __shared__ unsigned int lds[];
 		constexpr unsigned int R = ItemStageRows;
 		unsigned int* const stage = lds;                      // [k][row][65]
@@ -519,8 +726,11 @@ namespace SuperTerrainPlus::STPBatch {
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
 		const unsigned int lane = threadIdx.x, side = c.Side, ncols = side + 2u * radius;
-		const unsigned int col = t.y * ShfLineBlock + lane;
+		const unsigned int nb = (ncols + ShfLineBlock - 1u) / ShfLineBlock, seg = t.y / nb;
+		const unsigned int col = (t.y % nb) * ShfLineBlock + lane;
 		const bool active = col < ncols;
+		// this block's segment of the outputs; an unsplit line is the one segment [0, side)
+		const uint2 yr = segmentRange(side, c.Segs, seg);
 		const size_t lastRow = side + 2u * radius - 1u;
 		const unsigned short* const colc = biome + c.Halo + min(col, ncols - 1u);
 		const auto fetch = [&](unsigned int (&v)[VPrefetch], const long long row0) {
@@ -531,21 +741,24 @@ namespace SuperTerrainPlus::STPBatch {
 			}
 		};
 		ItemLine<MaxB> line;
+		if (seg > 0u && active) {
+			line.resume(state, c, ncols, seg, col);
+		}
 		bool over = false;
 		for (unsigned int j0 = 0u; j0 <= 2u * radius; j0 += VPrefetch) {
 			unsigned int v[VPrefetch];
-			fetch(v, j0);
+			fetch(v, static_cast<long long>(yr.x) + j0);
 #pragma unroll
 			for (unsigned int i = 0u; i < VPrefetch; i++) {
 				if (active && j0 + i <= 2u * radius) {
-					line.enter(v[i], static_cast<int>(j0 + i), 1u, 0u, radius);
+					line.enter(v[i], static_cast<int>(yr.x + j0 + i), 1u, 0u, radius);
 				}
 			}
 		}
 		unsigned int* const outBin = vbin + c.Line * vcap;
 		unsigned char* const outN = vcount + c.Line;
-		for (unsigned int y0 = 0u; y0 < side; y0 += R) {
-			const unsigned int rows = min(R, side - y0);
+		for (unsigned int y0 = yr.x; y0 < yr.y; y0 += R) {
+			const unsigned int rows = min(R, yr.y - y0);
 			unsigned int enter[VPrefetch], leave[VPrefetch];
 			fetch(enter, static_cast<long long>(y0) + 2 * static_cast<long long>(radius));
 			fetch(leave, static_cast<long long>(y0) - 1);
@@ -553,7 +766,7 @@ namespace SuperTerrainPlus::STPBatch {
 			for (unsigned int i = 0u; i < VPrefetch; i++) {
 				const unsigned int y = y0 + i;
 				if (active && i < rows) {
-					if (y > 0u) {
+					if (y > yr.x) {
 						line.enter(enter[i], static_cast<int>(y + 2u * radius), 1u, 0u, radius);
 						line.leave(leave[i], 1u);
 					}
@@ -573,7 +786,7 @@ namespace SuperTerrainPlus::STPBatch {
 			// consecutive lanes take consecutive rows of one column
 			for (unsigned int i = lane; i < ShfLineBlock * R; i += ShfLineBlock) {
 				const unsigned int cc = i / R, yy = i % R;
-				const unsigned int cg = t.y * ShfLineBlock + cc;
+				const unsigned int cg = (t.y % nb) * ShfLineBlock + cc;
 				if (cg < ncols && yy < rows) {
 					const unsigned int n = stageN[yy * 65u + cc];
 					const size_t y = static_cast<size_t>(y0) + yy;
@@ -598,7 +811,8 @@ namespace SuperTerrainPlus::STPBatch {
 	__global__ void __launch_bounds__(ShfLineBlock) horizontalItemKERNEL(const uint2* const tiles,
 		const Chunk* const chunks, const unsigned int radius, const unsigned int vcap, const unsigned int cap,
 		const unsigned int* const vbin, const unsigned char* const vcount, unsigned char* const bin_count,
-		unsigned short* const bin_item, float* const bin_weight, unsigned int* const overflow) {
+		unsigned short* const bin_item, float* const bin_weight, const unsigned int* const state,
+		unsigned int* const overflow) {
 		extern __shared__ unsigned int lds[];
 		constexpr unsigned int P = HItemPrefetch;
 		unsigned int* const outN = lds;                // [px][65]
@@ -607,8 +821,12 @@ namespace SuperTerrainPlus::STPBatch {
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
 		const unsigned int lane = threadIdx.x, side = c.Side, ncols = side + 2u * radius;
-		const unsigned int y0 = t.y * ShfLineBlock, y = y0 + lane;
+		const unsigned int nb = (side + ShfLineBlock - 1u) / ShfLineBlock, seg = t.y / nb;
+		const unsigned int y0 = (t.y % nb) * ShfLineBlock, y = y0 + lane;
 		const bool active = y < side;
+		// this block's segment of the outputs; an unsplit line is the one segment [0, side)
+		const uint2 xr = segmentRange(side, c.Segs, seg);
+		const int xBegin = static_cast<int>(xr.x), xEnd = static_cast<int>(xr.y);
 		const unsigned int* const inBin = vbin + c.Line * vcap;
 		const unsigned char* const inN = vcount + c.Line;
 		const unsigned int yc = min(y, side - 1u);
@@ -616,9 +834,12 @@ namespace SuperTerrainPlus::STPBatch {
 			return static_cast<size_t>(col < 0 ? 0 : (col >= static_cast<int>(ncols) ? static_cast<int>(ncols) - 1 : col));
 		};
 		ItemLine<MaxB> line;
+		if (seg > 0u && active) {
+			line.resume(state, c, side, seg, y);
+		}
 		bool over = false;
-		const int first = -2 * static_cast<int>(radius);
-		for (int s0 = first; s0 < static_cast<int>(side); s0 += static_cast<int>(P)) {
+		const int first = xBegin - 2 * static_cast<int>(radius);
+		for (int s0 = first; s0 < xEnd; s0 += static_cast<int>(P)) {
 			// column histograms of the block: entering s0 + 2r + j, leaving s0 - 1 + j
 			unsigned int eN[P], lN[P], eB[P][MaxB], lB[P][MaxB];
 #pragma unroll
@@ -633,7 +854,7 @@ namespace SuperTerrainPlus::STPBatch {
 					lB[j][k] = k < vcap ? inBin[(cl * vcap + k) * side + yc] : 0u;
 				}
 			}
-			const int sEnd = min(s0 + static_cast<int>(P), static_cast<int>(side));
+			const int sEnd = min(s0 + static_cast<int>(P), xEnd);
 #pragma unroll
 			for (unsigned int j = 0u; j < P; j++) {
 				const int s = s0 + static_cast<int>(j);
@@ -645,7 +866,7 @@ namespace SuperTerrainPlus::STPBatch {
 							line.enter(eB[j][k] & 0xFFFFu, ce, eB[j][k] >> 16, k, radius);
 						}
 					}
-					if (s >= 1) {
+					if (s >= xBegin + 1) {
 #pragma unroll
 						for (unsigned int k = 0u; k < MaxB; k++) {
 							if (k < lN[j]) {
@@ -653,7 +874,7 @@ namespace SuperTerrainPlus::STPBatch {
 							}
 						}
 					}
-					if (s >= 0) {
+					if (s >= xBegin) {
 						unsigned int item[MaxB], count[MaxB];
 						const unsigned int n = line.order(item, count);
 						over = over || n > cap;
@@ -676,7 +897,7 @@ namespace SuperTerrainPlus::STPBatch {
 			}
 			__syncthreads();
 			// row-contiguous stores of the staged pixels [xa, sEnd)
-			const int xa = max(s0, 0);
+			const int xa = max(s0, xBegin);
 			if (sEnd > xa) {
 				const unsigned int npx = static_cast<unsigned int>(sEnd - xa), jb = static_cast<unsigned int>(xa - s0);
 				const unsigned int rows = min(ShfLineBlock, side - y0), width = npx * cap;
@@ -1099,6 +1320,9 @@ void STPHeightfieldBatch::init(const STPEnvironment::STPRainDropSetting& rain) {
 	this->BiomeMap = STPSmartDeviceMemory::makeDevice<unsigned short>(halo);
 	this->VBin = STPSmartDeviceMemory::makeDevice<unsigned int>(line * this->VCap);
 	this->VCount = STPSmartDeviceMemory::makeDevice<unsigned char>(line);
+	// a split binding holds at most SegmentTarget slots of 3 words for each of at most 8 biomes
+	this->SegCapacity = SegmentTarget;
+	this->SegState = STPSmartDeviceMemory::makeDevice<unsigned int>(this->SegCapacity * 3ull * 8ull);
 	this->BinCount = STPSmartDeviceMemory::makeDevice<unsigned char>(pixel);
 	this->BinItem = STPSmartDeviceMemory::makeDevice<unsigned short>(pixel * binCap);
 	this->BinWeight = STPSmartDeviceMemory::makeDevice<float>(pixel * binCap);
@@ -1179,6 +1403,29 @@ void STPHeightfieldBatch::bind(const std::vector<STPBatchChunk>& chunks, const s
 	}
 	if (pixel > this->Capacity.Pixels || drops > std::max<unsigned long long>(this->Capacity.Drops, 1ull)) {
 		STP_THROW(STPInvalidArgument, "the chunk list exceeds the batch capacity");
+	}
+	// filter line segments: the largest power-of-two factor with lines * factor <= SegmentTarget,
+	// so the split binding fits the segment-state arena
+	unsigned long long lines = 0ull;
+	for (const Chunk& c : rec) {
+		lines += c.Side + 2ull * radius;
+	}
+	unsigned int factor = 1u;
+	while (this->itemPath() && factor < MaxSegments && lines * factor * 2ull <= SegmentTarget) {
+		factor *= 2u;
+	}
+	unsigned long long slots = 0ull;
+	this->Segmented = false;
+	for (Chunk& c : rec) {
+		c.Segs = factor > 1u ? std::max(1u, std::min(factor, c.Side / MinSegment)) : 1u;
+		c.SegLine = slots;
+		if (c.Segs > 1u) {
+			slots += static_cast<unsigned long long>(c.Segs) * (c.Side + 2u * radius);
+			this->Segmented = true;
+		}
+	}
+	if (slots > this->SegCapacity) {
+		STP_THROW(STPCapacityError, "filter line segments exceed the segment-state arena");
 	}
 	this->releaseGraphs();
 	this->Record = std::move(rec);
@@ -1305,6 +1552,16 @@ STPBatchTuning STPHeightfieldBatch::defaultTuning() const {
 	return STPBatchTuning {0u, ErosionBlock};
 }
 
+STPBatchTuning STPHeightfieldBatch::latencyTuning() const {
+	STPBatchTuning best = this->defaultTuning();
+	for (const unsigned int tile : ErosionTileCandidates) {
+		if (this->erosionLdsBytes(tile) != 0u) {
+			best = STPBatchTuning {tile, 1024u};
+		}
+	}
+	return best;
+}
+
 void STPHeightfieldBatch::setTuning(const STPBatchTuning& tuning) {
 	const unsigned int th = tuning.ErosionThreads;
 	if (th != 256u && th != 512u && th != 1024u) {
@@ -1361,8 +1618,9 @@ void STPHeightfieldBatch::buildTileTables(const hipStream_t stream) {
 			((c.Side + BlendX - 1u) / BlendX) * ((c.Side + BlendY * BlendPixel - 1u) / (BlendY * BlendPixel));
 		const unsigned int tx = tile ? (c.Side + tile - 1u) / tile : 0u;
 		const unsigned int bt = (w + BiomeSide - 1u) / BiomeSide;
-		const unsigned int n[5] = {bt * bt, (w + ShfLineBlock - 1u) / ShfLineBlock,
-			(c.Side + ShfLineBlock - 1u) / ShfLineBlock, blend, (c.Drops + ErosionBlock - 1u) / ErosionBlock};
+		// filter tables: line blocks x segments, segment-major
+		const unsigned int n[5] = {bt * bt, ((w + ShfLineBlock - 1u) / ShfLineBlock) * c.Segs,
+			((c.Side + ShfLineBlock - 1u) / ShfLineBlock) * c.Segs, blend, (c.Drops + ErosionBlock - 1u) / ErosionBlock};
 		for (int s = 0; s < 5; s++) {
 			range.Begin[s] = static_cast<unsigned int>(table[s].size());
 			for (unsigned int k = 0u; k < n[s]; k++) {
@@ -1511,12 +1769,19 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 			};
 			if (this->itemPath()) {
 				const unsigned int words = (this->VCap * ItemStageRows + ItemStageRows) * 65u;
+				unsigned int* const state = this->SegState.get();
+				const auto item = [&](auto summary, auto carry, auto filter) {
+					if (this->Segmented) {
+						summary<<<n, ShfLineBlock, 0, stream>>>(t, this->ChunkRecord, radius, this->BiomeMap.get(), state);
+						carry<<<n, ShfLineBlock, 0, stream>>>(t, this->ChunkRecord, radius, 2u * radius, state);
+					}
+					filter<<<n, ShfLineBlock, words * sizeof(unsigned int), stream>>>(t, this->ChunkRecord, radius,
+						this->VCap, this->BiomeMap.get(), this->VBin.get(), this->VCount.get(), state, this->Flag.get());
+				};
 				if (this->Setting.BiomeCount <= 4u) {
-					verticalItemKERNEL<4u><<<n, ShfLineBlock, words * sizeof(unsigned int), stream>>>(t, this->ChunkRecord,
-						radius, this->VCap, this->BiomeMap.get(), this->VBin.get(), this->VCount.get(), this->Flag.get());
+					item(verticalSummaryKERNEL<4u>, segmentCarryKERNEL<4u>, verticalItemKERNEL<4u>);
 				} else {
-					verticalItemKERNEL<8u><<<n, ShfLineBlock, words * sizeof(unsigned int), stream>>>(t, this->ChunkRecord,
-						radius, this->VCap, this->BiomeMap.get(), this->VBin.get(), this->VCount.get(), this->Flag.get());
+					item(verticalSummaryKERNEL<8u>, segmentCarryKERNEL<8u>, verticalItemKERNEL<8u>);
 				}
 				break;
 			}
@@ -1549,14 +1814,21 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 			};
 			if (this->itemPath()) {
 				const unsigned int words = HItemPrefetch * (1u + 2u * cap) * 65u;
+				unsigned int* const state = this->SegState.get();
+				const auto item = [&](auto summary, auto carry, auto filter) {
+					if (this->Segmented) {
+						summary<<<n, ShfLineBlock, 0, stream>>>(t, this->ChunkRecord, radius, this->VCap, this->VBin.get(),
+							this->VCount.get(), state);
+						carry<<<n, ShfLineBlock, 0, stream>>>(t, this->ChunkRecord, radius, 0u, state);
+					}
+					filter<<<n, ShfLineBlock, words * sizeof(unsigned int), stream>>>(t, this->ChunkRecord, radius,
+						this->VCap, cap, this->VBin.get(), this->VCount.get(), this->BinCount.get(), this->BinItem.get(),
+						this->BinWeight.get(), state, this->Flag.get());
+				};
 				if (this->Setting.BiomeCount <= 4u) {
-					horizontalItemKERNEL<4u><<<n, ShfLineBlock, words * sizeof(unsigned int), stream>>>(t, this->ChunkRecord,
-						radius, this->VCap, cap, this->VBin.get(), this->VCount.get(), this->BinCount.get(),
-						this->BinItem.get(), this->BinWeight.get(), this->Flag.get());
+					item(horizontalSummaryKERNEL<4u>, segmentCarryKERNEL<4u>, horizontalItemKERNEL<4u>);
 				} else {
-					horizontalItemKERNEL<8u><<<n, ShfLineBlock, words * sizeof(unsigned int), stream>>>(t, this->ChunkRecord,
-						radius, this->VCap, cap, this->VBin.get(), this->VCount.get(), this->BinCount.get(),
-						this->BinItem.get(), this->BinWeight.get(), this->Flag.get());
+					item(horizontalSummaryKERNEL<8u>, segmentCarryKERNEL<8u>, horizontalItemKERNEL<8u>);
 				}
 				break;
 			}

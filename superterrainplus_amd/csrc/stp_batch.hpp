@@ -58,6 +58,8 @@ namespace SuperTerrainPlus {
 			unsigned int StreamId;     // Philox counter word of this chunk's drops
 			unsigned int TileBase;     // first erosion tile of the chunk in the tile-bin arenas
 			unsigned int Step;         // world units per pixel
+			unsigned int Segs;         // filter lines are split into Segs segments of outputs
+			unsigned long long SegLine; // first slot of the chunk in the segment-state arena (Segs * (Side + 2r) slots)
 		};
 	}
 
@@ -118,6 +120,9 @@ namespace SuperTerrainPlus {
 		STPBatchTuning autotune(hipStream_t stream, unsigned int repeat = 2u);
 		// The shape a new batch starts with: the largest tile whose window fits in LDS.
 		STPBatchTuning defaultTuning() const;
+		// The smallest valid tile with 1024 threads: the most workgroups per colour batch, which
+		// shortens a small batch (a few chunks) at some cost in throughput.
+		STPBatchTuning latencyTuning() const;
 
 		unsigned long long pixels() const noexcept;
 		unsigned long long drops() const noexcept { return this->Drops; }
@@ -164,6 +169,11 @@ namespace SuperTerrainPlus {
 		STPSmartDeviceMemory::STPDevice<unsigned int> DropRank, DropBin, TileDrops, TileFirst;
 		STPSmartDeviceMemory::STPDevice<unsigned short> Fixed, BiomeMap;
 		STPSmartDeviceMemory::STPDevice<unsigned int> VBin;
+		// filter line segments: per slot (segment of a line) and biome, the run summary of the
+		// segment and then the state carried into it (3 words per biome, at most 8 biomes)
+		STPSmartDeviceMemory::STPDevice<unsigned int> SegState;
+		unsigned long long SegCapacity = 0ull; // slots
+		bool Segmented = false;                // some chunk of the binding has Segs > 1
 		STPSmartDeviceMemory::STPDevice<unsigned char> VCount, BinCount;
 		STPSmartDeviceMemory::STPDevice<unsigned short> BinItem;
 		STPSmartDeviceMemory::STPDevice<float> BinWeight, Ledger;

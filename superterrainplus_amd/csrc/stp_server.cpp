@@ -73,6 +73,9 @@ STPHeightfieldServer::STPHeightfieldServer(const STPEnvironment::STPSimplexNoise
 	this->Engines.resize(setting.Engines);
 	for (Engine& e : this->Engines) {
 		e.Batch = std::make_unique<STPHeightfieldBatch>(noise, biome, rain, batch, cap);
+		// a serving batch holds a few chunks: the smallest erosion tile spreads them over the most
+		// workgroups
+		e.Batch->setTuning(e.Batch->latencyTuning());
 		// default priority, non-blocking with respect to the null stream
 		STP_CHECK_HIP(hipStreamCreateWithFlags(&e.Stream, hipStreamNonBlocking));
 		STP_CHECK_HIP(hipEventCreateWithFlags(&e.Done, hipEventDisableTiming));

@@ -17,18 +17,18 @@ SIDES = [64, 128, 64, 96, 256, 64]
 ORIGINS = [(-1000, 500), (0, 0), (333, -777), (4096, 4096), (-2048, 1024), (64, 64)]
 
 
-def props():
-    return [(p[0], 8, p[2], p[3], p[4], p[5]) for p in W.biome_table(COUNT, SEED)]
+def props(count=COUNT):
+    return [(p[0], 8, p[2], p[3], p[4], p[5]) for p in W.biome_table(count, SEED)]
 
 
 class Batch:
-    def __init__(self, dpp, sides=SIDES, origins=ORIGINS, radius=RADIUS, cap=8):
+    def __init__(self, dpp, sides=SIDES, origins=ORIGINS, radius=RADIUS, cap=8, count=COUNT):
         self.sides, self.origins, self.radius = sides, origins, radius
-        setting = capi.BatchSetting(SEED, COUNT, CELL, radius, cap, dpp)
+        setting = capi.BatchSetting(SEED, count, CELL, radius, cap, dpp)
         chunks = (capi.BatchChunk * len(sides))(*[capi.BatchChunk(s, (C.c_int32 * 2)(*o))
                                                    for s, o in zip(sides, origins)])
         h = C.c_void_p()
-        capi.call("stp_batch_create", C.byref(NOISE), capi.biome_array(props()), COUNT,
+        capi.call("stp_batch_create", C.byref(NOISE), capi.biome_array(props(count)), count,
                   C.byref(W.erosion_setting(1)), C.byref(setting), chunks, len(sides), C.byref(h))
         self.h = h.value
 
@@ -66,12 +66,12 @@ class Batch:
             capi.lib().stp_batch_destroy(C.c_void_p(self.h))
 
 
-def oracle_chunk(oracle, i):
-    s, o, r = SIDES[i], ORIGINS[i], RADIUS
-    halo = oracle.voronoi_map(SEED, COUNT, CELL, (o[0] - r, o[1] - r), s + 2 * r, s + 2 * r)
+def oracle_chunk(oracle, i, sides=SIDES, origins=ORIGINS, count=COUNT):
+    s, o, r = sides[i], origins[i], RADIUS
+    halo = oracle.voronoi_map(SEED, count, CELL, (o[0] - r, o[1] - r), s + 2 * r, s + 2 * r)
     item, weight, off = oracle.shf_window(halo, (r, r), (s, s), r)
     perm, g32, _ = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
-    h = oracle.multi_biome(perm, g32, props(), item, weight, off, (s, s), (o[0] + s / 2, o[1] + s / 2))
+    h = oracle.multi_biome(perm, g32, props(count), item, weight, off, (s, s), (o[0] + s / 2, o[1] + s / 2))
     return halo, h
 
 
@@ -93,6 +93,25 @@ def test_batch_without_erosion_bit_exact(gpu, oracle, cap):
         assert (gf == oracle.format_heightmap(h)).all(), i
         for mode in (1, 2):
             assert all((x == y).all() for x, y in zip(results[mode][i], results[0][i])), (mode, i)
+
+
+@pytest.mark.parametrize("count", [4, 8])
+def test_batch_split_lines_bit_exact(gpu, oracle, count):
+    """A binding of few lines splits every filter line of its large chunks into segments (here
+    16 of 64 outputs for the 1024-pixel chunk, 5 uneven ones for the 333-pixel chunk), each
+    resumed from the state folded over the segments before it: bit-exact against the oracle, for
+    the 4- and the 8-biome kernels, in graph and plain mode."""
+    log(f"batch split lines count={count}")
+    sides, origins = [1024, 333], [(-5000, 700), (123, -456)]
+    b = Batch(0.0, sides=sides, origins=origins, count=count)
+    for mode in (0, 2):
+        b.run(mode)
+        for i in range(len(sides)):
+            gh, gf, gb = b.read(i)
+            halo, h = oracle_chunk(oracle, i, sides, origins, count)
+            assert (gb == halo).all(), (mode, i)
+            assert gh.tobytes() == h.tobytes(), (mode, i)
+            assert (gf == oracle.format_heightmap(h)).all(), (mode, i)
 
 
 def test_batch_graph_replays_are_stable(gpu):
