@@ -109,11 +109,13 @@ namespace SuperTerrainPlus::STPBatch {
 	// histogram output for row-contiguous stores.
 
 	constexpr unsigned int VPrefetch = 16u; // halo rows fetched at once by the vertical pass
-	// Filter line segments: a binding whose lines number fewer than SegmentTarget (a few large
-	// chunks) splits them into up to MaxSegments segments of at least MinSegment outputs, so
-	// that about SegmentTarget segments keep the GPU busy; fuller bindings keep whole lines.
+	// Filter line segments. Every line longer than LongSegment outputs is split into segments of
+	// at most LongSegment, so that long lines do not set the pass's duration; and a binding whose
+	// lines number fewer than SegmentTarget (a few large chunks) splits them further, into up to
+	// MaxSegments segments of at least MinSegment outputs, so that about SegmentTarget segments
+	// keep the GPU busy.
 	constexpr unsigned long long SegmentTarget = 131072ull;
-	constexpr unsigned int MaxSegments = 32u, MinSegment = 64u;
+	constexpr unsigned int MaxSegments = 32u, MinSegment = 64u, LongSegment = 256u;
 	constexpr unsigned int HItemPrefetch = 8u; // columns fetched at once by the biome-indexed horizontal pass
 	// columns fetched at once by the register horizontal pass (its registers scale with MaxCap)
 	__host__ __device__ constexpr unsigned int horizontalPrefetch(const unsigned int maxCap) {
@@ -1320,8 +1322,9 @@ void STPHeightfieldBatch::init(const STPEnvironment::STPRainDropSetting& rain) {
 	this->BiomeMap = STPSmartDeviceMemory::makeDevice<unsigned short>(halo);
 	this->VBin = STPSmartDeviceMemory::makeDevice<unsigned int>(line * this->VCap);
 	this->VCount = STPSmartDeviceMemory::makeDevice<unsigned char>(line);
-	// a split binding holds at most SegmentTarget slots of 3 words for each of at most 8 biomes
-	this->SegCapacity = SegmentTarget;
+	// slots of 3 words for each of at most 8 biomes: a few-line binding holds at most
+	// SegmentTarget slots, the long-line split at most sum (side / LongSegment + 1) (side + 2r)
+	this->SegCapacity = SegmentTarget + pixel / LongSegment + sideSum + sideSum * r2 / LongSegment + r2 * cap.Chunks + 2ull;
 	this->SegState = STPSmartDeviceMemory::makeDevice<unsigned int>(this->SegCapacity * 3ull * 8ull);
 	this->BinCount = STPSmartDeviceMemory::makeDevice<unsigned char>(pixel);
 	this->BinItem = STPSmartDeviceMemory::makeDevice<unsigned short>(pixel * binCap);
@@ -1417,7 +1420,8 @@ void STPHeightfieldBatch::bind(const std::vector<STPBatchChunk>& chunks, const s
 	unsigned long long slots = 0ull;
 	this->Segmented = false;
 	for (Chunk& c : rec) {
-		c.Segs = factor > 1u ? std::max(1u, std::min(factor, c.Side / MinSegment)) : 1u;
+		const unsigned int few = factor > 1u ? std::max(1u, std::min(factor, c.Side / MinSegment)) : 1u;
+		c.Segs = this->itemPath() ? std::max(few, (c.Side + LongSegment - 1u) / LongSegment) : 1u;
 		c.SegLine = slots;
 		if (c.Segs > 1u) {
 			slots += static_cast<unsigned long long>(c.Segs) * (c.Side + 2u * radius);
