@@ -115,7 +115,7 @@ namespace SuperTerrainPlus::STPBatch {
 	// MaxSegments segments of at least MinSegment outputs, so that about SegmentTarget segments
 	// keep the GPU busy.
 	constexpr unsigned long long SegmentTarget = 131072ull;
-	constexpr unsigned int MaxSegments = 32u, MinSegment = 64u, LongSegment = 256u;
+	constexpr unsigned int MaxSegments = 32u, MinSegment = 64u, LongSegment = 128u;
 	constexpr unsigned int HItemPrefetch = 8u; // columns fetched at once by the biome-indexed horizontal pass
 	// columns fetched at once by the register horizontal pass (its registers scale with MaxCap)
 	__host__ __device__ constexpr unsigned int horizontalPrefetch(const unsigned int maxCap) {
