@@ -31,21 +31,25 @@ namespace SuperTerrainPlus::STPBatch {
 	constexpr unsigned int BlendX = 64u, BlendY = 4u, BlendPixel = 2u;
 	constexpr unsigned int ErosionBlock = 256u;
 
-	// Synthetic biome map over a 16 x 16 tile of the halo map: the Voronoi sites of every grid cell
-	// the tile's samples look at (their cells +- 1) are drawn once into LDS, then each sample takes
-	// the nearest of its 3 x 3 cells in the order and with the tie rule of voronoiBiome. Distances
-	// are exact integers, so the result is the same as voronoiBiome per sample.
-	constexpr unsigned int BiomeSide = 16u, BiomeCells = 1024u;
+	// Synthetic biome map over a 32 x 32 tile of the halo map, BiomeTile threads with
+	// BiomeSide / BiomeRows samples each. The Voronoi sites of every grid cell the tile's samples
+	// look at (their cells +- 1) are drawn once into LDS, relative to the tile's first world sample
+	// and packed as two 16-bit integers; each sample then takes the nearest of its 3 x 3 cells in
+	// the order and with the tie rule of voronoiBiome. While the tile's extent in world units stays
+	// below 2^15 every offset fits 16 bits and every squared distance is an exact 32-bit integer, so
+	// the result is voronoiBiome's; larger extents (very coarse levels of detail, huge cells) call
+	// voronoiBiome per sample.
+	constexpr unsigned int BiomeSide = 32u, BiomeRows = BiomeTile / BiomeSide, BiomeCells = 1024u;
 	__global__ void __launch_bounds__(BiomeTile) biomeKERNEL(const uint2* const tiles, const Chunk* const chunks,
 		const unsigned long long seed, const unsigned int count, const unsigned int cell, const unsigned int radius,
 		unsigned short* const biome) {
-		__shared__ int siteX[BiomeCells], siteY[BiomeCells];
+		__shared__ unsigned int site[BiomeCells];
 		__shared__ unsigned short siteBiome[BiomeCells];
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
 		const unsigned int w = c.Side + 2u * radius, tilesX = (w + BiomeSide - 1u) / BiomeSide;
 		const unsigned int x0 = (t.y % tilesX) * BiomeSide, y0 = (t.y / tilesX) * BiomeSide;
-		const unsigned int x = x0 + threadIdx.x % BiomeSide, y = y0 + threadIdx.x / BiomeSide;
+		const unsigned int tx = threadIdx.x % BiomeSide, ty = threadIdx.x / BiomeSide;
 		// halo pixel (x, y) is chunk pixel (x - r, y - r), at world Origin + pixel * Step
 		const long long step = c.Step;
 		const long long wx0 = static_cast<long long>(c.OriginX) + (static_cast<long long>(x0) - radius) * step,
@@ -53,48 +57,65 @@ namespace SuperTerrainPlus::STPBatch {
 		const long long span = static_cast<long long>(BiomeSide - 1u) * step;
 		const long long cx0 = floorDiv(wx0, cell) - 1, cy0 = floorDiv(wy0, cell) - 1;
 		const long long nx = floorDiv(wx0 + span, cell) + 1 - cx0 + 1, ny = floorDiv(wy0 + span, cell) + 1 - cy0 + 1;
-		const long long wx = wx0 + static_cast<long long>(threadIdx.x % BiomeSide) * step,
-			wy = wy0 + static_cast<long long>(threadIdx.x / BiomeSide) * step;
-		const bool inside = x < w && y < w;
-		if (nx * ny > static_cast<long long>(BiomeCells) || cell > (1u << 20)) {
-			// very coarse levels of detail (or cells too large for 32-bit relative sites): sample by sample
-			if (inside) {
-				biome[c.Halo + static_cast<size_t>(y) * w + x] = voronoiBiome(seed, count, cell, wx, wy);
+		const unsigned int x = x0 + tx;
+		// every offset below lies within (-(2 cell + span), 2 cell + span)
+		if (nx * ny > static_cast<long long>(BiomeCells) || 2ll * cell + span >= 32768ll) {
+			for (unsigned int j = 0u; j < BiomeSide; j += BiomeRows) {
+				const unsigned int y = y0 + ty + j;
+				if (x < w && y < w) {
+					biome[c.Halo + static_cast<size_t>(y) * w + x] = voronoiBiome(seed, count, cell,
+						wx0 + static_cast<long long>(tx) * step, wy0 + static_cast<long long>(ty + j) * step);
+				}
 			}
 			return;
 		}
-		const unsigned int cells = static_cast<unsigned int>(nx * ny);
+		const unsigned int cells = static_cast<unsigned int>(nx * ny), cols = static_cast<unsigned int>(nx);
 		for (unsigned int k = threadIdx.x; k < cells; k += BiomeTile) {
-			const long long ncx = cx0 + static_cast<long long>(k % static_cast<unsigned int>(nx)),
-				ncy = cy0 + static_cast<long long>(k / static_cast<unsigned int>(nx));
+			const long long ncx = cx0 + static_cast<long long>(k % cols), ncy = cy0 + static_cast<long long>(k / cols);
 			const STPRandom::Philox4 r = STPRandom::philox4x32_10(static_cast<unsigned int>(ncx),
 				static_cast<unsigned int>(ncy), 0x564F524Fu, 0u, static_cast<unsigned int>(seed),
 				static_cast<unsigned int>(seed >> 32));
 			// site relative to the tile's first world sample
-			siteX[k] = static_cast<int>(ncx * cell + (r.v[0] % cell) - wx0);
-			siteY[k] = static_cast<int>(ncy * cell + (r.v[1] % cell) - wy0);
+			const int sx = static_cast<int>(ncx * cell + (r.v[0] % cell) - wx0),
+				sy = static_cast<int>(ncy * cell + (r.v[1] % cell) - wy0);
+			site[k] = (static_cast<unsigned int>(sx) & 0xFFFFu) | (static_cast<unsigned int>(sy) << 16);
 			siteBiome[k] = static_cast<unsigned short>(r.v[2] % count);
 		}
 		__syncthreads();
-		if (!inside) {
+		if (x >= w) {
 			return;
 		}
-		const long long px = wx - wx0, py = wy - wy0;
-		const long long lx = floorDiv(wx, cell) - cx0, ly = floorDiv(wy, cell) - cy0;
-		long long best = 0x7FFFFFFFFFFFFFFFll;
-		unsigned short b = 0u;
-		for (int dy = -1; dy <= 1; dy++) {
-			for (int dx = -1; dx <= 1; dx++) {
-				const unsigned int k = static_cast<unsigned int>((ly + dy) * nx + (lx + dx));
-				const long long ex = px - siteX[k], ey = py - siteY[k];
-				const long long d = ex * ex + ey * ey;
-				if (d < best) {
-					best = d;
-					b = siteBiome[k];
+		// 32-bit from here: sample offsets from the tile's first sample, and from cell (cx0, cy0)
+		const int istep = static_cast<int>(step);
+		const int px = static_cast<int>(tx) * istep;
+		const unsigned int lx = (static_cast<unsigned int>(wx0 - cx0 * cell) + static_cast<unsigned int>(px)) / cell;
+		const unsigned int ey0 = static_cast<unsigned int>(wy0 - cy0 * cell);
+		for (unsigned int j = 0u; j < BiomeSide; j += BiomeRows) {
+			const unsigned int y = y0 + ty + j;
+			if (y >= w) {
+				break;
+			}
+			const int py = static_cast<int>(ty + j) * istep;
+			const unsigned int ly = (ey0 + static_cast<unsigned int>(py)) / cell;
+			int best = 0x7FFFFFFF;
+			unsigned int bk = 0u;
+#pragma unroll
+			for (int dy = -1; dy <= 1; dy++) {
+#pragma unroll
+				for (int dx = -1; dx <= 1; dx++) {
+					const unsigned int k = (ly + dy) * cols + (lx + dx);
+					const unsigned int sk = site[k];
+					const int ex = px - static_cast<int>(static_cast<short>(sk & 0xFFFFu)),
+						ey = py - (static_cast<int>(sk) >> 16);
+					const int d = __mul24(ex, ex) + __mul24(ey, ey);
+					if (d < best) {
+						best = d;
+						bk = k;
+					}
 				}
 			}
+			biome[c.Halo + static_cast<size_t>(y) * w + x] = siteBiome[bk];
 		}
-		biome[c.Halo + static_cast<size_t>(y) * w + x] = b;
 	}
 
 	// ---- single histogram filter, batched form

@@ -114,6 +114,33 @@ def test_batch_split_lines_bit_exact(gpu, oracle, count):
             assert (gf == oracle.format_heightmap(h)).all(), (mode, i)
 
 
+def test_batch_biome_map_every_level_of_detail(gpu, oracle):
+    """The biome map stage draws each 32 x 32 tile's Voronoi sites once and measures 32-bit
+    distances while the tile spans less than 2^15 world units, and falls back to one Voronoi
+    search per sample beyond that (steps 1024 and 4096 here): the halo maps of every level are
+    bit-exact against the oracle."""
+    log("batch biome map levels of detail")
+    sides, steps = [64, 96, 64, 64], [1, 16, 1024, 4096]
+    origins = [(-777, 123), (5000, -9000), (1 << 20, -(1 << 21)), (-(1 << 24), 1 << 23)]
+    setting = capi.BatchSetting(SEED, COUNT, CELL, RADIUS, 8, 0.0)
+    chunks = (capi.BatchChunk * len(sides))(*[capi.BatchChunk(s, (C.c_int32 * 2)(*o), st)
+                                               for s, o, st in zip(sides, origins, steps)])
+    h = C.c_void_p()
+    capi.call("stp_batch_create", C.byref(NOISE), capi.biome_array(props()), COUNT,
+              C.byref(W.erosion_setting(1)), C.byref(setting), chunks, len(sides), C.byref(h))
+    try:
+        capi.call("stp_batch_run", h, 0, None)
+        capi.synchronize()
+        for i, (side, o, st) in enumerate(zip(sides, origins, steps)):
+            w = side + 2 * RADIUS
+            b = np.zeros((w, w), np.uint16)
+            capi.call("stp_batch_read_chunk", h, i, None, None, capi.ptr(b), None)
+            halo = oracle.voronoi_map(SEED, COUNT, CELL, (o[0] - RADIUS * st, o[1] - RADIUS * st), w, w, step=st)
+            assert (b == halo).all(), (side, st)
+    finally:
+        capi.lib().stp_batch_destroy(h)
+
+
 def test_batch_graph_replays_are_stable(gpu):
     """Replays of the captured graph with eager work in between give identical results."""
     log("batch graph replays")
