@@ -148,6 +148,9 @@ namespace SuperTerrainPlus::STPStage {
 		__device__ unsigned int stride() const { return this->W; }
 		__device__ float load(const unsigned int i) const { return loadHeight(this->Map + i); }
 		__device__ Raw raw(const unsigned int i) const { return loadHeight(this->Map + i); }
+		// height = unscaled(i) * Unit
+		static constexpr float Unit = 1.0f;
+		__device__ float unscaled(const unsigned int i) const { return loadHeight(this->Map + i); }
 		static constexpr float WeightScale = 1.0f;
 		__device__ float take(const unsigned int i, const Raw h, const float amount) const {
 			const float ds = (h < amount) ? h : amount;
@@ -158,6 +161,12 @@ namespace SuperTerrainPlus::STPStage {
 		__device__ void add(const unsigned int i, const float v) const { unsafeAtomicAdd(this->Map + i, v); }
 		// the chunk-level guards already keep every access inside the map
 		__device__ bool holds(unsigned int, unsigned int, unsigned int, unsigned int) const { return true; }
+		// a spawn may start; a step that left cell (cx, cy) for (px, py), inside the map's kill
+		// bounds, may erode at (cx, cy)
+		__device__ bool spawnGuard(const ErosionArgs& a, const DropState& d) const { return spawnInside(a, d); }
+		__device__ bool stepGuard(const ErosionArgs& a, const unsigned int cx, const unsigned int cy, float, float) const {
+			return cellInside(a, cx, cy);
+		}
 		// all taps of a step in one group: one HBM/L2 round trip
 		static constexpr unsigned int groupOf(const unsigned int taps) { return taps; }
 		template <unsigned int Taps>
@@ -288,11 +297,15 @@ namespace SuperTerrainPlus::STPStage {
 		int* Lds;
 		unsigned int X0, Y0, RW, RH;
 		__device__ unsigned int cell(const unsigned int x, const unsigned int y) const {
-			return (y - this->Y0) * this->RW + (x - this->X0);
+			// window sides are far below 2^24
+			return __umul24(y - this->Y0, this->RW) + (x - this->X0);
 		}
 		__device__ unsigned int stride() const { return this->RW; }
 		__device__ float load(const unsigned int i) const { return fromFixedPoint(this->Lds[i]); }
 		__device__ Raw raw(const unsigned int i) const { return this->Lds[i]; }
+		// height = unscaled(i) * Unit exactly (a power-of-two scale)
+		static constexpr float Unit = FixedStep;
+		__device__ float unscaled(const unsigned int i) const { return static_cast<float>(this->Lds[i]); }
 		// weights arrive pre-scaled to fixed point, so `amount` is already in fixed-point units
 		static constexpr float WeightScale = FixedScale;
 		__device__ float take(const unsigned int i, const Raw h, const float amount) const {
@@ -302,9 +315,30 @@ namespace SuperTerrainPlus::STPStage {
 			return static_cast<float>(dq);
 		}
 		__device__ static float height(const float taken) { return taken * FixedStep; }
-		__device__ void add(const unsigned int i, const float v) const { atomicAdd(this->Lds + i, toFixedPoint(v)); }
+		// deposits are >= 0: round half up
+		__device__ void add(const unsigned int i, const float v) const {
+			atomicAdd(this->Lds + i, static_cast<int>(fmaf(v, FixedScale, 0.5f)));
+		}
 		__device__ bool holds(const unsigned int x0, const unsigned int y0, const unsigned int x1, const unsigned int y1) const {
 			return x0 >= this->X0 && y0 >= this->Y0 && x1 < this->X0 + this->RW && y1 < this->Y0 + this->RH;
+		}
+		// The window must hold the footprint (brush reach and 2 x 2 cell) of the cell a drop will
+		// erode or deposit at. Checked when the drop arrives at the cell (spawn, or the step that
+		// moved it there), which also covers the 2 x 2 cell its gradient reads. Inside the window the
+		// footprint is inside the map, so cellInside needs no separate test; the window's halo is
+		// wide enough that in exact arithmetic no drop fails this test within its lifetime.
+		__device__ bool footprint(const ErosionArgs& a, const unsigned int x, const unsigned int y) const {
+			const unsigned int r = a.Rain.ErosionBrushRadius, lo = r - 1u, reach = r > 2u ? r - 1u : 1u;
+			return this->holds(x - lo, y - lo, x + reach, y + reach);
+		}
+		__device__ bool spawnGuard(const ErosionArgs& a, const DropState& d) const {
+			const unsigned int x = static_cast<unsigned int>(d.px), y = static_cast<unsigned int>(d.py);
+			// cellInside keeps x, y >= r - 1, so the footprint test cannot wrap
+			return spawnInside(a, d) && cellInside(a, x, y) && this->footprint(a, x, y);
+		}
+		// (px, py) passed the kill test, so it lies at least r inside the map
+		__device__ bool stepGuard(const ErosionArgs& a, unsigned int, unsigned int, const float px, const float py) const {
+			return this->footprint(a, static_cast<unsigned int>(px), static_cast<unsigned int>(py));
 		}
 		static constexpr unsigned int groupOf(const unsigned int taps) { return taps; }
 		// The 9/25/45-tap brushes go through shapedTake. Concurrent drops can see a cell's height
@@ -354,17 +388,25 @@ namespace SuperTerrainPlus::STPStage {
 		return Surface::height(taken);
 	}
 
-	// STPRainDrop.cu:23-51 on a surface.
+	// STPRainDrop.cu:23-51 on a surface, for the parallel modes, in an algebraically equal form
+	// with a third of the operations: with top = mix(hx, hy, cx) and bottom = mix(hz, hw, cx) the
+	// height is mix(top, bottom, cy), the x slope mix(hy - hx, hw - hz, cy), and the y slope
+	// (hz - hx)(1 - cx) + (hw - hy) cx is exactly bottom - top. Its roundings differ from the
+	// reference's expanded form by an ulp; parallel erosion is judged statistically, and the
+	// serialized kernel keeps the reference's expression (heightGradient).
+	// The heights are read unscaled and scaled once at the end (a power of two: exact).
 	template <class Surface>
 	__device__ inline float3 surfaceGradient(const Surface& s, const float px, const float py) {
 		const unsigned int rx = static_cast<unsigned int>(px), ry = static_cast<unsigned int>(py);
 		const float cx = px - static_cast<float>(rx), cy = py - static_cast<float>(ry);
 		const unsigned int b = s.cell(rx, ry), w = s.stride();
-		const float hx = s.load(b), hy = s.load(b + 1u), hz = s.load(b + w), hw = s.load(b + w + 1u);
+		const float hx = s.unscaled(b), hy = s.unscaled(b + 1u), hz = s.unscaled(b + w), hw = s.unscaled(b + w + 1u);
+		const float topSlope = hy - hx, bottomSlope = hw - hz;
+		const float top = fmaf(topSlope, cx, hx), bottom = fmaf(bottomSlope, cx, hz);
 		float3 r;
-		r.x = hx * (1 - cx) * (1 - cy) + hy * cx * (1 - cy) + hz * (1 - cx) * cy + hw * cx * cy;
-		r.y = (hy - hx) * (1 - cy) + (hw - hz) * cy;
-		r.z = (hz - hx) * (1 - cx) + (hw - hy) * cx;
+		r.x = fmaf(bottom - top, cy, top) * Surface::Unit;
+		r.y = fmaf(bottomSlope - topSlope, cy, topSlope) * Surface::Unit;
+		r.z = (bottom - top) * Surface::Unit;
 		return r;
 	}
 
@@ -377,10 +419,8 @@ namespace SuperTerrainPlus::STPStage {
 		DropState d = spawn(a, drop);
 		float lost = 0.0f;
 		unsigned int steps = 0u, erodeSteps = 0u, depositSteps = 0u, killed = 0u;
-		const unsigned int r = a.Rain.ErosionBrushRadius, lo = r - 1u, reach = r > 2u ? r - 1u : 1u;
-		if (!spawnInside(a, d)
-			|| !s.holds(static_cast<unsigned int>(d.px), static_cast<unsigned int>(d.py),
-				static_cast<unsigned int>(d.px) + 1u, static_cast<unsigned int>(d.py) + 1u)) {
+		const unsigned int r = a.Rain.ErosionBrushRadius;
+		if (!s.spawnGuard(a, d)) {
 			d.volume = 0.0f;
 			killed = 1u;
 		}
@@ -408,10 +448,7 @@ namespace SuperTerrainPlus::STPStage {
 			const bool inMap = !(d.dx == 0.0f && d.dy == 0.0f) && d.px >= radius
 				&& d.px < 1.0f * static_cast<float>(a.Total.x) - radius && d.py >= radius
 				&& d.py < 1.0f * static_cast<float>(a.Total.y) - radius;
-			if (!inMap || !cellInside(a, cellX, cellY)
-				|| !s.holds(cellX - lo, cellY - lo, cellX + reach, cellY + reach)
-				|| !s.holds(static_cast<unsigned int>(d.px), static_cast<unsigned int>(d.py),
-					static_cast<unsigned int>(d.px) + 1u, static_cast<unsigned int>(d.py) + 1u)) {
+			if (!inMap || !s.stepGuard(a, cellX, cellY, d.px, d.py)) {
 				lost = d.sediment;
 				d.volume = 0.0f;
 				d.sediment = 0.0f;
