@@ -108,8 +108,16 @@ namespace SuperTerrainPlus::STPStage {
 			&& static_cast<unsigned int>(d.py) + 1u <= a.Total.y - 1u;
 	}
 
+	// Fast = the hardware square root (1 ulp) of the parallel modes, which are judged
+	// statistically; the serialized kernel keeps the correctly rounded sqrtf.
+	template <bool Fast = false>
 	__device__ inline void advance(const ErosionArgs& a, DropState& d, const float deltaHeight) {
-		d.speed = sqrtf(fmaxf(0.0f, d.speed * d.speed + deltaHeight * a.Rain.Gravity));
+		const float v = fmaxf(0.0f, d.speed * d.speed + deltaHeight * a.Rain.Gravity);
+		if constexpr (Fast) {
+			d.speed = __builtin_amdgcn_sqrtf(v);
+		} else {
+			d.speed = sqrtf(v);
+		}
 		d.speed *= 1.0f - a.Rain.Friction;
 		d.volume *= (1.0f - a.Rain.EvaporateSpeed);
 	}
@@ -336,9 +344,12 @@ namespace SuperTerrainPlus::STPStage {
 			// cellInside keeps x, y >= r - 1, so the footprint test cannot wrap
 			return spawnInside(a, d) && cellInside(a, x, y) && this->footprint(a, x, y);
 		}
-		// (px, py) passed the kill test, so it lies at least r inside the map
+		// Evaluated beside the kill test, whose result decides: (px, py) may then be anything, so
+		// it is clamped before the integer conversion (NaN -> 0); when the kill test passes it
+		// lies at least r inside the map and the clamp changes nothing.
 		__device__ bool stepGuard(const ErosionArgs& a, unsigned int, unsigned int, const float px, const float py) const {
-			return this->footprint(a, static_cast<unsigned int>(px), static_cast<unsigned int>(py));
+			const float x = fminf(fmaxf(px, 0.0f), 16777216.0f), y = fminf(fmaxf(py, 0.0f), 16777216.0f);
+			return this->footprint(a, static_cast<unsigned int>(x), static_cast<unsigned int>(y));
 		}
 		static constexpr unsigned int groupOf(const unsigned int taps) { return taps; }
 		// The 9/25/45-tap brushes go through shapedTake. Concurrent drops can see a cell's height
@@ -445,10 +456,11 @@ namespace SuperTerrainPlus::STPStage {
 			d.px += d.dx;
 			d.py += d.dy;
 			const float radius = static_cast<float>(r);
-			const bool inMap = !(d.dx == 0.0f && d.dy == 0.0f) && d.px >= radius
-				&& d.px < 1.0f * static_cast<float>(a.Total.x) - radius && d.py >= radius
-				&& d.py < 1.0f * static_cast<float>(a.Total.y) - radius;
-			if (!inMap || !s.stepGuard(a, cellX, cellY, d.px, d.py)) {
+			// one branch: every term is evaluated (a NaN position fails the float tests)
+			const bool inMap = ((d.dx != 0.0f) | (d.dy != 0.0f)) & (d.px >= radius)
+				& (d.px < 1.0f * static_cast<float>(a.Total.x) - radius) & (d.py >= radius)
+				& (d.py < 1.0f * static_cast<float>(a.Total.y) - radius);
+			if (!(inMap & s.stepGuard(a, cellX, cellY, d.px, d.py))) {
 				lost = d.sediment;
 				d.volume = 0.0f;
 				d.sediment = 0.0f;
@@ -473,7 +485,7 @@ namespace SuperTerrainPlus::STPStage {
 				d.sediment += s.template erode<Taps>(mapIndex, amount, brush, a.BrushSize);
 				erodeSteps++;
 			}
-			advance(a, d, deltaHeight);
+			advance<true>(a, d, deltaHeight);
 		}
 		if (ledger) {
 			ledger[drop] = lost + d.sediment;
