@@ -693,7 +693,7 @@ namespace SuperTerrainPlus::STPBatch {
 	// run summaries of the horizontal pass's segments (all but each line's last)
 	template <unsigned int MaxB>
 	__global__ void __launch_bounds__(ShfLineBlock) horizontalSummaryKERNEL(const uint2* const tiles,
-		const Chunk* const chunks, const unsigned int radius, const unsigned int vcap, const unsigned int* const vbin,
+		const Chunk* const chunks, const unsigned int radius, const unsigned int vcap, const unsigned short* const vbin,
 		const unsigned char* const vcount, unsigned int* const state) {
 		constexpr unsigned int P = HItemPrefetch;
 		const uint2 t = tiles[blockIdx.x];
@@ -706,7 +706,7 @@ namespace SuperTerrainPlus::STPBatch {
 		const uint2 xr = segmentRange(side, c.Segs, seg);
 		const bool active = y < side;
 		const unsigned int yc = min(y, side - 1u);
-		const unsigned int* const inBin = vbin + c.Line * vcap;
+		const unsigned short* const inBin = vbin + c.Line * vcap;
 		const unsigned char* const inN = vcount + c.Line;
 		RunSummary<MaxB> sum;
 		for (unsigned int x0 = xr.x; x0 < xr.y; x0 += P) {
@@ -726,7 +726,7 @@ namespace SuperTerrainPlus::STPBatch {
 #pragma unroll
 					for (unsigned int k = 0u; k < MaxB; k++) {
 						if (k < n[j]) {
-							sum.occur(b[j][k] & 0xFFFFu, static_cast<int>(x0 + j), k, radius);
+							sum.occur(b[j][k] & 0xFFu, static_cast<int>(x0 + j), k, radius);
 						}
 					}
 				}
@@ -740,7 +740,7 @@ namespace SuperTerrainPlus::STPBatch {
 	// vertical pass, vcount[col * side + y], vbin[(col * vcap + k) * side + y] as verticalKERNEL
 	template <unsigned int MaxB>
 	__global__ void __launch_bounds__(ShfLineBlock) verticalItemKERNEL(const uint2* const tiles, const Chunk* const chunks,
-		const unsigned int radius, const unsigned int vcap, const unsigned short* const biome, unsigned int* const vbin,
+		const unsigned int radius, const unsigned int vcap, const unsigned short* const biome, unsigned short* const vbin,
 		unsigned char* const vcount, const unsigned int* const state, unsigned int* const overflow) {
 		extern __shared__ unsigned int lds[];
 		constexpr unsigned int R = ItemStageRows;
@@ -778,7 +778,7 @@ namespace SuperTerrainPlus::STPBatch {
 				}
 			}
 		}
-		unsigned int* const outBin = vbin + c.Line * vcap;
+		unsigned short* const outBin = vbin + c.Line * vcap;
 		unsigned char* const outN = vcount + c.Line;
 		for (unsigned int y0 = yr.x; y0 < yr.y; y0 += R) {
 			const unsigned int rows = min(R, yr.y - y0);
@@ -800,7 +800,7 @@ namespace SuperTerrainPlus::STPBatch {
 #pragma unroll
 					for (unsigned int k = 0u; k < MaxB; k++) {
 						if (k < n && k < vcap) {
-							stage[(k * R + i) * 65u + lane] = item[k] | (count[k] << 16);
+							stage[(k * R + i) * 65u + lane] = item[k] | (count[k] << 8);
 						}
 					}
 				}
@@ -817,7 +817,7 @@ namespace SuperTerrainPlus::STPBatch {
 #pragma unroll
 					for (unsigned int k = 0u; k < MaxB; k++) {
 						if (k < n) {
-							outBin[(static_cast<size_t>(cg) * vcap + k) * side + y] = stage[(k * R + yy) * 65u + cc];
+							outBin[(static_cast<size_t>(cg) * vcap + k) * side + y] = static_cast<unsigned short>(stage[(k * R + yy) * 65u + cc]);
 						}
 					}
 				}
@@ -833,7 +833,7 @@ namespace SuperTerrainPlus::STPBatch {
 	template <unsigned int MaxB>
 	__global__ void __launch_bounds__(ShfLineBlock) horizontalItemKERNEL(const uint2* const tiles,
 		const Chunk* const chunks, const unsigned int radius, const unsigned int vcap, const unsigned int cap,
-		const unsigned int* const vbin, const unsigned char* const vcount, unsigned char* const bin_count,
+		const unsigned short* const vbin, const unsigned char* const vcount, unsigned char* const bin_count,
 		unsigned short* const bin_item, float* const bin_weight, const unsigned int* const state,
 		unsigned int* const overflow) {
 		extern __shared__ unsigned int lds[];
@@ -850,7 +850,7 @@ namespace SuperTerrainPlus::STPBatch {
 		// this block's segment of the outputs; an unsplit line is the one segment [0, side)
 		const uint2 xr = segmentRange(side, c.Segs, seg);
 		const int xBegin = static_cast<int>(xr.x), xEnd = static_cast<int>(xr.y);
-		const unsigned int* const inBin = vbin + c.Line * vcap;
+		const unsigned short* const inBin = vbin + c.Line * vcap;
 		const unsigned char* const inN = vcount + c.Line;
 		const unsigned int yc = min(y, side - 1u);
 		const auto colIndex = [&](const int col) {
@@ -886,14 +886,14 @@ namespace SuperTerrainPlus::STPBatch {
 #pragma unroll
 					for (unsigned int k = 0u; k < MaxB; k++) {
 						if (k < eN[j]) {
-							line.enter(eB[j][k] & 0xFFFFu, ce, eB[j][k] >> 16, k, radius);
+							line.enter(eB[j][k] & 0xFFu, ce, eB[j][k] >> 8, k, radius);
 						}
 					}
 					if (s >= xBegin + 1) {
 #pragma unroll
 						for (unsigned int k = 0u; k < MaxB; k++) {
 							if (k < lN[j]) {
-								line.leave(lB[j][k] & 0xFFFFu, lB[j][k] >> 16);
+								line.leave(lB[j][k] & 0xFFu, lB[j][k] >> 8);
 							}
 						}
 					}
@@ -931,11 +931,14 @@ namespace SuperTerrainPlus::STPBatch {
 					const unsigned int r = r0 + lr;
 					if (lr < rowsPerPass && r < rows && lq < width) {
 						const size_t p = static_cast<size_t>(y0 + r) * side + static_cast<size_t>(xa) + px;
-						const unsigned int at = ((jb + px) * cap + k) * 65u + r;
-						bin_item[(c.Pixel + p) * cap + k] = static_cast<unsigned short>(outItem[at]);
-						bin_weight[(c.Pixel + p) * cap + k] = outW[at];
+						const unsigned int at = ((jb + px) * cap + k) * 65u + r, n = outN[(jb + px) * 65u + r];
+						// slots past the pixel's bin count are never read: not stored
+						if (k < n) {
+							bin_item[(c.Pixel + p) * cap + k] = static_cast<unsigned short>(outItem[at]);
+							bin_weight[(c.Pixel + p) * cap + k] = outW[at];
+						}
 						if (k == 0u) {
-							bin_count[c.Pixel + p] = static_cast<unsigned char>(outN[(jb + px) * 65u + r]);
+							bin_count[c.Pixel + p] = static_cast<unsigned char>(n);
 						}
 					}
 				}
@@ -1534,8 +1537,9 @@ namespace {
 } // namespace
 
 bool STPHeightfieldBatch::itemPath() const noexcept {
-	// biome ids come from the synthetic biome map, so they are < BiomeCount
-	return this->Setting.BiomeCount <= 8u && this->Setting.BinCapacity <= 8u;
+	// biome ids come from the synthetic biome map, so they are < BiomeCount; a column count is at
+	// most 2r + 1, which the 16-bit slots hold in 8 bits
+	return this->Setting.BiomeCount <= 8u && this->Setting.BinCapacity <= 8u && this->Setting.Radius <= 127u;
 }
 
 ErosionArgs STPHeightfieldBatch::erosionArgs() const {
@@ -1795,13 +1799,15 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 			if (this->itemPath()) {
 				const unsigned int words = (this->VCap * ItemStageRows + ItemStageRows) * 65u;
 				unsigned int* const state = this->SegState.get();
+				// the item path's column histograms are 16-bit slots (item | count << 8)
+				unsigned short* const vbin = reinterpret_cast<unsigned short*>(this->VBin.get());
 				const auto item = [&](auto summary, auto carry, auto filter) {
 					if (this->Segmented) {
 						summary<<<n, ShfLineBlock, 0, stream>>>(t, this->ChunkRecord, radius, this->BiomeMap.get(), state);
 						carry<<<n, ShfLineBlock, 0, stream>>>(t, this->ChunkRecord, radius, 2u * radius, state);
 					}
 					filter<<<n, ShfLineBlock, words * sizeof(unsigned int), stream>>>(t, this->ChunkRecord, radius,
-						this->VCap, this->BiomeMap.get(), this->VBin.get(), this->VCount.get(), state, this->Flag.get());
+						this->VCap, this->BiomeMap.get(), vbin, this->VCount.get(), state, this->Flag.get());
 				};
 				if (this->Setting.BiomeCount <= 4u) {
 					item(verticalSummaryKERNEL<4u>, segmentCarryKERNEL<4u>, verticalItemKERNEL<4u>);
@@ -1840,14 +1846,15 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 			if (this->itemPath()) {
 				const unsigned int words = HItemPrefetch * (1u + 2u * cap) * 65u;
 				unsigned int* const state = this->SegState.get();
+				const unsigned short* const vbin = reinterpret_cast<const unsigned short*>(this->VBin.get());
 				const auto item = [&](auto summary, auto carry, auto filter) {
 					if (this->Segmented) {
-						summary<<<n, ShfLineBlock, 0, stream>>>(t, this->ChunkRecord, radius, this->VCap, this->VBin.get(),
+						summary<<<n, ShfLineBlock, 0, stream>>>(t, this->ChunkRecord, radius, this->VCap, vbin,
 							this->VCount.get(), state);
 						carry<<<n, ShfLineBlock, 0, stream>>>(t, this->ChunkRecord, radius, 0u, state);
 					}
 					filter<<<n, ShfLineBlock, words * sizeof(unsigned int), stream>>>(t, this->ChunkRecord, radius,
-						this->VCap, cap, this->VBin.get(), this->VCount.get(), this->BinCount.get(), this->BinItem.get(),
+						this->VCap, cap, vbin, this->VCount.get(), this->BinCount.get(), this->BinItem.get(),
 						this->BinWeight.get(), state, this->Flag.get());
 				};
 				if (this->Setting.BiomeCount <= 4u) {
