@@ -829,6 +829,9 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 	}
 
+	// LDS words per staged pixel of horizontalItemKERNEL
+	__host__ __device__ constexpr unsigned int horizontalItemWords(const unsigned int cap) { return 2u + (cap + 1u) / 2u; }
+
 	// horizontal pass, dense output as horizontalKERNEL
 	template <unsigned int MaxB>
 	__global__ void __launch_bounds__(ShfLineBlock) horizontalItemKERNEL(const uint2* const tiles,
@@ -838,9 +841,10 @@ namespace SuperTerrainPlus::STPBatch {
 		unsigned int* const overflow) {
 		extern __shared__ unsigned int lds[];
 		constexpr unsigned int P = HItemPrefetch;
-		unsigned int* const outN = lds;                // [px][65]
-		unsigned int* const outItem = outN + P * 65u;  // [px][k][65]
-		float* const outW = reinterpret_cast<float*>(outItem + P * cap * 65u);
+		// staged pixels, [px][word][65]: word 0 = bin count | items (3 bits each) << 4, word 1 = the
+		// weight factor 1 / sum, words 2.. = the counts, two 16-bit halves per word
+		const unsigned int words = horizontalItemWords(cap);
+		unsigned int* const stage = lds;
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
 		const unsigned int lane = threadIdx.x, side = c.Side, ncols = side + 2u * radius;
@@ -907,12 +911,21 @@ namespace SuperTerrainPlus::STPBatch {
 							sum += k < n ? count[k] : 0u;
 						}
 						const float norm = 1.0f / static_cast<float>(sum);
-						outN[j * 65u + lane] = min(n, cap);
+						unsigned int head = min(n, cap), packed[(MaxB + 1u) / 2u] = {};
 #pragma unroll
 						for (unsigned int k = 0u; k < MaxB; k++) {
 							if (k < n && k < cap) {
-								outItem[(j * cap + k) * 65u + lane] = item[k];
-								outW[(j * cap + k) * 65u + lane] = static_cast<float>(count[k]) * norm;
+								head |= item[k] << (4u + 3u * k);
+								packed[k / 2u] |= count[k] << (16u * (k & 1u));
+							}
+						}
+						unsigned int* const at = stage + j * words * 65u + lane;
+						at[0] = head;
+						at[65] = __float_as_uint(norm);
+#pragma unroll
+						for (unsigned int w = 0u; w < (MaxB + 1u) / 2u; w++) {
+							if (2u * w < cap) {
+								at[(2u + w) * 65u] = packed[w];
 							}
 						}
 					}
@@ -923,22 +936,58 @@ namespace SuperTerrainPlus::STPBatch {
 			const int xa = max(s0, xBegin);
 			if (sEnd > xa) {
 				const unsigned int npx = static_cast<unsigned int>(sEnd - xa), jb = static_cast<unsigned int>(xa - s0);
-				const unsigned int rows = min(ShfLineBlock, side - y0), width = npx * cap;
-				// lane -> (pixel, slot) of one row; rows advance by the lanes that fit
-				const unsigned int perRow = width, rowsPerPass = max(ShfLineBlock / perRow, 1u);
-				const unsigned int lr = lane / perRow, lq = lane - lr * perRow, px = lq / cap, k = lq - px * cap;
-				for (unsigned int r0 = 0u; r0 < rows; r0 += rowsPerPass) {
-					const unsigned int r = r0 + lr;
-					if (lr < rowsPerPass && r < rows && lq < width) {
-						const size_t p = static_cast<size_t>(y0 + r) * side + static_cast<size_t>(xa) + px;
-						const unsigned int at = ((jb + px) * cap + k) * 65u + r, n = outN[(jb + px) * 65u + r];
-						// slots past the pixel's bin count are never read: not stored
-						if (k < n) {
-							bin_item[(c.Pixel + p) * cap + k] = static_cast<unsigned short>(outItem[at]);
-							bin_weight[(c.Pixel + p) * cap + k] = outW[at];
-						}
-						if (k == 0u) {
+				const unsigned int rows = min(ShfLineBlock, side - y0);
+				if (cap == 4u || cap == 8u) {
+					// lane -> (row, pixel): one pixel's slots in 8- and 16-byte stores; slots past the
+					// bin count hold zeros (they are never read)
+					constexpr unsigned int RowsPerPass = ShfLineBlock / P;
+					const unsigned int px = lane % P, lr = lane / P;
+					for (unsigned int r0 = 0u; r0 < rows; r0 += RowsPerPass) {
+						const unsigned int r = r0 + lr;
+						if (r < rows && px < npx) {
+							const size_t p = static_cast<size_t>(y0 + r) * side + static_cast<size_t>(xa) + px;
+							const unsigned int* const at = stage + (jb + px) * words * 65u + r;
+							const unsigned int head = at[0], n = head & 15u;
+							const float norm = __uint_as_float(at[65]);
 							bin_count[c.Pixel + p] = static_cast<unsigned char>(n);
+							for (unsigned int q = 0u; q < cap; q += 4u) {
+								const unsigned int lo = at[(2u + q / 2u) * 65u], hi = at[(3u + q / 2u) * 65u];
+								const unsigned int cnt[4] = {lo & 0xFFFFu, lo >> 16, hi & 0xFFFFu, hi >> 16};
+								unsigned short it[4];
+								float w[4];
+#pragma unroll
+								for (unsigned int k = 0u; k < 4u; k++) {
+									const bool used = q + k < n;
+									it[k] = used ? static_cast<unsigned short>((head >> (4u + 3u * (q + k))) & 7u) : 0u;
+									// the same product as the reference's count * (1 / sum)
+									w[k] = used ? static_cast<float>(cnt[k]) * norm : 0.0f;
+								}
+								*reinterpret_cast<ushort4*>(bin_item + (c.Pixel + p) * cap + q) = make_ushort4(it[0], it[1], it[2], it[3]);
+								*reinterpret_cast<float4*>(bin_weight + (c.Pixel + p) * cap + q) = make_float4(w[0], w[1], w[2], w[3]);
+							}
+						}
+					}
+				} else {
+					const unsigned int width = npx * cap;
+					// lane -> (pixel, slot) of one row; rows advance by the lanes that fit
+					const unsigned int perRow = width, rowsPerPass = max(ShfLineBlock / perRow, 1u);
+					const unsigned int lr = lane / perRow, lq = lane - lr * perRow, px = lq / cap, k = lq - px * cap;
+					for (unsigned int r0 = 0u; r0 < rows; r0 += rowsPerPass) {
+						const unsigned int r = r0 + lr;
+						if (lr < rowsPerPass && r < rows && lq < width) {
+							const size_t p = static_cast<size_t>(y0 + r) * side + static_cast<size_t>(xa) + px;
+							const unsigned int* const at = stage + (jb + px) * words * 65u + r;
+							const unsigned int head = at[0], n = head & 15u;
+							// slots past the pixel's bin count are never read: not stored
+							if (k < n) {
+								const unsigned int count = (at[(2u + k / 2u) * 65u] >> (16u * (k & 1u))) & 0xFFFFu;
+								bin_item[(c.Pixel + p) * cap + k] = static_cast<unsigned short>((head >> (4u + 3u * k)) & 7u);
+								// the same product as the reference's count * (1 / sum)
+								bin_weight[(c.Pixel + p) * cap + k] = static_cast<float>(count) * __uint_as_float(at[65]);
+							}
+							if (k == 0u) {
+								bin_count[c.Pixel + p] = static_cast<unsigned char>(n);
+							}
 						}
 					}
 				}
@@ -1844,7 +1893,7 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 					this->BinCount.get(), this->BinItem.get(), this->BinWeight.get(), this->Flag.get());
 			};
 			if (this->itemPath()) {
-				const unsigned int words = HItemPrefetch * (1u + 2u * cap) * 65u;
+				const unsigned int words = HItemPrefetch * horizontalItemWords(cap) * 65u;
 				unsigned int* const state = this->SegState.get();
 				const unsigned short* const vbin = reinterpret_cast<const unsigned short*>(this->VBin.get());
 				const auto item = [&](auto summary, auto carry, auto filter) {
