@@ -438,6 +438,45 @@ int stp_server_memory(stp_heightfield_server* server, uint64_t* engine_bytes, ui
 int stp_server_replay(stp_heightfield_server* server, const stp_chunk_request* h_requests, const double* h_arrival_s,
 	uint32_t count, double deadline_ms, double* h_latency_ms, stp_server_stats* out);
 
+/* ------------------------------------------------------------------ world pipeline
+ * The reference world pipeline's generation chain (STPWorldPipeline.cpp: requestChunk :640-663,
+ * erodeHeightmap :485-559, generateHeightmap :437-476, scheduleChunk :339-404) on resident device
+ * state. A requested chunk is complete once it has been eroded as the centre of its
+ * ErosionNeighbour x ErosionNeighbour neighbourhood (drops spawn in the centre chunk, the erosion
+ * runs free-slip over the merged neighbourhood and writes every chunk of it); that needs the
+ * heightmap of every chunk of the neighbourhood first. Centres whose neighbourhoods overlap are
+ * eroded in different batches. Chunk (X, Y) starts at world pixel (X, Y) * MapSize. */
+typedef struct {
+	uint32_t MapSize;          /* chunk side in pixels (STPChunkSetting::MapSize) */
+	uint32_t ErosionNeighbour; /* odd, <= 9 (STPChunkSetting::ErosionNearestNeighbour) */
+	uint32_t GenerateChunks;   /* heightmaps generated per batch */
+	uint32_t ErodeChunks;      /* neighbourhoods eroded per batch */
+} stp_world_setting;
+
+typedef struct stp_heightfield_world stp_heightfield_world;
+
+/* batch->DropsPerPixel gives the drops of one erosion: MapSize^2 * DropsPerPixel (the reference's
+ * RainDropCount); raindrop->RainDropCount is ignored. */
+int stp_world_create(const stp_simplex_noise_setting* noise, const stp_biome_property* h_table, uint32_t biome_count,
+	const stp_raindrop_setting* raindrop, const stp_batch_setting* batch, const stp_world_setting* setting,
+	stp_heightfield_world** out);
+int stp_world_destroy(stp_heightfield_world* world);
+/* Makes every requested chunk (h_chunks: count (X, Y) pairs) complete; writes up to `capacity`
+ * chunks whose maps changed (the union of the eroded neighbourhoods, the reference's
+ * RequireHeightmapReload) to h_changed as (X, Y) pairs and their number to *changed_count.
+ * Synchronous. */
+int stp_world_request(stp_heightfield_world* world, const int32_t* h_chunks, uint32_t count, int32_t* h_changed,
+	uint32_t capacity, uint32_t* changed_count);
+/* Copies a chunk with a map (either pointer may be NULL); STP_ERROR_INVALID_ARGUMENT if it has none. */
+int stp_world_read(stp_heightfield_world* world, int32_t x, int32_t y, float* h_height, uint16_t* h_fixed);
+/* 0 no map, 1 heightmap generated, 2 complete */
+int stp_world_state(stp_heightfield_world* world, int32_t x, int32_t y, uint32_t* state);
+/* The conflict-free schedule the world uses (host only): h_batch[i] = batch of centre i, first
+ * fit in the given order; centres in one batch have disjoint neighbourhoods. */
+int stp_world_schedule(const int32_t* h_centres, uint32_t count, uint32_t neighbour, uint32_t* h_batch);
+/* Philox stream word of the drops of the neighbourhood centred at chunk (X, Y). */
+int stp_world_stream_word(int32_t x, int32_t y, uint32_t* word);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,11 @@ class BatchChunk(C.Structure):
     _fields_ = [("Side", C.c_uint32), ("Origin", C.c_int32 * 2), ("Step", C.c_uint32)]
 
 
+class WorldSetting(C.Structure):
+    _fields_ = [("MapSize", C.c_uint32), ("ErosionNeighbour", C.c_uint32), ("GenerateChunks", C.c_uint32),
+                ("ErodeChunks", C.c_uint32)]
+
+
 class ServerSetting(C.Structure):
     _fields_ = [("MemoryBudget", C.c_uint64), ("BatchPixels", C.c_uint64), ("BatchChunks", C.c_uint32),
                 ("MaxSide", C.c_uint32), ("Engines", C.c_uint32), ("ChunkWorldSize", C.c_uint32)]
@@ -191,6 +196,13 @@ _SIGNATURES = {
     "stp_batch_set_tuning": (_I, [_P, _P]),
     "stp_batch_get_tuning": (_I, [_P, _P]),
     "stp_batch_autotune": (_I, [_P, _U32, _P, _P]),
+    "stp_world_create": (_I, [_P, _P, _U32, _P, _P, _P, _P]),
+    "stp_world_destroy": (_I, [_P]),
+    "stp_world_request": (_I, [_P, _P, _U32, _P, _U32, _P]),
+    "stp_world_read": (_I, [_P, C.c_int32, C.c_int32, _P, _P]),
+    "stp_world_state": (_I, [_P, C.c_int32, C.c_int32, _P]),
+    "stp_world_schedule": (_I, [_P, _U32, _U32, _P]),
+    "stp_world_stream_word": (_I, [C.c_int32, C.c_int32, _P]),
 }
 
 _lib = None

@@ -1031,6 +1031,18 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 	}
 
+	// the erosion arguments of chunk c: a map of Side / NN per chunk, merged NN x NN (Side), drops
+	// spawned in the centre map
+	__device__ inline ErosionArgs chunkArgs(ErosionArgs a, const Chunk& c) {
+		const unsigned int map = c.Side / c.NN;
+		a.MapSize = make_uint2(map, map);
+		a.NN = make_uint2(c.NN, c.NN);
+		a.Total = make_uint2(c.Side, c.Side);
+		a.StreamId = c.StreamId;
+		a.Rain.RainDropCount = c.Drops;
+		return a;
+	}
+
 	__global__ void __launch_bounds__(ErosionBlock) erosionKERNEL(const uint2* const tiles, const Chunk* const chunks,
 		ErosionArgs args, const int2* const brush_offset, const float* const brush_weight, float* const height,
 		float* const ledger, uint2* const drop_info) {
@@ -1046,12 +1058,7 @@ namespace SuperTerrainPlus::STPBatch {
 		if (drop >= c.Drops) {
 			return;
 		}
-		args.MapSize = make_uint2(c.Side, c.Side);
-		args.NN = make_uint2(1u, 1u);
-		args.Total = args.MapSize;
-		args.StreamId = c.StreamId;
-		args.Rain.RainDropCount = c.Drops;
-		erodeDropAnyBrush(GlobalSurface {height + c.Pixel, c.Side}, args, brush, drop, ledger + c.DropBase,
+		erodeDropAnyBrush(GlobalSurface {height + c.Pixel, c.Side}, chunkArgs(args, c), brush, drop, ledger + c.DropBase,
 			drop_info + c.DropBase);
 	}
 
@@ -1082,14 +1089,6 @@ namespace SuperTerrainPlus::STPBatch {
 		return y * tx + x;
 	}
 
-	__device__ inline ErosionArgs chunkArgs(ErosionArgs a, const Chunk& c) {
-		a.MapSize = make_uint2(c.Side, c.Side);
-		a.NN = make_uint2(1u, 1u);
-		a.Total = a.MapSize;
-		a.StreamId = c.StreamId;
-		a.Rain.RainDropCount = c.Drops;
-		return a;
-	}
 
 	// drop -> rank among the drops of its spawn tile. The workgroup's drops (one chunk) are
 	// counted per tile in LDS first; one global atomic per tile then reserves the workgroup's
@@ -1193,6 +1192,11 @@ namespace SuperTerrainPlus::STPBatch {
 		__shared__ int2 brush[MaxTiledTaps];
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
+		const unsigned int g = c.TileBase + t.y, n = tile_drops[g];
+		if (n == 0u) {
+			// a tile without drops leaves its cells alone (tiles outside a neighbourhood's centre map)
+			return;
+		}
 		const unsigned int side = c.Side, tilesX = (side + tile - 1u) / tile;
 		const unsigned int x0 = (t.y % tilesX) * tile, y0 = (t.y / tilesX) * tile;
 		LocalSurface s;
@@ -1213,7 +1217,7 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 		__syncthreads();
 		const ErosionArgs a = chunkArgs(args, c);
-		const unsigned int g = c.TileBase + t.y, first = tile_first[g], n = tile_drops[g];
+		const unsigned int first = tile_first[g];
 		for (unsigned int k = threadIdx.x; k < n; k += Threads) {
 			erodeDropAnyBrush(s, a, brush, bin[first + k], ledger + c.DropBase, drop_info + c.DropBase);
 		}
@@ -1457,12 +1461,18 @@ void STPHeightfieldBatch::bind(const std::vector<STPBatchChunk>& chunks, const s
 		if ((step & (step - 1u)) != 0u || step > 65536u) {
 			STP_THROW(STPInvalidArgument, "chunk step must be a power of two up to 65536");
 		}
+		const unsigned int nn = in.Neighbour == 0u ? 1u : in.Neighbour;
+		if ((nn & 1u) == 0u || in.Side % nn != 0u || in.Side / nn < 2u * this->Rain.ErosionBrushRadius + 2u) {
+			STP_THROW(STPInvalidArgument, "a neighbourhood is odd, divides the side and keeps maps of at least 2r + 2");
+		}
 		Chunk c;
 		c.Side = in.Side;
 		c.OriginX = in.OriginX;
 		c.OriginY = in.OriginY;
-		const unsigned long long px = static_cast<unsigned long long>(in.Side) * in.Side;
-		c.Drops = static_cast<unsigned int>(static_cast<double>(px) * this->Setting.DropsPerPixel);
+		c.NN = nn;
+		const unsigned long long px = static_cast<unsigned long long>(in.Side) * in.Side,
+			spawnPx = static_cast<unsigned long long>(in.Side / nn) * (in.Side / nn);
+		c.Drops = static_cast<unsigned int>(static_cast<double>(spawnPx) * this->Setting.DropsPerPixel);
 		c.Pixel = pixel;
 		c.Halo = halo;
 		c.Line = line;
@@ -2100,6 +2110,16 @@ void STPHeightfieldBatch::run(const STPRunMode mode, const hipStream_t stream) {
 	}
 	default:
 		STP_THROW(STPInvalidArgument, "unknown batch run mode");
+	}
+}
+
+void STPHeightfieldBatch::runStages(const int first, const int last, const hipStream_t stream) {
+	if (first < 0 || last > StageCount || first >= last) {
+		STP_THROW(STPInvalidArgument, "stage range outside [0, 7)");
+	}
+	this->LastTimed = false;
+	for (int s = first; s < last; s++) {
+		this->enqueueStage(stream, s, 0u, static_cast<unsigned int>(this->Record.size()), true);
 	}
 }
 

@@ -25,6 +25,10 @@ namespace SuperTerrainPlus {
 		unsigned int Side; // square chunk, pixels per side
 		int OriginX, OriginY;
 		unsigned int Step = 1u; // world units per pixel (level of detail); 0 is read as 1
+		// Free-slip neighbourhood (odd): the chunk is the merged map of Neighbour x Neighbour chunks
+		// of side Side / Neighbour, and its drops spawn in the centre one, as the reference's
+		// erosion of a neighbourhood (STPHeightfieldKernel.cu:93-102). 1 = a chunk on its own.
+		unsigned int Neighbour = 1u;
 	};
 
 	// Arena sizes of a batch that is bound to different chunk lists over its lifetime.
@@ -58,6 +62,7 @@ namespace SuperTerrainPlus {
 			unsigned int StreamId;     // Philox counter word of this chunk's drops
 			unsigned int TileBase;     // first erosion tile of the chunk in the tile-bin arenas
 			unsigned int Step;         // world units per pixel
+			unsigned int NN;           // free-slip neighbourhood: Side = NN * map side, drops spawn in the centre map
 			unsigned int Segs;         // filter lines are split into Segs segments of outputs
 			unsigned long long SegLine; // first slot of the chunk in the segment-state arena (Segs * (Side + 2r) slots)
 		};
@@ -92,6 +97,10 @@ namespace SuperTerrainPlus {
 		// Enqueues copies of every bound chunk's outputs to height[i] / fixed[i] (device memory, one
 		// pointer per chunk) on `stream`, in one launch.
 		void copyOut(const std::vector<float*>& height, const std::vector<unsigned short*>& fixed, hipStream_t stream);
+		// Enqueues stages [first, last) of every chunk on `stream` without a graph (eager). A caller
+		// that writes the heights itself (heightOf) runs the erosion binning, erosion and format
+		// stages only (4, 7).
+		void runStages(int first, int last, hipStream_t stream);
 		STPHeightfieldBatch(const STPHeightfieldBatch&) = delete;
 		STPHeightfieldBatch& operator=(const STPHeightfieldBatch&) = delete;
 		~STPHeightfieldBatch();

@@ -2,6 +2,7 @@
 // thread-local message, and plain C structs into the C++ API types.
 #include "stp_batch.hpp"
 #include "stp_server.hpp"
+#include "stp_world.hpp"
 #include "stp_internal.hpp"
 
 #include <stp/STPDeviceRuntime.hpp>
@@ -159,6 +160,10 @@ struct stp_rtc_program {
 
 struct stp_rtc_module {
 	STPDeviceRuntimeProgram::STPSmartModule Module;
+};
+
+struct stp_heightfield_world {
+	std::unique_ptr<STPHeightfieldWorld> World;
 };
 
 struct stp_heightfield_server {
@@ -954,6 +959,88 @@ int stp_server_replay(stp_heightfield_server* const server, const stp_chunk_requ
 			out->MeanBatchChunks = st.MeanBatchChunks;
 			out->PeakResidentBytes = st.PeakResidentBytes;
 		}
+	});
+}
+
+int stp_world_create(const stp_simplex_noise_setting* const noise, const stp_biome_property* const h_table,
+	const uint32_t biome_count, const stp_raindrop_setting* const raindrop, const stp_batch_setting* const batch,
+	const stp_world_setting* const setting, stp_heightfield_world** const out) {
+	return guarded([&] {
+		require(h_table && batch && setting && out, "null argument");
+		std::vector<STPBiomeProperty> t;
+		for (uint32_t i = 0u; i < biome_count; i++) {
+			t.push_back(toBiome(h_table[i]));
+		}
+		STPBatchSetting bs;
+		bs.BiomeSeed = batch->BiomeSeed;
+		bs.BiomeCount = batch->BiomeCount;
+		bs.BiomeCell = batch->BiomeCell;
+		bs.Radius = batch->Radius;
+		bs.BinCapacity = batch->BinCapacity;
+		bs.DropsPerPixel = batch->DropsPerPixel;
+		const STPWorldSetting ws {setting->MapSize, setting->ErosionNeighbour, setting->GenerateChunks, setting->ErodeChunks};
+		auto w = std::make_unique<stp_heightfield_world>();
+		w->World = std::make_unique<STPHeightfieldWorld>(toNoise(noise), t, toRain(raindrop), bs, ws);
+		*out = w.release();
+	});
+}
+
+int stp_world_destroy(stp_heightfield_world* const world) {
+	return guarded([&] { delete world; });
+}
+
+int stp_world_request(stp_heightfield_world* const world, const int32_t* const h_chunks, const uint32_t count,
+	int32_t* const h_changed, const uint32_t capacity, uint32_t* const changed_count) {
+	return guarded([&] {
+		require(world && (count == 0u || h_chunks) && (capacity == 0u || h_changed), "null argument");
+		std::vector<int2> req(count);
+		for (uint32_t i = 0u; i < count; i++) {
+			req[i] = make_int2(h_chunks[2u * i], h_chunks[2u * i + 1u]);
+		}
+		const std::vector<int2> changed = world->World->request(req);
+		const uint32_t n = static_cast<uint32_t>(std::min<size_t>(capacity, changed.size()));
+		for (uint32_t i = 0u; i < n; i++) {
+			h_changed[2u * i] = changed[i].x;
+			h_changed[2u * i + 1u] = changed[i].y;
+		}
+		if (changed_count) {
+			*changed_count = static_cast<uint32_t>(changed.size());
+		}
+	});
+}
+
+int stp_world_read(stp_heightfield_world* const world, const int32_t x, const int32_t y, float* const h_height,
+	uint16_t* const h_fixed) {
+	return guarded([&] {
+		require(world, "null world");
+		require(world->World->read(make_int2(x, y), h_height, h_fixed), "the chunk has no map");
+	});
+}
+
+int stp_world_state(stp_heightfield_world* const world, const int32_t x, const int32_t y, uint32_t* const state) {
+	return guarded([&] {
+		require(world && state, "null argument");
+		*state = static_cast<uint32_t>(world->World->state(make_int2(x, y)));
+	});
+}
+
+int stp_world_schedule(const int32_t* const h_centres, const uint32_t count, const uint32_t neighbour,
+	uint32_t* const h_batch) {
+	return guarded([&] {
+		require(count == 0u || (h_centres && h_batch), "null argument");
+		std::vector<int2> c(count);
+		for (uint32_t i = 0u; i < count; i++) {
+			c[i] = make_int2(h_centres[2u * i], h_centres[2u * i + 1u]);
+		}
+		const std::vector<unsigned int> b = STPHeightfieldWorld::schedule(c, neighbour);
+		std::copy(b.begin(), b.end(), h_batch);
+	});
+}
+
+int stp_world_stream_word(const int32_t x, const int32_t y, uint32_t* const word) {
+	return guarded([&] {
+		require(word, "null argument");
+		*word = STPHeightfieldWorld::streamWord(make_int2(x, y));
 	});
 }
 

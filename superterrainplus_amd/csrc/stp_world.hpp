@@ -1,0 +1,84 @@
+// The reference's world pipeline contract on resident device state: a requested chunk is complete
+// once it has been eroded as the centre of its free-slip erosion neighbourhood, which needs the
+// heightmap of every chunk of that neighbourhood first; centres whose neighbourhoods overlap are
+// never eroded at the same time.
+//
+// SuperTerrain+/SuperTerrain+/Private/World/STPWorldPipeline.cpp: requestChunk (:640-663) ->
+// erodeHeightmap (:485-559: heightmaps of the neighbourhoods, conflict-free schedule, erosion of
+// the centres in batches) -> generateHeightmap (:437-476) -> generateBiomemap (:411-430); the
+// schedule is scheduleChunk (:339-404). Here the biome map is procedural and sampled with its
+// filter halo inside the heightmap batch, the heightmaps of many chunks come from one batch, and
+// one batch erodes every centre of a conflict-free group at once, on merged neighbourhood maps.
+#pragma once
+#ifndef STP_AMD_WORLD_HPP
+#define STP_AMD_WORLD_HPP
+
+#include "stp_batch.hpp"
+
+#include <memory>
+#include <unordered_map>
+#include <vector>
+
+namespace SuperTerrainPlus {
+
+	struct STPWorldSetting {
+		unsigned int MapSize;          // chunk side in pixels (STPChunkSetting::MapSize)
+		unsigned int ErosionNeighbour; // odd: chunks per side of the erosion neighbourhood (STPChunkSetting::ErosionNearestNeighbour)
+		unsigned int GenerateChunks;   // heightmaps generated per batch
+		unsigned int ErodeChunks;      // neighbourhoods eroded per batch
+	};
+
+	class STPHeightfieldWorld {
+	public:
+		// chunk state: no map, heightmap generated (not yet eroded as a centre), complete
+		enum class STPChunkState : unsigned int { None = 0u, Heightmap = 1u, Complete = 2u };
+
+		STPHeightfieldWorld(const STPEnvironment::STPSimplexNoiseSetting& noise, const std::vector<STPBiomeProperty>& biome,
+			const STPEnvironment::STPRainDropSetting& rain, const STPBatchSetting& batch, const STPWorldSetting& setting);
+		STPHeightfieldWorld(const STPHeightfieldWorld&) = delete;
+		STPHeightfieldWorld& operator=(const STPHeightfieldWorld&) = delete;
+		~STPHeightfieldWorld();
+
+		// Makes every requested chunk complete and returns the chunks whose maps changed: the
+		// union of the eroded neighbourhoods, in first-seen order (the reference's
+		// RequireHeightmapReload). Synchronous.
+		std::vector<int2> request(const std::vector<int2>& chunks);
+		// Copies a resident chunk (heightmap or complete) to host memory; false if it has no map.
+		bool read(int2 chunk, float* height, unsigned short* fixed);
+		STPChunkState state(int2 chunk) const;
+		unsigned long long residentBytes() const noexcept { return this->Resident; }
+
+		// Conflict-free schedule: batch number of each centre, first fit in the given order; two
+		// centres share a batch only if their neighbourhoods (nn x nn chunks) are disjoint.
+		static std::vector<unsigned int> schedule(const std::vector<int2>& centres, unsigned int nn);
+		// Philox stream word of the drops of the neighbourhood centred at a chunk
+		static unsigned int streamWord(int2 chunk) noexcept;
+
+	private:
+		struct Key {
+			int X, Y;
+			bool operator==(const Key& k) const noexcept { return X == k.X && Y == k.Y; }
+		};
+		struct KeyHash {
+			size_t operator()(const Key& k) const noexcept;
+		};
+		struct Entry {
+			float* Height = nullptr;
+			unsigned short* Fixed = nullptr;
+			STPChunkState State = STPChunkState::None;
+		};
+
+		STPWorldSetting Setting;
+		std::unique_ptr<STPHeightfieldBatch> Generator, Eroder;
+		std::unordered_map<Key, Entry, KeyHash> Chunk;
+		hipStream_t Stream = nullptr;
+		hipMemPool_t Pool = nullptr;
+		unsigned long long Resident = 0ull;
+
+		void generate(const std::vector<Key>& chunks);
+		void erode(const std::vector<Key>& centres);
+	};
+
+} // namespace SuperTerrainPlus
+
+#endif // STP_AMD_WORLD_HPP
