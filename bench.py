@@ -143,6 +143,63 @@ def served(seconds_scale=1.0):
     return out
 
 
+def world(plain_frames=6):
+    """The world pipeline over a camera walk (workloads.WorldWalk): each frame's request makes the
+    render area complete; a warm-up walk elsewhere comes first. The same walk with one chunk per
+    batch (its first `plain_frames` frames) is the plain pattern."""
+    from superterrainplus_amd import capi, workloads as W
+    ww = W.world_walk()
+    cfg = ww.batch
+    noise = capi.SimplexNoiseSetting(cfg.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, cfg.engine)
+    props = [(p[0], cfg.fractal.octave, p[2], p[3], p[4], p[5]) for p in cfg.table()]
+    batch = capi.BatchSetting(cfg.biome_seed, cfg.biome_count, cfg.cell, cfg.radius, cfg.biome_count,
+                              cfg.drops_per_pixel)
+
+    def walk(generate, erode, frames):
+        h = C.c_void_p()
+        capi.call("stp_world_create", C.byref(noise), capi.biome_array(props), len(props),
+                  C.byref(W.erosion_setting(1, seed=cfg.erosion_seed)), C.byref(batch),
+                  C.byref(capi.WorldSetting(ww.map, ww.nn, generate, erode)), C.byref(h))
+        try:
+            def request(chunks):
+                flat = (C.c_int32 * (2 * len(chunks)))(*[v for c in chunks for v in c])
+                n = C.c_uint32()
+                capi.call("stp_world_request", h, flat, len(chunks), None, 0, C.byref(n))
+                return n.value
+            for f in ww.frames(origin=(5000, 5000))[:3]:
+                request(f)
+            done, lat, t_all = set(), [], time.perf_counter()
+            for f in frames:
+                new = [c for c in f if c not in done]
+                t0 = time.perf_counter()
+                request(f)
+                lat.append((time.perf_counter() - t0) * 1e3)
+                done.update(new)
+            wall = time.perf_counter() - t_all
+        finally:
+            capi.lib().stp_world_destroy(h)
+        return len(done), np.array(lat), wall
+
+    frames = ww.frames()
+    n, lat, wall = walk(ww.generate, ww.erode, frames)
+    pn, plat, pwall = walk(1, 1, frames[:plain_frames])
+    px = ww.map * ww.map
+    out = {"workload": f"camera walk {ww.steps} chunks (east, then north), render area {2 * ww.render + 1}^2 of "
+                       f"{ww.map}^2 chunks, {ww.nn}x{ww.nn} free-slip erosion neighbourhoods, "
+                       f"{int(px * cfg.drops_per_pixel)} drops per centre, batches of {ww.generate} heightmaps / "
+                       f"{ww.erode} neighbourhoods",
+           "frames": len(frames), "chunks_completed": n, "first_frame_ms": round(float(lat[0]), 2),
+           "frame_p50_ms": round(float(np.percentile(lat[1:], 50)), 2),
+           "frame_p99_ms": round(float(np.percentile(lat[1:], 99)), 2),
+           "mpixel_s": round(n * px / wall / 1e6, 1),
+           "plain_mpixel_s": round(pn * px / pwall / 1e6, 1),
+           "plain_frames": plain_frames}
+    out["batched_vs_plain"] = round(out["mpixel_s"] / out["plain_mpixel_s"], 2)
+    print(f"[bench] world: first frame {lat[0]:.1f} ms, frames p50 {out['frame_p50_ms']} p99 {out['frame_p99_ms']} ms, "
+          f"{out['mpixel_s']} Mpixel/s completed (plain {out['plain_mpixel_s']})", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +210,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default erosion launch shape")
     ap.add_argument("--no-served", action="store_true", help="leave out the camera-walk serving replays")
+    ap.add_argument("--no-world", action="store_true", help="leave out the world-pipeline camera walk")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -264,6 +322,8 @@ def main():
                         "Poisson arrivals, 16 ms deadline, whole path per chunk as the batched workload, "
                         "2 engines x 8 Mpx, 200 GB budget with LRU",
             "rates": served()}
+    if not args.no_world:
+        result["config"]["world"] = world()
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(cfg, sides, origins, props, args.cpu_seconds, threads)

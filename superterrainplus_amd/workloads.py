@@ -241,3 +241,34 @@ class Served:
 
 def served():
     return Served()
+
+
+@dataclass
+class WorldWalk:
+    """The world pipeline's request pattern at the free-slip configuration's scale: 512-pixel
+    chunks, 3 x 3 erosion neighbourhoods, 65536 drops per centre (0.25 per pixel of the centre
+    chunk, eroding the 1536^2 merged map), the batched workload's biome map and noise. A camera
+    walks `steps` chunks, east for the first half, then north; every frame requests the chunks of
+    the render area (radius 2: 5 x 5 chunks) around it, as the reference's pipeline does for its
+    render distance (STPWorldPipeline.cpp:640-663)."""
+    map: int = 512
+    nn: int = 3
+    render: int = 2
+    steps: int = 24
+    generate: int = 64
+    erode: int = 16
+    batch: Batched = field(default_factory=Batched)
+
+    def frames(self, origin=(0, 0)):
+        cam = list(origin)
+        out = []
+        for s in range(self.steps + 1):
+            out.append([(cam[0] + dx, cam[1] + dy) for dy in range(-self.render, self.render + 1)
+                        for dx in range(-self.render, self.render + 1)])
+            cam[0 if s < self.steps // 2 else 1] += 1
+        return out
+
+
+def world_walk():
+    return WorldWalk()
+
