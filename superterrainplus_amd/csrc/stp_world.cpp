@@ -125,6 +125,9 @@ STPHeightfieldWorld::STPHeightfieldWorld(const STPEnvironment::STPSimplexNoiseSe
 	ec.Chunks = setting.ErodeChunks;
 	ec.MaxSide = static_cast<unsigned int>(side);
 	this->Eroder = std::make_unique<STPHeightfieldBatch>(noise, biome, rain, batch, ec);
+	// a conflict-free group holds a few neighbourhoods, whose drops fill only the centre map's
+	// tiles: the smallest tile spreads them over the most workgroups
+	this->Eroder->setTuning(this->Eroder->latencyTuning());
 
 	STP_CHECK_HIP(hipStreamCreateWithFlags(&this->Stream, hipStreamNonBlocking));
 	int device = 0;
