@@ -438,6 +438,45 @@ int stp_server_memory(stp_heightfield_server* server, uint64_t* engine_bytes, ui
 int stp_server_replay(stp_heightfield_server* server, const stp_chunk_request* h_requests, const double* h_arrival_s,
 	uint32_t count, double deadline_ms, double* h_latency_ms, stp_server_stats* out);
 
+/* ------------------------------------------------------------------ splatmap
+ * The texture region of every pixel of the rendered area's chunks (STPWorldPipeline.cpp:671-707
+ * generateSplatmap running the demo script SuperDemo+/Script/STPSplatmapGenerator.cu:65-137; rule
+ * lookups STPTextureSplatRuleWrapper, SuperTerrain+/SuperAlgorithm+/Device/Private/STPTextureSplatRuleWrapper.cu:56-126;
+ * rule database STPTextureFactory.cpp:137-176). */
+typedef struct {
+	uint16_t Sample;    /* biome id */
+	float UpperBound;   /* altitudes up to this take Region */
+	uint32_t Region;
+} stp_altitude_rule;
+
+typedef struct {
+	uint16_t Sample;
+	float minGradient, maxGradient; /* the slope factor's range */
+	float LowerBound, UpperBound;   /* the altitude's range */
+	uint32_t Region;
+} stp_gradient_rule;
+
+/* A chunk of the rendered area (STPSplatGeneratorInformation::STPLocalChunkInformation,
+ * STPTextureInformation.hpp:120-128): its tile position in the area and its map offset. */
+typedef struct {
+	uint32_t LocalX, LocalY;
+	float OffsetX, OffsetY;
+} stp_splat_chunk;
+
+typedef struct stp_splat_database stp_splat_database;
+
+/* Builds the rule database in device memory: samples with rules ascending, altitude rules by
+ * (sample, upper bound), gradient rules by sample in the given order (STPTextureDatabase.cpp:357-411). */
+int stp_splat_database_create(const stp_altitude_rule* h_altitude, uint32_t altitude_count,
+	const stp_gradient_rule* h_gradient, uint32_t gradient_count, stp_splat_database** out);
+int stp_splat_database_destroy(stp_splat_database* database);
+/* d_biome (UINT16 biome ids) and d_height (UINT16 heights) cover the rendered area, area_x x area_y
+ * row-major; for every listed chunk, its map_x x map_y pixels get their region (255 = none) in
+ * d_splat. The noise's half dimension is map / 2. Asynchronous on `stream`. */
+int stp_splatmap_generate(const stp_permutation* perm, const stp_splat_database* database, const uint16_t* d_biome,
+	const uint16_t* d_height, uint8_t* d_splat, uint32_t area_x, uint32_t area_y, uint32_t map_x, uint32_t map_y,
+	const stp_splat_chunk* h_chunks, uint32_t count, void* stream);
+
 /* ------------------------------------------------------------------ world pipeline
  * The reference world pipeline's generation chain (STPWorldPipeline.cpp: requestChunk :640-663,
  * erodeHeightmap :485-559, generateHeightmap :437-476, scheduleChunk :339-404) on resident device

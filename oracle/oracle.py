@@ -109,6 +109,10 @@ def lib():
                                            C.POINTER(FractalDesc), C.c_uint32, C.c_uint32,
                                            C.POINTER(RainDropSetting), C.c_uint32, C.c_uint64,
                                            C.c_uint32]
+        L.orc_splatmap.restype = None
+        L.orc_splatmap.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                   C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                   C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32]
         _lib = L
     return _lib
 
@@ -284,3 +288,33 @@ def pipeline_threads(perm, grad, desc, chunk_count, side, setting, drops, seed, 
     return lib().orc_pipeline_threads(perm.ctypes.data, grad.ctypes.data, grad.size // 2,
                                       C.byref(desc), chunk_count, side, C.byref(setting), drops,
                                       seed, threads)
+
+
+class AltitudeRule(C.Structure):
+    _fields_ = [("Sample", C.c_uint16), ("UpperBound", C.c_float), ("Region", C.c_uint32)]
+
+
+class GradientRule(C.Structure):
+    _fields_ = [("Sample", C.c_uint16), ("minGradient", C.c_float), ("maxGradient", C.c_float),
+                ("LowerBound", C.c_float), ("UpperBound", C.c_float), ("Region", C.c_uint32)]
+
+
+class SplatChunk(C.Structure):
+    _fields_ = [("LocalX", C.c_uint32), ("LocalY", C.c_uint32), ("OffsetX", C.c_float), ("OffsetY", C.c_float)]
+
+
+def splatmap(perm, grad, altitude, gradient, biome, height, map_size, chunks, fill=0):
+    """The splatmap generator (SuperDemo+/Script/STPSplatmapGenerator.cu:65-137) over the rendered area:
+    `altitude` = [(sample, upper bound, region)], `gradient` = [(sample, min, max, lower, upper, region)],
+    `chunks` = [(local x, local y, offset x, offset y)]; pixels of unlisted chunks keep `fill`."""
+    b = np.ascontiguousarray(biome, np.uint16)
+    h = np.ascontiguousarray(height, np.uint16)
+    H, W = h.shape
+    out = np.full((H, W), fill, np.uint8)
+    a = (AltitudeRule * max(len(altitude), 1))(*[AltitudeRule(*r) for r in altitude])
+    g = (GradientRule * max(len(gradient), 1))(*[GradientRule(*r) for r in gradient])
+    c = (SplatChunk * max(len(chunks), 1))(*[SplatChunk(*r) for r in chunks])
+    lib().orc_splatmap(perm.ctypes.data, grad.ctypes.data, grad.size // 2, a, len(altitude), g, len(gradient),
+                       b.ctypes.data, h.ctypes.data, out.ctypes.data, W, H, map_size[0], map_size[1], c, len(chunks))
+    return out
+

@@ -872,3 +872,130 @@ double orc_pipeline_threads(const uint8_t* perm, const float* grad, const uint32
 }
 
 } // extern "C"
+
+// ------------------------------------------------------------------------------------
+// Splatmap
+// SuperDemo+/Script/STPSplatmapGenerator.cu:65-137 (constants :2-18), the rule lookups of
+// SuperTerrain+/SuperAlgorithm+/Device/Private/STPTextureSplatRuleWrapper.cu:56-126 and the rule
+// order of SuperTerrain+/SuperTerrain+/Private/World/Diversity/STPTextureDatabase.cpp:357-411.
+// The heightfield is read as CUDA's linear-filtered, clamped, normalised UINT16 texture at
+// unnormalised coordinates (STPWorldPipeline.cpp:682-696): texel centres at +0.5, value / 65535.
+// ------------------------------------------------------------------------------------
+struct orc_altitude_rule {
+	uint16_t Sample;
+	float UpperBound;
+	uint32_t Region;
+};
+struct orc_gradient_rule {
+	uint16_t Sample;
+	float minGradient, maxGradient, LowerBound, UpperBound;
+	uint32_t Region;
+};
+struct orc_splat_chunk {
+	uint32_t LocalX, LocalY;
+	float OffsetX, OffsetY;
+};
+
+static float splatTexel(const uint16_t* const m, const uint32_t W, const uint32_t H, int64_t x, int64_t y) {
+	x = std::min<int64_t>(std::max<int64_t>(x, 0), static_cast<int64_t>(W) - 1);
+	y = std::min<int64_t>(std::max<int64_t>(y, 0), static_cast<int64_t>(H) - 1);
+	return static_cast<float>(m[static_cast<size_t>(y) * W + static_cast<size_t>(x)]) / 65535.0f;
+}
+
+static float splatLinear(const uint16_t* const m, const uint32_t W, const uint32_t H, const float u, const float v) {
+	const float xb = u - 0.5f, yb = v - 0.5f, fx = std::floor(xb), fy = std::floor(yb);
+	const float a = xb - fx, b = yb - fy;
+	const int64_t i = static_cast<int64_t>(fx), j = static_cast<int64_t>(fy);
+	const float t00 = splatTexel(m, W, H, i, j), t10 = splatTexel(m, W, H, i + 1, j), t01 = splatTexel(m, W, H, i, j + 1),
+		t11 = splatTexel(m, W, H, i + 1, j + 1);
+	return (1.0f - a) * ((1.0f - b) * t00 + b * t01) + a * ((1.0f - b) * t10 + b * t11);
+}
+
+extern "C" {
+
+void orc_splatmap(const uint8_t* perm, const float* grad, const uint32_t n, const orc_altitude_rule* alt,
+	const uint32_t altCount, const orc_gradient_rule* gra, const uint32_t graCount, const uint16_t* biome,
+	const uint16_t* height, uint8_t* splat, const uint32_t W, const uint32_t H, const uint32_t mapX, const uint32_t mapY,
+	const orc_splat_chunk* chunks, const uint32_t count) {
+	constexpr float GradientBias = 5.5f, KernelRadius = 2.5f, NoiseScale = 91.5f, NoiseContribution = 0.15f,
+		HeightFactor = 875.5f;
+	constexpr uint32_t NoRegion = 0xFFFFFFFFu;
+	// the database order: altitude rules by (sample, upper bound), gradient rules by sample
+	std::vector<orc_altitude_rule> a(alt, alt + altCount);
+	std::vector<orc_gradient_rule> g(gra, gra + graCount);
+	std::stable_sort(a.begin(), a.end(), [](const orc_altitude_rule& p, const orc_altitude_rule& q) {
+		return p.Sample != q.Sample ? p.Sample < q.Sample : p.UpperBound < q.UpperBound;
+	});
+	std::stable_sort(g.begin(), g.end(), [](const orc_gradient_rule& p, const orc_gradient_rule& q) {
+		return p.Sample < q.Sample;
+	});
+	std::vector<uint16_t> samples;
+	for (const auto& r : a) {
+		samples.push_back(r.Sample);
+	}
+	for (const auto& r : g) {
+		samples.push_back(r.Sample);
+	}
+	std::sort(samples.begin(), samples.end());
+	samples.erase(std::unique(samples.begin(), samples.end()), samples.end());
+
+	const Permutation<float> p {perm, grad, n};
+	Fractal<float> f;
+	f.Persistence = 0.7f;
+	f.Lacunarity = 1.88f;
+	f.Octave = 3u;
+	f.Scale = NoiseScale;
+	f.HalfX = static_cast<float>(mapX) / 2.0f;
+	f.HalfY = static_cast<float>(mapY) / 2.0f;
+	f.Amplitude = 1.0f;
+	f.Frequency = 1.0f;
+	const int kernel[4][2] = {{0, -1}, {-1, 0}, {1, 0}, {0, 1}};
+	for (uint32_t c = 0u; c < count; c++) {
+		f.OffsetX = chunks[c].OffsetX;
+		f.OffsetY = chunks[c].OffsetY;
+		for (uint32_t y = 0u; y < mapY; y++) {
+			for (uint32_t x = 0u; x < mapX; x++) {
+				const uint32_t sx = x + mapX * chunks[c].LocalX, sy = y + mapY * chunks[c].LocalY;
+				float cell[4];
+				for (int i = 0; i < 4; i++) {
+					cell[i] = splatLinear(height, W, H, static_cast<float>(sx) + static_cast<float>(kernel[i][0]) * KernelRadius,
+						static_cast<float>(sy) + static_cast<float>(kernel[i][1]) * KernelRadius) * HeightFactor;
+				}
+				const float g0 = cell[0] - cell[3], g1 = GradientBias, g2 = cell[1] - cell[2];
+				const float slope = 1.0f - g1 * (1.0f / std::sqrt(g0 * g0 + g1 * g1 + g2 * g2));
+				const float noise = simplex2DFractal<float>(p, 1.0f * static_cast<float>(x), 1.0f * static_cast<float>(y), f)
+					* NoiseContribution;
+				const uint16_t s = biome[static_cast<size_t>(sy) * W + sx];
+				const float h = saturate<float>(splatLinear(height, W, H, static_cast<float>(sx), static_cast<float>(sy)) + noise);
+				uint32_t region = NoRegion;
+				// the first sample with rules that is not less than s
+				size_t k = 0u;
+				while (k < samples.size() && samples[k] < s) {
+					k++;
+				}
+				if (k < samples.size()) {
+					const uint16_t t = samples[k];
+					// gradient rules first, in order; then the altitude rule with the smallest upper bound >= h
+					for (const auto& r : g) {
+						if (r.Sample == t && slope >= r.minGradient && slope <= r.maxGradient && h >= r.LowerBound
+							&& h <= r.UpperBound) {
+							region = r.Region;
+							break;
+						}
+					}
+					if (region == NoRegion) {
+						for (const auto& r : a) {
+							if (r.Sample == t && !(r.UpperBound < h)) {
+								region = r.Region;
+								break;
+							}
+						}
+					}
+				}
+				splat[static_cast<size_t>(sy) * W + sx] = static_cast<uint8_t>(region);
+			}
+		}
+	}
+}
+
+} // extern "C"

@@ -71,6 +71,19 @@ class BatchChunk(C.Structure):
     _fields_ = [("Side", C.c_uint32), ("Origin", C.c_int32 * 2), ("Step", C.c_uint32)]
 
 
+class AltitudeRule(C.Structure):
+    _fields_ = [("Sample", C.c_uint16), ("UpperBound", C.c_float), ("Region", C.c_uint32)]
+
+
+class GradientRule(C.Structure):
+    _fields_ = [("Sample", C.c_uint16), ("minGradient", C.c_float), ("maxGradient", C.c_float),
+                ("LowerBound", C.c_float), ("UpperBound", C.c_float), ("Region", C.c_uint32)]
+
+
+class SplatChunk(C.Structure):
+    _fields_ = [("LocalX", C.c_uint32), ("LocalY", C.c_uint32), ("OffsetX", C.c_float), ("OffsetY", C.c_float)]
+
+
 class WorldSetting(C.Structure):
     _fields_ = [("MapSize", C.c_uint32), ("ErosionNeighbour", C.c_uint32), ("GenerateChunks", C.c_uint32),
                 ("ErodeChunks", C.c_uint32)]
@@ -196,6 +209,9 @@ _SIGNATURES = {
     "stp_batch_set_tuning": (_I, [_P, _P]),
     "stp_batch_get_tuning": (_I, [_P, _P]),
     "stp_batch_autotune": (_I, [_P, _U32, _P, _P]),
+    "stp_splat_database_create": (_I, [_P, _U32, _P, _U32, _P]),
+    "stp_splat_database_destroy": (_I, [_P]),
+    "stp_splatmap_generate": (_I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, _U32, _P, _U32, _P]),
     "stp_world_create": (_I, [_P, _P, _U32, _P, _P, _P, _P]),
     "stp_world_destroy": (_I, [_P]),
     "stp_world_request": (_I, [_P, _P, _U32, _P, _U32, _P]),

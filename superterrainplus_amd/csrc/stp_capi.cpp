@@ -2,6 +2,7 @@
 // thread-local message, and plain C structs into the C++ API types.
 #include "stp_batch.hpp"
 #include "stp_server.hpp"
+#include "stp_splat.hpp"
 #include "stp_world.hpp"
 #include "stp_internal.hpp"
 
@@ -160,6 +161,10 @@ struct stp_rtc_program {
 
 struct stp_rtc_module {
 	STPDeviceRuntimeProgram::STPSmartModule Module;
+};
+
+struct stp_splat_database {
+	std::unique_ptr<STPSplatRuleTable> Table;
 };
 
 struct stp_heightfield_world {
@@ -959,6 +964,45 @@ int stp_server_replay(stp_heightfield_server* const server, const stp_chunk_requ
 			out->MeanBatchChunks = st.MeanBatchChunks;
 			out->PeakResidentBytes = st.PeakResidentBytes;
 		}
+	});
+}
+
+int stp_splat_database_create(const stp_altitude_rule* const h_altitude, const uint32_t altitude_count,
+	const stp_gradient_rule* const h_gradient, const uint32_t gradient_count, stp_splat_database** const out) {
+	return guarded([&] {
+		require(out && (altitude_count == 0u || h_altitude) && (gradient_count == 0u || h_gradient), "null argument");
+		std::vector<STPAltitudeRule> alt(altitude_count);
+		for (uint32_t i = 0u; i < altitude_count; i++) {
+			alt[i] = STPAltitudeRule {h_altitude[i].Sample, h_altitude[i].UpperBound, h_altitude[i].Region};
+		}
+		std::vector<STPGradientRule> gra(gradient_count);
+		for (uint32_t i = 0u; i < gradient_count; i++) {
+			const stp_gradient_rule& g = h_gradient[i];
+			gra[i] = STPGradientRule {g.Sample, g.minGradient, g.maxGradient, g.LowerBound, g.UpperBound, g.Region};
+		}
+		auto d = std::make_unique<stp_splat_database>();
+		d->Table = std::make_unique<STPSplatRuleTable>(alt, gra);
+		*out = d.release();
+	});
+}
+
+int stp_splat_database_destroy(stp_splat_database* const database) {
+	return guarded([&] { delete database; });
+}
+
+int stp_splatmap_generate(const stp_permutation* const perm, const stp_splat_database* const database,
+	const uint16_t* const d_biome, const uint16_t* const d_height, uint8_t* const d_splat, const uint32_t area_x,
+	const uint32_t area_y, const uint32_t map_x, const uint32_t map_y, const stp_splat_chunk* const h_chunks,
+	const uint32_t count, void* const stream) {
+	return guarded([&] {
+		require(perm && database && (count == 0u || h_chunks), "null argument");
+		std::vector<STPSplatChunk> c(count);
+		for (uint32_t i = 0u; i < count; i++) {
+			c[i] = STPSplatChunk {h_chunks[i].LocalX, h_chunks[i].LocalY, h_chunks[i].OffsetX, h_chunks[i].OffsetY};
+		}
+		STPHeightfieldKernel::splatmap(toPerm(perm), database->Table->view(), d_biome, d_height, d_splat,
+			make_uint2(area_x, area_y), make_uint2(map_x, map_y),
+			make_float2(static_cast<float>(map_x) / 2.0f, static_cast<float>(map_y) / 2.0f), c, toStream(stream));
 	});
 }
 
