@@ -200,6 +200,52 @@ def world(plain_frames=6):
     return out
 
 
+def splat(reps=10):
+    """The splatmap generator over a rendered area of 5 x 5 chunks of 512^2 (6.6 Mpx): a smooth
+    synthetic UINT16 heightfield, biome patches of the 4 biomes, two altitude and one gradient rule
+    per biome; all 25 chunks per call."""
+    from superterrainplus_amd import capi, workloads as W
+    rng = np.random.default_rng(9)
+    m, n = 512, 5
+    side = m * n
+    coarse = rng.uniform(0, 65535, (n * 4 + 1, n * 4 + 1))
+    t = np.linspace(0, n * 4, side)
+    i0 = np.minimum(t.astype(int), n * 4 - 1)
+    f = t - i0
+    rows = coarse[i0] * (1 - f)[:, None] + coarse[i0 + 1] * f[:, None]
+    height = (rows[:, i0] * (1 - f)[None, :] + rows[:, i0 + 1] * f[None, :]).astype(np.uint16)
+    biome = np.repeat(np.repeat(rng.integers(0, 4, (side // 64, side // 64)), 64, 0), 64, 1).astype(np.uint16)
+    alt = [(b, u, 4 * b + k) for b in range(4) for k, u in enumerate((0.45, 1.0))]
+    gra = [(b, 0.2, 1.0, 0.0, 1.0, 4 * b + 3) for b in range(4)]
+    a = (capi.AltitudeRule * len(alt))(*[capi.AltitudeRule(*r) for r in alt])
+    g = (capi.GradientRule * len(gra))(*[capi.GradientRule(*r) for r in gra])
+    db = C.c_void_p()
+    capi.call("stp_splat_database_create", a, len(alt), g, len(gra), C.byref(db))
+    table = capi.PermutationTable(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+    try:
+        b, h = capi.DeviceArray.from_host(biome), capi.DeviceArray.from_host(height)
+        out = capi.DeviceArray((side, side), np.uint8)
+        chunks = (capi.SplatChunk * (n * n))(*[capi.SplatChunk(x, y, float(x * m), float(y * m))
+                                              for y in range(n) for x in range(n)])
+
+        def call():
+            capi.call("stp_splatmap_generate", C.byref(table.view), db, b.vp, h.vp, out.vp, side, side, m, m, chunks,
+                      n * n, None)
+        call()
+        capi.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        capi.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+    finally:
+        capi.lib().stp_splat_database_destroy(db)
+    mpx = side * side / ms / 1e3
+    print(f"[bench] splatmap: {ms:.3f} ms per 5x5 x 512^2 area, {mpx:.0f} Mpixel/s", file=sys.stderr, flush=True)
+    return {"workload": "splatmap of a 5x5 rendered area of 512^2 chunks (6.6 Mpx), 3-octave noise, 4 biomes x "
+                        "(2 altitude + 1 gradient) rules", "ms_per_area": round(ms, 3), "mpixel_s": round(mpx, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,6 +370,7 @@ def main():
             "rates": served()}
     if not args.no_world:
         result["config"]["world"] = world()
+        result["config"]["splatmap"] = splat()
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(cfg, sides, origins, props, args.cpu_seconds, threads)

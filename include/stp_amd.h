@@ -510,6 +510,12 @@ int stp_world_request(stp_heightfield_world* world, const int32_t* h_chunks, uin
 int stp_world_read(stp_heightfield_world* world, int32_t x, int32_t y, float* h_height, uint16_t* h_fixed);
 /* 0 no map, 1 heightmap generated, 2 complete */
 int stp_world_state(stp_heightfield_world* world, int32_t x, int32_t y, uint32_t* state);
+/* The rendered area around chunk (x, y): its (2 radius + 1)^2 chunks are made complete, then their
+ * UINT16 heightmaps and biome map are written side by side into device buffers of side
+ * (2 radius + 1) * MapSize, row-major (the reference's rendering buffer, STPWorldPipeline.cpp:671-707;
+ * the input of stp_splatmap_generate). Synchronous. */
+int stp_world_render_area(stp_heightfield_world* world, int32_t x, int32_t y, uint32_t radius, uint16_t* d_biome,
+	uint16_t* d_height);
 /* The conflict-free schedule the world uses (host only): h_batch[i] = batch of centre i, first
  * fit in the given order; centres in one batch have disjoint neighbourhoods. */
 int stp_world_schedule(const int32_t* h_centres, uint32_t count, uint32_t neighbour, uint32_t* h_batch);

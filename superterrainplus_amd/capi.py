@@ -218,6 +218,7 @@ _SIGNATURES = {
     "stp_world_read": (_I, [_P, C.c_int32, C.c_int32, _P, _P]),
     "stp_world_state": (_I, [_P, C.c_int32, C.c_int32, _P]),
     "stp_world_schedule": (_I, [_P, _U32, _U32, _P]),
+    "stp_world_render_area": (_I, [_P, C.c_int32, C.c_int32, _U32, _P, _P]),
     "stp_world_stream_word": (_I, [C.c_int32, C.c_int32, _P]),
 }
 

@@ -1068,6 +1068,14 @@ int stp_world_state(stp_heightfield_world* const world, const int32_t x, const i
 	});
 }
 
+int stp_world_render_area(stp_heightfield_world* const world, const int32_t x, const int32_t y, const uint32_t radius,
+	uint16_t* const d_biome, uint16_t* const d_height) {
+	return guarded([&] {
+		require(world, "null world");
+		world->World->renderArea(make_int2(x, y), radius, d_biome, d_height);
+	});
+}
+
 int stp_world_schedule(const int32_t* const h_centres, const uint32_t count, const uint32_t neighbour,
 	uint32_t* const h_batch) {
 	return guarded([&] {

@@ -92,7 +92,7 @@ std::vector<unsigned int> STPHeightfieldWorld::schedule(const std::vector<int2>&
 STPHeightfieldWorld::STPHeightfieldWorld(const STPEnvironment::STPSimplexNoiseSetting& noise,
 	const std::vector<STPBiomeProperty>& biome, const STPEnvironment::STPRainDropSetting& rain,
 	const STPBatchSetting& batch, const STPWorldSetting& setting) :
-	Setting(setting) {
+	Setting(setting), Batch(batch) {
 	const unsigned int map = setting.MapSize, nn = setting.ErosionNeighbour;
 	if (nn == 0u || (nn & 1u) == 0u || nn > 9u) {
 		STP_THROW(STPInvalidArgument, "the erosion neighbourhood is odd and at most 9 chunks per side");
@@ -302,3 +302,35 @@ bool STPHeightfieldWorld::read(const int2 chunk, float* const height, unsigned s
 	STP_CHECK_HIP(hipStreamSynchronize(this->Stream));
 	return true;
 }
+
+void STPHeightfieldWorld::renderArea(const int2 centre, const unsigned int radius, unsigned short* const biome,
+	unsigned short* const height) {
+	if (!biome || !height) {
+		STP_THROW(STPInvalidArgument, "null area buffer");
+	}
+	const unsigned int map = this->Setting.MapSize, n = 2u * radius + 1u;
+	if (radius > 64u || static_cast<unsigned long long>(n) * map > 65536ull) {
+		STP_THROW(STPInvalidArgument, "the rendered area is at most 129 chunks and 65536 pixels wide");
+	}
+	const int r = static_cast<int>(radius);
+	std::vector<int2> chunks;
+	for (int dy = -r; dy <= r; dy++) {
+		for (int dx = -r; dx <= r; dx++) {
+			chunks.push_back(make_int2(centre.x + dx, centre.y + dy));
+		}
+	}
+	this->request(chunks);
+	const size_t side = static_cast<size_t>(n) * map;
+	for (size_t i = 0u; i < chunks.size(); i++) {
+		const Entry& e = this->Chunk.at(Key {chunks[i].x, chunks[i].y});
+		const size_t lx = i % n, ly = i / n;
+		STP_CHECK_HIP(hipMemcpy2DAsync(height + ly * map * side + lx * map, side * sizeof(unsigned short), e.Fixed,
+			map * sizeof(unsigned short), map * sizeof(unsigned short), map, hipMemcpyDeviceToDevice, this->Stream));
+	}
+	// the biome map the heightmap batch sampled: chunk (X, Y) starts at world pixel (X, Y) * MapSize
+	STPHeightfieldKernel::voronoiBiomemap(this->Batch.BiomeSeed, this->Batch.BiomeCount, this->Batch.BiomeCell,
+		static_cast<long long>(centre.x - r) * map, static_cast<long long>(centre.y - r) * map, biome,
+		static_cast<unsigned int>(side), static_cast<unsigned int>(side), static_cast<unsigned int>(side), this->Stream);
+	STP_CHECK_HIP(hipStreamSynchronize(this->Stream));
+}
+

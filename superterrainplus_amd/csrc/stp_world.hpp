@@ -45,6 +45,11 @@ namespace SuperTerrainPlus {
 		std::vector<int2> request(const std::vector<int2>& chunks);
 		// Copies a resident chunk (heightmap or complete) to host memory; false if it has no map.
 		bool read(int2 chunk, float* height, unsigned short* fixed);
+		// The rendered area around `centre` ((2 radius + 1)^2 chunks, made complete first), as the
+		// reference's rendering buffer holds it (STPWorldPipeline.cpp:671-707): the chunks' UINT16
+		// heightmaps and their biome map side by side in device buffers of side
+		// (2 radius + 1) * MapSize, row-major. Synchronous.
+		void renderArea(int2 centre, unsigned int radius, unsigned short* biome, unsigned short* height);
 		STPChunkState state(int2 chunk) const;
 		unsigned long long residentBytes() const noexcept { return this->Resident; }
 
@@ -69,6 +74,7 @@ namespace SuperTerrainPlus {
 		};
 
 		STPWorldSetting Setting;
+		STPBatchSetting Batch;
 		std::unique_ptr<STPHeightfieldBatch> Generator, Eroder;
 		std::unordered_map<Key, Entry, KeyHash> Chunk;
 		hipStream_t Stream = nullptr;

@@ -174,6 +174,45 @@ def test_world_block_of_centres(gpu):
         assert np.isfinite(h).all()
 
 
+def test_world_render_area_and_splatmap(gpu, oracle):
+    """The rendered area around (3, -2) (3 x 3 chunks, made complete): the chunks' UINT16 maps side
+    by side, the area's biome map equal to the oracle's Voronoi map, and the splatmap of its chunks
+    bit-exact against the oracle on those buffers."""
+    log("world render area")
+    w = World(0.5)
+    n, side = 3, 3 * MAP
+    biome, height = capi.DeviceArray((side, side), np.uint16), capi.DeviceArray((side, side), np.uint16)
+    capi.call("stp_world_render_area", C.c_void_p(w.h), 3, -2, 1, biome.vp, height.vp)
+    hb, hh = biome.download(), height.download()
+    for ly in range(n):
+        for lx in range(n):
+            x, y = 3 - 1 + lx, -2 - 1 + ly
+            assert w.state(x, y) == 2
+            assert (hh[ly * MAP:(ly + 1) * MAP, lx * MAP:(lx + 1) * MAP] == w.read(x, y)[1]).all(), (x, y)
+    assert (hb == oracle.voronoi_map(SEED, COUNT, CELL, (2 * MAP, -3 * MAP), side, side)).all()
+    altitude = [(b, u, 3 * b + k) for b in range(COUNT) for k, u in enumerate((0.4, 0.7, 1.0))]
+    gradient = [(b, 0.05, 1.0, 0.0, 1.0, 20 + b) for b in range(COUNT)]
+    a = (capi.AltitudeRule * len(altitude))(*[capi.AltitudeRule(*r) for r in altitude])
+    g = (capi.GradientRule * len(gradient))(*[capi.GradientRule(*r) for r in gradient])
+    db = C.c_void_p()
+    capi.call("stp_splat_database_create", a, len(altitude), g, len(gradient), C.byref(db))
+    table = capi.PermutationTable(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+    try:
+        out = capi.DeviceArray.from_host(np.zeros((side, side), np.uint8))
+        chunks = [(lx, ly, float((3 - 1 + lx) * MAP), float((-2 - 1 + ly) * MAP)) for ly in range(n) for lx in range(n)]
+        c = (capi.SplatChunk * len(chunks))(*[capi.SplatChunk(*r) for r in chunks])
+        capi.call("stp_splatmap_generate", C.byref(table.view), db, biome.vp, height.vp, out.vp, side, side, MAP, MAP, c,
+                  len(chunks), None)
+        capi.synchronize()
+        p, g32, _ = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+        ref = oracle.splatmap(p, g32, altitude, gradient, hb, hh, (MAP, MAP), chunks)
+        gpu = out.download()
+        assert (gpu == ref).all()
+        assert len(np.unique(gpu)) > 2
+    finally:
+        capi.lib().stp_splat_database_destroy(db)
+
+
 def test_world_rejects_bad_settings(gpu):
     batch = capi.BatchSetting(SEED, COUNT, CELL, RADIUS, 8, 0.0)
     for bad in [capi.WorldSetting(MAP, 2, 4, 4), capi.WorldSetting(4, 3, 4, 4), capi.WorldSetting(MAP, 3, 0, 4)]:
