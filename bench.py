@@ -200,6 +200,54 @@ def world(plain_frames=6):
     return out
 
 
+def layers(reps=10, maps=64):
+    """Biome maps from the demo's layer chain (14 layers ending in 3 Voronoi zooms, seed 6666, the demo
+    registry): `maps` maps of 512^2 at the chunk offsets of an 8 x 8 block, in one call. The oracle's
+    recursive evaluation (the reference's algorithm, one production line, one thread) on one map of
+    the same block is timed beside it."""
+    from superterrainplus_amd import capi
+    side = 512
+    nodes, reg = capi.demo_layer_program(), capi.demo_biome_registry()
+    arr = (capi.LayerNode * len(nodes))(*nodes)
+    h = C.c_void_p()
+    capi.call("stp_layer_factory_create", arr, len(nodes), 6666, C.byref(reg), side, side, C.byref(h))
+    try:
+        offsets = [((i % 8) * side, (i // 8) * side) for i in range(maps)]
+        flat = (C.c_int32 * (2 * maps))(*[v for o in offsets for v in o])
+        out = capi.DeviceArray((maps, side, side), np.uint16)
+
+        def call():
+            capi.call("stp_layer_factory_generate_device", h, out.vp, flat, maps, None)
+        call()
+        capi.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        capi.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        first = out.download()[0]
+    finally:
+        capi.lib().stp_layer_factory_destroy(h)
+    mpx = maps * side * side / ms / 1e3
+    cpu = None
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        o = [O.layer_node(n.Kind, n.Salt, [p for p in n.Parent if p >= 0], n.Param, n.CacheSize) for n in nodes]
+        t0 = time.perf_counter()
+        ref = O.layer_biomemap(o, 6666, (np.array(reg.ID[:]), np.array(reg.Temperature[:]), np.array(reg.Precipitation[:])),
+                               [offsets[0]], side, side)[0]
+        cpu_s = time.perf_counter() - t0
+        cpu = {"mpixel_s": round(side * side / cpu_s / 1e6, 2), "cores": 1, "kind": "port",
+               "sample": "one 512^2 map of the same block, recursive cached evaluation", "matches_gpu": bool((ref == first).all())}
+    except OSError:
+        pass
+    print(f"[bench] biome layers: {ms:.3f} ms per {maps} x 512^2 maps, {mpx:.0f} Mpixel/s"
+          + (f" (CPU oracle {cpu['mpixel_s']} Mpixel/s)" if cpu else ""), file=sys.stderr, flush=True)
+    return {"workload": f"demo biome layer chain (14 layers, 3 Voronoi zooms), {maps} maps of 512^2 per call",
+            "ms_per_call": round(ms, 3), "mpixel_s": round(mpx, 1), "cpu_baseline": cpu}
+
+
 def splat(reps=10):
     """The splatmap generator over a rendered area of 5 x 5 chunks of 512^2 (6.6 Mpx): a smooth
     synthetic UINT16 heightfield, biome patches of the 4 biomes, two altitude and one gradient rule
@@ -371,6 +419,7 @@ def main():
     if not args.no_world:
         result["config"]["world"] = world()
         result["config"]["splatmap"] = splat()
+        result["config"]["biome_layers"] = layers()
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(cfg, sides, origins, props, args.cpu_seconds, threads)

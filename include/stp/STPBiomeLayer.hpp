@@ -1,0 +1,136 @@
+// Biome map generation by a chain of layers (the Minecraft-style layered algorithm) on the GPU.
+//
+// The reference builds the biome map on the host: an STPBiomeFactory
+// (SuperTerrain+/SuperTerrain+/Public/SuperTerrain+/World/Diversity/STPBiomeFactory.h:16-93,
+// SuperTerrain+/SuperTerrain+/Private/World/Diversity/STPBiomeFactory.cpp:24-41) samples the root of a
+// tree of STPLayer objects (SuperTerrain+/SuperTerrain+/Public/SuperTerrain+/World/Diversity/STPLayer.h,
+// SuperTerrain+/SuperTerrain+/Private/World/Diversity/STPLayer.cpp) pixel by pixel, each layer
+// recursing into its parents through a hash cache. The demo's layers are SuperDemo+/World/Layers/*.h and
+// its chain SuperDemo+/World/Layers/STPAllLayers.cpp:55-110.
+//
+// Here a chain is a list of nodes of built-in layer kinds (the demo's), parents before children, the
+// last node the root. The GPU evaluates it bottom-up: every node once over the rectangle of
+// coordinates its children read, for every chunk of a batch in one launch.
+#pragma once
+#ifndef STP_BIOME_LAYER_HPP
+#define STP_BIOME_LAYER_HPP
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+namespace SuperTerrainPlus::STPDiversity {
+
+	// The layer kinds of the demo (SuperDemo+/World/Layers). Parents: 0 for Continent and OceanNoise,
+	// 2 for Hills (land, noise), OceanMix (land, ocean) and RiverMix (land, river), 1 otherwise.
+	enum class STPLayerKind : unsigned int {
+		Continent = 0u,       // STPContinentLayer.h
+		Scale = 1u,           // STPScaleLayer.h; Param 0 normal, 1 fuzzy
+		Land = 2u,            // STPLandLayer.h (X cross)
+		Island = 3u,          // STPIslandLayer.h (cross)
+		Voronoi = 4u,         // STPVoronoiLayer.h, 2D only; Param 0 std::hash as libstdc++ (identity), 1 as MSVC (FNV-1a)
+		BaseBiome = 5u,       // STPBaseBiomeLayer.h
+		ClimateSingle = 6u,   // STPClimateLayer.h
+		ClimateModerate = 7u,
+		ClimateExtreme = 8u,
+		DeepOcean = 9u,       // STPDeepOceanLayer.h
+		EaseEdge = 10u,       // STPEaseEdgeLayer.h
+		EdgeBiome = 11u,      // STPEdgeBiomeLayer.h
+		Hills = 12u,          // STPHillsLayer.h
+		Noise = 13u,          // STPNoiseLayer.h
+		NoiseToRiver = 14u,   // STPNoiseToRiverLayer.h
+		OceanNoise = 15u,     // STPOceanTemperatureLayer.h
+		OceanTemperate = 16u,
+		OceanExtreme = 17u,
+		OceanTransition = 18u,
+		OceanMix = 19u,
+		RiverErode = 20u,     // STPRiverErodeLayer.h
+		RiverMix = 21u,       // STPRiverMixLayer.h
+		SmoothScale = 22u,    // STPSmoothScaleLayer.h
+		Count = 23u
+	};
+
+	// One layer: its kind, salt (STPLayer::Salt), parents (node indices, -1 for none), the kind's
+	// parameter, and the reference's cache size (0 or a power of 2; validated as the reference does,
+	// without effect here: the GPU evaluates whole rectangles).
+	struct STPLayerNode {
+		unsigned int Kind;
+		unsigned long long Salt;
+		int Parent[2];
+		unsigned int Param;
+		unsigned long long CacheSize;
+	};
+
+	// The demo's named biomes in the order the registry registers them (STPBiomeRegistry.cpp:69-104);
+	// the first biome registered under an id decides that id's precipitation type.
+	enum class STPNamedBiome : unsigned int {
+		Ocean, DeepOcean, WarmOcean, LukewarmOcean, ColdOcean, FrozenOcean, DeepWarmOcean, DeepLukewarmOcean,
+		DeepColdOcean, DeepFrozenOcean, River, FrozenRiver, Plains, Desert, Mountain, Forest, Taiga, SnowyTaiga,
+		SnowyTundra, Jungle, Savannah, Swamp, Badlands, DesertHills, TaigaHills, WoodedMountain, SnowyMountain,
+		ForestHills, JungleHills, SavannahPlateau, SwampHills, BadlandsPlateau, Beach, SnowyBeach, StoneShore,
+		Count
+	};
+	constexpr unsigned int NamedBiomeCount = static_cast<unsigned int>(STPNamedBiome::Count);
+
+	// Ids, temperatures and precipitations of the named biomes (STPBiome::ID/Temperature/Precipitation).
+	struct STPBiomeRegistry {
+		unsigned short ID[NamedBiomeCount];
+		float Temperature[NamedBiomeCount];
+		float Precipitation[NamedBiomeCount];
+
+		// The demo's registry after loading SuperDemo+/Biome.ini: ocean 0, plains 1, desert 2, forest 3
+		// (STPTerrainParaLoader.cpp:271-294); every other biome is never loaded and keeps the zero
+		// of its static storage, so its id is 0 (the ocean's).
+		static STPBiomeRegistry demo() noexcept;
+	};
+
+	// A list of layers; the last one added is the root.
+	class STPLayerProgram {
+	public:
+		std::vector<STPLayerNode> Node;
+
+		// Appends a node, returns its index. Parents must be added before.
+		int add(STPLayerKind kind, unsigned long long salt, int parent0 = -1, int parent1 = -1, unsigned int param = 0u,
+			unsigned long long cache_size = 0u);
+
+		// The demo's chain (STPAllLayers.cpp:55-110): Continent, fuzzy Scale, Land, Scale, 3 Land, Island,
+		// 3 Scale, 3 Voronoi, with the demo's salts and cache sizes.
+		static STPLayerProgram demo();
+	};
+
+	// The biome factory on the GPU (STPBiomeFactory): biome maps of BiomeDimension pixels whose pixel
+	// (x, y) is the root layer sampled at world (x + offset.x, 0, y + offset.y).
+	class STPLayerBiomeFactory {
+	public:
+		const uint2 BiomeDimension;
+
+		// Throws STPNumericDomainError for an empty dimension or a cache size that is not a power of 2,
+		// STPInvalidArgument for a malformed chain.
+		STPLayerBiomeFactory(const STPLayerProgram& program, unsigned long long global_seed, const STPBiomeRegistry& registry,
+			uint2 dimension);
+		~STPLayerBiomeFactory();
+
+		STPLayerBiomeFactory(const STPLayerBiomeFactory&) = delete;
+		STPLayerBiomeFactory& operator=(const STPLayerBiomeFactory&) = delete;
+
+		// STPBiomeFactory::operator(): one map into host memory, synchronous.
+		void operator()(unsigned short* biomemap, int2 offset);
+		// `count` maps into device memory (map i at biomemap + i * dimension.x * dimension.y), offsets in
+		// host memory; asynchronous on `stream`.
+		void generateDevice(unsigned short* biomemap, const int2* offset, unsigned int count, hipStream_t stream);
+		// The same over arbitrary rectangles: map i has origin[i] and size (width, height), row pitch
+		// `pitch` elements, map stride `stride` elements.
+		void generateDevice(unsigned short* biomemap, const int2* origin, unsigned int count, unsigned int width,
+			unsigned int height, size_t pitch, size_t stride, hipStream_t stream);
+
+		struct STPImplementation;
+
+	private:
+		std::unique_ptr<STPImplementation> Impl;
+	};
+
+} // namespace SuperTerrainPlus::STPDiversity
+
+#endif // STP_BIOME_LAYER_HPP

@@ -28,6 +28,7 @@
 #include <utility>
 #include <vector>
 
+#include <stp/STPBiomeLayer.hpp>
 #include <stp/STPBiomeProperty.hpp>
 #include <stp/STPSimplexNoise.hpp>
 #include <stp/STPSingleHistogramWrapper.hpp>

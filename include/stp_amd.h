@@ -477,6 +477,54 @@ int stp_splatmap_generate(const stp_permutation* perm, const stp_splat_database*
 	const uint16_t* d_height, uint8_t* d_splat, uint32_t area_x, uint32_t area_y, uint32_t map_x, uint32_t map_y,
 	const stp_splat_chunk* h_chunks, uint32_t count, void* stream);
 
+/* ------------------------------------------------------------------ biome layers
+ * Biome maps from a chain of layers (the reference's STPBiomeFactory over STPLayer trees,
+ * SuperTerrain+/SuperTerrain+/Public/SuperTerrain+/World/Diversity/STPBiomeFactory.h:16-93 and STPLayer.h:20-260,
+ * with the demo layers of SuperDemo+/World/Layers and its chain STPAllLayers.cpp:55-110), evaluated on the GPU.
+ * A chain is a list of nodes, parents before children, the last node the root. */
+typedef struct {
+	uint32_t Kind;      /* 0 Continent, 1 Scale, 2 Land, 3 Island, 4 Voronoi, 5 BaseBiome, 6 ClimateSingle,
+	                       7 ClimateModerate, 8 ClimateExtreme, 9 DeepOcean, 10 EaseEdge, 11 EdgeBiome, 12 Hills,
+	                       13 Noise, 14 NoiseToRiver, 15 OceanNoise, 16 OceanTemperate, 17 OceanExtreme,
+	                       18 OceanTransition, 19 OceanMix, 20 RiverErode, 21 RiverMix, 22 SmoothScale */
+	uint32_t Param;     /* Scale: 0 normal, 1 fuzzy. Voronoi (2D): seed hash 0 identity (libstdc++), 1 FNV-1a (MSVC) */
+	int32_t Parent[2];  /* node indices, -1 for none; Hills (land, noise), OceanMix (land, ocean), RiverMix (land, river) */
+	uint64_t Salt;      /* STPLayer::Salt */
+	uint64_t CacheSize; /* STPLayer cache size: 0 or a power of 2 (validated; no effect on the GPU) */
+} stp_layer_node;
+
+/* Ids, temperatures and precipitations of the demo's 35 named biomes in registration order
+ * (STPBiomeRegistry.cpp:66-104): Ocean, DeepOcean, WarmOcean, LukewarmOcean, ColdOcean, FrozenOcean,
+ * DeepWarmOcean, DeepLukewarmOcean, DeepColdOcean, DeepFrozenOcean, River, FrozenRiver, Plains, Desert,
+ * Mountain, Forest, Taiga, SnowyTaiga, SnowyTundra, Jungle, Savannah, Swamp, Badlands, DesertHills, TaigaHills,
+ * WoodedMountain, SnowyMountain, ForestHills, JungleHills, SavannahPlateau, SwampHills, BadlandsPlateau,
+ * Beach, SnowyBeach, StoneShore. */
+typedef struct {
+	uint16_t ID[35];
+	float Temperature[35];
+	float Precipitation[35];
+} stp_biome_registry;
+
+typedef struct stp_layer_factory stp_layer_factory;
+
+/* The demo's chain (STPAllLayers.cpp:55-110, 14 nodes) into h_nodes (capacity >= 14). */
+int stp_layer_program_demo(stp_layer_node* h_nodes, uint32_t capacity, uint32_t* count);
+/* The demo's registry after loading SuperDemo+/Biome.ini (ocean 0, plains 1, desert 2, forest 3, the
+ * other biomes id 0). */
+int stp_biome_registry_demo(stp_biome_registry* out);
+/* STPBiomeFactory(dimension) with the chain as its production line; STP_ERROR_NUMERIC_DOMAIN for an
+ * empty dimension or a cache size that is not a power of 2 (STPBiomeFactory.cpp:20-22, STPLayer.cpp:68-73). */
+int stp_layer_factory_create(const stp_layer_node* h_nodes, uint32_t count, uint64_t global_seed,
+	const stp_biome_registry* registry, uint32_t dimension_x, uint32_t dimension_y, stp_layer_factory** out);
+int stp_layer_factory_destroy(stp_layer_factory* factory);
+/* STPBiomeFactory::operator()(biomemap, offset): pixel (x, y) is the root sampled at world
+ * (x + offset_x, 0, y + offset_y). Synchronous, host memory. */
+int stp_layer_factory_generate(stp_layer_factory* factory, uint16_t* h_biomemap, int32_t offset_x, int32_t offset_y);
+/* `count` maps into device memory one after another, h_offsets = count (x, y) pairs. Asynchronous
+ * on `stream`; one factory is used by one thread at a time. */
+int stp_layer_factory_generate_device(stp_layer_factory* factory, uint16_t* d_biomemap, const int32_t* h_offsets,
+	uint32_t count, void* stream);
+
 /* ------------------------------------------------------------------ world pipeline
  * The reference world pipeline's generation chain (STPWorldPipeline.cpp: requestChunk :640-663,
  * erodeHeightmap :485-559, generateHeightmap :437-476, scheduleChunk :339-404) on resident device

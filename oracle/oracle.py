@@ -318,3 +318,54 @@ def splatmap(perm, grad, altitude, gradient, biome, height, map_size, chunks, fi
                        b.ctypes.data, h.ctypes.data, out.ctypes.data, W, H, map_size[0], map_size[1], c, len(chunks))
     return out
 
+
+
+LAYER_KINDS = ["Continent", "Scale", "Land", "Island", "Voronoi", "BaseBiome", "ClimateSingle", "ClimateModerate",
+               "ClimateExtreme", "DeepOcean", "EaseEdge", "EdgeBiome", "Hills", "Noise", "NoiseToRiver", "OceanNoise",
+               "OceanTemperate", "OceanExtreme", "OceanTransition", "OceanMix", "RiverErode", "RiverMix", "SmoothScale"]
+# registration order of SuperDemo+/World/Biomes/STPBiomeRegistry.cpp:66-104
+NAMED_BIOMES = ["Ocean", "DeepOcean", "WarmOcean", "LukewarmOcean", "ColdOcean", "FrozenOcean", "DeepWarmOcean",
+                "DeepLukewarmOcean", "DeepColdOcean", "DeepFrozenOcean", "River", "FrozenRiver", "Plains", "Desert",
+                "Mountain", "Forest", "Taiga", "SnowyTaiga", "SnowyTundra", "Jungle", "Savannah", "Swamp", "Badlands",
+                "DesertHills", "TaigaHills", "WoodedMountain", "SnowyMountain", "ForestHills", "JungleHills",
+                "SavannahPlateau", "SwampHills", "BadlandsPlateau", "Beach", "SnowyBeach", "StoneShore"]
+
+
+class LayerNode(C.Structure):
+    _fields_ = [("Kind", C.c_uint32), ("Param", C.c_uint32), ("Parent", C.c_int32 * 2), ("Salt", C.c_uint64),
+                ("CacheSize", C.c_uint64)]
+
+
+def layer_node(kind, salt, parents=(), param=0, cache=0):
+    k = LAYER_KINDS.index(kind) if isinstance(kind, str) else int(kind)
+    p = list(parents) + [-1] * (2 - len(parents))
+    return LayerNode(k, param, (C.c_int32 * 2)(*p), salt, cache)
+
+
+def mix_seed(s, fac):
+    """STPLayer::mixSeed (SuperTerrain+/SuperTerrain+/Private/World/Diversity/STPLayer.cpp:174-178)."""
+    lib().orc_layer_mix_seed.restype = C.c_uint64
+    lib().orc_layer_mix_seed.argtypes = [C.c_uint64, C.c_int64]
+    return int(lib().orc_layer_mix_seed(s, fac))
+
+
+def layer_biomemap(nodes, global_seed, registry, origins, width, height):
+    """Biome maps of a layer chain's root (the last node), the reference's way: every pixel through
+    the cached recursion of STPLayer::retrieve (STPBiomeFactory.cpp:24-41). `registry` =
+    (ids, temperatures, precipitations) over NAMED_BIOMES; returns (maps, height, width) uint16."""
+    ids, temp, prec = registry
+    n = (LayerNode * len(nodes))(*nodes)
+    o = np.ascontiguousarray(np.asarray(origins, np.int32).reshape(-1, 2))
+    out = np.zeros((len(o), height, width), np.uint16)
+    i = np.ascontiguousarray(ids, np.uint16)
+    t = np.ascontiguousarray(temp, np.float32)
+    p = np.ascontiguousarray(prec, np.float32)
+    f = lib().orc_layer_biomemap
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                  C.c_uint32, C.c_uint32, C.c_void_p]
+    rc = f(n, len(nodes), global_seed, i.ctypes.data, t.ctypes.data, p.ctypes.data, o.ctypes.data, len(o), width, height,
+           out.ctypes.data)
+    if rc:
+        raise ValueError("malformed layer chain")
+    return out

@@ -999,3 +999,495 @@ void orc_splatmap(const uint8_t* perm, const float* grad, const uint32_t n, cons
 }
 
 } // extern "C"
+
+// ------------------------------------------------------------------------------------
+// Biome layer chain: the reference's evaluation, pixel by pixel through cached recursion.
+// STPBiomeFactory.cpp:24-41 (map layout, y = 0), STPLayer.cpp:47-186 (key hash, MurmurHash3
+// finaliser index, cache, seeds, local sampler), SuperDemo+/World/Layers/*.h (sample rules),
+// STPBiomeRegistry.cpp:61-141 (registry, predicates, precipitation type, CAS).
+// Third-party semantics: std::hash<uint64_t> is libstdc++'s identity (param 0) or the MSVC
+// library's FNV-1a over the 8 little-endian bytes (param 1).
+// ------------------------------------------------------------------------------------
+
+struct orc_layer_node {
+	uint32_t Kind, Param;
+	int32_t Parent[2];
+	uint64_t Salt, CacheSize;
+};
+
+namespace {
+namespace layer {
+
+using Seed = uint64_t;
+using Sample = uint16_t;
+
+namespace K {
+enum : uint32_t {
+	Continent, Scale, Land, Island, Voronoi, BaseBiome, ClimateSingle, ClimateModerate, ClimateExtreme, DeepOcean, EaseEdge,
+	EdgeBiome, Hills, Noise, NoiseToRiver, OceanNoise, OceanTemperate, OceanExtreme, OceanTransition, OceanMix, RiverErode,
+	RiverMix, SmoothScale, KindCount
+};
+}
+// registration order of STPBiomeRegistry.cpp:66-104
+enum B : uint32_t {
+	Ocean, DeepOcean, WarmOcean, LukewarmOcean, ColdOcean, FrozenOcean, DeepWarmOcean, DeepLukewarmOcean, DeepColdOcean,
+	DeepFrozenOcean, River, FrozenRiver, Plains, Desert, Mountain, Forest, Taiga, SnowyTaiga, SnowyTundra, Jungle, Savannah,
+	Swamp, Badlands, DesertHills, TaigaHills, WoodedMountain, SnowyMountain, ForestHills, JungleHills, SavannahPlateau,
+	SwampHills, BadlandsPlateau, Beach, SnowyBeach, StoneShore, BCount
+};
+
+Seed mixSeed(Seed s, const long long fac) {
+	s *= s * 6364136223846793005ull + 1442695040888963407ull;
+	s += static_cast<Seed>(fac);
+	return s;
+}
+
+struct LocalSampler {
+	Seed LayerSeed, LocalSeed;
+	Sample nextValue(const Sample range) {
+		const Sample v = static_cast<Sample>((this->LocalSeed >> 24u) % static_cast<Seed>(range));
+		this->LocalSeed = mixSeed(this->LocalSeed, static_cast<long long>(this->LayerSeed));
+		return v;
+	}
+	Sample choose(const Sample a, const Sample b) {
+		return this->nextValue(2u) == 0u ? a : b;
+	}
+	Sample choose(const Sample a, const Sample b, const Sample c, const Sample d) {
+		const Sample i = this->nextValue(4u);
+		return i == 0u ? a : i == 1u ? b : i == 2u ? c : d;
+	}
+};
+
+struct Chain {
+	struct Node {
+		uint32_t Kind, Param;
+		int32_t Parent[2];
+		Seed LayerSeed, VoronoiSeed;
+		std::vector<unsigned long long> Key;
+		std::vector<Sample> Value;
+	};
+	std::vector<Node> N;
+	Sample Id[BCount];
+	const uint8_t* Type; // precipitation type per id: 0 none, 1 rain, 2 snow
+
+	Sample id(const B b) const { return this->Id[b]; }
+	bool shallowOcean(const Sample v) const {
+		return v == id(Ocean) || v == id(FrozenOcean) || v == id(WarmOcean) || v == id(LukewarmOcean) || v == id(ColdOcean);
+	}
+	bool ocean(const Sample v) const {
+		return shallowOcean(v) || v == id(DeepOcean) || v == id(DeepWarmOcean) || v == id(DeepLukewarmOcean)
+			|| v == id(DeepColdOcean) || v == id(DeepFrozenOcean);
+	}
+	bool snowy(const Sample v) const { return this->Type[v] == 2u; }
+
+	static Seed seedLocal(const Seed layer, const int x, const int z) {
+		Seed s = mixSeed(layer, x);
+		s = mixSeed(s, z);
+		s = mixSeed(s, x);
+		return mixSeed(s, z);
+	}
+	LocalSampler rng(const Node& n, const int x, const int z) const { return LocalSampler {n.LayerSeed, seedLocal(n.LayerSeed, x, z)}; }
+
+	// STPLayer::retrieve through STPLayerCache::get
+	Sample retrieve(const int32_t n, const int x, const int y, const int z) {
+		Node& node = this->N[static_cast<size_t>(n)];
+		if (node.Key.empty()) {
+			return this->sample(n, x, y, z);
+		}
+		unsigned long long key = static_cast<unsigned long long>(x) & ((1ull << 28u) - 1ull);
+		key |= (static_cast<unsigned long long>(z) & ((1ull << 28u) - 1ull)) << 28u;
+		key |= (static_cast<unsigned long long>(y) & 0xFFull) << 56u;
+		unsigned long long h = key;
+		h ^= h >> 33u;
+		h *= 0xFF51AFD7ED558CCDull;
+		h ^= h >> 33u;
+		h *= 0xC4CEB9FE1A85EC53ull;
+		h ^= h >> 33u;
+		const size_t index = static_cast<size_t>(h & (node.Key.size() - 1u));
+		if (node.Key[index] == key) {
+			return node.Value[index];
+		}
+		const Sample s = this->sample(n, x, y, z);
+		Node& again = this->N[static_cast<size_t>(n)];
+		again.Key[index] = key;
+		again.Value[index] = s;
+		return s;
+	}
+
+	Sample cross(const Node& n, const Sample c, const Sample north, const Sample e, const Sample s, const Sample w,
+		const Seed local) {
+		LocalSampler r {n.LayerSeed, local};
+		switch (n.Kind) {
+		case K::Island:
+			return shallowOcean(c) && shallowOcean(north) && shallowOcean(e) && shallowOcean(s) && shallowOcean(w)
+				&& r.nextValue(2u) == 0u ? id(Plains) : c;
+		case K::ClimateModerate: {
+			const auto mf = [this](const Sample v) { return v == id(Mountain) || v == id(Forest); };
+			if (c == id(Plains) && (mf(north) || mf(e) || mf(s) || mf(w))) {
+				return id(Desert);
+			}
+			return c;
+		}
+		case K::ClimateExtreme: {
+			const auto pd = [this](const Sample v) { return v == id(Plains) || v == id(Desert); };
+			if (c != id(Forest) || (!pd(north) && !pd(e) && !pd(s) && !pd(w))) {
+				return c;
+			}
+			return id(Mountain);
+		}
+		case K::DeepOcean: {
+			if (!shallowOcean(c)) {
+				return c;
+			}
+			Sample i = 0u;
+			i += shallowOcean(north);
+			i += shallowOcean(e);
+			i += shallowOcean(s);
+			i += shallowOcean(w);
+			if (i > 3u) {
+				if (c == id(Ocean)) return id(DeepOcean);
+				if (c == id(WarmOcean)) return id(DeepWarmOcean);
+				if (c == id(LukewarmOcean)) return id(DeepLukewarmOcean);
+				if (c == id(ColdOcean)) return id(DeepColdOcean);
+				if (c == id(FrozenOcean)) return id(DeepFrozenOcean);
+			}
+			return c;
+		}
+		case K::EaseEdge: {
+			const auto any = [&](const Sample v) { return v == north || v == e || v == s || v == w; };
+			if (c == id(Desert) && any(id(SnowyTundra))) {
+				return id(WoodedMountain);
+			}
+			if (c == id(Swamp) && (any(id(Desert)) || any(id(SnowyTundra)) || any(id(SnowyTaiga)))) {
+				return id(Plains);
+			}
+			return c;
+		}
+		case K::EdgeBiome: {
+			if (ocean(c)) {
+				return c;
+			}
+			const bool snow = snowy(c);
+			if (ocean(north) || ocean(e) || ocean(s) || ocean(w)) {
+				if (snow) return id(SnowyBeach);
+				if (c == id(Mountain)) return id(StoneShore);
+				return id(Beach);
+			}
+			return c;
+		}
+		case K::NoiseToRiver: {
+			const auto f = [](const Sample v) -> Sample { return v >= 2u ? static_cast<Sample>(2u + (v & 1u)) : v; };
+			const Sample i = f(c);
+			return i == f(north) && i == f(e) && i == f(s) && i == f(w) ? Sample {65535u} : id(River);
+		}
+		case K::OceanTemperate: {
+			const auto none = [&](const Sample v) { return north != v && e != v && s != v && w != v; };
+			if (c != id(LukewarmOcean) || none(id(ColdOcean))) return c;
+			if (c != id(ColdOcean) || none(id(LukewarmOcean))) return c;
+			return id(Ocean);
+		}
+		case K::OceanExtreme: {
+			const auto any = [&](const Sample v) { return north == v || e == v || s == v || w == v; };
+			if (c == id(WarmOcean) && any(id(FrozenOcean))) return id(LukewarmOcean);
+			if (c == id(FrozenOcean) && any(id(WarmOcean))) return id(ColdOcean);
+			return c;
+		}
+		case K::RiverErode:
+			return north == id(River) || e == id(River) || s == id(River) || w == id(River) ? id(River) : c;
+		case K::SmoothScale: {
+			const bool xm = w == e, zm = north == s;
+			if (xm == zm) {
+				return xm ? r.choose(w, north) : c;
+			}
+			return xm ? w : north;
+		}
+		default: return c;
+		}
+	}
+
+	Sample land(const Node& n, const Sample center, const Sample ne, const Sample se, const Sample sw, const Sample nw,
+		const Seed local) {
+		LocalSampler rng {n.LayerSeed, local};
+		const auto so = [this](const Sample v) { return shallowOcean(v); };
+		const auto cas = [](const Sample a, const Sample b, const Sample f) { return a == b ? b : f; };
+		if (!so(center) || (so(sw) && so(se) && so(ne) && so(nw))) {
+			if (so(center) || (!so(sw) && !so(se) && !so(ne) && !so(nw)) || rng.nextValue(5u) != 0u) {
+				return center;
+			}
+			if (so(nw)) return cas(center, id(Forest), nw);
+			if (so(sw)) return cas(center, id(Forest), sw);
+			if (so(ne)) return cas(center, id(Forest), ne);
+			if (so(se)) return cas(center, id(Forest), se);
+			return center;
+		}
+		Sample i = 1u, j = 1u;
+		if (!so(nw) && rng.nextValue(i++) == 0u) j = nw;
+		if (!so(ne) && rng.nextValue(i++) == 0u) j = ne;
+		if (!so(sw) && rng.nextValue(i++) == 0u) j = sw;
+		if (!so(se) && rng.nextValue(i) == 0u) j = se;
+		if (rng.nextValue(3u) == 0u) {
+			return j;
+		}
+		return j == id(Forest) ? id(Forest) : center;
+	}
+
+	static double voronoiDist(const Seed seed, const int x, const int y, const int z, const double xf, const double yf,
+		const double zf) {
+		const auto distribute = [](const Seed s) {
+			const double d = static_cast<double>(static_cast<unsigned int>((s >> 24u) % 1024u)) / 1024.0;
+			return (d - 0.5) * 0.9;
+		};
+		Seed m = mixSeed(seed, x);
+		m = mixSeed(m, y);
+		m = mixSeed(m, z);
+		m = mixSeed(m, x);
+		m = mixSeed(m, y);
+		m = mixSeed(m, z);
+		const double d = distribute(m);
+		m = mixSeed(m, static_cast<long long>(seed));
+		const double e = distribute(m);
+		m = mixSeed(m, static_cast<long long>(seed));
+		const double f = distribute(m);
+		return (zf + f) * (zf + f) + (yf + e) * (yf + e) + (xf + d) * (xf + d);
+	}
+
+	Sample sample(const int32_t ni, const int x, const int y, const int z) {
+		const Node n = this->N[static_cast<size_t>(ni)]; // a copy: recursion may touch the vector's caches
+		const int32_t p0 = n.Parent[0], p1 = n.Parent[1];
+		switch (n.Kind) {
+		case K::Continent: {
+			LocalSampler r = rng(n, x, z);
+			return r.nextValue(10u) == 0u ? id(Plains) : id(Ocean);
+		}
+		case K::OceanNoise: {
+			LocalSampler r = rng(n, x, z);
+			const Sample i = r.nextValue(3u);
+			return i == 0u ? id(FrozenOcean) : i == 1u ? id(WarmOcean) : id(Ocean);
+		}
+		case K::Scale: {
+			const Sample i = retrieve(p0, x >> 1, y, z >> 1);
+			const int xb = x & 1, zb = z & 1;
+			LocalSampler r = rng(n, x & -2, z & -2);
+			if (xb == 0 && zb == 0) {
+				return i;
+			}
+			const Sample l = retrieve(p0, x >> 1, y, (z + 1) >> 1);
+			const Sample m = r.choose(i, l);
+			if (xb == 0) {
+				return m;
+			}
+			const Sample nn = retrieve(p0, (x + 1) >> 1, y, z >> 1);
+			const Sample o = r.choose(i, nn);
+			if (zb == 0) {
+				return o;
+			}
+			const Sample p = retrieve(p0, (x + 1) >> 1, y, (z + 1) >> 1);
+			// the rule on (center, e, s, se) = (i, n, l, p)
+			const Sample center = i, e = nn, s = l, se = p;
+			const Sample ret = r.choose(center, e, s, se);
+			if (n.Param != 0u) return ret;
+			if (e == s && e == se) return e;
+			if (center == e && (center == se || s != se)) return center;
+			if (center == s && (center == se || e != se)) return center;
+			if (center == se && e != s) return center;
+			if (e == s && center != se) return e;
+			if (e == se && center != s) return e;
+			if (s == se && center != e) return s;
+			return ret;
+		}
+		case K::Land:
+			return land(n, retrieve(p0, x, y, z), retrieve(p0, x + 1, y, z - 1), retrieve(p0, x + 1, y, z + 1),
+				retrieve(p0, x - 1, y, z + 1), retrieve(p0, x - 1, y, z - 1), seedLocal(n.LayerSeed, x, z));
+		case K::Voronoi: {
+			const int ijk[3] = {x - 2, y - 2, z - 2};
+			const int lmn[3] = {ijk[0] >> 2, ijk[1] >> 2, ijk[2] >> 2};
+			const double def[3] = {static_cast<double>(ijk[0] & 3) / 4.0, static_cast<double>(ijk[1] & 3) / 4.0,
+				static_cast<double>(ijk[2] & 3) / 4.0};
+			double ds[8];
+			for (unsigned int c = 0u; c < 8u; c++) {
+				const bool bl[3] = {(c & 4u) == 0u, (c & 2u) == 0u, (c & 1u) == 0u};
+				ds[c] = voronoiDist(n.VoronoiSeed, bl[0] ? lmn[0] : lmn[0] + 1, bl[1] ? lmn[1] : lmn[1] + 1,
+					bl[2] ? lmn[2] : lmn[2] + 1, bl[0] ? def[0] : def[0] - 1.0, bl[1] ? def[1] : def[1] - 1.0,
+					bl[2] ? def[2] : def[2] - 1.0);
+			}
+			unsigned int index = 0u;
+			double min = ds[0];
+			for (unsigned int c = 1u; c < 8u; c++) {
+				if (ds[c] >= min) continue;
+				index = c;
+				min = ds[c];
+			}
+			return retrieve(p0, (index & 4u) == 0u ? lmn[0] : lmn[0] + 1, 0, (index & 1u) == 0u ? lmn[2] : lmn[2] + 1);
+		}
+		case K::BaseBiome: {
+			LocalSampler r = rng(n, x, z);
+			const Sample c = retrieve(p0, x, y, z);
+			if (ocean(c)) return c;
+			if (c == id(Plains)) {
+				const Sample l[4] = {id(Desert), id(Savannah), id(Plains), id(Badlands)};
+				return l[r.nextValue(4u)];
+			}
+			if (c == id(Desert)) {
+				const Sample l[5] = {id(Jungle), id(Forest), id(Mountain), id(Plains), id(Swamp)};
+				return l[r.nextValue(5u)];
+			}
+			if (c == id(Mountain)) {
+				const Sample l[4] = {id(Plains), id(Mountain), id(Forest), id(Taiga)};
+				return l[r.nextValue(4u)];
+			}
+			if (c == id(Forest)) {
+				const Sample l[3] = {id(SnowyTaiga), id(SnowyTundra), id(SnowyMountain)};
+				return l[r.nextValue(3u)];
+			}
+			return c;
+		}
+		case K::ClimateSingle: {
+			const Sample v = retrieve(p0, x, y, z);
+			LocalSampler r = rng(n, x, z);
+			if (shallowOcean(v)) return v;
+			const Sample i = r.nextValue(6u);
+			return i == 0u ? id(Forest) : i == 1u ? id(Mountain) : id(Plains);
+		}
+		case K::Noise: {
+			LocalSampler r = rng(n, x, z);
+			const Sample v = retrieve(p0, x, y, z);
+			return shallowOcean(v) ? v : static_cast<Sample>(r.nextValue(static_cast<Sample>(65535u / 2u)) + 2u);
+		}
+		case K::OceanTransition: {
+			const Sample v = retrieve(p0, x, y, z);
+			if (!ocean(v)) return v;
+			for (int rx = -8; rx <= 8; rx += 4) {
+				for (int rz = -8; rz <= 8; rz += 4) {
+					if (ocean(retrieve(p0, x + rx, y, z + rz))) continue;
+					if (v == id(WarmOcean)) return id(LukewarmOcean);
+					if (v == id(FrozenOcean)) return id(ColdOcean);
+				}
+			}
+			return v;
+		}
+		case K::OceanMix: {
+			const Sample l = retrieve(p0, x, y, z);
+			if (!ocean(l)) return l;
+			return retrieve(p1, x, y, z);
+		}
+		case K::RiverMix: {
+			const Sample l = retrieve(p0, x, y, z), rv = retrieve(p1, x, y, z);
+			if (ocean(l)) return l;
+			if (rv == id(River) && snowy(l)) return id(FrozenRiver);
+			return l;
+		}
+		case K::Hills: {
+			LocalSampler r = rng(n, x, z);
+			const Sample lv = retrieve(p0, x, y, z), nv = retrieve(p1, x, y, z);
+			const Sample is_hill = static_cast<Sample>((nv - 2) % 29);
+			if (r.nextValue(3u) == 0u || is_hill == 0u) {
+				Sample l = lv;
+				if (lv == id(Desert)) l = id(DesertHills);
+				else if (lv == id(Taiga)) l = id(TaigaHills);
+				else if (lv == id(Mountain)) l = id(WoodedMountain);
+				else if (lv == id(SnowyTundra) || lv == id(SnowyTaiga)) l = id(SnowyMountain);
+				else if (lv == id(Plains)) l = r.nextValue(3u) == 0u ? id(ForestHills) : id(Forest);
+				else if (lv == id(Forest)) l = id(ForestHills);
+				else if (lv == id(Jungle)) l = id(JungleHills);
+				else if (lv == id(Savannah)) l = id(SavannahPlateau);
+				else if (lv == id(Swamp)) l = id(SwampHills);
+				else if (lv == id(Badlands)) l = id(BadlandsPlateau);
+				else if (lv == id(Ocean)) l = id(DeepOcean);
+				else if (lv == id(WarmOcean)) l = id(DeepWarmOcean);
+				else if (lv == id(LukewarmOcean)) l = id(DeepLukewarmOcean);
+				else if (lv == id(ColdOcean)) l = id(DeepColdOcean);
+				else if (lv == id(FrozenOcean)) l = id(DeepFrozenOcean);
+				if (ocean(lv) && !shallowOcean(lv) && r.nextValue(3u) == 0u) {
+					l = r.nextValue(2u) == 0u ? id(Plains) : id(Forest);
+				}
+				if (l != lv) {
+					unsigned char m = 0u;
+					if (lv == retrieve(p0, x, y, z - 1)) m++;
+					if (lv == retrieve(p0, x + 1, y, z)) m++;
+					if (lv == retrieve(p0, x - 1, y, z)) m++;
+					if (lv == retrieve(p0, x, y, z + 1)) m++;
+					if (m >= 3u) return l;
+				}
+			}
+			return lv;
+		}
+		default:
+			return cross(n, retrieve(p0, x, y, z), retrieve(p0, x, y, z - 1), retrieve(p0, x + 1, y, z),
+				retrieve(p0, x, y, z + 1), retrieve(p0, x - 1, y, z), seedLocal(n.LayerSeed, x, z));
+		}
+	}
+};
+
+} // namespace layer
+} // namespace
+
+extern "C" {
+
+// STPLayer::mixSeed, for the pins of the layer RNG
+uint64_t orc_layer_mix_seed(const uint64_t s, const int64_t fac) {
+	return layer::mixSeed(s, static_cast<long long>(fac));
+}
+
+// Biome maps of the chain's root (the last node) at each origin, width x height, the maps one after
+// another; one production line (caches kept across maps, as the reference's pooled layers).
+// Returns 0, or 1 for a malformed chain (parents after the node, cache not a power of 2).
+int orc_layer_biomemap(const orc_layer_node* nodes, const uint32_t count, const uint64_t global, const uint16_t* ids,
+	const float* temperature, const float* precipitation, const int32_t* origin, const uint32_t maps, const uint32_t width,
+	const uint32_t height, uint16_t* out) {
+	using namespace layer;
+	Chain ch;
+	std::vector<uint8_t> type(65536u, 0u), seen(65536u, 0u);
+	for (uint32_t b = 0u; b < BCount; b++) {
+		ch.Id[b] = ids[b];
+		if (!seen[ids[b]]) {
+			seen[ids[b]] = 1u;
+			type[ids[b]] = precipitation[b] < 1.0f ? 0u : (temperature[b] < 1.0f ? 2u : 1u);
+		}
+	}
+	ch.Type = type.data();
+	for (uint32_t i = 0u; i < count; i++) {
+		const orc_layer_node& d = nodes[i];
+		if (d.Kind >= K::KindCount || d.Parent[0] >= static_cast<int32_t>(i) || d.Parent[1] >= static_cast<int32_t>(i)
+			|| (d.CacheSize & (d.CacheSize - 1u)) != 0u) {
+			return 1;
+		}
+		Chain::Node n;
+		n.Kind = d.Kind;
+		n.Param = d.Param;
+		n.Parent[0] = d.Parent[0];
+		n.Parent[1] = d.Parent[1];
+		// STPLayer.cpp:143-151
+		Seed mid = mixSeed(d.Salt, static_cast<long long>(d.Salt));
+		mid = mixSeed(mid, static_cast<long long>(mid));
+		mid = mixSeed(mid, static_cast<long long>(mid));
+		Seed s = mixSeed(global, static_cast<long long>(mid));
+		s = mixSeed(s, static_cast<long long>(mid));
+		n.LayerSeed = mixSeed(s, static_cast<long long>(mid));
+		Seed h = global;
+		if (d.Param == 1u) {
+			h = 14695981039346656037ull;
+			for (unsigned int k = 0u; k < 8u; k++) {
+				h ^= (global >> (8u * k)) & 0xFFu;
+				h *= 1099511628211ull;
+			}
+		}
+		n.VoronoiSeed = h;
+		if (d.CacheSize) {
+			n.Key.assign(d.CacheSize, ~0ull);
+			n.Value.assign(d.CacheSize, 0u);
+		}
+		ch.N.push_back(std::move(n));
+	}
+	const int32_t root = static_cast<int32_t>(count) - 1;
+	for (uint32_t m = 0u; m < maps; m++) {
+		for (uint32_t z = 0u; z < height; z++) {
+			for (uint32_t x = 0u; x < width; x++) {
+				out[static_cast<size_t>(m) * width * height + x + static_cast<size_t>(z) * width] =
+					ch.retrieve(root, static_cast<int>(x) + origin[2 * m], 0, static_cast<int>(z) + origin[2 * m + 1]);
+			}
+		}
+	}
+	return 0;
+}
+
+} // extern "C"

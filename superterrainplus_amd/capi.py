@@ -89,6 +89,30 @@ class WorldSetting(C.Structure):
                 ("ErodeChunks", C.c_uint32)]
 
 
+class LayerNode(C.Structure):
+    """stp_layer_node: one layer of a biome layer chain."""
+    _fields_ = [("Kind", C.c_uint32), ("Param", C.c_uint32), ("Parent", C.c_int32 * 2), ("Salt", C.c_uint64),
+                ("CacheSize", C.c_uint64)]
+
+
+class BiomeRegistry(C.Structure):
+    """stp_biome_registry: ids, temperatures, precipitations of the 35 named biomes."""
+    _fields_ = [("ID", C.c_uint16 * 35), ("Temperature", C.c_float * 35), ("Precipitation", C.c_float * 35)]
+
+
+def demo_layer_program():
+    """The demo's 14-layer chain (SuperDemo+/World/Layers/STPAllLayers.cpp:55-110) as LayerNode list."""
+    nodes, n = (LayerNode * 64)(), C.c_uint32()
+    call("stp_layer_program_demo", nodes, 64, C.byref(n))
+    return list(nodes[:n.value])
+
+
+def demo_biome_registry():
+    r = BiomeRegistry()
+    call("stp_biome_registry_demo", C.byref(r))
+    return r
+
+
 class ServerSetting(C.Structure):
     _fields_ = [("MemoryBudget", C.c_uint64), ("BatchPixels", C.c_uint64), ("BatchChunks", C.c_uint32),
                 ("MaxSide", C.c_uint32), ("Engines", C.c_uint32), ("ChunkWorldSize", C.c_uint32)]
@@ -220,6 +244,12 @@ _SIGNATURES = {
     "stp_world_schedule": (_I, [_P, _U32, _U32, _P]),
     "stp_world_render_area": (_I, [_P, C.c_int32, C.c_int32, _U32, _P, _P]),
     "stp_world_stream_word": (_I, [C.c_int32, C.c_int32, _P]),
+    "stp_layer_program_demo": (_I, [_P, _U32, _P]),
+    "stp_biome_registry_demo": (_I, [_P]),
+    "stp_layer_factory_create": (_I, [_P, _U32, _U64, _P, _U32, _U32, _P]),
+    "stp_layer_factory_destroy": (_I, [_P]),
+    "stp_layer_factory_generate": (_I, [_P, _P, C.c_int32, C.c_int32]),
+    "stp_layer_factory_generate_device": (_I, [_P, _P, _P, _U32, _P]),
 }
 
 _lib = None

@@ -4,6 +4,8 @@
 #include "stp_server.hpp"
 #include "stp_splat.hpp"
 #include "stp_world.hpp"
+
+#include <stp/STPBiomeLayer.hpp>
 #include "stp_internal.hpp"
 
 #include <stp/STPDeviceRuntime.hpp>
@@ -165,6 +167,10 @@ struct stp_rtc_module {
 
 struct stp_splat_database {
 	std::unique_ptr<STPSplatRuleTable> Table;
+};
+
+struct stp_layer_factory {
+	std::unique_ptr<STPDiversity::STPLayerBiomeFactory> Factory;
 };
 
 struct stp_heightfield_world {
@@ -1097,3 +1103,73 @@ int stp_world_stream_word(const int32_t x, const int32_t y, uint32_t* const word
 }
 
 } // extern "C"
+
+/* ---------------------------------------------------------------------------- biome layers */
+
+int stp_layer_program_demo(stp_layer_node* const h_nodes, const uint32_t capacity, uint32_t* const count) {
+	return guarded([&] {
+		require(count, "null count");
+		const STPDiversity::STPLayerProgram p = STPDiversity::STPLayerProgram::demo();
+		*count = static_cast<uint32_t>(p.Node.size());
+		require(h_nodes && capacity >= p.Node.size(), "node capacity too small");
+		for (size_t i = 0u; i < p.Node.size(); i++) {
+			const STPDiversity::STPLayerNode& n = p.Node[i];
+			h_nodes[i] = stp_layer_node {n.Kind, n.Param, {n.Parent[0], n.Parent[1]}, n.Salt, n.CacheSize};
+		}
+	});
+}
+
+int stp_biome_registry_demo(stp_biome_registry* const out) {
+	return guarded([&] {
+		require(out, "null registry");
+		const STPDiversity::STPBiomeRegistry r = STPDiversity::STPBiomeRegistry::demo();
+		static_assert(sizeof(out->ID) == sizeof(r.ID), "registry size");
+		std::memcpy(out->ID, r.ID, sizeof r.ID);
+		std::memcpy(out->Temperature, r.Temperature, sizeof r.Temperature);
+		std::memcpy(out->Precipitation, r.Precipitation, sizeof r.Precipitation);
+	});
+}
+
+int stp_layer_factory_create(const stp_layer_node* const h_nodes, const uint32_t count, const uint64_t global_seed,
+	const stp_biome_registry* const registry, const uint32_t dimension_x, const uint32_t dimension_y,
+	stp_layer_factory** const out) {
+	return guarded([&] {
+		require(out && h_nodes && registry, "null argument");
+		STPDiversity::STPLayerProgram p;
+		for (uint32_t i = 0u; i < count; i++) {
+			const stp_layer_node& n = h_nodes[i];
+			p.Node.push_back(STPDiversity::STPLayerNode {n.Kind, n.Salt, {n.Parent[0], n.Parent[1]}, n.Param, n.CacheSize});
+		}
+		STPDiversity::STPBiomeRegistry r {};
+		std::memcpy(r.ID, registry->ID, sizeof r.ID);
+		std::memcpy(r.Temperature, registry->Temperature, sizeof r.Temperature);
+		std::memcpy(r.Precipitation, registry->Precipitation, sizeof r.Precipitation);
+		auto f = std::make_unique<stp_layer_factory>();
+		f->Factory = std::make_unique<STPDiversity::STPLayerBiomeFactory>(p, global_seed, r, make_uint2(dimension_x, dimension_y));
+		*out = f.release();
+	});
+}
+
+int stp_layer_factory_destroy(stp_layer_factory* const factory) {
+	return guarded([&] { delete factory; });
+}
+
+int stp_layer_factory_generate(stp_layer_factory* const factory, uint16_t* const h_biomemap, const int32_t offset_x,
+	const int32_t offset_y) {
+	return guarded([&] {
+		require(factory && h_biomemap, "null argument");
+		(*factory->Factory)(h_biomemap, make_int2(offset_x, offset_y));
+	});
+}
+
+int stp_layer_factory_generate_device(stp_layer_factory* const factory, uint16_t* const d_biomemap,
+	const int32_t* const h_offsets, const uint32_t count, void* const stream) {
+	return guarded([&] {
+		require(factory && (count == 0u || (d_biomemap && h_offsets)), "null argument");
+		std::vector<int2> o(count);
+		for (uint32_t i = 0u; i < count; i++) {
+			o[i] = make_int2(h_offsets[2u * i], h_offsets[2u * i + 1u]);
+		}
+		factory->Factory->generateDevice(d_biomemap, o.data(), count, static_cast<hipStream_t>(stream));
+	});
+}
