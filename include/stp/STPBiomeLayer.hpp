@@ -100,6 +100,33 @@ namespace SuperTerrainPlus::STPDiversity {
 		static STPLayerProgram demo();
 	};
 
+	// A map the factory writes: sample (i, j) is the root layer at world
+	// (Origin.x + i * Step, 0, Origin.y + j * Step), stored at out[Offset + j * Pitch + i].
+	struct STPLayerMap {
+		int2 Origin;
+		uint2 Size;
+		unsigned int Step;
+		unsigned int Pitch;
+		unsigned long long Offset;
+	};
+
+	class STPLayerBiomeFactory;
+
+	// The rectangles every layer evaluates for a list of maps, their tiles and buffers, in device
+	// memory; rebuilt in place for a new list (its memory grows, never shrinks).
+	class STPLayerPlan {
+	public:
+		STPLayerPlan();
+		~STPLayerPlan();
+		STPLayerPlan(const STPLayerPlan&) = delete;
+		STPLayerPlan& operator=(const STPLayerPlan&) = delete;
+
+		unsigned int count() const noexcept;
+
+		struct STPImplementation;
+		std::unique_ptr<STPImplementation> Impl;
+	};
+
 	// The biome factory on the GPU (STPBiomeFactory): biome maps of BiomeDimension pixels whose pixel
 	// (x, y) is the root layer sampled at world (x + offset.x, 0, y + offset.y).
 	class STPLayerBiomeFactory {
@@ -124,6 +151,17 @@ namespace SuperTerrainPlus::STPDiversity {
 		// `pitch` elements, map stride `stride` elements.
 		void generateDevice(unsigned short* biomemap, const int2* origin, unsigned int count, unsigned int width,
 			unsigned int height, size_t pitch, size_t stride, hipStream_t stream);
+
+		// Builds `plan` for `maps` (uploads on `stream`). Throws STPInvalidArgument for empty maps or
+		// rectangles beyond 2^32 samples or the int coordinate range.
+		void plan(STPLayerPlan& plan, const std::vector<STPLayerMap>& maps, hipStream_t stream) const;
+		// Writes maps [first, last) of `plan` into `out`. A root id >= `limit` is written as 0 and
+		// flagged (status bit 1). Launches only, so a graph may capture it.
+		void run(const STPLayerPlan& plan, unsigned short* out, unsigned int first, unsigned int last, unsigned int limit,
+			hipStream_t stream) const;
+		// Bit 0: a layer read outside its parent's rectangle (an internal error); bit 1: a root id at
+		// or beyond the limit. Reading clears them; synchronous.
+		unsigned int status();
 
 		struct STPImplementation;
 

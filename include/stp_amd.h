@@ -321,6 +321,8 @@ int stp_rtc_launch(void* function, const uint32_t grid[3], const uint32_t block[
  * uneven size resident in device memory and runs the whole path for all of them: Voronoi biome
  * map with a radius halo -> single histogram filter -> multi-biome blend -> parallel erosion on
  * the chunk alone -> UINT16. */
+typedef struct stp_layer_factory stp_layer_factory;
+
 typedef struct {
 	uint64_t BiomeSeed;
 	uint32_t BiomeCount;
@@ -328,6 +330,11 @@ typedef struct {
 	uint32_t Radius;       /* filter radius, even */
 	uint32_t BinCapacity;  /* 1..64 */
 	float DropsPerPixel;   /* drops of a chunk = side^2 * DropsPerPixel */
+	/* NULL: the synthetic Voronoi biome map (BiomeSeed, BiomeCell). Otherwise the biome map of every
+	 * chunk and its filter halo comes from this layer chain (the reference's biome factory); ids at
+	 * or beyond BiomeCount are written as 0 and flagged (stp_layer_factory_status). The factory
+	 * must outlive the batch, server or world. */
+	stp_layer_factory* BiomeLayers;
 } stp_batch_setting;
 
 typedef struct {
@@ -505,8 +512,6 @@ typedef struct {
 	float Precipitation[35];
 } stp_biome_registry;
 
-typedef struct stp_layer_factory stp_layer_factory;
-
 /* The demo's chain (STPAllLayers.cpp:55-110, 14 nodes) into h_nodes (capacity >= 14). */
 int stp_layer_program_demo(stp_layer_node* h_nodes, uint32_t capacity, uint32_t* count);
 /* The demo's registry after loading SuperDemo+/Biome.ini (ocean 0, plains 1, desert 2, forest 3, the
@@ -521,9 +526,13 @@ int stp_layer_factory_destroy(stp_layer_factory* factory);
  * (x + offset_x, 0, y + offset_y). Synchronous, host memory. */
 int stp_layer_factory_generate(stp_layer_factory* factory, uint16_t* h_biomemap, int32_t offset_x, int32_t offset_y);
 /* `count` maps into device memory one after another, h_offsets = count (x, y) pairs. Asynchronous
- * on `stream`; one factory is used by one thread at a time. */
+ * on `stream`; one factory is used by one thread and one stream at a time (batches, servers and
+ * worlds that share it keep their own layer buffers). */
 int stp_layer_factory_generate_device(stp_layer_factory* factory, uint16_t* d_biomemap, const int32_t* h_offsets,
 	uint32_t count, void* stream);
+/* Bit 0: a layer read outside its rectangle (an internal error); bit 1: an id at or beyond a batch's
+ * BiomeCount was replaced by 0. Reading clears them. Synchronous. */
+int stp_layer_factory_status(stp_layer_factory* factory, uint32_t* status);
 
 /* ------------------------------------------------------------------ world pipeline
  * The reference world pipeline's generation chain (STPWorldPipeline.cpp: requestChunk :640-663,

@@ -64,7 +64,8 @@ class BiomeProperty(C.Structure):
 
 class BatchSetting(C.Structure):
     _fields_ = [("BiomeSeed", C.c_uint64), ("BiomeCount", C.c_uint32), ("BiomeCell", C.c_uint32),
-                ("Radius", C.c_uint32), ("BinCapacity", C.c_uint32), ("DropsPerPixel", C.c_float)]
+                ("Radius", C.c_uint32), ("BinCapacity", C.c_uint32), ("DropsPerPixel", C.c_float),
+                ("BiomeLayers", C.c_void_p)]
 
 
 class BatchChunk(C.Structure):
@@ -250,6 +251,7 @@ _SIGNATURES = {
     "stp_layer_factory_destroy": (_I, [_P]),
     "stp_layer_factory_generate": (_I, [_P, _P, C.c_int32, C.c_int32]),
     "stp_layer_factory_generate_device": (_I, [_P, _P, _P, _U32, _P]),
+    "stp_layer_factory_status": (_I, [_P, _P]),
 }
 
 _lib = None

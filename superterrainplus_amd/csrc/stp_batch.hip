@@ -1515,6 +1515,25 @@ void STPHeightfieldBatch::bind(const std::vector<STPBatchChunk>& chunks, const s
 		STP_THROW(STPCapacityError, "filter line segments exceed the segment-state arena");
 	}
 	this->releaseGraphs();
+	if (this->Setting.BiomeLayers) {
+		// halo sample (x, y) of a chunk is at world Origin + (x - r, y - r) * Step
+		std::vector<STPDiversity::STPLayerMap> maps(rec.size());
+		for (size_t i = 0u; i < rec.size(); i++) {
+			const Chunk& c = rec[i];
+			const unsigned int w = c.Side + 2u * radius;
+			const long long ox = c.OriginX - static_cast<long long>(radius) * c.Step,
+				oy = c.OriginY - static_cast<long long>(radius) * c.Step;
+			if (ox < INT32_MIN || oy < INT32_MIN) {
+				STP_THROW(STPInvalidArgument, "a chunk's biome halo leaves the int coordinate range of the biome layers");
+			}
+			maps[i] = STPDiversity::STPLayerMap {make_int2(static_cast<int>(ox), static_cast<int>(oy)), make_uint2(w, w), c.Step,
+				w, c.Halo};
+		}
+		if (!this->LayerPlan) {
+			this->LayerPlan = std::make_unique<STPDiversity::STPLayerPlan>();
+		}
+		this->Setting.BiomeLayers->plan(*this->LayerPlan, maps, stream);
+	}
 	this->Record = std::move(rec);
 	this->Pixels = pixel;
 	this->Drops = drops;
@@ -1596,7 +1615,8 @@ namespace {
 } // namespace
 
 bool STPHeightfieldBatch::itemPath() const noexcept {
-	// biome ids come from the synthetic biome map, so they are < BiomeCount; a column count is at
+	// biome ids come from the synthetic biome map or a layer chain that replaces ids >= BiomeCount by
+	// 0, so they are < BiomeCount; a column count is at
 	// most 2r + 1, which the 16-bit slots hold in 8 bits
 	return this->Setting.BiomeCount <= 8u && this->Setting.BinCapacity <= 8u && this->Setting.Radius <= 127u;
 }
@@ -1837,6 +1857,10 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 	case 0: {
 		if (clear_flag) {
 			zeroKERNEL<<<1, 64, 0, stream>>>(this->Flag.get(), 4u);
+		}
+		if (this->Setting.BiomeLayers) {
+			this->Setting.BiomeLayers->run(*this->LayerPlan, this->BiomeMap.get(), first, last, this->Setting.BiomeCount, stream);
+			break;
 		}
 		const auto [t, n] = range(0);
 		if (n) {

@@ -19,6 +19,9 @@ namespace SuperTerrainPlus {
 		unsigned int Radius;          // single histogram filter radius (even)
 		unsigned int BinCapacity;     // most distinct biomes one pixel's histogram may hold
 		float DropsPerPixel;          // raindrops of a chunk = side^2 * DropsPerPixel
+		// null: the synthetic Voronoi biome map; otherwise every chunk's biome map and filter halo come
+		// from this layer chain (ids >= BiomeCount are written as 0 and flagged by the factory)
+		STPDiversity::STPLayerBiomeFactory* BiomeLayers = nullptr;
 	};
 
 	struct STPBatchChunk {
@@ -189,6 +192,8 @@ namespace SuperTerrainPlus {
 		STPSmartDeviceMemory::STPDevice<uint2> DropInfo;
 		STPSmartDeviceMemory::STPDevice<unsigned int> Flag;
 		STPSmartDeviceMemory::STPDevice<unsigned long long> Counter;
+		// the layer chain's plan of the bound chunks' halo windows (biome maps from a layer chain)
+		std::unique_ptr<STPDiversity::STPLayerPlan> LayerPlan;
 
 		hipStream_t CaptureStream = nullptr;
 		hipGraph_t Graph = nullptr;

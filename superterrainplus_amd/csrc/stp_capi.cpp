@@ -59,6 +59,8 @@ namespace {
 		return n;
 	}
 
+	STPBatchSetting toBatchSetting(const stp_batch_setting* s);
+
 	STPEnvironment::STPRainDropSetting toRain(const stp_raindrop_setting* s) {
 		require(s, "null raindrop setting");
 		STPEnvironment::STPRainDropSetting r;
@@ -172,6 +174,21 @@ struct stp_splat_database {
 struct stp_layer_factory {
 	std::unique_ptr<STPDiversity::STPLayerBiomeFactory> Factory;
 };
+
+namespace {
+	STPBatchSetting toBatchSetting(const stp_batch_setting* const s) {
+		require(s, "null batch setting");
+		STPBatchSetting bs;
+		bs.BiomeSeed = s->BiomeSeed;
+		bs.BiomeCount = s->BiomeCount;
+		bs.BiomeCell = s->BiomeCell;
+		bs.Radius = s->Radius;
+		bs.BinCapacity = s->BinCapacity;
+		bs.DropsPerPixel = s->DropsPerPixel;
+		bs.BiomeLayers = s->BiomeLayers ? s->BiomeLayers->Factory.get() : nullptr;
+		return bs;
+	}
+} // namespace
 
 struct stp_heightfield_world {
 	std::unique_ptr<STPHeightfieldWorld> World;
@@ -751,13 +768,7 @@ int stp_batch_create(const stp_simplex_noise_setting* const noise, const stp_bio
 		for (uint32_t i = 0u; i < biome_count; i++) {
 			t.push_back(toBiome(h_table[i]));
 		}
-		STPBatchSetting bs;
-		bs.BiomeSeed = setting->BiomeSeed;
-		bs.BiomeCount = setting->BiomeCount;
-		bs.BiomeCell = setting->BiomeCell;
-		bs.Radius = setting->Radius;
-		bs.BinCapacity = setting->BinCapacity;
-		bs.DropsPerPixel = setting->DropsPerPixel;
+		const STPBatchSetting bs = toBatchSetting(setting);
 		std::vector<STPBatchChunk> c(chunk_count);
 		for (uint32_t i = 0u; i < chunk_count; i++) {
 			c[i] = STPBatchChunk {h_chunks[i].Side, h_chunks[i].Origin[0], h_chunks[i].Origin[1], h_chunks[i].Step};
@@ -863,13 +874,7 @@ int stp_server_create(const stp_simplex_noise_setting* const noise, const stp_bi
 		for (uint32_t i = 0u; i < biome_count; i++) {
 			t.push_back(toBiome(h_table[i]));
 		}
-		STPBatchSetting bs;
-		bs.BiomeSeed = batch->BiomeSeed;
-		bs.BiomeCount = batch->BiomeCount;
-		bs.BiomeCell = batch->BiomeCell;
-		bs.Radius = batch->Radius;
-		bs.BinCapacity = batch->BinCapacity;
-		bs.DropsPerPixel = batch->DropsPerPixel;
+		const STPBatchSetting bs = toBatchSetting(batch);
 		const STPServerSetting ss {setting->MemoryBudget, setting->BatchPixels, setting->BatchChunks, setting->MaxSide,
 			setting->Engines, setting->ChunkWorldSize};
 		auto s = std::make_unique<stp_heightfield_server>();
@@ -1021,13 +1026,7 @@ int stp_world_create(const stp_simplex_noise_setting* const noise, const stp_bio
 		for (uint32_t i = 0u; i < biome_count; i++) {
 			t.push_back(toBiome(h_table[i]));
 		}
-		STPBatchSetting bs;
-		bs.BiomeSeed = batch->BiomeSeed;
-		bs.BiomeCount = batch->BiomeCount;
-		bs.BiomeCell = batch->BiomeCell;
-		bs.Radius = batch->Radius;
-		bs.BinCapacity = batch->BinCapacity;
-		bs.DropsPerPixel = batch->DropsPerPixel;
+		const STPBatchSetting bs = toBatchSetting(batch);
 		const STPWorldSetting ws {setting->MapSize, setting->ErosionNeighbour, setting->GenerateChunks, setting->ErodeChunks};
 		auto w = std::make_unique<stp_heightfield_world>();
 		w->World = std::make_unique<STPHeightfieldWorld>(toNoise(noise), t, toRain(raindrop), bs, ws);
@@ -1171,5 +1170,12 @@ int stp_layer_factory_generate_device(stp_layer_factory* const factory, uint16_t
 			o[i] = make_int2(h_offsets[2u * i], h_offsets[2u * i + 1u]);
 		}
 		factory->Factory->generateDevice(d_biomemap, o.data(), count, static_cast<hipStream_t>(stream));
+	});
+}
+
+int stp_layer_factory_status(stp_layer_factory* const factory, uint32_t* const status) {
+	return guarded([&] {
+		require(factory && status, "null argument");
+		*status = factory->Factory->status();
 	});
 }

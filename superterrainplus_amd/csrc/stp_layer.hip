@@ -1,11 +1,13 @@
 // Biome layer chain on the GPU (STPBiomeLayer.hpp): bottom-up evaluation over rectangles.
 //
-// For a batch of maps, the host walks the chain from the root to the leaves and gives every node,
-// per map, the bounding rectangle of the coordinates its children read (a scale layer reads
-// x >> 1 .. (x + 1) >> 1, a cross layer one pixel around, a Voronoi layer ((x - 2) >> 2) + {0, 1},
-// the ocean transition 8 around). Each node is then one launch over all maps, parents first; the
-// root writes the maps. Every parent read is checked against the parent's rectangle (a miss sets a
-// flag and reads 0), so a wrong rectangle is reported instead of reading foreign memory.
+// A plan walks the chain from the root to the leaves and gives every node, per map, the bounding
+// rectangle of the coordinates its children read (a scale layer reads x >> 1 .. (x + 1) >> 1, a
+// cross layer one pixel around, a Voronoi layer ((x - 2) >> 2) + {0, 1}, the ocean transition 8
+// around), a place in the plan's buffer and a list of tiles. Running the plan is one launch per
+// node over the tiles of a range of maps, parents first; the root writes the caller's maps, at
+// a step between samples for coarse levels of detail. Every parent read is checked against the
+// parent's rectangle (a miss sets a flag and reads 0), so a wrong rectangle is reported instead of
+// reading foreign memory.
 //
 // Roofline: the root is the only large node (a Voronoi layer over the whole map, every lower node
 // is 1/16 of the one below it or smaller). Per pixel it reads one parent sample (L2-resident) and
@@ -31,22 +33,23 @@ namespace {
 	// two lattice heights of a 2D map (y = -1, 0)
 	constexpr unsigned int VoronoiRows = 16u, LatticeX = 18u, LatticeZ = 6u, LatticePoints = LatticeX * LatticeZ * 2u;
 
-	// A node's output buffer and rectangle table: map c's rectangle is Rect[Node * Count + c]
-	// (x0, z0, width, height); its pixel (x, z) is at Data[c * Stride + (z - z0) * Pitch + (x - x0)].
-	struct View {
-		const Sample* Data;
-		size_t Stride;
-		unsigned int Pitch, Node;
+	// Where map c's samples of a node lie: sample (i, j) is the layer at (X0 + i Step, Z0 + j Step),
+	// stored at base[Offset + j Pitch + i]; a node's rectangles are Rect[node * Count + c].
+	struct LayerRect {
+		int X0, Z0;
+		unsigned int W, H, Pitch, Step;
+		unsigned long long Offset;
 	};
+	static_assert(sizeof(LayerRect) == 32u, "rectangle layout");
 
 	struct NodeArgs {
-		unsigned int Kind, Param, Count;
+		unsigned int Kind, Param, Count, Node, Limit, Root;
 		Seed LayerSeed, VoronoiSeed;
-		Sample* Out;
-		size_t OutStride;
-		unsigned int OutPitch, OutNode;
-		View In[2];
-		const int4* Rect;
+		Sample* Out;       // the caller's maps for the root, the plan's buffer otherwise
+		const Sample* In;  // the plan's buffer (parents)
+		int Parent[2];
+		const LayerRect* Rect;
+		const uint2* Tile; // (map, tile) of this launch
 		L::Registry Reg;
 		unsigned int* Fault;
 	};
@@ -59,14 +62,14 @@ namespace {
 		unsigned int* Fault;
 
 		__device__ Reader(const NodeArgs& a, const unsigned int k, const unsigned int c) {
-			const View& v = a.In[k];
-			const int4 r = v.Data ? a.Rect[static_cast<size_t>(v.Node) * a.Count + c] : make_int4(0, 0, 0, 0);
-			this->Data = v.Data + c * v.Stride;
-			this->X0 = r.x;
-			this->Z0 = r.y;
-			this->W = static_cast<unsigned int>(r.z);
-			this->H = static_cast<unsigned int>(r.w);
-			this->Pitch = v.Pitch;
+			const int p = a.Parent[k];
+			const LayerRect r = p >= 0 ? a.Rect[static_cast<size_t>(p) * a.Count + c] : LayerRect {};
+			this->Data = a.In + r.Offset;
+			this->X0 = r.X0;
+			this->Z0 = r.Z0;
+			this->W = r.W;
+			this->H = r.H;
+			this->Pitch = r.Pitch;
 			this->Fault = a.Fault;
 		}
 		__device__ inline Sample operator()(const int x, const int z) const {
@@ -78,6 +81,15 @@ namespace {
 			return this->Data[static_cast<size_t>(v) * this->Pitch + u];
 		}
 	};
+
+	// the root's ids index the caller's biome table: one out of range is flagged and written as 0
+	__device__ inline Sample limited(const NodeArgs& a, const Sample v) {
+		if (a.Root && v >= a.Limit) {
+			atomicOr(a.Fault, 2u);
+			return 0u;
+		}
+		return v;
+	}
 
 	// One pixel of a non-Voronoi node (the demo layers' sample(x, y, z) with y = 0).
 	__device__ Sample samplePixel(const NodeArgs& a, const Reader& p, const Reader& q, const int x, const int z) {
@@ -228,59 +240,69 @@ namespace {
 	}
 
 	__global__ void __launch_bounds__(LayerX * LayerY) layerKERNEL(const NodeArgs a) {
-		const unsigned int c = blockIdx.z;
-		const int4 rect = a.Rect[static_cast<size_t>(a.OutNode) * a.Count + c];
-		const unsigned int lx = blockIdx.x * LayerX + threadIdx.x, lz = blockIdx.y * LayerY + threadIdx.y;
-		if (lx >= static_cast<unsigned int>(rect.z) || lz >= static_cast<unsigned int>(rect.w)) {
+		const uint2 t = a.Tile[blockIdx.x];
+		const LayerRect rect = a.Rect[static_cast<size_t>(a.Node) * a.Count + t.x];
+		const unsigned int tilesX = (rect.W + LayerX - 1u) / LayerX;
+		const unsigned int lx = (t.y % tilesX) * LayerX + threadIdx.x, lz = (t.y / tilesX) * LayerY + threadIdx.y;
+		if (lx >= rect.W || lz >= rect.H) {
 			return;
 		}
-		const Reader p(a, 0u, c), q(a, 1u, c);
-		a.Out[c * a.OutStride + static_cast<size_t>(lz) * a.OutPitch + lx] =
-			samplePixel(a, p, q, rect.x + static_cast<int>(lx), rect.y + static_cast<int>(lz));
+		const Reader p(a, 0u, t.x), q(a, 1u, t.x);
+		const Sample v = samplePixel(a, p, q, rect.X0 + static_cast<int>(lx * rect.Step),
+			rect.Z0 + static_cast<int>(lz * rect.Step));
+		a.Out[rect.Offset + static_cast<size_t>(lz) * rect.Pitch + lx] = limited(a, v);
+	}
+
+	// The jitter of lattice point (px, py, pz) (STPVoronoiLayer.h:16-43), x, y and z components.
+	__device__ inline void voronoiPoint(const Seed seed, const int px, const int py, const int pz, double* const j) {
+		Seed m = L::mixSeed(seed, px);
+		m = L::mixSeed(m, py);
+		m = L::mixSeed(m, pz);
+		m = L::mixSeed(m, px);
+		m = L::mixSeed(m, py);
+		m = L::mixSeed(m, pz);
+		j[0] = L::voronoiJitter(m);
+		m = L::mixSeed(m, static_cast<long long>(seed));
+		j[1] = L::voronoiJitter(m);
+		m = L::mixSeed(m, static_cast<long long>(seed));
+		j[2] = L::voronoiJitter(m);
 	}
 
 	// STPVoronoiLayer.h:59-103 in 2D (y = 0): the nearest of the 8 jittered lattice points around
-	// ((x - 2) / 4, -2 / 4, (z - 2) / 4) picks which parent sample the pixel takes. The jitter of a
-	// lattice point (STPVoronoiLayer.h:16-43) depends on the point only, so the tile's points are
-	// jittered once, into LDS.
+	// ((x - 2) / 4, -2 / 4, (z - 2) / 4) picks which parent sample the pixel takes. At step 1 the
+	// tile's lattice points are jittered once, into LDS, and shared by 16 pixels each; at coarser
+	// steps every pixel jitters its 8 points itself.
 	__global__ void __launch_bounds__(LayerX * LayerY) voronoiKERNEL(const NodeArgs a) {
 		__shared__ double jitter[LatticePoints][3];
-		const unsigned int c = blockIdx.z;
-		const int4 rect = a.Rect[static_cast<size_t>(a.OutNode) * a.Count + c];
-		const int tx0 = rect.x + static_cast<int>(blockIdx.x * LayerX), tz0 = rect.y + static_cast<int>(blockIdx.y * VoronoiRows);
+		const uint2 t = a.Tile[blockIdx.x];
+		const LayerRect rect = a.Rect[static_cast<size_t>(a.Node) * a.Count + t.x];
+		const unsigned int tilesX = (rect.W + LayerX - 1u) / LayerX;
+		const unsigned int bx = (t.y % tilesX) * LayerX, bz = (t.y / tilesX) * VoronoiRows;
+		const bool staged = rect.Step == 1u; // uniform over the block: one map per tile
+		const int tx0 = rect.X0 + static_cast<int>(bx), tz0 = rect.Z0 + static_cast<int>(bz);
 		const int ax = (tx0 - 2) >> 2, az = (tz0 - 2) >> 2;
-		const unsigned int tid = threadIdx.y * LayerX + threadIdx.x;
-		if (tid < LatticePoints) {
-			const int i = static_cast<int>(tid % LatticeX), rest = static_cast<int>(tid / LatticeX);
-			const int px = ax + i, py = rest % 2 - 1, pz = az + rest / 2;
-			const Seed seed = a.VoronoiSeed;
-			Seed m = L::mixSeed(seed, px);
-			m = L::mixSeed(m, py);
-			m = L::mixSeed(m, pz);
-			m = L::mixSeed(m, px);
-			m = L::mixSeed(m, py);
-			m = L::mixSeed(m, pz);
-			jitter[tid][0] = L::voronoiJitter(m);
-			m = L::mixSeed(m, static_cast<long long>(seed));
-			jitter[tid][1] = L::voronoiJitter(m);
-			m = L::mixSeed(m, static_cast<long long>(seed));
-			jitter[tid][2] = L::voronoiJitter(m);
+		if (staged) {
+			const unsigned int tid = threadIdx.y * LayerX + threadIdx.x;
+			if (tid < LatticePoints) {
+				const int i = static_cast<int>(tid % LatticeX), rest = static_cast<int>(tid / LatticeX);
+				voronoiPoint(a.VoronoiSeed, ax + i, rest % 2 - 1, az + rest / 2, jitter[tid]);
+			}
+			__syncthreads();
 		}
-		__syncthreads();
 
-		const Reader p(a, 0u, c);
-		const unsigned int lx = blockIdx.x * LayerX + threadIdx.x;
-		if (lx >= static_cast<unsigned int>(rect.z)) {
+		const Reader p(a, 0u, t.x);
+		const unsigned int lx = bx + threadIdx.x;
+		if (lx >= rect.W) {
 			return;
 		}
-		const int x = rect.x + static_cast<int>(lx), ix = x - 2, lmx = ix >> 2;
+		const int x = rect.X0 + static_cast<int>(lx * rect.Step), ix = x - 2, lmx = ix >> 2;
 		const double fx = static_cast<double>(ix & 3) / 4.0;
 		for (unsigned int k = 0u; k < VoronoiRows / LayerY; k++) {
-			const unsigned int lz = blockIdx.y * VoronoiRows + threadIdx.y + k * LayerY;
-			if (lz >= static_cast<unsigned int>(rect.w)) {
+			const unsigned int lz = bz + threadIdx.y + k * LayerY;
+			if (lz >= rect.H) {
 				break;
 			}
-			const int z = rect.y + static_cast<int>(lz), iz = z - 2, lmz = iz >> 2;
+			const int z = rect.Z0 + static_cast<int>(lz * rect.Step), iz = z - 2, lmz = iz >> 2;
 			const double fz = static_cast<double>(iz & 3) / 4.0;
 			unsigned int best = 0u;
 			double min = 0.0;
@@ -288,17 +310,29 @@ namespace {
 			for (unsigned int q = 0u; q < 8u; q++) {
 				const unsigned int dx = (q >> 2u) & 1u, dy = (q >> 1u) & 1u, dz = q & 1u;
 				const double gx = dx ? fx - 1.0 : fx, gy = dy ? 0.5 - 1.0 : 0.5, gz = dz ? fz - 1.0 : fz;
-				const double* const j = jitter[((lmz + static_cast<int>(dz) - az) * 2 + static_cast<int>(dy)) * static_cast<int>(LatticeX)
-					+ (lmx + static_cast<int>(dx) - ax)];
-				const double sz = gz + j[2], sy = gy + j[1], sx = gx + j[0];
+				double jx, jy, jz;
+				if (staged) {
+					const double* const j = jitter[((lmz + static_cast<int>(dz) - az) * 2 + static_cast<int>(dy))
+						* static_cast<int>(LatticeX) + (lmx + static_cast<int>(dx) - ax)];
+					jx = j[0];
+					jy = j[1];
+					jz = j[2];
+				} else {
+					double own[3];
+					voronoiPoint(a.VoronoiSeed, lmx + static_cast<int>(dx), static_cast<int>(dy) - 1, lmz + static_cast<int>(dz), own);
+					jx = own[0];
+					jy = own[1];
+					jz = own[2];
+				}
+				const double sz = gz + jz, sy = gy + jy, sx = gx + jx;
 				const double d = sz * sz + sy * sy + sx * sx;
 				if (q == 0u || d < min) {
 					best = q;
 					min = d;
 				}
 			}
-			a.Out[c * a.OutStride + static_cast<size_t>(lz) * a.OutPitch + lx] =
-				p(lmx + static_cast<int>((best >> 2u) & 1u), lmz + static_cast<int>(best & 1u));
+			a.Out[rect.Offset + static_cast<size_t>(lz) * rect.Pitch + lx] =
+				limited(a, p(lmx + static_cast<int>((best >> 2u) & 1u), lmz + static_cast<int>(best & 1u)));
 		}
 	}
 
@@ -387,18 +421,15 @@ STPLayerProgram STPLayerProgram::demo() {
 	return p;
 }
 
-struct STPLayerBiomeFactory::STPImplementation {
-	std::vector<STPLayerNode> Node;
-	std::vector<Seed> LayerSeed, VoronoiSeed;
-	std::vector<unsigned char> Reachable;
-	L::Registry Reg {};
-	STPSmartDeviceMemory::STPDevice<unsigned char> Precipitation;
-	STPSmartDeviceMemory::STPDevice<unsigned int> Fault;
-
-	// per call: rectangles (device, via pinned staging), node buffers
-	STPSmartDeviceMemory::STPHost<int4> Staging;
-	STPSmartDeviceMemory::STPDevice<int4> Rect;
-	size_t RectCapacity = 0u;
+struct STPLayerPlan::STPImplementation {
+	unsigned int Count = 0u;
+	// per node and map, the node's first tile of the map in the tile table; [node * (Count + 1) + map]
+	std::vector<unsigned int> TileBegin;
+	size_t RectBytes = 0u;
+	// rectangles, then tiles, in one upload
+	STPSmartDeviceMemory::STPHost<unsigned char> Staging;
+	STPSmartDeviceMemory::STPDevice<unsigned char> Table;
+	size_t TableCapacity = 0u;
 	STPSmartDeviceMemory::STPDevice<Sample> Arena;
 	size_t ArenaCapacity = 0u;
 	hipEvent_t Uploaded = nullptr;
@@ -409,6 +440,27 @@ struct STPLayerBiomeFactory::STPImplementation {
 			(void)hipEventDestroy(this->Uploaded);
 		}
 	}
+};
+
+STPLayerPlan::STPLayerPlan() : Impl(std::make_unique<STPImplementation>()) {
+	STP_CHECK_HIP(hipEventCreateWithFlags(&this->Impl->Uploaded, hipEventDisableTiming));
+}
+
+STPLayerPlan::~STPLayerPlan() = default;
+
+unsigned int STPLayerPlan::count() const noexcept {
+	return this->Impl->Count;
+}
+
+struct STPLayerBiomeFactory::STPImplementation {
+	std::vector<STPLayerNode> Node;
+	std::vector<Seed> LayerSeed, VoronoiSeed;
+	std::vector<unsigned char> Reachable;
+	L::Registry Reg {};
+	STPSmartDeviceMemory::STPDevice<unsigned char> Precipitation;
+	STPSmartDeviceMemory::STPDevice<unsigned int> Fault;
+	// the plan of generateDevice
+	STPLayerPlan Plan;
 };
 
 STPLayerBiomeFactory::STPLayerBiomeFactory(const STPLayerProgram& program, const unsigned long long global_seed,
@@ -491,7 +543,6 @@ STPLayerBiomeFactory::STPLayerBiomeFactory(const STPLayerProgram& program, const
 	m.Reg.Precipitation = m.Precipitation.get();
 	m.Fault = STPSmartDeviceMemory::makeDevice<unsigned int>(1u);
 	STP_CHECK_HIP(hipMemset(m.Fault.get(), 0, sizeof(unsigned int)));
-	STP_CHECK_HIP(hipEventCreateWithFlags(&m.Uploaded, hipEventDisableTiming));
 }
 
 STPLayerBiomeFactory::~STPLayerBiomeFactory() = default;
@@ -504,9 +555,7 @@ void STPLayerBiomeFactory::operator()(unsigned short* const biomemap, const int2
 	auto out = STPSmartDeviceMemory::makeDevice<unsigned short>(pixel);
 	this->generateDevice(out.get(), &offset, 1u, nullptr);
 	STP_CHECK_HIP(hipMemcpy(biomemap, out.get(), pixel * sizeof(unsigned short), hipMemcpyDeviceToHost));
-	unsigned int fault = 0u;
-	STP_CHECK_HIP(hipMemcpy(&fault, this->Impl->Fault.get(), sizeof fault, hipMemcpyDeviceToHost));
-	if (fault) {
+	if (this->status() & 1u) {
 		throw std::logic_error("biome layer read outside its parent's rectangle");
 	}
 }
@@ -525,26 +574,44 @@ void STPLayerBiomeFactory::generateDevice(unsigned short* const biomemap, const 
 	if (!biomemap || !origin) {
 		STP_THROW(STPInvalidArgument, "null biome map or origins");
 	}
-	if (width == 0u || height == 0u || pitch < width || (count > 1u && stride < pitch * (height - 1u) + width)) {
+	if (width == 0u || height == 0u || pitch < width || pitch > 0xFFFFFFFFu
+		|| (count > 1u && stride < pitch * (height - 1u) + width)) {
 		STP_THROW(STPInvalidArgument, "biome map rectangle: width and height positive, pitch >= width, maps not overlapping");
 	}
-	STPImplementation& m = *this->Impl;
-	const size_t n = m.Node.size(), root = n - 1u;
-
-	// rectangles: from the root down, every node gets the union of what its children read
-	std::vector<Box> box(n * count);
+	std::vector<STPLayerMap> maps(count);
 	for (unsigned int c = 0u; c < count; c++) {
-		box[root * count + c] = Box {origin[c].x, origin[c].y, static_cast<long long>(origin[c].x) + width - 1,
-			static_cast<long long>(origin[c].y) + height - 1};
+		maps[c] = STPLayerMap {origin[c], make_uint2(width, height), 1u, static_cast<unsigned int>(pitch), c * stride};
+	}
+	this->plan(this->Impl->Plan, maps, stream);
+	this->run(this->Impl->Plan, biomemap, 0u, count, 65536u, stream);
+}
+
+void STPLayerBiomeFactory::plan(STPLayerPlan& plan, const std::vector<STPLayerMap>& maps, const hipStream_t stream) const {
+	const STPImplementation& m = *this->Impl;
+	STPLayerPlan::STPImplementation& q = *plan.Impl;
+	const size_t n = m.Node.size(), root = n - 1u, count = maps.size();
+	if (count == 0u || count > 0xFFFFFFFFu) {
+		STP_THROW(STPInvalidArgument, "a plan needs at least one map");
+	}
+	// rectangles: from the root down, every node gets the bounding box of what its children read
+	std::vector<Box> box(n * count);
+	for (size_t c = 0u; c < count; c++) {
+		const STPLayerMap& map = maps[c];
+		const unsigned int step = map.Step == 0u ? 1u : map.Step;
+		if (map.Size.x == 0u || map.Size.y == 0u || map.Pitch < map.Size.x) {
+			STP_THROW(STPInvalidArgument, "a map needs a positive size and a pitch of at least its width");
+		}
+		box[root * count + c] = Box {map.Origin.x, map.Origin.y,
+			map.Origin.x + static_cast<long long>(map.Size.x - 1u) * step, map.Origin.y + static_cast<long long>(map.Size.y - 1u) * step};
 	}
 	for (size_t i = n; i-- > 0u;) {
+		if (!m.Reachable[i]) {
+			continue;
+		}
 		const STPLayerNode& node = m.Node[i];
 		const auto kind = static_cast<STPLayerKind>(node.Kind);
-		for (unsigned int c = 0u; c < count; c++) {
+		for (size_t c = 0u; c < count; c++) {
 			const Box& b = box[i * count + c];
-			if (b.empty() || !m.Reachable[i]) {
-				continue;
-			}
 			Box need;
 			switch (kind) {
 			case STPLayerKind::Continent:
@@ -570,98 +637,134 @@ void STPLayerBiomeFactory::generateDevice(unsigned short* const biomemap, const 
 			}
 		}
 	}
-	// buffer of every node below the root: its widest rectangle in every map
-	std::vector<unsigned int> pitchOf(n, 0u), heightOf(n, 0u);
-	std::vector<size_t> offsetOf(n, 0u);
-	size_t arena = 0u;
-	for (size_t i = 0u; i < root; i++) {
-		if (!m.Reachable[i]) {
-			continue;
-		}
-		unsigned long long w = 0u, h = 0u;
-		for (unsigned int c = 0u; c < count; c++) {
-			const Box& b = box[i * count + c];
-			w = std::max<unsigned long long>(w, static_cast<unsigned long long>(b.X1 - b.X0 + 1));
-			h = std::max<unsigned long long>(h, static_cast<unsigned long long>(b.Z1 - b.Z0 + 1));
-		}
-		if (w > (1ull << 20u) || h > (1ull << 20u) || w * h * count > (1ull << 32u)) {
-			STP_THROW(STPInvalidArgument, "a biome layer's rectangles are too large (more than 2^32 samples in a batch)");
-		}
-		pitchOf[i] = static_cast<unsigned int>(w);
-		heightOf[i] = static_cast<unsigned int>(h);
-		offsetOf[i] = arena;
-		arena += (static_cast<size_t>(w) * h * count + 127u) & ~size_t {127u};
-	}
 	// any coordinate outside int is outside what the reference's int arithmetic can address
 	for (const Box& b : box) {
 		if (!b.empty() && (b.X0 < INT32_MIN || b.Z0 < INT32_MIN || b.X1 > INT32_MAX || b.Z1 > INT32_MAX)) {
 			STP_THROW(STPInvalidArgument, "biome map coordinates out of the int range");
 		}
 	}
+	// rectangles, buffer places and tiles
+	std::vector<LayerRect> rect(n * count, LayerRect {});
+	std::vector<uint2> tile;
+	q.TileBegin.assign(n * (count + 1u), 0u);
+	unsigned long long arena = 0u;
+	for (size_t i = 0u; i < n; i++) {
+		const bool voronoi = m.Node[i].Kind == static_cast<unsigned int>(STPLayerKind::Voronoi);
+		const unsigned int rows = voronoi ? VoronoiRows : LayerY;
+		for (size_t c = 0u; c < count; c++) {
+			q.TileBegin[i * (count + 1u) + c] = static_cast<unsigned int>(tile.size());
+			if (!m.Reachable[i]) {
+				continue;
+			}
+			const Box& b = box[i * count + c];
+			LayerRect& r = rect[i * count + c];
+			r.X0 = static_cast<int>(b.X0);
+			r.Z0 = static_cast<int>(b.Z0);
+			if (i == root) {
+				const STPLayerMap& map = maps[c];
+				r.W = map.Size.x;
+				r.H = map.Size.y;
+				r.Pitch = map.Pitch;
+				r.Step = map.Step == 0u ? 1u : map.Step;
+				r.Offset = map.Offset;
+			} else {
+				const unsigned long long w = static_cast<unsigned long long>(b.X1 - b.X0 + 1),
+					h = static_cast<unsigned long long>(b.Z1 - b.Z0 + 1);
+				if (w > (1ull << 24u) || h > (1ull << 24u)) {
+					STP_THROW(STPInvalidArgument, "a biome layer's rectangle is too large");
+				}
+				r.W = static_cast<unsigned int>(w);
+				r.H = static_cast<unsigned int>(h);
+				r.Pitch = r.W;
+				r.Step = 1u;
+				r.Offset = arena;
+				arena += w * h;
+			}
+			const unsigned long long tiles = static_cast<unsigned long long>((r.W + LayerX - 1u) / LayerX) * ((r.H + rows - 1u) / rows);
+			if (arena > (1ull << 32u) || tile.size() + tiles > 0x7FFFFFFFull) {
+				STP_THROW(STPInvalidArgument, "a batch of biome maps needs too many layer samples (more than 2^32)");
+			}
+			for (unsigned long long t = 0u; t < tiles; t++) {
+				tile.push_back(make_uint2(static_cast<unsigned int>(c), static_cast<unsigned int>(t)));
+			}
+		}
+		q.TileBegin[i * (count + 1u) + count] = static_cast<unsigned int>(tile.size());
+	}
 
-	// upload the rectangles through pinned staging, once the previous upload has finished
-	const size_t rects = n * count;
-	STP_CHECK_HIP(hipEventSynchronize(m.Uploaded));
-	if (rects > m.RectCapacity) {
+	// upload through pinned staging, once the previous upload has left it; grow after the device
+	// has finished with the old tables
+	const size_t rectBytes = rect.size() * sizeof(LayerRect), bytes = rectBytes + tile.size() * sizeof(uint2);
+	STP_CHECK_HIP(hipEventSynchronize(q.Uploaded));
+	if (bytes > q.TableCapacity || arena > q.ArenaCapacity) {
 		STP_CHECK_HIP(hipDeviceSynchronize());
-		m.Staging = STPSmartDeviceMemory::makeHost<int4>(rects);
-		m.Rect = STPSmartDeviceMemory::makeDevice<int4>(rects);
-		m.RectCapacity = rects;
+		if (bytes > q.TableCapacity) {
+			q.TableCapacity = bytes + bytes / 4u;
+			q.Staging = STPSmartDeviceMemory::makeHost<unsigned char>(q.TableCapacity);
+			q.Table = STPSmartDeviceMemory::makeDevice<unsigned char>(q.TableCapacity);
+		}
+		if (arena > q.ArenaCapacity) {
+			q.ArenaCapacity = static_cast<size_t>(arena + arena / 4u + 1u);
+			q.Arena = STPSmartDeviceMemory::makeDevice<Sample>(q.ArenaCapacity);
+		}
 	}
-	if (arena > m.ArenaCapacity) {
-		STP_CHECK_HIP(hipDeviceSynchronize());
-		m.Arena = STPSmartDeviceMemory::makeDevice<Sample>(arena);
-		m.ArenaCapacity = arena;
-	}
-	for (size_t r = 0u; r < rects; r++) {
-		const Box& b = box[r];
-		m.Staging[r] = b.empty() ? make_int4(0, 0, 0, 0)
-			: make_int4(static_cast<int>(b.X0), static_cast<int>(b.Z0), static_cast<int>(b.X1 - b.X0 + 1),
-				static_cast<int>(b.Z1 - b.Z0 + 1));
-	}
-	STP_CHECK_HIP(hipMemcpyAsync(m.Rect.get(), m.Staging.get(), rects * sizeof(int4), hipMemcpyHostToDevice, stream));
-	STP_CHECK_HIP(hipEventRecord(m.Uploaded, stream));
+	std::memcpy(q.Staging.get(), rect.data(), rectBytes);
+	std::memcpy(q.Staging.get() + rectBytes, tile.data(), tile.size() * sizeof(uint2));
+	STP_CHECK_HIP(hipMemcpyAsync(q.Table.get(), q.Staging.get(), bytes, hipMemcpyHostToDevice, stream));
+	STP_CHECK_HIP(hipEventRecord(q.Uploaded, stream));
+	q.RectBytes = rectBytes;
+	q.Count = static_cast<unsigned int>(count);
+}
 
+void STPLayerBiomeFactory::run(const STPLayerPlan& plan, unsigned short* const out, const unsigned int first,
+	const unsigned int last, const unsigned int limit, const hipStream_t stream) const {
+	const STPImplementation& m = *this->Impl;
+	const STPLayerPlan::STPImplementation& q = *plan.Impl;
+	if (!out || first > last || last > q.Count) {
+		STP_THROW(STPInvalidArgument, "maps [first, last) of the plan, into a non-null buffer");
+	}
+	const size_t n = m.Node.size(), root = n - 1u, count = q.Count;
+	const auto* const rects = reinterpret_cast<const LayerRect*>(q.Table.get());
+	const auto* const tiles = reinterpret_cast<const uint2*>(q.Table.get() + q.RectBytes);
 	for (size_t i = 0u; i < n; i++) {
 		if (!m.Reachable[i]) {
+			continue;
+		}
+		const unsigned int tb = q.TileBegin[i * (count + 1u) + first], te = q.TileBegin[i * (count + 1u) + last];
+		if (te == tb) {
 			continue;
 		}
 		const STPLayerNode& node = m.Node[i];
 		NodeArgs a {};
 		a.Kind = node.Kind;
 		a.Param = node.Param;
-		a.Count = count;
+		a.Count = static_cast<unsigned int>(count);
+		a.Node = static_cast<unsigned int>(i);
+		a.Limit = limit;
+		a.Root = i == root;
 		a.LayerSeed = m.LayerSeed[i];
 		a.VoronoiSeed = m.VoronoiSeed[i];
-		a.OutNode = static_cast<unsigned int>(i);
-		if (i == root) {
-			a.Out = biomemap;
-			a.OutStride = stride;
-			a.OutPitch = static_cast<unsigned int>(pitch);
-		} else {
-			a.Out = m.Arena.get() + offsetOf[i];
-			a.OutPitch = pitchOf[i];
-			a.OutStride = static_cast<size_t>(pitchOf[i]) * heightOf[i];
-		}
-		for (unsigned int k = 0u; k < 2u; k++) {
-			const int p = node.Parent[k];
-			if (p >= 0) {
-				a.In[k] = View {m.Arena.get() + offsetOf[static_cast<size_t>(p)],
-					static_cast<size_t>(pitchOf[static_cast<size_t>(p)]) * heightOf[static_cast<size_t>(p)],
-					pitchOf[static_cast<size_t>(p)], static_cast<unsigned int>(p)};
-			}
-		}
-		a.Rect = m.Rect.get();
+		a.Out = i == root ? out : q.Arena.get();
+		a.In = q.Arena.get();
+		a.Parent[0] = node.Parent[0];
+		a.Parent[1] = node.Parent[1];
+		a.Rect = rects;
+		a.Tile = tiles + tb;
 		a.Reg = m.Reg;
 		a.Fault = m.Fault.get();
-		const unsigned int w = i == root ? width : pitchOf[i], h = i == root ? height : heightOf[i];
 		if (node.Kind == static_cast<unsigned int>(STPLayerKind::Voronoi)) {
-			const dim3 grid((w + LayerX - 1u) / LayerX, (h + VoronoiRows - 1u) / VoronoiRows, count);
-			voronoiKERNEL<<<grid, dim3(LayerX, LayerY), 0, stream>>>(a);
+			voronoiKERNEL<<<te - tb, dim3(LayerX, LayerY), 0, stream>>>(a);
 		} else {
-			const dim3 grid((w + LayerX - 1u) / LayerX, (h + LayerY - 1u) / LayerY, count);
-			layerKERNEL<<<grid, dim3(LayerX, LayerY), 0, stream>>>(a);
+			layerKERNEL<<<te - tb, dim3(LayerX, LayerY), 0, stream>>>(a);
 		}
 		STP_CHECK_HIP(hipGetLastError());
 	}
+}
+
+unsigned int STPLayerBiomeFactory::status() {
+	unsigned int fault = 0u;
+	STP_CHECK_HIP(hipMemcpy(&fault, this->Impl->Fault.get(), sizeof fault, hipMemcpyDeviceToHost));
+	if (fault) {
+		STP_CHECK_HIP(hipMemset(this->Impl->Fault.get(), 0, sizeof(unsigned int)));
+	}
+	return fault;
 }

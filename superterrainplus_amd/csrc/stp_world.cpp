@@ -328,9 +328,20 @@ void STPHeightfieldWorld::renderArea(const int2 centre, const unsigned int radiu
 			map * sizeof(unsigned short), map * sizeof(unsigned short), map, hipMemcpyDeviceToDevice, this->Stream));
 	}
 	// the biome map the heightmap batch sampled: chunk (X, Y) starts at world pixel (X, Y) * MapSize
-	STPHeightfieldKernel::voronoiBiomemap(this->Batch.BiomeSeed, this->Batch.BiomeCount, this->Batch.BiomeCell,
-		static_cast<long long>(centre.x - r) * map, static_cast<long long>(centre.y - r) * map, biome,
-		static_cast<unsigned int>(side), static_cast<unsigned int>(side), static_cast<unsigned int>(side), this->Stream);
+	const long long ox = static_cast<long long>(centre.x - r) * map, oy = static_cast<long long>(centre.y - r) * map;
+	if (this->Batch.BiomeLayers) {
+		if (ox < INT32_MIN || oy < INT32_MIN || ox + static_cast<long long>(side) > INT32_MAX
+			|| oy + static_cast<long long>(side) > INT32_MAX) {
+			STP_THROW(STPInvalidArgument, "the rendered area leaves the int coordinate range of the biome layers");
+		}
+		const int2 origin = make_int2(static_cast<int>(ox), static_cast<int>(oy));
+		this->Batch.BiomeLayers->generateDevice(biome, &origin, 1u, static_cast<unsigned int>(side),
+			static_cast<unsigned int>(side), side, side * side, this->Stream);
+	} else {
+		STPHeightfieldKernel::voronoiBiomemap(this->Batch.BiomeSeed, this->Batch.BiomeCount, this->Batch.BiomeCell, ox, oy,
+			biome, static_cast<unsigned int>(side), static_cast<unsigned int>(side), static_cast<unsigned int>(side),
+			this->Stream);
+	}
 	STP_CHECK_HIP(hipStreamSynchronize(this->Stream));
 }
 
