@@ -51,12 +51,12 @@ def workload():
     return cfg, sides, origins, props
 
 
-def make_batch(cfg, sides, origins, props):
+def make_batch(cfg, sides, origins, props, layers=None):
     from superterrainplus_amd import capi, workloads as W
     noise = capi.SimplexNoiseSetting(cfg.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, cfg.engine)
     # a pixel's histogram can hold at most biome_count distinct biomes
     setting = capi.BatchSetting(cfg.biome_seed, cfg.biome_count, cfg.cell, cfg.radius, cfg.biome_count,
-                                cfg.drops_per_pixel)
+                                cfg.drops_per_pixel, layers)
     chunks = (capi.BatchChunk * len(sides))(*[capi.BatchChunk(s, (C.c_int32 * 2)(*o)) for s, o in zip(sides, origins)])
     h = C.c_void_p()
     capi.call("stp_batch_create", C.byref(noise), capi.biome_array(props), len(props),
@@ -200,7 +200,7 @@ def world(plain_frames=6):
     return out
 
 
-def layers(reps=10, maps=64):
+def layers(reps=10, maps=64, tuning=None, steps=5):
     """Biome maps from the demo's layer chain (14 layers ending in 3 Voronoi zooms, seed 6666, the demo
     registry): `maps` maps of 512^2 at the chunk offsets of an 8 x 8 block, in one call. The oracle's
     recursive evaluation (the reference's algorithm, one production line, one thread) on one map of
@@ -244,8 +244,40 @@ def layers(reps=10, maps=64):
         pass
     print(f"[bench] biome layers: {ms:.3f} ms per {maps} x 512^2 maps, {mpx:.0f} Mpixel/s"
           + (f" (CPU oracle {cpu['mpixel_s']} Mpixel/s)" if cpu else ""), file=sys.stderr, flush=True)
+    # the batched workload with its biome maps (chunks and filter halos) from the demo chain
+    cfg, sides, origins, props = workload()
+    capi.call("stp_layer_factory_create", arr, len(nodes), 6666, C.byref(reg), side, side, C.byref(h))
+    batch = make_batch(cfg, sides, origins, props, layers=h)
+    try:
+        if tuning is not None:
+            capi.call("stp_batch_set_tuning", batch, C.byref(tuning))
+        capi.call("stp_batch_set_timing", batch, 1)
+        capi.call("stp_batch_run", batch, 0, None)
+        capi.synchronize()
+        stage = np.zeros(len(STAGES))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            capi.call("stp_batch_run", batch, 0, None)
+            capi.synchronize()
+            t = (C.c_float * len(STAGES))()
+            capi.call("stp_batch_stage_times", batch, t, len(STAGES))
+            stage += np.array(t, np.float64)
+        wall = (time.perf_counter() - t0) / steps
+        status = C.c_uint32()
+        capi.call("stp_layer_factory_status", h, C.byref(status))
+    finally:
+        capi.lib().stp_batch_destroy(batch)
+        capi.lib().stp_layer_factory_destroy(h)
+    path_mpx = sum(s * s for s in sides) / wall / 1e6
+    stage /= steps
+    print(f"[bench] batched path with layer biomes: {path_mpx:.0f} Mpixel/s, biome stage {stage[0]:.3f} ms",
+          file=sys.stderr, flush=True)
     return {"workload": f"demo biome layer chain (14 layers, 3 Voronoi zooms), {maps} maps of 512^2 per call",
-            "ms_per_call": round(ms, 3), "mpixel_s": round(mpx, 1), "cpu_baseline": cpu}
+            "ms_per_call": round(ms, 3), "mpixel_s": round(mpx, 1), "cpu_baseline": cpu,
+            "batched_path": {"workload": "the batched workload with chunk and halo biome maps from the demo chain "
+                             "(4 biome ids) instead of the Voronoi map", "mpixel_s": round(path_mpx, 1),
+                             "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES, stage)},
+                             "flagged_ids": int(status.value)}}
 
 
 def splat(reps=10):
@@ -419,7 +451,7 @@ def main():
     if not args.no_world:
         result["config"]["world"] = world()
         result["config"]["splatmap"] = splat()
-        result["config"]["biome_layers"] = layers()
+        result["config"]["biome_layers"] = layers(tuning=tuning)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(cfg, sides, origins, props, args.cpu_seconds, threads)

@@ -426,24 +426,35 @@ struct STPLayerPlan::STPImplementation {
 	// per node and map, the node's first tile of the map in the tile table; [node * (Count + 1) + map]
 	std::vector<unsigned int> TileBegin;
 	size_t RectBytes = 0u;
-	// rectangles, then tiles, in one upload
-	STPSmartDeviceMemory::STPHost<unsigned char> Staging;
+	// rectangles, then tiles, in one upload from one of two pinned staging buffers in turn (a
+	// plan built while the previous upload is in flight does not wait for it)
+	STPSmartDeviceMemory::STPHost<unsigned char> Staging[2];
+	hipEvent_t Uploaded[2] {};
+	size_t StagingCapacity[2] {};
+	unsigned int Slot = 0u;
 	STPSmartDeviceMemory::STPDevice<unsigned char> Table;
 	size_t TableCapacity = 0u;
 	STPSmartDeviceMemory::STPDevice<Sample> Arena;
 	size_t ArenaCapacity = 0u;
-	hipEvent_t Uploaded = nullptr;
+	// host work space, kept between builds
+	std::vector<Box> Bound;
+	std::vector<LayerRect> Rect;
+	std::vector<uint2> Tile;
 
 	~STPImplementation() {
-		if (this->Uploaded) {
-			(void)hipEventSynchronize(this->Uploaded);
-			(void)hipEventDestroy(this->Uploaded);
+		for (hipEvent_t& e : this->Uploaded) {
+			if (e) {
+				(void)hipEventSynchronize(e);
+				(void)hipEventDestroy(e);
+			}
 		}
 	}
 };
 
 STPLayerPlan::STPLayerPlan() : Impl(std::make_unique<STPImplementation>()) {
-	STP_CHECK_HIP(hipEventCreateWithFlags(&this->Impl->Uploaded, hipEventDisableTiming));
+	for (hipEvent_t& e : this->Impl->Uploaded) {
+		STP_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+	}
 }
 
 STPLayerPlan::~STPLayerPlan() = default;
@@ -594,7 +605,8 @@ void STPLayerBiomeFactory::plan(STPLayerPlan& plan, const std::vector<STPLayerMa
 		STP_THROW(STPInvalidArgument, "a plan needs at least one map");
 	}
 	// rectangles: from the root down, every node gets the bounding box of what its children read
-	std::vector<Box> box(n * count);
+	std::vector<Box>& box = q.Bound;
+	box.assign(n * count, Box {});
 	for (size_t c = 0u; c < count; c++) {
 		const STPLayerMap& map = maps[c];
 		const unsigned int step = map.Step == 0u ? 1u : map.Step;
@@ -644,8 +656,10 @@ void STPLayerBiomeFactory::plan(STPLayerPlan& plan, const std::vector<STPLayerMa
 		}
 	}
 	// rectangles, buffer places and tiles
-	std::vector<LayerRect> rect(n * count, LayerRect {});
-	std::vector<uint2> tile;
+	std::vector<LayerRect>& rect = q.Rect;
+	rect.assign(n * count, LayerRect {});
+	std::vector<uint2>& tile = q.Tile;
+	tile.clear();
 	q.TileBegin.assign(n * (count + 1u), 0u);
 	unsigned long long arena = 0u;
 	for (size_t i = 0u; i < n; i++) {
@@ -691,15 +705,16 @@ void STPLayerBiomeFactory::plan(STPLayerPlan& plan, const std::vector<STPLayerMa
 		q.TileBegin[i * (count + 1u) + count] = static_cast<unsigned int>(tile.size());
 	}
 
-	// upload through pinned staging, once the previous upload has left it; grow after the device
-	// has finished with the old tables
+	// upload through the staging buffer whose previous upload has finished; grow the device tables
+	// after the device has finished with the old ones
 	const size_t rectBytes = rect.size() * sizeof(LayerRect), bytes = rectBytes + tile.size() * sizeof(uint2);
-	STP_CHECK_HIP(hipEventSynchronize(q.Uploaded));
+	const unsigned int slot = q.Slot;
+	q.Slot ^= 1u;
+	STP_CHECK_HIP(hipEventSynchronize(q.Uploaded[slot]));
 	if (bytes > q.TableCapacity || arena > q.ArenaCapacity) {
 		STP_CHECK_HIP(hipDeviceSynchronize());
 		if (bytes > q.TableCapacity) {
 			q.TableCapacity = bytes + bytes / 4u;
-			q.Staging = STPSmartDeviceMemory::makeHost<unsigned char>(q.TableCapacity);
 			q.Table = STPSmartDeviceMemory::makeDevice<unsigned char>(q.TableCapacity);
 		}
 		if (arena > q.ArenaCapacity) {
@@ -707,10 +722,15 @@ void STPLayerBiomeFactory::plan(STPLayerPlan& plan, const std::vector<STPLayerMa
 			q.Arena = STPSmartDeviceMemory::makeDevice<Sample>(q.ArenaCapacity);
 		}
 	}
-	std::memcpy(q.Staging.get(), rect.data(), rectBytes);
-	std::memcpy(q.Staging.get() + rectBytes, tile.data(), tile.size() * sizeof(uint2));
-	STP_CHECK_HIP(hipMemcpyAsync(q.Table.get(), q.Staging.get(), bytes, hipMemcpyHostToDevice, stream));
-	STP_CHECK_HIP(hipEventRecord(q.Uploaded, stream));
+	if (bytes > q.StagingCapacity[slot]) {
+		q.StagingCapacity[slot] = bytes + bytes / 4u;
+		q.Staging[slot] = STPSmartDeviceMemory::makeHost<unsigned char>(q.StagingCapacity[slot]);
+	}
+	unsigned char* const staging = q.Staging[slot].get();
+	std::memcpy(staging, rect.data(), rectBytes);
+	std::memcpy(staging + rectBytes, tile.data(), tile.size() * sizeof(uint2));
+	STP_CHECK_HIP(hipMemcpyAsync(q.Table.get(), staging, bytes, hipMemcpyHostToDevice, stream));
+	STP_CHECK_HIP(hipEventRecord(q.Uploaded[slot], stream));
 	q.RectBytes = rectBytes;
 	q.Count = static_cast<unsigned int>(count);
 }
