@@ -155,7 +155,9 @@ def world(plain_frames=6):
     batch = capi.BatchSetting(cfg.biome_seed, cfg.biome_count, cfg.cell, cfg.radius, cfg.biome_count,
                               cfg.drops_per_pixel)
 
-    def walk(generate, erode, frames):
+    export = {}
+
+    def walk(generate, erode, frames, measure_export=False):
         h = C.c_void_p()
         capi.call("stp_world_create", C.byref(noise), capi.biome_array(props), len(props),
                   C.byref(W.erosion_setting(1, seed=cfg.erosion_seed)), C.byref(batch),
@@ -176,12 +178,34 @@ def world(plain_frames=6):
                 lat.append((time.perf_counter() - t0) * 1e3)
                 done.update(new)
             wall = time.perf_counter() - t_all
+            if measure_export:
+                # chunk records of the last frame's render area, every map, into pinned host memory
+                last = frames[-1]
+                size = C.c_uint64()
+                capi.call("stp_chunk_record_bytes", ww.map, ww.map, 7, C.byref(size))
+                total = size.value * len(last)
+                host = C.c_void_p()
+                capi.call("stp_host_malloc_pinned", C.byref(host), total)
+                flat = (C.c_int32 * (2 * len(last)))(*[v for c in last for v in c])
+                written = C.c_uint64()
+                try:
+                    capi.call("stp_world_export", h, flat, len(last), 7, host, total, C.byref(written))
+                    t0 = time.perf_counter()
+                    reps = 5
+                    for _ in range(reps):
+                        capi.call("stp_world_export", h, flat, len(last), 7, host, total, C.byref(written))
+                    ms = (time.perf_counter() - t0) / reps * 1e3
+                finally:
+                    capi.lib().stp_host_free_pinned(host)
+                export.update({"workload": f"chunk records of the {len(last)} render-area chunks (biome, FP32 and UINT16 "
+                                           f"heights, {total / 1e6:.1f} MB) into pinned host memory",
+                               "ms": round(ms, 3), "gb_s": round(total / ms / 1e6, 2)})
         finally:
             capi.lib().stp_world_destroy(h)
         return len(done), np.array(lat), wall
 
     frames = ww.frames()
-    n, lat, wall = walk(ww.generate, ww.erode, frames)
+    n, lat, wall = walk(ww.generate, ww.erode, frames, measure_export=True)
     pn, plat, pwall = walk(1, 1, frames[:plain_frames])
     px = ww.map * ww.map
     out = {"workload": f"camera walk {ww.steps} chunks (east, then north), render area {2 * ww.render + 1}^2 of "
@@ -195,8 +219,10 @@ def world(plain_frames=6):
            "plain_mpixel_s": round(pn * px / pwall / 1e6, 1),
            "plain_frames": plain_frames}
     out["batched_vs_plain"] = round(out["mpixel_s"] / out["plain_mpixel_s"], 2)
+    out["export"] = export
     print(f"[bench] world: first frame {lat[0]:.1f} ms, frames p50 {out['frame_p50_ms']} p99 {out['frame_p99_ms']} ms, "
           f"{out['mpixel_s']} Mpixel/s completed (plain {out['plain_mpixel_s']})", file=sys.stderr, flush=True)
+    print(f"[bench] world export: {export.get('ms')} ms, {export.get('gb_s')} GB/s", file=sys.stderr, flush=True)
     return out
 
 

@@ -573,6 +573,35 @@ int stp_world_state(stp_heightfield_world* world, int32_t x, int32_t y, uint32_t
  * the input of stp_splatmap_generate). Synchronous. */
 int stp_world_render_area(stp_heightfield_world* world, int32_t x, int32_t y, uint32_t radius, uint16_t* d_biome,
 	uint16_t* d_height);
+/* ------------------------------------------------------------------ chunk records
+ * The byte format in which chunk maps leave the device (files, sockets): the reference's map pixel
+ * formats (STPWorldMapPixelFormat.hpp:12-21: biome u16, height f32 and u16, row-major per chunk as
+ * STPChunk holds them, STPChunk.cpp:17-25) behind a 64-byte little-endian header; the layout is this
+ * library's own (the reference keeps chunks in memory). Header: "STPC", u16 version 1, u16 header
+ * bytes 64, i32 chunk x, y, i32 world pixel x, y of the first pixel, u32 width, height, u32
+ * completeness (STPChunkCompleteness: 0 empty, 1 biome map, 2 heightmap, 3 complete), u32 maps
+ * (bit 0 biome, 1 FP32 height, 2 UINT16 height), u64 payload bytes, u32 check, 12 zero bytes. The
+ * payload holds the present maps in bit order, each zero-padded to a multiple of 16 bytes;
+ * check = sum of the payload's 32-bit words w_i times (2 i + 1), modulo 2^32. */
+typedef struct {
+	int32_t X, Y, OriginX, OriginY;
+	uint32_t Width, Height, Completeness, Maps, Check;
+	const uint16_t* BiomeMap;   /* NULL when absent; pointers into the parsed bytes */
+	const float* HeightMap;
+	const uint16_t* FixedMap;
+	uint64_t Bytes;             /* of the whole record */
+} stp_chunk_record;
+
+/* Bytes of one record of a width x height chunk with `maps`. */
+int stp_chunk_record_bytes(uint32_t width, uint32_t height, uint32_t maps, uint64_t* bytes);
+/* Records of the listed chunks (count (X, Y) pairs, each with a map) one after another into h_out
+ * (capacity bytes); *written = count * the record size. Synchronous. */
+int stp_world_export(stp_heightfield_world* world, const int32_t* h_chunks, uint32_t count, uint32_t maps, void* h_out,
+	uint64_t capacity, uint64_t* written);
+/* Validates the record at `data` (size bytes available: magic, version, sizes, check) and points
+ * *record into it; STP_ERROR_INVALID_ARGUMENT on any mismatch. Host only. */
+int stp_chunk_record_read(const void* data, uint64_t size, stp_chunk_record* record);
+
 /* The conflict-free schedule the world uses (host only): h_batch[i] = batch of centre i, first
  * fit in the given order; centres in one batch have disjoint neighbourhoods. */
 int stp_world_schedule(const int32_t* h_centres, uint32_t count, uint32_t neighbour, uint32_t* h_batch);

@@ -114,6 +114,38 @@ def demo_biome_registry():
     return r
 
 
+class ChunkRecord(C.Structure):
+    """stp_chunk_record: a parsed chunk record (pointers into the parsed bytes)."""
+    _fields_ = [("X", C.c_int32), ("Y", C.c_int32), ("OriginX", C.c_int32), ("OriginY", C.c_int32),
+                ("Width", C.c_uint32), ("Height", C.c_uint32), ("Completeness", C.c_uint32), ("Maps", C.c_uint32),
+                ("Check", C.c_uint32), ("BiomeMap", C.c_void_p), ("HeightMap", C.c_void_p), ("FixedMap", C.c_void_p),
+                ("Bytes", C.c_uint64)]
+
+
+def read_chunk_records(data):
+    """Every record of a byte string of chunk records, validated by the library: a list of dicts
+    with the header fields and the present maps as numpy arrays (copies)."""
+    import numpy as np
+    buf = np.frombuffer(data, np.uint8)
+    base = buf.ctypes.data
+    out, at = [], 0
+    while at < buf.size:
+        r = ChunkRecord()
+        call("stp_chunk_record_read", C.c_void_p(base + at), buf.size - at, C.byref(r))
+        shape = (r.Height, r.Width)
+        n = r.Width * r.Height
+
+        def arr(ptr, dtype):
+            if not ptr:
+                return None
+            return np.frombuffer(buf, dtype, n, ptr - base).reshape(shape).copy()
+        out.append({"x": r.X, "y": r.Y, "origin": (r.OriginX, r.OriginY), "completeness": r.Completeness,
+                    "maps": r.Maps, "check": r.Check, "biome": arr(r.BiomeMap, np.uint16),
+                    "height": arr(r.HeightMap, np.float32), "fixed": arr(r.FixedMap, np.uint16)})
+        at += r.Bytes
+    return out
+
+
 class ServerSetting(C.Structure):
     _fields_ = [("MemoryBudget", C.c_uint64), ("BatchPixels", C.c_uint64), ("BatchChunks", C.c_uint32),
                 ("MaxSide", C.c_uint32), ("Engines", C.c_uint32), ("ChunkWorldSize", C.c_uint32)]
@@ -252,6 +284,9 @@ _SIGNATURES = {
     "stp_layer_factory_generate": (_I, [_P, _P, C.c_int32, C.c_int32]),
     "stp_layer_factory_generate_device": (_I, [_P, _P, _P, _U32, _P]),
     "stp_layer_factory_status": (_I, [_P, _P]),
+    "stp_chunk_record_bytes": (_I, [_U32, _U32, _U32, _P]),
+    "stp_world_export": (_I, [_P, _P, _U32, _U32, _P, _U64, _P]),
+    "stp_chunk_record_read": (_I, [_P, _U64, _P]),
 }
 
 _lib = None

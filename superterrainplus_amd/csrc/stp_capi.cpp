@@ -4,6 +4,7 @@
 #include "stp_server.hpp"
 #include "stp_splat.hpp"
 #include "stp_world.hpp"
+#include "stp_chunk_record.hpp"
 
 #include <stp/STPBiomeLayer.hpp>
 #include "stp_internal.hpp"
@@ -1177,5 +1178,36 @@ int stp_layer_factory_status(stp_layer_factory* const factory, uint32_t* const s
 	return guarded([&] {
 		require(factory && status, "null argument");
 		*status = factory->Factory->status();
+	});
+}
+
+/* ---------------------------------------------------------------------------- chunk records */
+
+int stp_chunk_record_bytes(const uint32_t width, const uint32_t height, const uint32_t maps, uint64_t* const bytes) {
+	return guarded([&] {
+		require(bytes && width > 0u && height > 0u && maps != 0u && (maps & ~STPChunkRecord::MapAll) == 0u,
+			"positive size, maps a non-empty combination of 1, 2, 4");
+		*bytes = STPChunkRecord::Layout(width, height, maps).record();
+	});
+}
+
+int stp_world_export(stp_heightfield_world* const world, const int32_t* const h_chunks, const uint32_t count,
+	const uint32_t maps, void* const h_out, const uint64_t capacity, uint64_t* const written) {
+	return guarded([&] {
+		require(world && written && (count == 0u || h_chunks), "null argument");
+		std::vector<int2> c(count);
+		for (uint32_t i = 0u; i < count; i++) {
+			c[i] = make_int2(h_chunks[2u * i], h_chunks[2u * i + 1u]);
+		}
+		*written = world->World->exportRecords(c, maps, static_cast<unsigned char*>(h_out), capacity);
+	});
+}
+
+int stp_chunk_record_read(const void* const data, const uint64_t size, stp_chunk_record* const record) {
+	return guarded([&] {
+		require(record, "null record");
+		const STPChunkRecord::View v = STPChunkRecord::read(static_cast<const unsigned char*>(data), size);
+		*record = stp_chunk_record {v.X, v.Y, v.OriginX, v.OriginY, v.Width, v.Height, v.Completeness, v.Maps, v.Check,
+			v.Biome, v.HeightMap, v.Fixed, v.Bytes};
 	});
 }

@@ -1,6 +1,7 @@
 // STPHeightfieldWorld: the world pipeline's generation chain on resident device state (see
 // stp_world.hpp for the reference functions it follows).
 #include "stp_world.hpp"
+#include "stp_chunk_record.hpp"
 #include "stp_internal.hpp"
 
 #include <algorithm>
@@ -345,3 +346,96 @@ void STPHeightfieldWorld::renderArea(const int2 centre, const unsigned int radiu
 	STP_CHECK_HIP(hipStreamSynchronize(this->Stream));
 }
 
+
+unsigned long long STPHeightfieldWorld::exportRecords(const std::vector<int2>& chunks, const unsigned int maps,
+	unsigned char* const out, const unsigned long long capacity) {
+	namespace R = STPChunkRecord;
+	if (maps == 0u || (maps & ~R::MapAll) != 0u) {
+		STP_THROW(STPInvalidArgument, "maps: a non-empty combination of biome (1), FP32 height (2), UINT16 height (4)");
+	}
+	const unsigned int m = this->Setting.MapSize;
+	const R::Layout l(m, m, maps);
+	const size_t n = chunks.size();
+	const unsigned long long record = l.record(), total = record * n;
+	if (n == 0u) {
+		return 0u;
+	}
+	if (!out || total > capacity) {
+		STP_THROW(STPInvalidArgument, "the export needs " + std::to_string(total) + " bytes of host memory");
+	}
+	// pack table: where each record's payload lies, and the chunk's resident maps
+	const size_t entryBytes = n * sizeof(R::PackEntry), checkAt = (entryBytes + 255u) & ~size_t {255u},
+		tableBytes = checkAt + n * sizeof(unsigned int);
+	std::vector<const Entry*> entry(n);
+	for (size_t i = 0u; i < n; i++) {
+		const auto it = this->Chunk.find(Key {chunks[i].x, chunks[i].y});
+		if (it == this->Chunk.end() || it->second.State == STPChunkState::None) {
+			STP_THROW(STPInvalidArgument, "chunk (" + std::to_string(chunks[i].x) + ", " + std::to_string(chunks[i].y)
+				+ ") has no map to export");
+		}
+		const long long ox = static_cast<long long>(chunks[i].x) * m, oy = static_cast<long long>(chunks[i].y) * m;
+		if (ox < INT32_MIN || oy < INT32_MIN || ox + m - 1 > INT32_MAX || oy + m - 1 > INT32_MAX) {
+			STP_THROW(STPInvalidArgument, "a chunk's world pixels leave the 32-bit range of the record");
+		}
+		entry[i] = &it->second;
+	}
+	if (total > this->ExportCapacity || tableBytes > this->ExportTableCapacity) {
+		STP_CHECK_HIP(hipStreamSynchronize(this->Stream));
+		if (total > this->ExportCapacity) {
+			this->ExportCapacity = total + total / 4u;
+			this->ExportStream = STPSmartDeviceMemory::makeDevice<unsigned char>(this->ExportCapacity);
+		}
+		if (tableBytes > this->ExportTableCapacity) {
+			this->ExportTableCapacity = tableBytes + tableBytes / 2u;
+			this->ExportTable = STPSmartDeviceMemory::makeDevice<unsigned char>(this->ExportTableCapacity);
+			this->ExportStaging = STPSmartDeviceMemory::makeHost<unsigned char>(this->ExportTableCapacity);
+		}
+	}
+	auto* const staged = reinterpret_cast<R::PackEntry*>(this->ExportStaging.get());
+	for (size_t i = 0u; i < n; i++) {
+		staged[i] = R::PackEntry {entry[i]->Height, entry[i]->Fixed, i * record + R::HeaderBytes, l.Height, l.Fixed, l.Payload};
+	}
+	unsigned char* const bytes = this->ExportStream.get();
+	auto* const table = reinterpret_cast<R::PackEntry*>(this->ExportTable.get());
+	auto* const check = reinterpret_cast<unsigned int*>(this->ExportTable.get() + checkAt);
+	STP_CHECK_HIP(hipMemcpyAsync(table, staged, entryBytes, hipMemcpyHostToDevice, this->Stream));
+	STP_CHECK_HIP(hipMemsetAsync(check, 0, n * sizeof(unsigned int), this->Stream));
+	// padding is zero, so the checks do not depend on stale bytes
+	STP_CHECK_HIP(hipMemsetAsync(bytes, 0, total, this->Stream));
+	if (maps & R::MapBiome) {
+		// the biome map the heightmap batch sampled: chunk (X, Y) starts at world pixel (X, Y) * MapSize
+		if (this->Batch.BiomeLayers) {
+			std::vector<STPDiversity::STPLayerMap> area(n);
+			for (size_t i = 0u; i < n; i++) {
+				area[i] = STPDiversity::STPLayerMap {make_int2(chunks[i].x * static_cast<int>(m), chunks[i].y * static_cast<int>(m)),
+					make_uint2(m, m), 1u, m, (i * record + R::HeaderBytes + l.Biome) / 2u};
+			}
+			if (!this->ExportPlan) {
+				this->ExportPlan = std::make_unique<STPDiversity::STPLayerPlan>();
+			}
+			this->Batch.BiomeLayers->plan(*this->ExportPlan, area, this->Stream);
+			this->Batch.BiomeLayers->run(*this->ExportPlan, reinterpret_cast<unsigned short*>(bytes), 0u,
+				static_cast<unsigned int>(n), this->Batch.BiomeCount, this->Stream);
+		} else {
+			for (size_t i = 0u; i < n; i++) {
+				STPHeightfieldKernel::voronoiBiomemap(this->Batch.BiomeSeed, this->Batch.BiomeCount, this->Batch.BiomeCell,
+					static_cast<long long>(chunks[i].x) * m, static_cast<long long>(chunks[i].y) * m,
+					reinterpret_cast<unsigned short*>(bytes + i * record + R::HeaderBytes + l.Biome), m, m, m, this->Stream);
+			}
+		}
+	}
+	R::pack(table, static_cast<unsigned int>(n), static_cast<unsigned long long>(m) * m, maps, bytes, this->Stream);
+	R::check(table, static_cast<unsigned int>(n), l.Payload, bytes, check, this->Stream);
+	STP_CHECK_HIP(hipMemcpyAsync(out, bytes, total, hipMemcpyDeviceToHost, this->Stream));
+	STP_CHECK_HIP(hipMemcpyAsync(this->ExportStaging.get(), check, n * sizeof(unsigned int), hipMemcpyDeviceToHost,
+		this->Stream));
+	STP_CHECK_HIP(hipStreamSynchronize(this->Stream));
+	const auto* const sum = reinterpret_cast<const unsigned int*>(this->ExportStaging.get());
+	for (size_t i = 0u; i < n; i++) {
+		// our states: heightmap generated -> HeightmapReady (2), eroded and formatted -> Complete (3)
+		const unsigned int completeness = entry[i]->State == STPChunkState::Complete ? 3u : 2u;
+		R::writeHeader(out + i * record, chunks[i].x, chunks[i].y, chunks[i].x * static_cast<int>(m),
+			chunks[i].y * static_cast<int>(m), m, m, completeness, maps, l.Payload, sum[i]);
+	}
+	return total;
+}

@@ -52,6 +52,11 @@ namespace SuperTerrainPlus {
 		void renderArea(int2 centre, unsigned int radius, unsigned short* biome, unsigned short* height);
 		STPChunkState state(int2 chunk) const;
 		unsigned long long residentBytes() const noexcept { return this->Resident; }
+		// Chunk records (stp_chunk_record.hpp) of the listed chunks, each with a map, one after
+		// another into host memory of `capacity` bytes: `maps` selects the biome map (the one the
+		// heightmap sampled), FP32 and UINT16 heights. Returns the bytes written. Synchronous.
+		unsigned long long exportRecords(const std::vector<int2>& chunks, unsigned int maps, unsigned char* out,
+			unsigned long long capacity);
 
 		// Conflict-free schedule: batch number of each centre, first fit in the given order; two
 		// centres share a batch only if their neighbourhoods (nn x nn chunks) are disjoint.
@@ -80,6 +85,13 @@ namespace SuperTerrainPlus {
 		hipStream_t Stream = nullptr;
 		hipMemPool_t Pool = nullptr;
 		unsigned long long Resident = 0ull;
+		// chunk record export: the byte stream as it leaves the device, its pack table and checks
+		STPSmartDeviceMemory::STPDevice<unsigned char> ExportStream;
+		unsigned long long ExportCapacity = 0ull;
+		STPSmartDeviceMemory::STPDevice<unsigned char> ExportTable;
+		STPSmartDeviceMemory::STPHost<unsigned char> ExportStaging;
+		size_t ExportTableCapacity = 0u;
+		std::unique_ptr<STPDiversity::STPLayerPlan> ExportPlan;
 
 		void generate(const std::vector<Key>& chunks);
 		void erode(const std::vector<Key>& centres);

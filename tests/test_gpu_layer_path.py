@@ -138,6 +138,13 @@ def test_world_with_layer_biomes(gpu, oracle):
             capi.call("stp_world_read", w, x, y, capi.ptr(hh), None)
             halo = oracle_halo(oracle, nodes, reg, (x * m - RADIUS, y * m - RADIUS), m + 2 * RADIUS)
             assert hh.tobytes() == oracle_heights(oracle, halo, m, (x * m, y * m)).tobytes(), (x, y)
+        # exported records carry the chain's biome maps
+        size, written = C.c_uint64(), C.c_uint64()
+        capi.call("stp_chunk_record_bytes", m, m, 1, C.byref(size))
+        buf = (C.c_uint8 * (2 * size.value))()
+        capi.call("stp_world_export", w, (C.c_int32 * 4)(-1, 5, 1, 7), 2, 1, buf, len(buf), C.byref(written))
+        for r, (x, y) in zip(capi.read_chunk_records(bytes(buf)), [(-1, 5), (1, 7)]):
+            assert (r["biome"] == oracle_halo(oracle, nodes, reg, (x * m, y * m), m)).all()
     finally:
         capi.lib().stp_world_destroy(w)
         capi.lib().stp_layer_factory_destroy(f)

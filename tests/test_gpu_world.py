@@ -213,6 +213,44 @@ def test_world_render_area_and_splatmap(gpu, oracle):
         capi.lib().stp_splat_database_destroy(db)
 
 
+def export(w, chunks, maps):
+    size = C.c_uint64()
+    capi.call("stp_chunk_record_bytes", MAP, MAP, maps, C.byref(size))
+    buf = (C.c_uint8 * (size.value * len(chunks)))()
+    flat = (C.c_int32 * (2 * len(chunks)))(*[v for c in chunks for v in c])
+    written = C.c_uint64()
+    capi.call("stp_world_export", C.c_void_p(w.h), flat, len(chunks), maps, buf, len(buf), C.byref(written))
+    assert written.value == len(buf)
+    return bytes(buf)
+
+
+def test_world_export_records(gpu, oracle):
+    """Chunk records of a complete chunk and of neighbourhood chunks with heightmaps only: the maps
+    equal the world's reads and the oracle's biome map, the completeness follows the chunk's state,
+    every record passes the reader's check; a chunk without a map and a short buffer are refused."""
+    log("world export records")
+    w = World(0.5)
+    w.request([(2, -1)])
+    chunks = [(2, -1), (1, -2), (3, 0)]
+    recs = capi.read_chunk_records(export(w, chunks, 7))
+    assert len(recs) == 3
+    for (x, y), r in zip(chunks, recs):
+        h, f = w.read(x, y)
+        assert (r["x"], r["y"], r["origin"]) == (x, y, (x * MAP, y * MAP))
+        assert r["completeness"] == (3 if (x, y) == (2, -1) else 2)
+        assert r["height"].tobytes() == h.tobytes() and (r["fixed"] == f).all()
+        assert (r["biome"] == oracle.voronoi_map(SEED, COUNT, CELL, (x * MAP, y * MAP), MAP, MAP)).all()
+    only = capi.read_chunk_records(export(w, chunks[:1], 4))[0]
+    assert only["biome"] is None and only["height"] is None and (only["fixed"] == w.read(2, -1)[1]).all()
+    flat = (C.c_int32 * 2)(9, 9)
+    buf, written = (C.c_uint8 * (1 << 20))(), C.c_uint64()
+    with pytest.raises(capi.STPError):
+        capi.call("stp_world_export", C.c_void_p(w.h), flat, 1, 7, buf, len(buf), C.byref(written))
+    flat = (C.c_int32 * 2)(2, -1)
+    with pytest.raises(capi.STPError):
+        capi.call("stp_world_export", C.c_void_p(w.h), flat, 1, 7, buf, 100, C.byref(written))
+
+
 def test_world_rejects_bad_settings(gpu):
     batch = capi.BatchSetting(SEED, COUNT, CELL, RADIUS, 8, 0.0)
     for bad in [capi.WorldSetting(MAP, 2, 4, 4), capi.WorldSetting(4, 3, 4, 4), capi.WorldSetting(MAP, 3, 0, 4)]:
