@@ -832,8 +832,10 @@ namespace SuperTerrainPlus::STPBatch {
 	// LDS words per staged pixel of horizontalItemKERNEL
 	__host__ __device__ constexpr unsigned int horizontalItemWords(const unsigned int cap) { return 2u + (cap + 1u) / 2u; }
 
-	// horizontal pass, dense output as horizontalKERNEL
-	template <unsigned int MaxB>
+	// horizontal pass, dense output as horizontalKERNEL; Packed: a slot holds item | count << 3 in
+	// bin_item and bin_weight is not written (the weight is count * (1 / (2r + 1)^2), the window's
+	// sum being (2r + 1)^2 at every pixel)
+	template <unsigned int MaxB, bool Packed>
 	__global__ void __launch_bounds__(ShfLineBlock) horizontalItemKERNEL(const uint2* const tiles,
 		const Chunk* const chunks, const unsigned int radius, const unsigned int vcap, const unsigned int cap,
 		const unsigned short* const vbin, const unsigned char* const vcount, unsigned char* const bin_count,
@@ -958,12 +960,15 @@ namespace SuperTerrainPlus::STPBatch {
 #pragma unroll
 								for (unsigned int k = 0u; k < 4u; k++) {
 									const bool used = q + k < n;
-									it[k] = used ? static_cast<unsigned short>((head >> (4u + 3u * (q + k))) & 7u) : 0u;
+									const unsigned int item = (head >> (4u + 3u * (q + k))) & 7u;
+									it[k] = used ? static_cast<unsigned short>(Packed ? item | cnt[k] << 3u : item) : 0u;
 									// the same product as the reference's count * (1 / sum)
 									w[k] = used ? static_cast<float>(cnt[k]) * norm : 0.0f;
 								}
 								*reinterpret_cast<ushort4*>(bin_item + (c.Pixel + p) * cap + q) = make_ushort4(it[0], it[1], it[2], it[3]);
-								*reinterpret_cast<float4*>(bin_weight + (c.Pixel + p) * cap + q) = make_float4(w[0], w[1], w[2], w[3]);
+								if (!Packed) {
+									*reinterpret_cast<float4*>(bin_weight + (c.Pixel + p) * cap + q) = make_float4(w[0], w[1], w[2], w[3]);
+								}
 							}
 						}
 					}
@@ -981,9 +986,12 @@ namespace SuperTerrainPlus::STPBatch {
 							// slots past the pixel's bin count are never read: not stored
 							if (k < n) {
 								const unsigned int count = (at[(2u + k / 2u) * 65u] >> (16u * (k & 1u))) & 0xFFFFu;
-								bin_item[(c.Pixel + p) * cap + k] = static_cast<unsigned short>((head >> (4u + 3u * k)) & 7u);
-								// the same product as the reference's count * (1 / sum)
-								bin_weight[(c.Pixel + p) * cap + k] = static_cast<float>(count) * __uint_as_float(at[65]);
+								const unsigned int item = (head >> (4u + 3u * k)) & 7u;
+								bin_item[(c.Pixel + p) * cap + k] = static_cast<unsigned short>(Packed ? item | count << 3u : item);
+								if (!Packed) {
+									// the same product as the reference's count * (1 / sum)
+									bin_weight[(c.Pixel + p) * cap + k] = static_cast<float>(count) * __uint_as_float(at[65]);
+								}
 							}
 							if (k == 0u) {
 								bin_count[c.Pixel + p] = static_cast<unsigned char>(n);
@@ -999,10 +1007,11 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 	}
 
+	template <bool Packed>
 	__global__ void __launch_bounds__(BlendX * BlendY) blendKERNEL(const uint2* const tiles, const Chunk* const chunks,
 		const STPAlgorithm::STPPermutation perm, const BiomeTable biomes, const unsigned int biome_count,
-		const unsigned int cap, const unsigned char* const bin_count, const unsigned short* const bin_item,
-		const float* const bin_weight, float* const height) {
+		const unsigned int cap, const unsigned int window, const unsigned char* const bin_count,
+		const unsigned short* const bin_item, const float* const bin_weight, float* const height) {
 		__shared__ Noise::STPStagedPermutation table;
 		__shared__ STPBiomeProperty prop[STPHeightfieldKernel::MaxBiome];
 		const unsigned int tid = threadIdx.x + threadIdx.y * BlendX;
@@ -1025,8 +1034,14 @@ namespace SuperTerrainPlus::STPBatch {
 			const unsigned int y = y0 + k;
 			if (y < c.Side) {
 				const unsigned long long i = c.Pixel + static_cast<unsigned long long>(y) * c.Side + x;
-				height[i] = blendPixel(table, prop, biome_count, bin_count[i], bin_item + i * cap, bin_weight + i * cap,
-					x * c.Step, y * c.Step, offset, make_float2(half, half));
+				if (Packed) {
+					// the horizontal pass's 1 / sum, the window (2r + 1)^2 being every pixel's sum
+					height[i] = blendPacked(table, prop, biome_count, bin_count[i], bin_item + i * cap,
+						1.0f / static_cast<float>(window), x * c.Step, y * c.Step, offset, make_float2(half, half));
+				} else {
+					height[i] = blendPixel(table, prop, biome_count, bin_count[i], bin_item + i * cap, bin_weight + i * cap,
+						x * c.Step, y * c.Step, offset, make_float2(half, half));
+				}
 			}
 		}
 	}
@@ -1614,6 +1629,12 @@ namespace {
 	}
 } // namespace
 
+bool STPHeightfieldBatch::packedBins() const noexcept {
+	// a slot holds item (3 bits) | count << 3 in 16 bits: counts below 2^13
+	const unsigned int w = 2u * this->Setting.Radius + 1u;
+	return this->itemPath() && w * w < 8192u;
+}
+
 bool STPHeightfieldBatch::itemPath() const noexcept {
 	// biome ids come from the synthetic biome map or a layer chain that replaces ids >= BiomeCount by
 	// 0, so they are < BiomeCount; a column count is at
@@ -1940,10 +1961,17 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 						this->VCap, cap, vbin, this->VCount.get(), this->BinCount.get(), this->BinItem.get(),
 						this->BinWeight.get(), state, this->Flag.get());
 				};
+				const bool packed = this->packedBins();
 				if (this->Setting.BiomeCount <= 4u) {
-					item(horizontalSummaryKERNEL<4u>, segmentCarryKERNEL<4u>, horizontalItemKERNEL<4u>);
+					if (packed) {
+						item(horizontalSummaryKERNEL<4u>, segmentCarryKERNEL<4u>, horizontalItemKERNEL<4u, true>);
+					} else {
+						item(horizontalSummaryKERNEL<4u>, segmentCarryKERNEL<4u>, horizontalItemKERNEL<4u, false>);
+					}
+				} else if (packed) {
+					item(horizontalSummaryKERNEL<8u>, segmentCarryKERNEL<8u>, horizontalItemKERNEL<8u, true>);
 				} else {
-					item(horizontalSummaryKERNEL<8u>, segmentCarryKERNEL<8u>, horizontalItemKERNEL<8u>);
+					item(horizontalSummaryKERNEL<8u>, segmentCarryKERNEL<8u>, horizontalItemKERNEL<8u, false>);
 				}
 				break;
 			}
@@ -1962,11 +1990,21 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 		break;
 	}
 	case 3: {
-		const auto [t, n] = range(3);
+		const auto tn = range(3);
+		const uint2* const t = tn.first;
+		const unsigned int n = tn.second;
 		if (n) {
-			blendKERNEL<<<n, dim3(BlendX, BlendY), 0, stream>>>(t, this->ChunkRecord, this->Permutation.Permutation,
-				this->Biome, this->BiomeCount, cap, this->BinCount.get(), this->BinItem.get(), this->BinWeight.get(),
-				this->Height.get());
+			const unsigned int window = (2u * radius + 1u) * (2u * radius + 1u);
+			const auto launch = [&](auto kernel) {
+				kernel<<<n, dim3(BlendX, BlendY), 0, stream>>>(t, this->ChunkRecord, this->Permutation.Permutation,
+					this->Biome, this->BiomeCount, cap, window, this->BinCount.get(), this->BinItem.get(),
+					this->BinWeight.get(), this->Height.get());
+			};
+			if (this->packedBins()) {
+				launch(blendKERNEL<true>);
+			} else {
+				launch(blendKERNEL<false>);
+			}
 		}
 		break;
 	}

@@ -216,6 +216,8 @@ namespace SuperTerrainPlus {
 		STPStage::ErosionArgs erosionArgs() const;
 		// the filter passes with per-biome registers (at most 8 biomes and 8 bins per pixel)
 		bool itemPath() const noexcept;
+		// the item path with each histogram slot packed as item | count << 3 (no weight array)
+		bool packedBins() const noexcept;
 	};
 
 } // namespace SuperTerrainPlus

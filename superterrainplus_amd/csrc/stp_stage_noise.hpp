@@ -60,6 +60,22 @@ namespace SuperTerrainPlus::STPStage {
 		return h;
 	}
 
+	// blendPixel over packed slots (item | count << 3): weight = count * (1 / window sum), the
+	// horizontal pass's product
+	__device__ inline float blendPacked(const Noise::STPStagedPermutation& table, const STPBiomeProperty* const prop,
+		const unsigned int biome_count, const unsigned int n, const unsigned short* const slot, const float norm,
+		const unsigned int x, const unsigned int y, const float2 offset, const float2 half) {
+		float h = 0.0f;
+		for (unsigned int b = 0u; b < n; b++) {
+			const unsigned int s = slot[b], it = s & 7u;
+			if (it < biome_count) {
+				h += static_cast<float>(s >> 3u) * norm
+					* biomeSample(table, prop[it], 1.0f * static_cast<float>(x), 1.0f * static_cast<float>(y), offset, half);
+			}
+		}
+		return h;
+	}
+
 	__device__ inline long long floorDiv(const long long a, const long long b) {
 		long long q = a / b;
 		if ((a % b != 0) && ((a < 0) != (b < 0))) {
