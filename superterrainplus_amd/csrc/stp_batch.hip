@@ -1757,12 +1757,17 @@ void STPHeightfieldBatch::buildTileTables(const hipStream_t stream) {
 			}
 			range.End[s] = static_cast<unsigned int>(table[s].size());
 		}
-		// erosion tiles by colour: (tile x parity) + 2 (tile y parity)
+		// erosion tiles by colour: (tile x parity) + 2 (tile y parity), rotated by the chunk's stream
+		// word. Any relabelling keeps same-colour tiles apart; the rotation spreads the tiles of chunks
+		// with few tiles (a one-tile chunk has colour 0 before it) over the four launches, and it
+		// depends on the chunk alone, so a chunk's tiles run in the same order in any batch.
+		const unsigned int rotation = c.StreamId & 3u;
 		for (int colour = 0; colour < 4; colour++) {
 			const int s = 5 + colour;
 			range.Begin[s] = static_cast<unsigned int>(table[s].size());
 			for (unsigned int k = 0u; k < tx * tx; k++) {
-				if (static_cast<int>((k % tx) & 1u) + 2 * static_cast<int>((k / tx) & 1u) == colour) {
+				const unsigned int parity = ((k % tx) & 1u) + 2u * ((k / tx) & 1u);
+				if (static_cast<int>((parity + rotation) & 3u) == colour) {
 					table[s].push_back(make_uint2(i, k));
 				}
 			}
