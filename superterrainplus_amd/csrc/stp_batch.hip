@@ -1109,42 +1109,69 @@ namespace SuperTerrainPlus::STPBatch {
 	// counted per tile in LDS first; one global atomic per tile then reserves the workgroup's
 	// ranks, so a counter sees one atomic per workgroup instead of one per drop.
 	constexpr unsigned int BinLds = 4096u;
-	__global__ void __launch_bounds__(ErosionBlock) binCountKERNEL(const uint2* const tiles, const Chunk* const chunks,
-		const ErosionArgs args, const unsigned int tile, unsigned int* const tile_drops, unsigned int* const rank) {
+	// A workgroup counts the drops of up to BinGroups consecutive drop-table entries (ErosionBlock
+	// drops each) per spawn tile: per run of entries of one chunk, an LDS histogram, then one global
+	// atomic per tile and run for the base of the run's ranks. Wider workgroup coverage keeps the
+	// number of global atomics on a tile's counter low (a 1024^2 chunk has 1024 entries).
+	constexpr unsigned int BinGroups = 16u;
+	__global__ void __launch_bounds__(ErosionBlock) binCountKERNEL(const uint2* const tiles, const unsigned int entries,
+		const Chunk* const chunks, const ErosionArgs args, const unsigned int tile, unsigned int* const tile_drops,
+		unsigned int* const rank) {
 		__shared__ unsigned int local[BinLds];
-		const uint2 t = tiles[blockIdx.x];
-		const Chunk c = chunks[t.x];
-		const unsigned int drop = t.y * ErosionBlock + threadIdx.x;
-		const unsigned int tx = (c.Side + tile - 1u) / tile, ntile = tx * tx;
-		const bool aggregate = ntile <= BinLds; // uniform over the workgroup
-		if (aggregate) {
-			for (unsigned int i = threadIdx.x; i < ntile; i += ErosionBlock) {
-				local[i] = 0u;
+		const unsigned int e0 = blockIdx.x * BinGroups, e1 = min(e0 + BinGroups, entries);
+		unsigned int myTile[BinGroups], myRank[BinGroups];
+		unsigned int e = e0;
+		while (e < e1) {
+			// the run [e, r) of entries of one chunk (uniform over the workgroup)
+			const unsigned int chunk = tiles[e].x;
+			unsigned int r = e + 1u;
+			while (r < e1 && tiles[r].x == chunk) {
+				r++;
 			}
-		}
-		__syncthreads();
-		unsigned int my = 0u, localRank = 0u;
-		if (drop < c.Drops) {
-			my = dropTile(spawn(chunkArgs(args, c), drop), c.Side, tile);
+			const Chunk c = chunks[chunk];
+			const unsigned int tx = (c.Side + tile - 1u) / tile, ntile = tx * tx;
+			const bool aggregate = ntile <= BinLds;
 			if (aggregate) {
-				localRank = atomicAdd(local + my, 1u);
-			} else {
-				rank[c.DropBase + drop] = atomicAdd(tile_drops + c.TileBase + my, 1u);
+				for (unsigned int i = threadIdx.x; i < ntile; i += ErosionBlock) {
+					local[i] = 0u;
+				}
 			}
-		}
-		if (!aggregate) {
-			return;
-		}
-		__syncthreads();
-		for (unsigned int i = threadIdx.x; i < ntile; i += ErosionBlock) {
-			const unsigned int n = local[i];
-			if (n != 0u) {
-				local[i] = atomicAdd(tile_drops + c.TileBase + i, n);
+			__syncthreads();
+#pragma unroll
+			for (unsigned int g = 0u; g < BinGroups; g++) {
+				const unsigned int k = e0 + g;
+				const unsigned int drop = tiles[min(k, e1 - 1u)].y * ErosionBlock + threadIdx.x;
+				if (k >= e && k < r && drop < c.Drops) {
+					const unsigned int my = dropTile(spawn(chunkArgs(args, c), drop), c.Side, tile);
+					if (aggregate) {
+						myTile[g] = my;
+						myRank[g] = atomicAdd(local + my, 1u);
+					} else {
+						rank[c.DropBase + drop] = atomicAdd(tile_drops + c.TileBase + my, 1u);
+					}
+				}
 			}
-		}
-		__syncthreads();
-		if (drop < c.Drops) {
-			rank[c.DropBase + drop] = local[my] + localRank;
+			if (aggregate) {
+				__syncthreads();
+				for (unsigned int i = threadIdx.x; i < ntile; i += ErosionBlock) {
+					const unsigned int n = local[i];
+					if (n != 0u) {
+						local[i] = atomicAdd(tile_drops + c.TileBase + i, n);
+					}
+				}
+				__syncthreads();
+#pragma unroll
+				for (unsigned int g = 0u; g < BinGroups; g++) {
+					const unsigned int k = e0 + g;
+					const unsigned int drop = tiles[min(k, e1 - 1u)].y * ErosionBlock + threadIdx.x;
+					if (k >= e && k < r && drop < c.Drops) {
+						rank[c.DropBase + drop] = local[myTile[g]] + myRank[g];
+					}
+				}
+				// the next run clears the histogram
+				__syncthreads();
+			}
+			e = r;
 		}
 	}
 
@@ -2028,8 +2055,8 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 		if (te > tb) {
 			zeroRangeKERNEL<<<(te - tb + 255u) / 256u, 256u, 0, stream>>>(this->TileDrops.get() + tb, te - tb);
 		}
-		binCountKERNEL<<<n, ErosionBlock, 0, stream>>>(t, this->ChunkRecord, args, tile, this->TileDrops.get(),
-			this->DropRank.get());
+		binCountKERNEL<<<(n + BinGroups - 1u) / BinGroups, ErosionBlock, 0, stream>>>(t, n, this->ChunkRecord, args, tile,
+			this->TileDrops.get(), this->DropRank.get());
 		binScanKERNEL<<<last - first, ScanBlock, 0, stream>>>(this->ChunkRecord, first, tile,
 			this->TileDrops.get(), this->TileFirst.get());
 		binScatterKERNEL<<<n, ErosionBlock, 0, stream>>>(t, this->ChunkRecord, args, tile, this->TileFirst.get(),
