@@ -101,6 +101,35 @@ def cpu_baseline(cfg, sides, origins, props, seconds, threads):
                       f"(noise, filter, blend, serialized erosion, format), {wall:.1f} s wall"}
 
 
+def cpu_baseline_fp64(cfg, sides, origins, props, seconds):
+    """The FP64 oracle pipeline on one thread (noise, filter and blend in FP64, serialized FP64
+    erosion, format), chunks in workload order until `seconds` of wall time have passed."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from superterrainplus_amd import workloads as W
+    perm, _, g64 = O.permutation(cfg.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, cfg.engine)
+    s = W.erosion_setting(1, seed=cfg.erosion_seed)
+    rain = O.RainDropSetting()
+    for f, _ in O.RainDropSetting._fields_:
+        setattr(rain, f, getattr(s, f))
+    r = cfg.radius
+    t0, px, n = time.perf_counter(), 0, 0
+    for i, (side, o) in enumerate(zip(sides, origins)):
+        if time.perf_counter() - t0 > seconds:
+            break
+        halo = O.voronoi_map(cfg.biome_seed, cfg.biome_count, cfg.cell, (o[0] - r, o[1] - r), side + 2 * r, side + 2 * r)
+        item, weight, off = O.shf_window(halo, (r, r), (side, side), r, cap_per_pixel=8)
+        h = O.multi_biome(perm, g64, props, item, weight, off, (side, side), (o[0] + side / 2, o[1] + side / 2), f64=True)
+        O.erode(h, (1, 1), (side, side), rain, int(side * side * cfg.drops_per_pixel), s.Seed, stream_id=i, f64=True)
+        O.format_heightmap(h)
+        px += side * side
+        n += 1
+    wall = time.perf_counter() - t0
+    return {"value": round(px / wall / 1e6, 3), "unit": "Mpixel/s", "cores": 1, "kind": "port",
+            "sample": f"first {n} of {len(sides)} chunks ({px / 1e6:.2f} Mpixel) of the same workload, FP64 oracle on one "
+                      f"thread, {wall:.1f} s wall"}
+
+
 def served(seconds_scale=1.0):
     """Camera-walk request stream replayed in real time at each rate on a fresh server; a short
     warm-up trace far from the walk comes first so that the timed replay finds warm engines."""
@@ -481,6 +510,10 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(cfg, sides, origins, props, args.cpu_seconds, threads)
+        result["config"]["cpu_baseline_fp64_1core"] = cpu_baseline_fp64(cfg, sides, origins, props,
+                                                                         min(8.0, args.cpu_seconds / 2))
+        print(f"[bench] cpu: FP32 {result['cpu_baseline']['value']:.2f} Mpixel/s on {threads} threads, FP64 "
+              f"{result['config']['cpu_baseline_fp64_1core']['value']} Mpixel/s on one", file=sys.stderr, flush=True)
     capi.call("stp_batch_destroy", batch)
     print(json.dumps(result))
 

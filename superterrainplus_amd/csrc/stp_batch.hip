@@ -28,7 +28,7 @@ using namespace SuperTerrainPlus::STPStage;
 namespace SuperTerrainPlus::STPBatch {
 
 	constexpr unsigned int BiomeTile = 256u;
-	constexpr unsigned int BlendX = 64u, BlendY = 4u, BlendPixel = 2u;
+	constexpr unsigned int BlendX = 64u, BlendY = 4u, BlendPixel = 4u;
 	constexpr unsigned int ErosionBlock = 256u;
 
 	// Synthetic biome map over a 32 x 32 tile of the halo map, BiomeTile threads with
