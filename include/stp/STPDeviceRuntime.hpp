@@ -20,6 +20,7 @@
 #include <hip/hiprtc.h>
 
 #include <stp/SuperTerrainPlus.hpp>
+#include <stp/STPTextureSplat.hpp>
 
 #include <memory>
 #include <string>
@@ -151,6 +152,41 @@ namespace SuperTerrainPlus {
 		std::string Log;
 		STPDeviceRuntimeProgram::STPSmartModule Module;
 		hipFunction_t Kernel = nullptr;
+	};
+
+	// The demo's runtime-compiled splatmap generator (SuperDemo+/World/Biomes/STPSplatmapGenerator.cpp:27-100
+	// compiling SuperDemo+/Script/STPSplatmapGenerator.cu) on hipRTC: the script reads the rendered
+	// area's biome map and UINT16 heightfield from device arrays instead of texture objects.
+	class STPRuntimeSplatmapGenerator {
+	public:
+		using STPChunkInformation = STPDiversity::STPTextureInformation::STPSplatGeneratorInformation::STPLocalChunkInformation;
+
+		// `perm` and `database` are device views that must outlive the generator. `script`: empty =
+		// the built-in script; `options`: extra compiler options.
+		STPRuntimeSplatmapGenerator(const STPAlgorithm::STPPermutation& perm,
+			const STPDiversity::STPTextureInformation::STPSplatRuleDatabase& database, uint2 map_size,
+			const std::string& script = std::string(), const std::vector<std::string>& options = {});
+		~STPRuntimeSplatmapGenerator();
+		STPRuntimeSplatmapGenerator(const STPRuntimeSplatmapGenerator&) = delete;
+		STPRuntimeSplatmapGenerator& operator=(const STPRuntimeSplatmapGenerator&) = delete;
+
+		// STPTextureFactory::splat: the region (255 = none) of every pixel of the listed chunks of the
+		// rendered area (area.x x area.y, row-major). Asynchronous on `stream`.
+		void splat(const unsigned short* biome, const unsigned short* height, unsigned char* splat, uint2 area,
+			const std::vector<STPChunkInformation>& chunks, hipStream_t stream);
+		const std::string& log() const noexcept { return this->Log; }
+		static const char* builtinScript() noexcept;
+
+	private:
+		uint2 MapSize;
+		std::string Log;
+		STPDeviceRuntimeProgram::STPSmartModule Module;
+		hipFunction_t Kernel = nullptr;
+		// the chunk list of a launch, through pinned staging
+		STPSmartDeviceMemory::STPDevice<STPChunkInformation> Chunk;
+		STPSmartDeviceMemory::STPHost<STPChunkInformation> Staging;
+		size_t Capacity = 0u;
+		hipEvent_t Uploaded = nullptr;
 	};
 
 } // namespace SuperTerrainPlus

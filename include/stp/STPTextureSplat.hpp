@@ -41,6 +41,17 @@ namespace SuperTerrainPlus {
 			const STPGradientNode* GradientRegistry;
 		};
 
+		// The chunks of the rendered area a splatmap launch covers (STPTextureInformation.hpp:114-135):
+		// each chunk's position in the area (in chunks) and its map offset; device array.
+		struct STPSplatGeneratorInformation {
+			struct STPLocalChunkInformation {
+				unsigned int LocalChunkCoordinateX, LocalChunkCoordinateY;
+				float ChunkMapOffsetX, ChunkMapOffsetY;
+			};
+			const STPLocalChunkInformation* RequestingLocalInfo;
+			unsigned int LocalCount;
+		};
+
 	} // namespace STPDiversity::STPTextureInformation
 
 	namespace STPAlgorithm {

@@ -534,6 +534,18 @@ int stp_layer_factory_generate_device(stp_layer_factory* factory, uint16_t* d_bi
  * BiomeCount was replaced by 0. Reading clears them. Synchronous. */
 int stp_layer_factory_status(stp_layer_factory* factory, uint32_t* status);
 
+/* The demo's runtime-compiled splatmap generator (SuperDemo+/World/Biomes/STPSplatmapGenerator.cpp:27-100
+ * over SuperDemo+/Script/STPSplatmapGenerator.cu) on hipRTC. `script` NULL = the built-in script
+ * (superterrainplus_amd/rtc/STPSplatmapGenerator.rtc); `perm` and `database` must outlive it. A
+ * script that does not compile gives STP_ERROR_COMPILATION with the compiler log. */
+typedef struct stp_splatmap_runtime stp_splatmap_runtime;
+int stp_splatmap_runtime_create(const stp_permutation* perm, const stp_splat_database* database, uint32_t map_x,
+	uint32_t map_y, const char* script, const char* const* options, uint32_t option_count, stp_splatmap_runtime** out);
+int stp_splatmap_runtime_destroy(stp_splatmap_runtime* generator);
+/* As stp_splatmap_generate, through the compiled script. Asynchronous on `stream`. */
+int stp_splatmap_runtime_generate(stp_splatmap_runtime* generator, const uint16_t* d_biome, const uint16_t* d_height,
+	uint8_t* d_splat, uint32_t area_x, uint32_t area_y, const stp_splat_chunk* h_chunks, uint32_t count, void* stream);
+
 /* ------------------------------------------------------------------ world pipeline
  * The reference world pipeline's generation chain (STPWorldPipeline.cpp: requestChunk :640-663,
  * erodeHeightmap :485-559, generateHeightmap :437-476, scheduleChunk :339-404) on resident device

@@ -287,6 +287,9 @@ _SIGNATURES = {
     "stp_chunk_record_bytes": (_I, [_U32, _U32, _U32, _P]),
     "stp_world_export": (_I, [_P, _P, _U32, _U32, _P, _U64, _P]),
     "stp_chunk_record_read": (_I, [_P, _U64, _P]),
+    "stp_splatmap_runtime_create": (_I, [_P, _P, _U32, _U32, C.c_char_p, _P, _U32, _P]),
+    "stp_splatmap_runtime_destroy": (_I, [_P]),
+    "stp_splatmap_runtime_generate": (_I, [_P, _P, _P, _P, _U32, _U32, _P, _U32, _P]),
 }
 
 _lib = None

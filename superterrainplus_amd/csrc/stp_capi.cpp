@@ -172,6 +172,10 @@ struct stp_splat_database {
 	std::unique_ptr<STPSplatRuleTable> Table;
 };
 
+struct stp_splatmap_runtime {
+	std::unique_ptr<STPRuntimeSplatmapGenerator> Generator;
+};
+
 struct stp_layer_factory {
 	std::unique_ptr<STPDiversity::STPLayerBiomeFactory> Factory;
 };
@@ -1209,5 +1213,44 @@ int stp_chunk_record_read(const void* const data, const uint64_t size, stp_chunk
 		const STPChunkRecord::View v = STPChunkRecord::read(static_cast<const unsigned char*>(data), size);
 		*record = stp_chunk_record {v.X, v.Y, v.OriginX, v.OriginY, v.Width, v.Height, v.Completeness, v.Maps, v.Check,
 			v.Biome, v.HeightMap, v.Fixed, v.Bytes};
+	});
+}
+
+/* ---------------------------------------------------------------------------- runtime splatmap */
+
+int stp_splatmap_runtime_create(const stp_permutation* const perm, const stp_splat_database* const database,
+	const uint32_t map_x, const uint32_t map_y, const char* const script, const char* const* const options,
+	const uint32_t option_count, stp_splatmap_runtime** const out) {
+	return guarded([&] {
+		require(perm && database && out && (options || option_count == 0u), "null argument");
+		std::vector<std::string> opt;
+		for (uint32_t i = 0u; i < option_count; i++) {
+			require(options[i], "null option");
+			opt.emplace_back(options[i]);
+		}
+		const STPAlgorithm::STPPermutation p {perm->Permutation, perm->Gradient2D, perm->Gradient2DSize};
+		auto g = std::make_unique<stp_splatmap_runtime>();
+		g->Generator = std::make_unique<STPRuntimeSplatmapGenerator>(p, database->Table->view(), make_uint2(map_x, map_y),
+			script ? std::string(script) : std::string(), opt);
+		*out = g.release();
+	});
+}
+
+int stp_splatmap_runtime_destroy(stp_splatmap_runtime* const generator) {
+	return guarded([&] { delete generator; });
+}
+
+int stp_splatmap_runtime_generate(stp_splatmap_runtime* const generator, const uint16_t* const d_biome,
+	const uint16_t* const d_height, uint8_t* const d_splat, const uint32_t area_x, const uint32_t area_y,
+	const stp_splat_chunk* const h_chunks, const uint32_t count, void* const stream) {
+	return guarded([&] {
+		require(generator && (count == 0u || h_chunks), "null argument");
+		std::vector<STPRuntimeSplatmapGenerator::STPChunkInformation> c(count);
+		for (uint32_t i = 0u; i < count; i++) {
+			c[i] = STPRuntimeSplatmapGenerator::STPChunkInformation {h_chunks[i].LocalX, h_chunks[i].LocalY,
+				h_chunks[i].OffsetX, h_chunks[i].OffsetY};
+		}
+		generator->Generator->splat(d_biome, d_height, d_splat, make_uint2(area_x, area_y), c,
+			static_cast<hipStream_t>(stream));
 	});
 }

@@ -394,3 +394,111 @@ void STPRuntimeBiomefieldGenerator::operator()(const STPNearestNeighbourHeightFl
 			+ std::to_string(this->BinCapacity) + ")");
 	}
 }
+
+// ---------------------------------------------------------------- STPRuntimeSplatmapGenerator
+
+extern "C" {
+extern const char* const STPLibrarySplatmapScript;
+}
+
+namespace {
+	constexpr char SplatKernelName[] = "generateTextureSplatmap";
+	constexpr unsigned int SplatTileX = 64u, SplatTileY = 4u;
+} // namespace
+
+const char* STPRuntimeSplatmapGenerator::builtinScript() noexcept {
+	return STPLibrarySplatmapScript;
+}
+
+STPRuntimeSplatmapGenerator::STPRuntimeSplatmapGenerator(const STPAlgorithm::STPPermutation& perm,
+	const STPDiversity::STPTextureInformation::STPSplatRuleDatabase& database, const uint2 map_size,
+	const std::string& script, const std::vector<std::string>& options) :
+	MapSize(map_size) {
+	if (map_size.x == 0u || map_size.y == 0u || !perm.Permutation || !perm.Gradient2D) {
+		STP_THROW(STPInvalidArgument, "the runtime splatmap generator needs a map size and a permutation table");
+	}
+	STPDeviceRuntimeBinary::STPSourceInformation info;
+	info.Option["-std=c++17"];
+	for (const std::string& o : options) {
+		info.Option[o];
+	}
+	info.NameExpression[SplatKernelName]["Dimension"]["HalfDimension"]["Permutation"]["SplatDatabase"];
+	const STPDeviceRuntimeBinary::STPExternalHeaderSource& headers = STPDeviceRuntimeBinary::libraryHeaders();
+	for (const auto& h : headers) {
+		info.ExternalHeader[h.first];
+	}
+	STPDeviceRuntimeBinary::STPCompilationOutput out = STPDeviceRuntimeBinary::compile("STPSplatmapGenerator",
+		script.empty() ? std::string(STPLibrarySplatmapScript) : script, info, headers);
+	this->Log = out.Log;
+	STPDeviceRuntimeProgram::STPLinkerInformation link;
+	link.DataOption.emplace_back(&out.ProgramObject, STPDeviceRuntimeProgram::STPBinaryType::CodeObject,
+		STPDeviceRuntimeProgram::STPLinkerInformation::STPDataJitOption {});
+	this->Module = STPDeviceRuntimeProgram::link(link);
+	const auto kernel = out.LoweredName.find(SplatKernelName);
+	if (kernel == out.LoweredName.end()) {
+		STP_THROW(STPCompilationError, std::string("the splatmap script does not define ") + SplatKernelName);
+	}
+	STP_CHECK_HIP(hipModuleGetFunction(&this->Kernel, this->Module.get(), kernel->second.c_str()));
+	// STPSplatmapGenerator.cpp:44-53: the globals are filled once
+	const float2 half = make_float2(static_cast<float>(map_size.x) / 2.0f, static_cast<float>(map_size.y) / 2.0f);
+	const auto put = [&](const char* name, const void* src, const size_t bytes) {
+		const ScriptGlobal g = scriptGlobal(this->Module.get(), out.LoweredName, name, bytes);
+		STP_CHECK_HIP(hipMemcpyHtoD(g.Pointer, const_cast<void*>(src), bytes));
+	};
+	put("Dimension", &map_size, sizeof(uint2));
+	put("HalfDimension", &half, sizeof(float2));
+	put("Permutation", &perm, sizeof(perm));
+	put("SplatDatabase", &database, sizeof(database));
+	STP_CHECK_HIP(hipEventCreateWithFlags(&this->Uploaded, hipEventDisableTiming));
+}
+
+STPRuntimeSplatmapGenerator::~STPRuntimeSplatmapGenerator() {
+	if (this->Uploaded) {
+		(void)hipEventSynchronize(this->Uploaded);
+		(void)hipEventDestroy(this->Uploaded);
+	}
+}
+
+void STPRuntimeSplatmapGenerator::splat(const unsigned short* const biome, const unsigned short* const height,
+	unsigned char* const splat, const uint2 area, const std::vector<STPChunkInformation>& chunks, const hipStream_t stream) {
+	if (!biome || !height || !splat) {
+		STP_THROW(STPInvalidArgument, "null map");
+	}
+	if (area.x == 0u || area.y == 0u) {
+		STP_THROW(STPInvalidArgument, "empty rendered area");
+	}
+	const uint2 m = this->MapSize;
+	for (const STPChunkInformation& c : chunks) {
+		if ((static_cast<unsigned long long>(c.LocalChunkCoordinateX) + 1ull) * m.x > area.x
+			|| (static_cast<unsigned long long>(c.LocalChunkCoordinateY) + 1ull) * m.y > area.y) {
+			STP_THROW(STPInvalidArgument, "a splat chunk lies outside the rendered area");
+		}
+	}
+	if (chunks.empty()) {
+		return;
+	}
+	if (chunks.size() > 65535u) {
+		STP_THROW(STPInvalidArgument, "at most 65535 chunks per splatmap launch");
+	}
+	// the staging buffer may still feed the previous upload
+	STP_CHECK_HIP(hipEventSynchronize(this->Uploaded));
+	if (chunks.size() > this->Capacity) {
+		STP_CHECK_HIP(hipDeviceSynchronize());
+		this->Capacity = chunks.size() + chunks.size() / 2u;
+		this->Chunk = STPSmartDeviceMemory::makeDevice<STPChunkInformation>(this->Capacity);
+		this->Staging = STPSmartDeviceMemory::makeHost<STPChunkInformation>(this->Capacity);
+	}
+	std::copy(chunks.begin(), chunks.end(), this->Staging.get());
+	STP_CHECK_HIP(hipMemcpyAsync(this->Chunk.get(), this->Staging.get(), chunks.size() * sizeof(STPChunkInformation),
+		hipMemcpyHostToDevice, stream));
+	STP_CHECK_HIP(hipEventRecord(this->Uploaded, stream));
+	STPDiversity::STPTextureInformation::STPSplatGeneratorInformation info {this->Chunk.get(),
+		static_cast<unsigned int>(chunks.size())};
+	const unsigned short* b = biome;
+	const unsigned short* h = height;
+	unsigned char* sp = splat;
+	uint2 a = area;
+	void* args[] = {&b, &h, &sp, &a, &info};
+	STP_CHECK_HIP(hipModuleLaunchKernel(this->Kernel, (m.x + SplatTileX - 1u) / SplatTileX, (m.y + SplatTileY - 1u) / SplatTileY,
+		static_cast<unsigned int>(chunks.size()), SplatTileX, SplatTileY, 1u, 0u, stream, args, nullptr));
+}

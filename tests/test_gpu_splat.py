@@ -111,3 +111,45 @@ def test_splatmap_rejects_bad_input(gpu):
             capi.call("stp_splat_database_create", None, 0, bad, 1, C.byref(h))
     finally:
         capi.lib().stp_splat_database_destroy(db)
+
+
+def test_runtime_splatmap_script_matches_builtin(gpu, oracle):
+    """The hipRTC-compiled splatmap script (the demo's runtime-compiled generator) against the
+    built-in kernel and the oracle, bit for bit; 70 chunks exceed the built-in kernel's 64 per
+    launch, the script takes them in one."""
+    log("runtime splatmap script")
+    rng = np.random.default_rng(11)
+    mx = my = 24
+    H, W_ = 7 * my, 10 * mx
+    height = smooth_field(rng, H, W_)
+    biome = np.repeat(np.repeat(rng.integers(0, 8, (H // 8, W_ // 8)), 8, 0), 8, 1).astype(np.uint16)
+    altitude, gradient = random_rules(rng, [0, 1, 3, 4, 6])
+    db = database(altitude, gradient)
+    table = capi.PermutationTable(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+    rt = C.c_void_p()
+    try:
+        capi.call("stp_splatmap_runtime_create", C.byref(table.view), db, mx, my, None, None, 0, C.byref(rt))
+        chunks = [(x, y, 24.0 * x - 100.0, 24.0 * y + 7.0) for y in range(7) for x in range(10)]
+        builtin = splat_gpu(table, db, biome, height, (mx, my), chunks)
+        b, h = capi.DeviceArray.from_host(biome), capi.DeviceArray.from_host(height)
+        out = capi.DeviceArray.from_host(np.zeros((H, W_), np.uint8))
+        c = (capi.SplatChunk * len(chunks))(*[capi.SplatChunk(*r) for r in chunks])
+        capi.call("stp_splatmap_runtime_generate", rt, b.vp, h.vp, out.vp, W_, H, c, len(chunks), None)
+        capi.synchronize()
+        got = out.download()
+        assert (got == builtin).all(), np.argwhere(got != builtin)[:5]
+        p, g32, _ = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+        assert (got == oracle.splatmap(p, g32, altitude, gradient, biome, height, (mx, my), chunks)).all()
+        assert len(np.unique(got)) > 3
+        with pytest.raises(capi.STPError):  # a chunk outside the area
+            capi.call("stp_splatmap_runtime_generate", rt, b.vp, h.vp, out.vp, W_, H,
+                      (capi.SplatChunk * 1)(capi.SplatChunk(10, 0, 0.0, 0.0)), 1, None)
+        bad = C.c_void_p()
+        with pytest.raises(capi.STPError) as e:
+            capi.call("stp_splatmap_runtime_create", C.byref(table.view), db, mx, my, b"__global__ void k( {", None, 0,
+                      C.byref(bad))
+        assert e.value.code == 5
+    finally:
+        if rt.value:
+            capi.lib().stp_splatmap_runtime_destroy(rt)
+        capi.lib().stp_splat_database_destroy(db)

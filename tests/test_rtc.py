@@ -68,3 +68,13 @@ def test_relocatable_program_has_bitcode_only():
     so the link itself is covered by the GPU tests."""
     p = capi.RtcProgram("MatrixOperator", M.SOURCE, ["-std=c++17", "-DSTP_TEST_ENABLE", "-fgpu-rdc"], M.NAMES)
     assert p.lowered("scale").startswith("_Z")
+
+
+def test_builtin_splatmap_script_compiles_with_library_headers():
+    """The splatmap script (SuperDemo+/Script/STPSplatmapGenerator.cu's contract) compiles against the
+    library's hipRTC-safe headers and lowers its kernel and globals."""
+    path = os.path.join(os.path.dirname(SCRIPT), "STPSplatmapGenerator.rtc")
+    names = ["generateTextureSplatmap", "Dimension", "HalfDimension", "Permutation", "SplatDatabase"]
+    p = capi.RtcProgram("STPSplatmapGenerator", open(path).read(), ["-std=c++17"], names, library_headers=True)
+    assert p.lowered("generateTextureSplatmap").startswith("_Z")
+    assert p.lowered("SplatDatabase") == "SplatDatabase"
