@@ -16,8 +16,10 @@
 #define STP_BIOME_LAYER_HPP
 
 #include <hip/hip_runtime.h>
+#include <stp/STPLayerDevice.hpp>
 
 #include <cstdint>
+#include <string>
 #include <memory>
 #include <vector>
 
@@ -49,7 +51,24 @@ namespace SuperTerrainPlus::STPDiversity {
 		RiverErode = 20u,     // STPRiverErodeLayer.h
 		RiverMix = 21u,       // STPRiverMixLayer.h
 		SmoothScale = 22u,    // STPSmoothScaleLayer.h
-		Count = 23u
+		Custom = 23u,         // a runtime-compiled layer (STPCustomLayer); Param = its index
+		Count = 24u
+	};
+
+	// A layer of the caller's own (an STPLayer subclass, STPLayer.h:153-165), compiled at run time
+	// (hipRTC). Source defines
+	//   __device__ unsigned short sample(const SuperTerrainPlus::STPLayerDevice::STPLayerContext& c, int x, int z);
+	// with the library's device headers (stp/STPLayerDevice.hpp and the others) available. The
+	// footprint declares which parent coordinates a sample at (x, z) may read: from ((x + Lo) >> Shift,
+	// (z + Lo) >> Shift) to ((x + Hi) >> Shift, (z + Hi) >> Shift), for both parents (Parents = 1 or 2).
+	// A read outside it gives 0 and status bit 0. E.g. a cross layer: Lo -1, Hi 1, Shift 0; a zoom:
+	// Lo 0, Hi 1, Shift 1.
+	struct STPCustomLayer {
+		std::string Source;
+		int Lo = 0, Hi = 0;
+		unsigned int Shift = 0u;
+		unsigned int Parents = 1u;
+		std::vector<std::string> Options;
 	};
 
 	// One layer: its kind, salt (STPLayer::Salt), parents (node indices, -1 for none), the kind's
@@ -62,17 +81,6 @@ namespace SuperTerrainPlus::STPDiversity {
 		unsigned int Param;
 		unsigned long long CacheSize;
 	};
-
-	// The demo's named biomes in the order the registry registers them (STPBiomeRegistry.cpp:69-104);
-	// the first biome registered under an id decides that id's precipitation type.
-	enum class STPNamedBiome : unsigned int {
-		Ocean, DeepOcean, WarmOcean, LukewarmOcean, ColdOcean, FrozenOcean, DeepWarmOcean, DeepLukewarmOcean,
-		DeepColdOcean, DeepFrozenOcean, River, FrozenRiver, Plains, Desert, Mountain, Forest, Taiga, SnowyTaiga,
-		SnowyTundra, Jungle, Savannah, Swamp, Badlands, DesertHills, TaigaHills, WoodedMountain, SnowyMountain,
-		ForestHills, JungleHills, SavannahPlateau, SwampHills, BadlandsPlateau, Beach, SnowyBeach, StoneShore,
-		Count
-	};
-	constexpr unsigned int NamedBiomeCount = static_cast<unsigned int>(STPNamedBiome::Count);
 
 	// Ids, temperatures and precipitations of the named biomes (STPBiome::ID/Temperature/Precipitation).
 	struct STPBiomeRegistry {
@@ -134,9 +142,10 @@ namespace SuperTerrainPlus::STPDiversity {
 		const uint2 BiomeDimension;
 
 		// Throws STPNumericDomainError for an empty dimension or a cache size that is not a power of 2,
-		// STPInvalidArgument for a malformed chain.
+		// STPInvalidArgument for a malformed chain, STPCompilationError for a custom layer that does not
+		// compile (with the compiler log).
 		STPLayerBiomeFactory(const STPLayerProgram& program, unsigned long long global_seed, const STPBiomeRegistry& registry,
-			uint2 dimension);
+			uint2 dimension, const std::vector<STPCustomLayer>& custom = {});
 		~STPLayerBiomeFactory();
 
 		STPLayerBiomeFactory(const STPLayerBiomeFactory&) = delete;

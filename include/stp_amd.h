@@ -493,7 +493,8 @@ typedef struct {
 	uint32_t Kind;      /* 0 Continent, 1 Scale, 2 Land, 3 Island, 4 Voronoi, 5 BaseBiome, 6 ClimateSingle,
 	                       7 ClimateModerate, 8 ClimateExtreme, 9 DeepOcean, 10 EaseEdge, 11 EdgeBiome, 12 Hills,
 	                       13 Noise, 14 NoiseToRiver, 15 OceanNoise, 16 OceanTemperate, 17 OceanExtreme,
-	                       18 OceanTransition, 19 OceanMix, 20 RiverErode, 21 RiverMix, 22 SmoothScale */
+	                       18 OceanTransition, 19 OceanMix, 20 RiverErode, 21 RiverMix, 22 SmoothScale,
+	                       23 custom (Param = the custom layer's index) */
 	uint32_t Param;     /* Scale: 0 normal, 1 fuzzy. Voronoi (2D): seed hash 0 identity (libstdc++), 1 FNV-1a (MSVC) */
 	int32_t Parent[2];  /* node indices, -1 for none; Hills (land, noise), OceanMix (land, ocean), RiverMix (land, river) */
 	uint64_t Salt;      /* STPLayer::Salt */
@@ -512,6 +513,20 @@ typedef struct {
 	float Precipitation[35];
 } stp_biome_registry;
 
+/* A layer of the caller's own (an STPLayer subclass, STPLayer.h:153-165), compiled at run time: Source
+ * defines `__device__ unsigned short sample(const SuperTerrainPlus::STPLayerDevice::STPLayerContext& c, int x,
+ * int z)` (stp/STPLayerDevice.hpp); a node of Kind 23 with Param = the custom layer's index runs it.
+ * A sample at (x, z) may read parent coordinates from ((x + Lo) >> Shift, (z + Lo) >> Shift) to
+ * ((x + Hi) >> Shift, (z + Hi) >> Shift) of its 1 or 2 parents; a read outside gives 0 and status bit 0. */
+typedef struct {
+	const char* Source;
+	int32_t Lo, Hi;
+	uint32_t Shift;
+	uint32_t Parents;
+	const char* const* Options;
+	uint32_t OptionCount;
+} stp_custom_layer;
+
 /* The demo's chain (STPAllLayers.cpp:55-110, 14 nodes) into h_nodes (capacity >= 14). */
 int stp_layer_program_demo(stp_layer_node* h_nodes, uint32_t capacity, uint32_t* count);
 /* The demo's registry after loading SuperDemo+/Biome.ini (ocean 0, plains 1, desert 2, forest 3, the
@@ -521,6 +536,11 @@ int stp_biome_registry_demo(stp_biome_registry* out);
  * empty dimension or a cache size that is not a power of 2 (STPBiomeFactory.cpp:20-22, STPLayer.cpp:68-73). */
 int stp_layer_factory_create(const stp_layer_node* h_nodes, uint32_t count, uint64_t global_seed,
 	const stp_biome_registry* registry, uint32_t dimension_x, uint32_t dimension_y, stp_layer_factory** out);
+/* As stp_layer_factory_create with custom layers; STP_ERROR_COMPILATION (compiler log in
+ * stp_last_error()) for one that does not compile. */
+int stp_layer_factory_create_custom(const stp_layer_node* h_nodes, uint32_t count, uint64_t global_seed,
+	const stp_biome_registry* registry, uint32_t dimension_x, uint32_t dimension_y, const stp_custom_layer* h_custom,
+	uint32_t custom_count, stp_layer_factory** out);
 int stp_layer_factory_destroy(stp_layer_factory* factory);
 /* STPBiomeFactory::operator()(biomemap, offset): pixel (x, y) is the root sampled at world
  * (x + offset_x, 0, y + offset_y). Synchronous, host memory. */

@@ -101,6 +101,12 @@ class BiomeRegistry(C.Structure):
     _fields_ = [("ID", C.c_uint16 * 35), ("Temperature", C.c_float * 35), ("Precipitation", C.c_float * 35)]
 
 
+class CustomLayer(C.Structure):
+    """stp_custom_layer: a runtime-compiled layer's source and parent footprint."""
+    _fields_ = [("Source", C.c_char_p), ("Lo", C.c_int32), ("Hi", C.c_int32), ("Shift", C.c_uint32),
+                ("Parents", C.c_uint32), ("Options", C.c_void_p), ("OptionCount", C.c_uint32)]
+
+
 def demo_layer_program():
     """The demo's 14-layer chain (SuperDemo+/World/Layers/STPAllLayers.cpp:55-110) as LayerNode list."""
     nodes, n = (LayerNode * 64)(), C.c_uint32()
@@ -284,6 +290,7 @@ _SIGNATURES = {
     "stp_layer_factory_generate": (_I, [_P, _P, C.c_int32, C.c_int32]),
     "stp_layer_factory_generate_device": (_I, [_P, _P, _P, _U32, _P]),
     "stp_layer_factory_status": (_I, [_P, _P]),
+    "stp_layer_factory_create_custom": (_I, [_P, _U32, _U64, _P, _U32, _U32, _P, _U32, _P]),
     "stp_chunk_record_bytes": (_I, [_U32, _U32, _U32, _P]),
     "stp_world_export": (_I, [_P, _P, _U32, _U32, _P, _U64, _P]),
     "stp_chunk_record_read": (_I, [_P, _U64, _P]),

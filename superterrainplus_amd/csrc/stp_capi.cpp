@@ -1137,7 +1137,28 @@ int stp_biome_registry_demo(stp_biome_registry* const out) {
 int stp_layer_factory_create(const stp_layer_node* const h_nodes, const uint32_t count, const uint64_t global_seed,
 	const stp_biome_registry* const registry, const uint32_t dimension_x, const uint32_t dimension_y,
 	stp_layer_factory** const out) {
+	return stp_layer_factory_create_custom(h_nodes, count, global_seed, registry, dimension_x, dimension_y, nullptr, 0u, out);
+}
+
+int stp_layer_factory_create_custom(const stp_layer_node* const h_nodes, const uint32_t count, const uint64_t global_seed,
+	const stp_biome_registry* const registry, const uint32_t dimension_x, const uint32_t dimension_y,
+	const stp_custom_layer* const h_custom, const uint32_t custom_count, stp_layer_factory** const out) {
 	return guarded([&] {
+		require(custom_count == 0u || h_custom, "null custom layers");
+		std::vector<STPDiversity::STPCustomLayer> custom(custom_count);
+		for (uint32_t i = 0u; i < custom_count; i++) {
+			const stp_custom_layer& c = h_custom[i];
+			require(c.Source && (c.OptionCount == 0u || c.Options), "a custom layer needs its source");
+			custom[i].Source = c.Source;
+			custom[i].Lo = c.Lo;
+			custom[i].Hi = c.Hi;
+			custom[i].Shift = c.Shift;
+			custom[i].Parents = c.Parents;
+			for (uint32_t k = 0u; k < c.OptionCount; k++) {
+				require(c.Options[k], "null option");
+				custom[i].Options.emplace_back(c.Options[k]);
+			}
+		}
 		require(out && h_nodes && registry, "null argument");
 		STPDiversity::STPLayerProgram p;
 		for (uint32_t i = 0u; i < count; i++) {
@@ -1149,7 +1170,8 @@ int stp_layer_factory_create(const stp_layer_node* const h_nodes, const uint32_t
 		std::memcpy(r.Temperature, registry->Temperature, sizeof r.Temperature);
 		std::memcpy(r.Precipitation, registry->Precipitation, sizeof r.Precipitation);
 		auto f = std::make_unique<stp_layer_factory>();
-		f->Factory = std::make_unique<STPDiversity::STPLayerBiomeFactory>(p, global_seed, r, make_uint2(dimension_x, dimension_y));
+		f->Factory = std::make_unique<STPDiversity::STPLayerBiomeFactory>(p, global_seed, r, make_uint2(dimension_x, dimension_y),
+			custom);
 		*out = f.release();
 	});
 }

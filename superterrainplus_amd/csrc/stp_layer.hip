@@ -16,6 +16,8 @@
 #include "stp_internal.hpp"
 #include "stp_stage_layer.hpp"
 
+#include <stp/STPDeviceRuntime.hpp>
+
 #include <algorithm>
 #include <cstring>
 
@@ -28,68 +30,15 @@ namespace {
 	using L::Sample;
 	using L::Seed;
 
-	constexpr unsigned int LayerX = 64u, LayerY = 4u;
+	constexpr unsigned int LayerX = L::TileX, LayerY = L::TileY;
 	// Voronoi tile: 64 x 16 pixels, 4 rows per thread; its lattice in LDS spans 18 x 6 points at the
 	// two lattice heights of a 2D map (y = -1, 0)
 	constexpr unsigned int VoronoiRows = 16u, LatticeX = 18u, LatticeZ = 6u, LatticePoints = LatticeX * LatticeZ * 2u;
 
-	// Where map c's samples of a node lie: sample (i, j) is the layer at (X0 + i Step, Z0 + j Step),
-	// stored at base[Offset + j Pitch + i]; a node's rectangles are Rect[node * Count + c].
-	struct LayerRect {
-		int X0, Z0;
-		unsigned int W, H, Pitch, Step;
-		unsigned long long Offset;
-	};
-	static_assert(sizeof(LayerRect) == 32u, "rectangle layout");
-
-	struct NodeArgs {
-		unsigned int Kind, Param, Count, Node, Limit, Root;
-		Seed LayerSeed, VoronoiSeed;
-		Sample* Out;       // the caller's maps for the root, the plan's buffer otherwise
-		const Sample* In;  // the plan's buffer (parents)
-		int Parent[2];
-		const LayerRect* Rect;
-		const uint2* Tile; // (map, tile) of this launch
-		L::Registry Reg;
-		unsigned int* Fault;
-	};
-
-	// Parent reads of map c, checked against the parent's rectangle.
-	struct Reader {
-		const Sample* Data;
-		int X0, Z0;
-		unsigned int W, H, Pitch;
-		unsigned int* Fault;
-
-		__device__ Reader(const NodeArgs& a, const unsigned int k, const unsigned int c) {
-			const int p = a.Parent[k];
-			const LayerRect r = p >= 0 ? a.Rect[static_cast<size_t>(p) * a.Count + c] : LayerRect {};
-			this->Data = a.In + r.Offset;
-			this->X0 = r.X0;
-			this->Z0 = r.Z0;
-			this->W = r.W;
-			this->H = r.H;
-			this->Pitch = r.Pitch;
-			this->Fault = a.Fault;
-		}
-		__device__ inline Sample operator()(const int x, const int z) const {
-			const unsigned int u = static_cast<unsigned int>(x - this->X0), v = static_cast<unsigned int>(z - this->Z0);
-			if (u >= this->W || v >= this->H) {
-				atomicOr(this->Fault, 1u);
-				return 0u;
-			}
-			return this->Data[static_cast<size_t>(v) * this->Pitch + u];
-		}
-	};
-
-	// the root's ids index the caller's biome table: one out of range is flagged and written as 0
-	__device__ inline Sample limited(const NodeArgs& a, const Sample v) {
-		if (a.Root && v >= a.Limit) {
-			atomicOr(a.Fault, 2u);
-			return 0u;
-		}
-		return v;
-	}
+	using L::LayerRect;
+	using L::NodeArgs;
+	using L::Reader;
+	using L::limited;
 
 	// One pixel of a non-Voronoi node (the demo layers' sample(x, y, z) with y = 0).
 	__device__ Sample samplePixel(const NodeArgs& a, const Reader& p, const Reader& q, const int x, const int z) {
@@ -240,17 +189,9 @@ namespace {
 	}
 
 	__global__ void __launch_bounds__(LayerX * LayerY) layerKERNEL(const NodeArgs a) {
-		const uint2 t = a.Tile[blockIdx.x];
-		const LayerRect rect = a.Rect[static_cast<size_t>(a.Node) * a.Count + t.x];
-		const unsigned int tilesX = (rect.W + LayerX - 1u) / LayerX;
-		const unsigned int lx = (t.y % tilesX) * LayerX + threadIdx.x, lz = (t.y / tilesX) * LayerY + threadIdx.y;
-		if (lx >= rect.W || lz >= rect.H) {
-			return;
-		}
-		const Reader p(a, 0u, t.x), q(a, 1u, t.x);
-		const Sample v = samplePixel(a, p, q, rect.X0 + static_cast<int>(lx * rect.Step),
-			rect.Z0 + static_cast<int>(lz * rect.Step));
-		a.Out[rect.Offset + static_cast<size_t>(lz) * rect.Pitch + lx] = limited(a, v);
+		L::runNode(a, [](const NodeArgs& args, const Reader& p, const Reader& q, const int x, const int z) {
+			return samplePixel(args, p, q, x, z);
+		});
 	}
 
 	// The jitter of lattice point (px, py, pz) (STPVoronoiLayer.h:16-43), x, y and z components.
@@ -463,7 +404,20 @@ unsigned int STPLayerPlan::count() const noexcept {
 	return this->Impl->Count;
 }
 
+namespace {
+	// a compiled custom layer
+	struct CustomKind {
+		int Lo, Hi;
+		unsigned int Shift, Parents;
+		STPDeviceRuntimeProgram::STPSmartModule Module;
+		hipFunction_t Kernel = nullptr;
+	};
+
+	constexpr char CustomKernelName[] = "STPCustomLayerKERNEL";
+} // namespace
+
 struct STPLayerBiomeFactory::STPImplementation {
+	std::vector<CustomKind> Custom;
 	std::vector<STPLayerNode> Node;
 	std::vector<Seed> LayerSeed, VoronoiSeed;
 	std::vector<unsigned char> Reachable;
@@ -475,7 +429,7 @@ struct STPLayerBiomeFactory::STPImplementation {
 };
 
 STPLayerBiomeFactory::STPLayerBiomeFactory(const STPLayerProgram& program, const unsigned long long global_seed,
-	const STPBiomeRegistry& registry, const uint2 dimension) :
+	const STPBiomeRegistry& registry, const uint2 dimension, const std::vector<STPCustomLayer>& custom) :
 	BiomeDimension(dimension), Impl(std::make_unique<STPImplementation>()) {
 	// STPBiomeFactory.cpp:20-22
 	STP_ASSERT_DOMAIN(dimension.x > 0u && dimension.y > 0u,
@@ -486,6 +440,38 @@ STPLayerBiomeFactory::STPLayerBiomeFactory(const STPLayerProgram& program, const
 	if (n == 0u || n > 4096u) {
 		STP_THROW(STPInvalidArgument, "a layer chain needs between 1 and 4096 layers");
 	}
+	// custom layers: the caller's sample function inside the library's node driver, one code object each
+	for (size_t k = 0u; k < custom.size(); k++) {
+		const STPCustomLayer& cl = custom[k];
+		if (cl.Parents < 1u || cl.Parents > 2u || cl.Shift > 16u || cl.Lo > cl.Hi || cl.Lo < -1024 || cl.Hi > 1024) {
+			STP_THROW(STPInvalidArgument, "custom layer " + std::to_string(k)
+				+ ": 1 or 2 parents, Lo <= Hi within [-1024, 1024], Shift <= 16");
+		}
+		const std::string source = "#include <stp/STPLayerDevice.hpp>\n#line 1 \"custom layer " + std::to_string(k) + "\"\n"
+			+ cl.Source + "\nextern \"C\" __global__ void __launch_bounds__(256) " + CustomKernelName
+			+ "(const SuperTerrainPlus::STPLayerDevice::NodeArgs a) {\n"
+			  "\tSuperTerrainPlus::STPLayerDevice::runNode(a, [](const SuperTerrainPlus::STPLayerDevice::NodeArgs& args,\n"
+			  "\t\tconst SuperTerrainPlus::STPLayerDevice::Reader& p, const SuperTerrainPlus::STPLayerDevice::Reader& q,\n"
+			  "\t\tconst int x, const int z) {\n"
+			  "\t\treturn sample(SuperTerrainPlus::STPLayerDevice::STPLayerContext {args, p, q}, x, z);\n\t});\n}\n";
+		STPDeviceRuntimeBinary::STPSourceInformation info;
+		info.Option["-std=c++17"];
+		for (const std::string& o : cl.Options) {
+			info.Option[o];
+		}
+		const STPDeviceRuntimeBinary::STPExternalHeaderSource& headers = STPDeviceRuntimeBinary::libraryHeaders();
+		for (const auto& h : headers) {
+			info.ExternalHeader[h.first];
+		}
+		STPDeviceRuntimeBinary::STPCompilationOutput out =
+			STPDeviceRuntimeBinary::compile("STPCustomLayer" + std::to_string(k), std::string(source), info, headers);
+		STPDeviceRuntimeProgram::STPLinkerInformation link;
+		link.DataOption.emplace_back(&out.ProgramObject, STPDeviceRuntimeProgram::STPBinaryType::CodeObject,
+			STPDeviceRuntimeProgram::STPLinkerInformation::STPDataJitOption {});
+		CustomKind kind {cl.Lo, cl.Hi, cl.Shift, cl.Parents, STPDeviceRuntimeProgram::link(link), nullptr};
+		STP_CHECK_HIP(hipModuleGetFunction(&kind.Kernel, kind.Module.get(), CustomKernelName));
+		m.Custom.push_back(std::move(kind));
+	}
 	for (size_t i = 0u; i < n; i++) {
 		const STPLayerNode& node = m.Node[i];
 		// STPLayer.cpp:68-73: a cache capacity is 0 or a power of 2
@@ -494,7 +480,11 @@ STPLayerBiomeFactory::STPLayerBiomeFactory(const STPLayerProgram& program, const
 		if (node.Kind >= static_cast<unsigned int>(STPLayerKind::Count)) {
 			STP_THROW(STPInvalidArgument, "unknown layer kind");
 		}
-		const unsigned int want = parentCount(static_cast<STPLayerKind>(node.Kind));
+		if (node.Kind == static_cast<unsigned int>(STPLayerKind::Custom) && node.Param >= m.Custom.size()) {
+			STP_THROW(STPInvalidArgument, "layer " + std::to_string(i) + " names a custom layer that was not given");
+		}
+		const unsigned int want = node.Kind == static_cast<unsigned int>(STPLayerKind::Custom) ? m.Custom[node.Param].Parents
+			: parentCount(static_cast<STPLayerKind>(node.Kind));
 		for (unsigned int k = 0u; k < 2u; k++) {
 			const int p = node.Parent[k];
 			if (k < want ? (p < 0 || static_cast<size_t>(p) >= i) : p != -1) {
@@ -635,6 +625,16 @@ void STPLayerBiomeFactory::plan(STPLayerPlan& plan, const std::vector<STPLayerMa
 				need = Box {shr(b.X0 - 2, 2), shr(b.Z0 - 2, 2), shr(b.X1 - 2, 2) + 1, shr(b.Z1 - 2, 2) + 1};
 				break;
 			case STPLayerKind::OceanTransition: need = b.grown(8); break;
+			case STPLayerKind::Custom: {
+				const CustomKind& k = m.Custom[node.Param];
+				need = Box {shr(b.X0 + k.Lo, static_cast<int>(k.Shift)), shr(b.Z0 + k.Lo, static_cast<int>(k.Shift)),
+					shr(b.X1 + k.Hi, static_cast<int>(k.Shift)), shr(b.Z1 + k.Hi, static_cast<int>(k.Shift))};
+				box[static_cast<size_t>(node.Parent[0]) * count + c].join(need);
+				if (node.Parent[1] >= 0) {
+					box[static_cast<size_t>(node.Parent[1]) * count + c].join(need);
+				}
+				continue;
+			}
 			case STPLayerKind::BaseBiome:
 			case STPLayerKind::ClimateSingle:
 			case STPLayerKind::Noise:
@@ -771,6 +771,12 @@ void STPLayerBiomeFactory::run(const STPLayerPlan& plan, unsigned short* const o
 		a.Tile = tiles + tb;
 		a.Reg = m.Reg;
 		a.Fault = m.Fault.get();
+		if (node.Kind == static_cast<unsigned int>(STPLayerKind::Custom)) {
+			void* args[] = {&a};
+			STP_CHECK_HIP(hipModuleLaunchKernel(m.Custom[node.Param].Kernel, te - tb, 1u, 1u, LayerX, LayerY, 1u, 0u, stream,
+				args, nullptr));
+			continue;
+		}
 		if (node.Kind == static_cast<unsigned int>(STPLayerKind::Voronoi)) {
 			voronoiKERNEL<<<te - tb, dim3(LayerX, LayerY), 0, stream>>>(a);
 		} else {

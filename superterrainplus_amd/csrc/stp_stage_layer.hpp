@@ -1,5 +1,5 @@
-// Device side of the biome layers: the layer RNG (STPLayer.cpp:111-178) and the sample rule of every
-// demo layer kind (SuperDemo+/World/Layers/*.h), as functions of parent values already read.
+// The sample rule of every demo layer kind (SuperDemo+/World/Layers/*.h), as functions of parent
+// values already read; the layer RNG and registry are in stp/STPLayerDevice.hpp.
 #pragma once
 #ifndef STP_AMD_STAGE_LAYER_HPP
 #define STP_AMD_STAGE_LAYER_HPP
@@ -8,85 +8,7 @@
 
 namespace SuperTerrainPlus::STPStage::Layer {
 
-	using Seed = unsigned long long;
-	using Sample = unsigned short;
-	using STPDiversity::STPNamedBiome;
-
-	// STPLayer::mixSeed (STPLayer.cpp:174-178): s <- s (s A + C) + fac over 64-bit words.
-	__host__ __device__ inline Seed mixSeed(Seed s, const long long fac) {
-		s *= s * 6364136223846793005ull + 1442695040888963407ull;
-		s += static_cast<Seed>(fac);
-		return s;
-	}
-
-	// STPLayer::seedLayer (STPLayer.cpp:143-151).
-	__host__ __device__ inline Seed seedLayer(const Seed global, const Seed salt) {
-		Seed mid = mixSeed(salt, static_cast<long long>(salt));
-		mid = mixSeed(mid, static_cast<long long>(mid));
-		mid = mixSeed(mid, static_cast<long long>(mid));
-		Seed s = mixSeed(global, static_cast<long long>(mid));
-		s = mixSeed(s, static_cast<long long>(mid));
-		return mixSeed(s, static_cast<long long>(mid));
-	}
-
-	// STPLayer::seedLocal (STPLayer.cpp:153-159).
-	__host__ __device__ inline Seed seedLocal(const Seed layer, const int x, const int z) {
-		Seed s = mixSeed(layer, x);
-		s = mixSeed(s, z);
-		s = mixSeed(s, x);
-		return mixSeed(s, z);
-	}
-
-	// STPLayer::STPLocalSampler (STPLayer.cpp:107-132): a value is (state >> 24) mod range, then the
-	// state advances by mixSeed(state, layer seed). Ranges are compile-time constants so that the
-	// 64-bit remainder becomes a multiply.
-	struct Sampler {
-		Seed Layer, Local;
-
-		template <unsigned int R>
-		__device__ inline unsigned int next() {
-			const unsigned int v = static_cast<unsigned int>((this->Local >> 24u) % static_cast<Seed>(R));
-			this->Local = mixSeed(this->Local, static_cast<long long>(this->Layer));
-			return v;
-		}
-		__device__ inline Sample choose(const Sample a, const Sample b) {
-			return this->next<2u>() == 0u ? a : b;
-		}
-		__device__ inline Sample choose(const Sample a, const Sample b, const Sample c, const Sample d) {
-			const unsigned int i = this->next<4u>();
-			return i == 0u ? a : i == 1u ? b : i == 2u ? c : d;
-		}
-	};
-
-	__device__ inline Sampler sampler(const Seed layer, const int x, const int z) {
-		return Sampler {layer, seedLocal(layer, x, z)};
-	}
-
-	// The biome ids the layers compare against, and each id's precipitation type (0 none, 1 rain,
-	// 2 snow; STPBiomeRegistry.cpp:125-137) in a 65536-entry table.
-	struct Registry {
-		Sample Id[STPDiversity::NamedBiomeCount];
-		const unsigned char* Precipitation;
-
-		__device__ inline Sample operator[](const STPNamedBiome b) const {
-			return this->Id[static_cast<unsigned int>(b)];
-		}
-		// STPBiomeRegistry::isShallowOcean / isOcean (STPBiomeRegistry.cpp:109-119)
-		__device__ inline bool shallowOcean(const Sample v) const {
-			const Registry& r = *this;
-			return v == r[STPNamedBiome::Ocean] || v == r[STPNamedBiome::FrozenOcean] || v == r[STPNamedBiome::WarmOcean]
-				|| v == r[STPNamedBiome::LukewarmOcean] || v == r[STPNamedBiome::ColdOcean];
-		}
-		__device__ inline bool ocean(const Sample v) const {
-			const Registry& r = *this;
-			return this->shallowOcean(v) || v == r[STPNamedBiome::DeepOcean] || v == r[STPNamedBiome::DeepWarmOcean]
-				|| v == r[STPNamedBiome::DeepLukewarmOcean] || v == r[STPNamedBiome::DeepColdOcean]
-				|| v == r[STPNamedBiome::DeepFrozenOcean];
-		}
-		__device__ inline bool snowy(const Sample v) const {
-			return this->Precipitation[v] == 2u;
-		}
-	};
+	using namespace STPLayerDevice;
 
 	// ---- rules of the single-parent and cross layers, on parent values --------------------------
 

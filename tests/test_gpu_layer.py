@@ -161,3 +161,86 @@ def test_layer_factory_errors(gpu):
         Factory([node("Continent", 1), node("Scale", 2, [1])], 0, reg, (4, 4))  # parent after the node
     with pytest.raises(capi.STPError):
         Factory([node("Continent", 1), node("Hills", 2, [0])], 0, reg, (4, 4))  # hills needs two parents
+
+
+ISLAND = b"""
+__device__ unsigned short sample(const SuperTerrainPlus::STPLayerDevice::STPLayerContext& c, int x, int z) {
+	using SuperTerrainPlus::STPDiversity::STPNamedBiome;
+	const auto& r = c.registry();
+	const unsigned short v = c.parent(0, x, z);
+	auto s = c.sampler(x, z);
+	return r.shallowOcean(v) && r.shallowOcean(c.parent(0, x, z - 1)) && r.shallowOcean(c.parent(0, x + 1, z))
+		&& r.shallowOcean(c.parent(0, x, z + 1)) && r.shallowOcean(c.parent(0, x - 1, z)) && s.nextValue(2) == 0u
+		? r[STPNamedBiome::Plains] : v;
+}
+"""
+
+MIX = b"""
+__device__ unsigned short sample(const SuperTerrainPlus::STPLayerDevice::STPLayerContext& c, int x, int z) {
+	return static_cast<unsigned short>((c.parent(0, x, z) * 7u + c.parent(1, x, z)) & 0xFFu);
+}
+"""
+
+# reads 5 pixels east of its declared footprint
+WIDE = b"""
+__device__ unsigned short sample(const SuperTerrainPlus::STPLayerDevice::STPLayerContext& c, int x, int z) {
+	return static_cast<unsigned short>(c.parent(0, x + 5, z) + 1u);
+}
+"""
+
+
+def custom_factory(nodes, reg, dim, custom):
+    arr = (capi.LayerNode * len(nodes))(*nodes)
+    cl = (capi.CustomLayer * len(custom))(*[capi.CustomLayer(src, lo, hi, shift, parents, None, 0)
+                                            for src, lo, hi, shift, parents in custom])
+    h = C.c_void_p()
+    capi.call("stp_layer_factory_create_custom", arr, len(nodes), 6666, C.byref(reg), dim[0], dim[1], cl, len(custom),
+              C.byref(h))
+    f = Factory.__new__(Factory)
+    f.h, f.dim = h, dim
+    return f
+
+
+def test_custom_layer_replaces_a_builtin_one(gpu, oracle):
+    """The demo chain with its island layer written as a custom layer (runtime compiled): the maps
+    equal the demo chain's on the oracle, bit for bit."""
+    log("layer custom island")
+    nodes, reg = capi.demo_layer_program(), capi.demo_biome_registry()
+    custom = list(nodes)
+    custom[7] = capi.LayerNode(23, 0, (C.c_int32 * 2)(6, -1), nodes[7].Salt, nodes[7].CacheSize)
+    dim, offsets = (96, 80), [(0, 400), (-333, 77), (5000, -7000)]
+    got = custom_factory(custom, reg, dim, [(ISLAND, -1, 1, 0, 1)]).device(offsets)
+    assert (got == oracle_maps(oracle, nodes, 6666, reg, offsets, dim)).all()
+
+
+def test_custom_two_parent_layer(gpu):
+    """A custom layer of two parents against its parents' own maps: (a * 7 + b) & 255."""
+    log("layer custom two parents")
+    reg = capi.demo_biome_registry()
+    for i in range(35):
+        reg.ID[i] = i
+    a, b = node("Continent", 11), node("OceanNoise", 12)
+    mix = capi.LayerNode(23, 0, (C.c_int32 * 2)(0, 1), 13, 0)
+    dim, offsets = (64, 48), [(-50, 70), (900, -3)]
+    got = custom_factory([a, b, mix], reg, dim, [(MIX, 0, 0, 0, 2)]).device(offsets)
+    ma = Factory([a], 6666, reg, dim).device(offsets).astype(np.uint32)
+    mb = Factory([a, b], 6666, reg, dim).device(offsets).astype(np.uint32)
+    assert (got == ((ma * 7 + mb) & 0xFF)).all()
+    assert len(np.unique(got)) >= 3
+
+
+def test_custom_layer_outside_its_footprint_and_errors(gpu):
+    log("layer custom footprint, errors")
+    reg = capi.demo_biome_registry()
+    nodes = [node("Continent", 11), capi.LayerNode(23, 0, (C.c_int32 * 2)(0, -1), 13, 0)]
+    f = custom_factory(nodes, reg, (32, 32), [(WIDE, 0, 0, 0, 1)])
+    got = f.device([(0, 0)])
+    status = C.c_uint32()
+    capi.call("stp_layer_factory_status", f.h, C.byref(status))
+    assert status.value & 1
+    assert (got[0][:, 27:] == 1).all()  # reads past the parent's rectangle give 0
+    with pytest.raises(capi.STPError) as e:
+        custom_factory(nodes, reg, (32, 32), [(b"__device__ unsigned short sample(", 0, 0, 0, 1)])
+    assert e.value.code == 5
+    with pytest.raises(capi.STPError):
+        custom_factory(nodes, reg, (32, 32), [])  # the node names custom layer 0, none given
