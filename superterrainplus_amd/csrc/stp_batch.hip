@@ -475,6 +475,7 @@ namespace SuperTerrainPlus::STPBatch {
 
 	constexpr unsigned int NoRun = 0xFFFFFFFFu;
 	constexpr unsigned int ItemStageRows = 16u;
+	static_assert(ItemStageRows == VPrefetch, "the vertical item pass stages the rows it fetches at once");
 
 	// Per-biome window state of one line.
 	template <unsigned int MaxB>
@@ -780,11 +781,17 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 		unsigned short* const outBin = vbin + c.Line * vcap;
 		unsigned char* const outN = vcount + c.Line;
+		// software pipelined: the rows of the next block are fetched before this block is processed
+		unsigned int enter[VPrefetch], leave[VPrefetch];
+		fetch(enter, static_cast<long long>(yr.x) + 2 * static_cast<long long>(radius));
+		fetch(leave, static_cast<long long>(yr.x) - 1);
 		for (unsigned int y0 = yr.x; y0 < yr.y; y0 += R) {
 			const unsigned int rows = min(R, yr.y - y0);
-			unsigned int enter[VPrefetch], leave[VPrefetch];
-			fetch(enter, static_cast<long long>(y0) + 2 * static_cast<long long>(radius));
-			fetch(leave, static_cast<long long>(y0) - 1);
+			unsigned int nextEnter[VPrefetch] = {}, nextLeave[VPrefetch] = {};
+			if (y0 + R < yr.y) {
+				fetch(nextEnter, static_cast<long long>(y0 + R) + 2 * static_cast<long long>(radius));
+				fetch(nextLeave, static_cast<long long>(y0 + R) - 1);
+			}
 #pragma unroll
 			for (unsigned int i = 0u; i < VPrefetch; i++) {
 				const unsigned int y = y0 + i;
@@ -823,6 +830,11 @@ namespace SuperTerrainPlus::STPBatch {
 				}
 			}
 			__syncthreads();
+#pragma unroll
+			for (unsigned int i = 0u; i < VPrefetch; i++) {
+				enter[i] = nextEnter[i];
+				leave[i] = nextLeave[i];
+			}
 		}
 		if (over) {
 			atomicOr(overflow, 1u);
