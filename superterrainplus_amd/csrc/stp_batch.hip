@@ -781,6 +781,8 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 		unsigned short* const outBin = vbin + c.Line * vcap;
 		unsigned char* const outN = vcount + c.Line;
+		// 4-row vector stores when every output row block starts 4-aligned in the arenas
+		const bool quads = side % 4u == 0u && yr.x % 4u == 0u && c.Line % 4u == 0u;
 		// software pipelined: the rows of the next block are fetched before this block is processed
 		unsigned int enter[VPrefetch], leave[VPrefetch];
 		fetch(enter, static_cast<long long>(yr.x) + 2 * static_cast<long long>(radius));
@@ -813,6 +815,46 @@ namespace SuperTerrainPlus::STPBatch {
 				}
 			}
 			__syncthreads();
+			if (quads && rows == R) {
+				// lane -> (column, 4 consecutive rows): 4- and 8-byte stores; a slot plane is written for the
+				// rows of a quad when one of them uses the slot (the others get zeros, never read)
+				constexpr unsigned int Q = R / 4u;
+				for (unsigned int i = lane; i < ShfLineBlock * Q; i += ShfLineBlock) {
+					const unsigned int cc = i / Q, q = i % Q;
+					const unsigned int cg = (t.y % nb) * ShfLineBlock + cc;
+					if (cg < ncols) {
+						unsigned int n[4];
+#pragma unroll
+						for (unsigned int r = 0u; r < 4u; r++) {
+							n[r] = stageN[(4u * q + r) * 65u + cc];
+						}
+						const size_t y = static_cast<size_t>(y0) + 4u * q;
+						*reinterpret_cast<uchar4*>(outN + static_cast<size_t>(cg) * side + y) = make_uchar4(
+							static_cast<unsigned char>(n[0]), static_cast<unsigned char>(n[1]), static_cast<unsigned char>(n[2]),
+							static_cast<unsigned char>(n[3]));
+						const unsigned int most = max(max(n[0], n[1]), max(n[2], n[3]));
+#pragma unroll
+						for (unsigned int k = 0u; k < MaxB; k++) {
+							if (k < most) {
+								unsigned short v[4];
+#pragma unroll
+								for (unsigned int r = 0u; r < 4u; r++) {
+									v[r] = k < n[r] ? static_cast<unsigned short>(stage[(k * R + 4u * q + r) * 65u + cc]) : 0u;
+								}
+								*reinterpret_cast<ushort4*>(outBin + (static_cast<size_t>(cg) * vcap + k) * side + y) =
+									make_ushort4(v[0], v[1], v[2], v[3]);
+							}
+						}
+					}
+				}
+				__syncthreads();
+#pragma unroll
+				for (unsigned int i = 0u; i < VPrefetch; i++) {
+					enter[i] = nextEnter[i];
+					leave[i] = nextLeave[i];
+				}
+				continue;
+			}
 			// consecutive lanes take consecutive rows of one column
 			for (unsigned int i = lane; i < ShfLineBlock * R; i += ShfLineBlock) {
 				const unsigned int cc = i / R, yy = i % R;
