@@ -1,7 +1,8 @@
 """ctypes binding of the C ABI in ``include/stp_amd.h``.
 
 This is the harness-side binding a Python caller (the tests, ``bench.py``) uses; it is also the
-reference shape for a cgo/JNI/N-API binding. It loads the in-tree
+reference shape for a cgo/JNI/N-API binding: one declaration per entry point, plain pointers
+and sizes, status codes checked after every call. It loads the in-tree
 ``libstp_amd.so`` and raises if it is missing: there is no fallback path.
 """
 import ctypes as C

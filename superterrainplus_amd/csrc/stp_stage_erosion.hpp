@@ -103,6 +103,14 @@ namespace SuperTerrainPlus::STPStage {
 		return cx + 1u >= r && cy + 1u >= r && cx + reach <= a.Total.x - 1u && cy + reach <= a.Total.y - 1u;
 	}
 
+	// The kill test's bounds on the moved position (STPRainDrop.cu:96-105): x in [r, Total.x - r),
+	// y in [r, Total.y - r), as (x lo, x hi, y lo, y hi).
+	__device__ inline float4 killBounds(const ErosionArgs& a) {
+		const float radius = static_cast<float>(a.Rain.ErosionBrushRadius);
+		return make_float4(radius, 1.0f * static_cast<float>(a.Total.x) - radius, radius,
+			1.0f * static_cast<float>(a.Total.y) - radius);
+	}
+
 	__device__ inline bool spawnInside(const ErosionArgs& a, const DropState& d) {
 		return static_cast<unsigned int>(d.px) + 1u <= a.Total.x - 1u
 			&& static_cast<unsigned int>(d.py) + 1u <= a.Total.y - 1u;
@@ -169,10 +177,11 @@ namespace SuperTerrainPlus::STPStage {
 		__device__ void add(const unsigned int i, const float v) const { unsafeAtomicAdd(this->Map + i, v); }
 		// the chunk-level guards already keep every access inside the map
 		__device__ bool holds(unsigned int, unsigned int, unsigned int, unsigned int) const { return true; }
-		// a spawn may start; a step that left cell (cx, cy) for (px, py), inside the map's kill
-		// bounds, may erode at (cx, cy)
+		// a spawn may start; a step that left cell (cx, cy) for a position inside the kill bounds
+		// (STPRainDrop.cu:96-105: [r, Total - r) on both axes) may erode at (cx, cy)
 		__device__ bool spawnGuard(const ErosionArgs& a, const DropState& d) const { return spawnInside(a, d); }
-		__device__ bool stepGuard(const ErosionArgs& a, const unsigned int cx, const unsigned int cy, float, float) const {
+		__device__ float4 moveBounds(const ErosionArgs& a) const { return killBounds(a); }
+		__device__ bool stepGuard(const ErosionArgs& a, const unsigned int cx, const unsigned int cy) const {
 			return cellInside(a, cx, cy);
 		}
 		// all taps of a step in one group: one HBM/L2 round trip
@@ -258,44 +267,78 @@ namespace SuperTerrainPlus::STPStage {
 		return taps == 9u ? 2 : (taps == 25u ? 3 : (taps == 45u ? 4 : 0));
 	}
 
-	// Erode taps of rows Dy .. R-1 around `cell` (LDS, row stride rw); returns the fixed-point
-	// sediment taken. Per tap: want = round(amount * w) and one returning atomic subtraction; the
-	// old height it returns gives min(old, want), and a tap with old < want gets the excess back, so
-	// the cell ends at old - min(old, want), the reference's erode update. Within a row the taps
-	// are consecutive words (one address, immediate offsets); a row is settled before the next is
-	// issued, which bounds the live registers to one row.
-	template <int R, int Dy>
-	__device__ inline int shapedTake(int* const cell, const int rw, const float amount) {
+	// taps of the brush of radius R
+	constexpr int shapedTaps(const int R) {
+		int n = 0;
+		for (int dy = 1 - R; dy < R; dy++) {
+			n += 2 * brushRowHalf(R, dy) + 1;
+		}
+		return n;
+	}
+
+	// The erode update of the brush of radius R around `cell` (LDS, row stride rw), in fixed point.
+	// Per tap: want = round(amount * w) and one returning atomic subtraction; the old height it
+	// returns gives min(old, want), and a tap with old < want gets the excess back, so the cell ends
+	// at old - min(old, want), the reference's erode update (STPRainDrop.cu:150-155). Within a row
+	// the taps are consecutive words (one address, immediate offsets).
+	//
+	// shapedIssue: rows Dy .. R-1, every tap's atomic issued, want and old kept at index J + j for
+	// the row's j-th tap.
+	template <int R, int Dy, int J>
+	__device__ inline void shapedIssue(int* const cell, const int rw, const float amount, int* const want, int* const old) {
 		constexpr int H = brushRowHalf(R, Dy), N = 2 * H + 1;
 		// the row's first tap; opaque to the optimiser, so that it keeps one address per row with
 		// non-negative immediate offsets instead of one address per tap left of the centre
 		int first = Dy * rw - H;
 		asm("" : "+s"(first));
 		int* const p = cell + first;
-		int want[N], old[N];
 #pragma unroll
 		for (int j = 0; j < N; j++) {
-			want[j] = static_cast<int>(fmaf(amount, shapedWeight(R, (j - H) * (j - H) + Dy * Dy), 0.5f));
-			old[j] = atomicSub(p + j, want[j]);
+			want[J + j] = static_cast<int>(fmaf(amount, shapedWeight(R, (j - H) * (j - H) + Dy * Dy), 0.5f));
+			old[J + j] = atomicSub(p + j, want[J + j]);
 		}
-		int sum = 0;
-		bool refund = false;
+		if constexpr (Dy < R - 1) {
+			shapedIssue<R, Dy + 1, J + N>(cell, rw, amount, want, old);
+		}
+	}
+
+	// Gives back want - old to every tap of rows Dy .. R-1 with old < want; returns what it gave.
+	template <int R, int Dy, int J>
+	__device__ inline int shapedRefund(int* const cell, const int rw, const int* const want, const int* const old) {
+		constexpr int H = brushRowHalf(R, Dy), N = 2 * H + 1;
+		int* const p = cell + (Dy * rw - H);
+		int back = 0;
 #pragma unroll
 		for (int j = 0; j < N; j++) {
-			sum += want[j];
-			refund |= old[j] < want[j];
-		}
-		if (refund) {
-#pragma unroll
-			for (int j = 0; j < N; j++) {
-				if (old[j] < want[j]) {
-					atomicAdd(p + j, want[j] - old[j]);
-					sum -= want[j] - old[j];
-				}
+			if (old[J + j] < want[J + j]) {
+				atomicAdd(p + j, want[J + j] - old[J + j]);
+				back += want[J + j] - old[J + j];
 			}
 		}
 		if constexpr (Dy < R - 1) {
-			sum += shapedTake<R, Dy + 1>(cell, rw, amount);
+			back += shapedRefund<R, Dy + 1, J + N>(cell, rw, want, old);
+		}
+		return back;
+	}
+
+	// The whole brush with every tap's atomic in flight at once (one wait per step; settling one
+	// row at a time took 2% longer). Returns the sediment taken. min(old) >= max(want) proves that
+	// no tap needs its excess back, so the per-tap refund test runs only when it fails (a cell
+	// lower than the amount the brush takes from it: rare).
+	template <int R>
+	__device__ inline int shapedTakeAll(int* const cell, const int rw, const float amount) {
+		constexpr int T = shapedTaps(R);
+		int want[T], old[T];
+		shapedIssue<R, 1 - R, 0>(cell, rw, amount, want, old);
+		int sum = 0, low = old[0], high = want[0];
+#pragma unroll
+		for (int j = 0; j < T; j++) {
+			sum += want[j];
+			low = min(low, old[j]);
+			high = max(high, want[j]);
+		}
+		if (low < high) {
+			sum -= shapedRefund<R, 1 - R, 0>(cell, rw, want, old);
 		}
 		return sum;
 	}
@@ -344,15 +387,20 @@ namespace SuperTerrainPlus::STPStage {
 			// cellInside keeps x, y >= r - 1, so the footprint test cannot wrap
 			return spawnInside(a, d) && cellInside(a, x, y) && this->footprint(a, x, y);
 		}
-		// Evaluated beside the kill test, whose result decides: (px, py) may then be anything, so
-		// it is clamped before the integer conversion (NaN -> 0); when the kill test passes it
-		// lies at least r inside the map and the clamp changes nothing.
-		__device__ bool stepGuard(const ErosionArgs& a, unsigned int, unsigned int, const float px, const float py) const {
-			const float x = fminf(fmaxf(px, 0.0f), 16777216.0f), y = fminf(fmaxf(py, 0.0f), 16777216.0f);
-			return this->footprint(a, static_cast<unsigned int>(x), static_cast<unsigned int>(y));
+		// The footprint test of the cell a step moves to, folded into the kill test's bounds: for a
+		// position p >= 0 and an integer k, floor(p) >= k <=> p >= k and floor(p) < k <=> p < k, so
+		// footprint(floor(px), floor(py)) within the kill bounds is px in [X0 + r - 1, X0 + RW - reach)
+		// and likewise for py. The bounds are integers below 2^24 (exact in FP32), and a NaN
+		// position fails every comparison.
+		__device__ float4 moveBounds(const ErosionArgs& a) const {
+			const unsigned int r = a.Rain.ErosionBrushRadius, lo = r - 1u, reach = r > 2u ? r - 1u : 1u;
+			const float4 k = killBounds(a);
+			return make_float4(fmaxf(k.x, static_cast<float>(this->X0 + lo)), fminf(k.y, static_cast<float>(this->X0 + this->RW - reach)),
+				fmaxf(k.z, static_cast<float>(this->Y0 + lo)), fminf(k.w, static_cast<float>(this->Y0 + this->RH - reach)));
 		}
+		__device__ bool stepGuard(const ErosionArgs&, unsigned int, unsigned int) const { return true; }
 		static constexpr unsigned int groupOf(const unsigned int taps) { return taps; }
-		// The 9/25/45-tap brushes go through shapedTake. Concurrent drops can see a cell's height
+		// The 9/25/45-tap brushes go through shapedTakeAll. Concurrent drops can see a cell's height
 		// transiently lowered by a refunded excess, a race of the same kind as the reference's
 		// read-then-update. `amount` is held below 8 (the window's height range) so the fixed-point
 		// tap sum stays below 2^31.
@@ -363,7 +411,7 @@ namespace SuperTerrainPlus::STPStage {
 			if constexpr (R == 0) {
 				return erodeGrouped<LocalSurface, Taps>(*this, i, amount, brush, count);
 			} else {
-				const int sum = shapedTake<R, 1 - R>(this->Lds + i, static_cast<int>(this->RW), fminf(amount, 7.5f));
+				const int sum = shapedTakeAll<R>(this->Lds + i, static_cast<int>(this->RW), fminf(amount, 7.5f));
 				return static_cast<float>(sum) * FixedStep;
 			}
 		}
@@ -406,17 +454,33 @@ namespace SuperTerrainPlus::STPStage {
 	// reference's expanded form by an ulp; parallel erosion is judged statistically, and the
 	// serialized kernel keeps the reference's expression (heightGradient).
 	// The heights are read unscaled and scaled once at the end (a power of two: exact).
+	//
+	// A position's cell: its corner (X, Y), the offsets in it and its surface index. A drop's
+	// position is >= 0 wherever it is read (spawned inside the map, or inside the kill bounds), so
+	// the unsigned conversion is the floor.
+	struct DropCell {
+		unsigned int X, Y, Index;
+		float Ox, Oy;
+	};
 	template <class Surface>
-	__device__ inline float3 surfaceGradient(const Surface& s, const float px, const float py) {
-		const unsigned int rx = static_cast<unsigned int>(px), ry = static_cast<unsigned int>(py);
-		const float cx = px - static_cast<float>(rx), cy = py - static_cast<float>(ry);
-		const unsigned int b = s.cell(rx, ry), w = s.stride();
+	__device__ inline DropCell dropCell(const Surface& s, const float px, const float py) {
+		DropCell c;
+		c.X = static_cast<unsigned int>(px);
+		c.Y = static_cast<unsigned int>(py);
+		c.Ox = px - static_cast<float>(c.X);
+		c.Oy = py - static_cast<float>(c.Y);
+		c.Index = s.cell(c.X, c.Y);
+		return c;
+	}
+	template <class Surface>
+	__device__ inline float3 surfaceGradient(const Surface& s, const DropCell& c) {
+		const unsigned int b = c.Index, w = s.stride();
 		const float hx = s.unscaled(b), hy = s.unscaled(b + 1u), hz = s.unscaled(b + w), hw = s.unscaled(b + w + 1u);
 		const float topSlope = hy - hx, bottomSlope = hw - hz;
-		const float top = fmaf(topSlope, cx, hx), bottom = fmaf(bottomSlope, cx, hz);
+		const float top = fmaf(topSlope, c.Ox, hx), bottom = fmaf(bottomSlope, c.Ox, hz);
 		float3 r;
-		r.x = fmaf(bottom - top, cy, top) * Surface::Unit;
-		r.y = fmaf(bottomSlope - topSlope, cy, topSlope) * Surface::Unit;
+		r.x = fmaf(bottom - top, c.Oy, top) * Surface::Unit;
+		r.y = fmaf(bottomSlope - topSlope, c.Oy, topSlope) * Surface::Unit;
 		r.z = (bottom - top) * Surface::Unit;
 		return r;
 	}
@@ -430,17 +494,17 @@ namespace SuperTerrainPlus::STPStage {
 		DropState d = spawn(a, drop);
 		float lost = 0.0f;
 		unsigned int steps = 0u, erodeSteps = 0u, depositSteps = 0u, killed = 0u;
-		const unsigned int r = a.Rain.ErosionBrushRadius;
 		if (!s.spawnGuard(a, d)) {
 			d.volume = 0.0f;
 			killed = 1u;
 		}
+		// the kill test's bounds, with the surface's own footprint test folded in where it has one
+		const float4 bounds = s.moveBounds(a);
+		// the drop's cell, carried from the step that moved the drop there
+		DropCell c = dropCell(s, d.px, d.py);
 		for (unsigned int life = 0u; alive(a, d, life); life++) {
 			steps++;
-			const unsigned int cellX = static_cast<unsigned int>(d.px), cellY = static_cast<unsigned int>(d.py);
-			const float ox = d.px - static_cast<float>(static_cast<int>(d.px)),
-				oy = d.py - static_cast<float>(static_cast<int>(d.py));
-			const float3 hg = surfaceGradient(s, d.px, d.py);
+			const float3 hg = surfaceGradient(s, c);
 			// STPRainDrop.cu:86-105 (direction, move, kill test)
 			const float inertia = a.Rain.Inertia;
 			d.dx = (-hg.y) * (1 - inertia) + d.dx * inertia;
@@ -455,20 +519,20 @@ namespace SuperTerrainPlus::STPStage {
 			}
 			d.px += d.dx;
 			d.py += d.dy;
-			const float radius = static_cast<float>(r);
 			// one branch: every term is evaluated (a NaN position fails the float tests)
-			const bool inMap = ((d.dx != 0.0f) | (d.dy != 0.0f)) & (d.px >= radius)
-				& (d.px < 1.0f * static_cast<float>(a.Total.x) - radius) & (d.py >= radius)
-				& (d.py < 1.0f * static_cast<float>(a.Total.y) - radius);
-			if (!(inMap & s.stepGuard(a, cellX, cellY, d.px, d.py))) {
+			const bool inMap = ((d.dx != 0.0f) | (d.dy != 0.0f)) & (d.px >= bounds.x) & (d.px < bounds.y)
+				& (d.py >= bounds.z) & (d.py < bounds.w);
+			if (!(inMap & s.stepGuard(a, c.X, c.Y))) {
 				lost = d.sediment;
 				d.volume = 0.0f;
 				d.sediment = 0.0f;
 				killed = 1u;
 				break;
 			}
-			const unsigned int mapIndex = s.cell(cellX, cellY), W = s.stride();
-			const float deltaHeight = surfaceGradient(s, d.px, d.py).x - hg.x;
+			const DropCell next = dropCell(s, d.px, d.py);
+			const unsigned int mapIndex = c.Index, W = s.stride();
+			const float ox = c.Ox, oy = c.Oy;
+			const float deltaHeight = surfaceGradient(s, next).x - hg.x;
 			const float cap = fmaxf(-deltaHeight * d.speed * d.volume * a.Rain.SedimentCapacityFactor,
 				a.Rain.minSedimentCapacity);
 			if (d.sediment > cap || deltaHeight > 0.0f) {
@@ -486,6 +550,7 @@ namespace SuperTerrainPlus::STPStage {
 				erodeSteps++;
 			}
 			advance<true>(a, d, deltaHeight);
+			c = next;
 		}
 		if (ledger) {
 			ledger[drop] = lost + d.sediment;

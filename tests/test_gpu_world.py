@@ -4,7 +4,7 @@ chunk, the whole merged map eroded and written back), centres with overlapping n
 different batches (SuperTerrain+/SuperTerrain+/Private/World/STPWorldPipeline.cpp:339-663).
 
 Without erosion every chunk is bit-exact against the oracle. With erosion each neighbourhood is
-judged, as every parallel erosion here, against the serialized FP32 oracle erosion of the same
+checked, as every parallel erosion here, against the serialized FP32 oracle erosion of the same
 merged map with the same drops (seed, stream word of the centre): RMS within 1.5 x the RMS between
 two serialized runs with different drops, sediment moved within 4/sqrt(drops) + 0.12 of the
 serialized run, UINT16 maps the format of the heights."""
@@ -83,7 +83,7 @@ def neighbourhood(cx, cy):
     return [(cx + dx, cy + dy) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
 
 
-def judge(oracle, before, after, centre, dpp):
+def check_region(oracle, before, after, centre, dpp):
     """The erosion of one merged neighbourhood against the serialized oracle (see the module doc)."""
     setting = W.erosion_setting(1)
     drops = int(MAP * MAP * dpp)
@@ -130,7 +130,7 @@ def test_world_overlapping_centres_in_order(gpu, oracle):
         [np.concatenate([oracle_chunk(oracle, x, y) for x in (-1, 0, 1)], axis=1) for y in (-1, 0, 1)], axis=0))
     assert sorted(w.request([(0, 0)])) == sorted(neighbourhood(0, 0))
     after = w.merged(0, 0)
-    judge(oracle, before, after, (0, 0), dpp)
+    check_region(oracle, before, after, (0, 0), dpp)
     for x, y in neighbourhood(0, 0):
         assert (w.read(x, y)[1] == oracle.format_heightmap(w.read(x, y)[0])).all()
     # (1, 0): its column x = 2 is new, the rest is what the first erosion left
@@ -140,13 +140,13 @@ def test_world_overlapping_centres_in_order(gpu, oracle):
                                    axis=1))
     before = np.ascontiguousarray(np.concatenate(rows, axis=0))
     assert sorted(w.request([(1, 0)])) == sorted(neighbourhood(1, 0))
-    judge(oracle, before, w.merged(1, 0), (1, 0), dpp)
+    check_region(oracle, before, w.merged(1, 0), (1, 0), dpp)
     assert w.state(0, 0) == 2 and w.state(1, 0) == 2 and w.state(2, 0) == 1
 
 
 def test_world_disjoint_centres_share_one_erosion(gpu, oracle):
     """Three centres 3 chunks apart have disjoint neighbourhoods: one batch erodes them together;
-    each neighbourhood is judged on its own."""
+    each neighbourhood is checked on its own."""
     log("world disjoint centres")
     dpp = 0.5
     w = World(dpp)
@@ -157,7 +157,7 @@ def test_world_disjoint_centres_share_one_erosion(gpu, oracle):
     changed = w.request(centres)
     assert len(changed) == 27
     for c in centres:
-        judge(oracle, before[c], w.merged(*c), c, dpp)
+        check_region(oracle, before[c], w.merged(*c), c, dpp)
 
 
 def test_world_block_of_centres(gpu):
