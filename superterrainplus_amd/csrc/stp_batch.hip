@@ -1428,7 +1428,11 @@ STPHeightfieldBatch::STPHeightfieldBatch(const STPEnvironment::STPSimplexNoiseSe
 	const STPBatchSetting& setting, const STPBatchCapacity& capacity) :
 	Setting(setting), Capacity(capacity), Permutation(STPAlgorithm::STPPermutationGenerator::generate(noise)),
 	Biome(packBiomeTable(biome.data(), static_cast<unsigned int>(biome.size()))),
-	BiomeCount(static_cast<unsigned int>(biome.size())) {
+	BiomeCount(static_cast<unsigned int>(biome.size())),
+	WindowHeights(std::all_of(biome.begin(), biome.end(), [](const STPBiomeProperty& b) {
+		// a biome's heights are fractal * Variation + Depth with fractal in [0, 1]
+		return b.Depth >= 0.0f && b.Variation >= 0.0f && b.Depth + b.Variation <= STPStage::WindowHeightLimit;
+	})) {
 	this->init(rain);
 }
 
@@ -1737,7 +1741,7 @@ ErosionArgs STPHeightfieldBatch::erosionArgs() const {
 
 unsigned int STPHeightfieldBatch::erosionLdsBytes(const unsigned int tile) const {
 	const unsigned int r = this->Rain.ErosionBrushRadius, life = this->Rain.MaxLifetime;
-	if (tile == 0u || life == 0u || this->BrushSize > MaxTiledTaps) {
+	if (tile == 0u || life == 0u || this->BrushSize > MaxTiledTaps || !this->WindowHeights) {
 		return 0u;
 	}
 	const unsigned long long halo = static_cast<unsigned long long>(life) + std::max(r, 2u) + 1u;
@@ -1780,7 +1784,8 @@ void STPHeightfieldBatch::setTuning(const STPBatchTuning& tuning) {
 	if (tuning.ErosionTile != 0u
 		&& (tuning.ErosionTile < 16u || tuning.ErosionTile > 4096u || this->erosionLdsBytes(tuning.ErosionTile) == 0u)) {
 		STP_THROW(STPInvalidArgument,
-			"tiled erosion needs a tile side in [16, 4096], MaxLifetime > 0 and a window that fits in LDS");
+			"tiled erosion needs a tile side in [16, 4096], MaxLifetime > 0, a window that fits in LDS and biome "
+			"heights (Depth to Depth + Variation) within [0, 4]");
 	}
 	this->releaseGraphs();
 	this->Tuning = tuning;

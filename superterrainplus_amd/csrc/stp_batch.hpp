@@ -153,6 +153,8 @@ namespace SuperTerrainPlus {
 		STPAlgorithm::STPPermutationGenerator::STPPermutationTable Permutation;
 		STPStage::BiomeTable Biome;
 		unsigned int BiomeCount;
+		// every biome's heights lie in [0, WindowHeightLimit], so the LDS-tiled erosion may run
+		bool WindowHeights;
 		STPRainDropParameter Rain {};
 		unsigned int VCap = 0u, BrushSize = 0u;
 		std::vector<STPBatch::Chunk> Record;

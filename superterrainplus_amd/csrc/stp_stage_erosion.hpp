@@ -201,6 +201,12 @@ namespace SuperTerrainPlus::STPStage {
 	// leave the window.
 	constexpr int FixedBits = 28; // heights in [-8, 8), resolution 2^-28
 	constexpr float FixedScale = 268435456.0f, FixedStep = 1.0f / 268435456.0f;
+	// The heights a window may start from: [0, 4]. From heights >= 0 a take leaves a cell at
+	// old - min(old, want) >= 0 (only a concurrent refund can make `old` transiently negative, by
+	// at most one tap's amount), and 4 leaves as much again for what the drops pile up. Below 0
+	// the reference's update takes negative amounts and feeds them back as negative deposits
+	// without bound: FP32 maps follow that, a fixed-point window would wrap.
+	constexpr float WindowHeightLimit = 4.0f;
 
 	__device__ inline int toFixedPoint(const float h) { return __float2int_rn(h * FixedScale); }
 	__device__ inline float fromFixedPoint(const int q) { return static_cast<float>(q) * FixedStep; }
@@ -282,10 +288,10 @@ namespace SuperTerrainPlus::STPStage {
 	// at old - min(old, want), the reference's erode update (STPRainDrop.cu:150-155). Within a row
 	// the taps are consecutive words (one address, immediate offsets).
 	//
-	// shapedIssue: rows Dy .. R-1, every tap's atomic issued, want and old kept at index J + j for
-	// the row's j-th tap.
+	// shapedIssue: rows Dy .. R-1, every tap's atomic issued, -want (the atomic's addend) and old
+	// kept at index J + j for the row's j-th tap.
 	template <int R, int Dy, int J>
-	__device__ inline void shapedIssue(int* const cell, const int rw, const float amount, int* const want, int* const old) {
+	__device__ inline void shapedIssue(int* const cell, const int rw, const float amount, int* const take, int* const old) {
 		constexpr int H = brushRowHalf(R, Dy), N = 2 * H + 1;
 		// the row's first tap; opaque to the optimiser, so that it keeps one address per row with
 		// non-negative immediate offsets instead of one address per tap left of the centre
@@ -294,29 +300,31 @@ namespace SuperTerrainPlus::STPStage {
 		int* const p = cell + first;
 #pragma unroll
 		for (int j = 0; j < N; j++) {
-			want[J + j] = static_cast<int>(fmaf(amount, shapedWeight(R, (j - H) * (j - H) + Dy * Dy), 0.5f));
-			old[J + j] = atomicSub(p + j, want[J + j]);
+			// -want, rounded the same way: fma(a, -w, -0.5) = -fma(a, w, 0.5) and truncation is odd
+			take[J + j] = static_cast<int>(fmaf(amount, -shapedWeight(R, (j - H) * (j - H) + Dy * Dy), -0.5f));
+			old[J + j] = atomicAdd(p + j, take[J + j]);
 		}
 		if constexpr (Dy < R - 1) {
-			shapedIssue<R, Dy + 1, J + N>(cell, rw, amount, want, old);
+			shapedIssue<R, Dy + 1, J + N>(cell, rw, amount, take, old);
 		}
 	}
 
 	// Gives back want - old to every tap of rows Dy .. R-1 with old < want; returns what it gave.
 	template <int R, int Dy, int J>
-	__device__ inline int shapedRefund(int* const cell, const int rw, const int* const want, const int* const old) {
+	__device__ inline int shapedRefund(int* const cell, const int rw, const int* const take, const int* const old) {
 		constexpr int H = brushRowHalf(R, Dy), N = 2 * H + 1;
 		int* const p = cell + (Dy * rw - H);
 		int back = 0;
 #pragma unroll
 		for (int j = 0; j < N; j++) {
-			if (old[J + j] < want[J + j]) {
-				atomicAdd(p + j, want[J + j] - old[J + j]);
-				back += want[J + j] - old[J + j];
+			const int want = -take[J + j];
+			if (old[J + j] < want) {
+				atomicAdd(p + j, want - old[J + j]);
+				back += want - old[J + j];
 			}
 		}
 		if constexpr (Dy < R - 1) {
-			back += shapedRefund<R, Dy + 1, J + N>(cell, rw, want, old);
+			back += shapedRefund<R, Dy + 1, J + N>(cell, rw, take, old);
 		}
 		return back;
 	}
@@ -328,17 +336,19 @@ namespace SuperTerrainPlus::STPStage {
 	template <int R>
 	__device__ inline int shapedTakeAll(int* const cell, const int rw, const float amount) {
 		constexpr int T = shapedTaps(R);
-		int want[T], old[T];
-		shapedIssue<R, 1 - R, 0>(cell, rw, amount, want, old);
-		int sum = 0, low = old[0], high = want[0];
+		int take[T], old[T];
+		shapedIssue<R, 1 - R, 0>(cell, rw, amount, take, old);
+		// sum = -sum(take), max(want) = -min(take)
+		int sum = 0, low = old[0], least = take[0];
 #pragma unroll
 		for (int j = 0; j < T; j++) {
-			sum += want[j];
+			sum += take[j];
 			low = min(low, old[j]);
-			high = max(high, want[j]);
+			least = min(least, take[j]);
 		}
-		if (low < high) {
-			sum -= shapedRefund<R, 1 - R, 0>(cell, rw, want, old);
+		sum = -sum;
+		if (low < -least) {
+			sum -= shapedRefund<R, 1 - R, 0>(cell, rw, take, old);
 		}
 		return sum;
 	}
@@ -453,7 +463,9 @@ namespace SuperTerrainPlus::STPStage {
 	// (hz - hx)(1 - cx) + (hw - hy) cx is exactly bottom - top. Its roundings differ from the
 	// reference's expanded form by an ulp; parallel erosion is judged statistically, and the
 	// serialized kernel keeps the reference's expression (heightGradient).
-	// The heights are read unscaled and scaled once at the end (a power of two: exact).
+	// The heights are read unscaled and the results stay unscaled: the caller applies the surface's
+	// Unit, a power of two, where it multiplies or subtracts anyway (exact: scaling by a power of
+	// two commutes with rounding, and no value here is small enough to lose bits to it).
 	//
 	// A position's cell: its corner (X, Y), the offsets in it and its surface index. A drop's
 	// position is >= 0 wherever it is read (spawned inside the map, or inside the kill bounds), so
@@ -479,9 +491,9 @@ namespace SuperTerrainPlus::STPStage {
 		const float topSlope = hy - hx, bottomSlope = hw - hz;
 		const float top = fmaf(topSlope, c.Ox, hx), bottom = fmaf(bottomSlope, c.Ox, hz);
 		float3 r;
-		r.x = fmaf(bottom - top, c.Oy, top) * Surface::Unit;
-		r.y = fmaf(bottomSlope - topSlope, c.Oy, topSlope) * Surface::Unit;
-		r.z = (bottom - top) * Surface::Unit;
+		r.x = fmaf(bottom - top, c.Oy, top);
+		r.y = fmaf(bottomSlope - topSlope, c.Oy, topSlope);
+		r.z = bottom - top;
 		return r;
 	}
 
@@ -502,13 +514,14 @@ namespace SuperTerrainPlus::STPStage {
 		const float4 bounds = s.moveBounds(a);
 		// the drop's cell, carried from the step that moved the drop there
 		DropCell c = dropCell(s, d.px, d.py);
+		const float inertia = a.Rain.Inertia, gradientScale = (1 - inertia) * Surface::Unit;
 		for (unsigned int life = 0u; alive(a, d, life); life++) {
 			steps++;
+			// unscaled: (-g Unit)(1 - inertia) = (-g)((1 - inertia) Unit) exactly
 			const float3 hg = surfaceGradient(s, c);
 			// STPRainDrop.cu:86-105 (direction, move, kill test)
-			const float inertia = a.Rain.Inertia;
-			d.dx = (-hg.y) * (1 - inertia) + d.dx * inertia;
-			d.dy = (-hg.z) * (1 - inertia) + d.dy * inertia;
+			d.dx = (-hg.y) * gradientScale + d.dx * inertia;
+			d.dy = (-hg.z) * gradientScale + d.dy * inertia;
 			// parallel mode normalises with the hardware reciprocal square root (1 ulp): its result
 			// is judged statistically; the serialized kernel keeps the reference's division
 			const float length2 = d.dx * d.dx + d.dy * d.dy;
@@ -532,7 +545,7 @@ namespace SuperTerrainPlus::STPStage {
 			const DropCell next = dropCell(s, d.px, d.py);
 			const unsigned int mapIndex = c.Index, W = s.stride();
 			const float ox = c.Ox, oy = c.Oy;
-			const float deltaHeight = surfaceGradient(s, next).x - hg.x;
+			const float deltaHeight = (surfaceGradient(s, next).x - hg.x) * Surface::Unit;
 			const float cap = fmaxf(-deltaHeight * d.speed * d.volume * a.Rain.SedimentCapacityFactor,
 				a.Rain.minSedimentCapacity);
 			if (d.sediment > cap || deltaHeight > 0.0f) {

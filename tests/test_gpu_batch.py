@@ -22,13 +22,13 @@ def props(count=COUNT):
 
 
 class Batch:
-    def __init__(self, dpp, sides=SIDES, origins=ORIGINS, radius=RADIUS, cap=8, count=COUNT):
+    def __init__(self, dpp, sides=SIDES, origins=ORIGINS, radius=RADIUS, cap=8, count=COUNT, biomes=None):
         self.sides, self.origins, self.radius = sides, origins, radius
         setting = capi.BatchSetting(SEED, count, CELL, radius, cap, dpp)
         chunks = (capi.BatchChunk * len(sides))(*[capi.BatchChunk(s, (C.c_int32 * 2)(*o))
                                                    for s, o in zip(sides, origins)])
         h = C.c_void_p()
-        capi.call("stp_batch_create", C.byref(NOISE), capi.biome_array(props(count)), count,
+        capi.call("stp_batch_create", C.byref(NOISE), capi.biome_array(biomes or props(count)), count,
                   C.byref(W.erosion_setting(1)), C.byref(setting), chunks, len(sides), C.byref(h))
         self.h = h.value
 
@@ -209,6 +209,46 @@ def test_batch_erosion_against_serialized_oracle(gpu, oracle, tuning):
         floor_hist = np.abs(height_histogram(ser) - height_histogram(other)).sum()
         assert np.abs(height_histogram(after) - height_histogram(ser)).sum() <= 2 * floor_hist + 0.005 * after.size
         assert (fixed == oracle.format_heightmap(after)).all()
+
+
+def test_batch_erosion_refunds_balance(gpu):
+    """Steep biomes (variation 0.9) beside nearly flat ones at depth 0 (variation 1e-4): drops
+    running off a slope erode with brush taps on flat cells lower than what the taps want, which
+    take the whole cell (the reference's min(height, amount * weight)) through the LDS window's
+    take-and-refund path and leave it at exactly 0. Such cells must exist, the heights stay at or
+    above 0 (up to one tap's amount, from a refund in flight), and the drops' ledger balances the
+    heights to 2e-4 relative."""
+    log("batch erosion refunds on flat cells")
+    biomes = [(60.0, 6, 0.5, 2.0, 0.0, 0.9), (60.0, 6, 0.5, 2.0, 0.0, 1e-4)] * (COUNT // 2)
+    dpp = 1.0
+    pre = Batch(0.0, biomes=biomes)
+    pre.run(0)
+    b = Batch(dpp, biomes=biomes)
+    assert b.tuning()[0] > 0
+    b.run(0)
+    zeroed = 0
+    for i, s in enumerate(SIDES):
+        before = pre.read(i)[0]
+        after = b.read(i)[0]
+        assert before.min() >= 0.0
+        moved = float(b.ledger(i, int(s * s * dpp)).astype(np.float64).sum())
+        assert abs((after.astype(np.float64).sum() - before.astype(np.float64).sum()) + moved) <= 2e-4 * moved + 1e-6, i
+        assert after.min() >= -1e-3, i
+        zeroed += int(((before > 0) & (after == 0)).sum())
+    log(f"cells taken to 0: {zeroed}")
+    assert zeroed > 0
+
+
+def test_batch_negative_heights_use_global_erosion(gpu):
+    """A biome reaching below 0 (depth -0.4): the reference's update can drive such terrain
+    without bound, which an FP32 map follows and the fixed-point LDS window cannot hold, so the
+    batch erodes it with the global-atomic kernel and refuses a tiled tuning."""
+    biomes = [(60.0, 6, 0.5, 2.0, -0.4, 0.9)] * COUNT
+    b = Batch(0.25, biomes=biomes)
+    assert b.tuning()[0] == 0
+    with pytest.raises(RuntimeError):
+        b.tuning(72, 256)
+    b.run(0)
 
 
 def test_batch_plain_matches_graph_statistically(gpu, oracle):
