@@ -37,7 +37,7 @@ STAGES = ["biome", "filter_vertical", "filter_horizontal", "blend", "erosion_bin
 BYTES_STEP, BYTES_ERODE, BYTES_DEPOSIT = 32, 25 * 12, 4 * 8
 # HBM bytes per launch of the tiled erosion kernel (tile 128, 1024 threads) on this workload, from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a kernel of known size (profiles/erosion_hbm_traffic.md)
-TILED_EROSION_TRAFFIC = 492.1e6
+TILED_EROSION_TRAFFIC = 493.0e6
 # serving loop: two engines of 8 Mpx / 512 chunks each
 SERVER_BATCH_PIXELS, SERVER_BATCH_CHUNKS, SERVER_ENGINES = 8 << 20, 512, 2
 
