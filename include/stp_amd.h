@@ -367,9 +367,12 @@ int stp_batch_counters(stp_heightfield_batch* batch, uint64_t h_out[6], void* st
 /* Launch shape of the erosion stage. ErosionTile > 0: LDS-tiled erosion -- the drops that spawn
  * in one ErosionTile x ErosionTile tile run on an LDS copy of the tile plus a halo of
  * H = MaxLifetime + max(r, 2) + 1 cells, and the tiles run in four colour batches (tile
- * coordinate parities) whose windows are disjoint. Needs MaxLifetime > 0, ErosionTile >= 2 H and
- * a window that fits in LDS. 0: every drop works on the heightmap in device memory with global
- * atomics. ErosionThreads: threads of one tile workgroup, 256, 512 or 1024. */
+ * coordinate parities) whose windows are disjoint. Needs MaxLifetime > 0, ErosionTile >= 2 H, a
+ * window that fits in LDS, and every biome's heights (Depth to Depth + Variation) within [0, 4]:
+ * the window holds heights in 32-bit fixed point, and below 0 the erode update can drive them
+ * without bound. A batch outside that range erodes with 0. 0: every drop works on the heightmap in
+ * device memory with global atomics. ErosionThreads: threads of one tile workgroup, 256, 512 or
+ * 1024. */
 typedef struct {
 	uint32_t ErosionTile;
 	uint32_t ErosionThreads;
