@@ -35,6 +35,13 @@ STAGES = ["biome", "filter_vertical", "filter_horizontal", "blend", "erosion_bin
 # algorithmic bytes of one drop step (r = 3 brush, 25 taps): two 2x2 bilinear height reads;
 # an erode step adds 25 x (4 B read + 8 B atomic read-modify-write); a deposit step 4 x 8 B.
 BYTES_STEP, BYTES_ERODE, BYTES_DEPOSIT = 32, 25 * 12, 4 * 8
+# LDS integer atomics of the tiled erosion: 25 per erode step (one per brush tap), 4 per deposit step
+ATOMICS_ERODE, ATOMICS_DEPOSIT = 25, 4
+# algorithmic FLOP of the blend (STPSimplexNoise.cu:5-110 as written, FMA = 2, division = 1): 53 per
+# simplex2D sample + 13 per octave of the fractal loop, and per histogram bin 3 more (the weight
+# product and the sum); one bin of a pixel costs octave * 66 + 3
+FLOP_PER_OCTAVE, FLOP_PER_BIN = 66, 3
+FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch of the tiled erosion kernel (tile 128, 1024 threads) on this workload, from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a kernel of known size (profiles/erosion_hbm_traffic.md)
 TILED_EROSION_TRAFFIC = 493.0e6
@@ -443,6 +450,9 @@ def main():
     counters = (C.c_uint64 * 6)()
     capi.call("stp_batch_counters", batch, counters, stream)
     pixels, drops, steps, erode, deposit, killed = [int(v) for v in counters]
+    bins = C.c_uint64()
+    capi.call("stp_batch_bin_total", batch, C.byref(bins), stream)
+    bins = int(bins.value)
     ms_per_step = wall / args.steps * 1e3
     batched_mpx = pixels / (wall / args.steps) / 1e6
     print(f"[bench] batched {batched_mpx:.1f} Mpixel/s ({ms_per_step:.2f} ms/step)", file=sys.stderr, flush=True)
@@ -469,6 +479,11 @@ def main():
     launches = 4 if tuning.ErosionTile else 1
     erosion_s = stage_ms[STAGES.index("erosion")] / 1e3
     achieved = erosion_bytes / erosion_s / 1e9 if erosion_s > 0 else 0.0
+    # every biome of the workload has cfg.fractal.octave octaves
+    blend_flop = bins * (cfg.fractal.octave * FLOP_PER_OCTAVE + FLOP_PER_BIN)
+    blend_s = stage_ms[STAGES.index("blend")] / 1e3
+    blend_tflops = blend_flop / blend_s / 1e12 if blend_s > 0 else 0.0
+    atomics = erode * ATOMICS_ERODE + deposit * ATOMICS_DEPOSIT
     result = {
         "metric": METRIC, "value": round(batched_mpx, 2), "unit": "Mpixel/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
@@ -487,6 +502,14 @@ def main():
             "erosion_tuning": {"tile": tuning.ErosionTile, "threads": tuning.ErosionThreads,
                                "autotuned": not args.no_autotune},
             "erosion_counters": {"steps": steps, "erode_steps": erode, "deposit_steps": deposit, "killed": killed},
+            "erosion_atomics": {"per_step": atomics, "per_s": round(atomics / erosion_s, 1) if erosion_s > 0 else None,
+                                "note": "LDS integer atomics (25 per erode step, 4 per deposit step) over the "
+                                        "erosion stage's event time"},
+            "blend_roofline": {"bound": "fp32 valu", "achieved": round(blend_tflops, 2), "peak": FP32_PEAK_TFLOPS,
+                               "unit": "TFLOP/s", "frac": round(blend_tflops / FP32_PEAK_TFLOPS, 4),
+                               "bins": bins, "flop_per_step": blend_flop,
+                               "note": "algorithmic FLOP = bins x (octaves x 66 + 3); the peak counts packed "
+                                       "FP32, which this build does not emit"},
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TILED_EROSION_TRAFFIC if (tuning.ErosionTile, tuning.ErosionThreads) == (128, 1024) else None,

@@ -364,6 +364,10 @@ int stp_batch_chunk_latencies(stp_heightfield_batch* batch, double* h_ms, uint32
 /* h_out = {pixels, drops, drop steps, erode steps, deposit steps, killed drops} of the last run
  * (synchronises `stream`); fails with STP_ERROR_CAPACITY if a histogram overflowed. */
 int stp_batch_counters(stp_heightfield_batch* batch, uint64_t h_out[6], void* stream);
+/* No reference counterpart (a measurement aid): *h_out = the sum over every pixel of the bound
+ * chunks of its histogram bin count in the last run, i.e. the (pixel, biome) noise evaluations of
+ * the blend (synchronises `stream`; STP_ERROR_CAPACITY if a histogram overflowed). */
+int stp_batch_bin_total(stp_heightfield_batch* batch, uint64_t* h_out, void* stream);
 /* Launch shape of the erosion stage. ErosionTile > 0: LDS-tiled erosion -- the drops that spawn
  * in one ErosionTile x ErosionTile tile run on an LDS copy of the tile plus a halo of
  * H = MaxLifetime + max(r, 2) + 1 cells, and the tiles run in four colour batches (tile

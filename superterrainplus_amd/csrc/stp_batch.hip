@@ -1387,8 +1387,26 @@ namespace SuperTerrainPlus::STPBatch {
 	}
 
 	__global__ void zero64KERNEL(unsigned long long* const p) {
-		if (threadIdx.x < 4u) {
+		if (threadIdx.x < 5u) {
 			p[threadIdx.x] = 0ull;
+		}
+	}
+
+	// Sum of the per-pixel bin counts (the histogram bins the blend evaluates) of chunk blockIdx.y's
+	// Side^2 pixels (not the padding after them).
+	__global__ void __launch_bounds__(256) binTotalKERNEL(const Chunk* const chunks, const unsigned char* const count,
+		unsigned long long* const out) {
+		const Chunk c = chunks[blockIdx.y];
+		const unsigned long long n = static_cast<unsigned long long>(c.Side) * c.Side;
+		unsigned long long s = 0ull;
+		for (unsigned long long i = blockIdx.x * 256ull + threadIdx.x; i < n; i += 256ull * gridDim.x) {
+			s += count[c.Pixel + i];
+		}
+		for (int o = 32; o > 0; o >>= 1) {
+			s += __shfl_down(s, o, 64);
+		}
+		if ((threadIdx.x & 63u) == 0u) {
+			atomicAdd(out, s);
 		}
 	}
 
@@ -1517,7 +1535,8 @@ void STPHeightfieldBatch::init(const STPEnvironment::STPRainDropSetting& rain) {
 	this->Ledger = STPSmartDeviceMemory::makeDevice<float>(drops);
 	this->DropInfo = STPSmartDeviceMemory::makeDevice<uint2>(drops);
 	this->Flag = STPSmartDeviceMemory::makeDevice<unsigned int>(4u);
-	this->Counter = STPSmartDeviceMemory::makeDevice<unsigned long long>(4u);
+	// 0..3 the drop counters, 4 the bin total
+	this->Counter = STPSmartDeviceMemory::makeDevice<unsigned long long>(5u);
 	STP_CHECK_HIP(hipMemset(this->Fixed.get(), 0, pixel * sizeof(unsigned short)));
 	STP_CHECK_HIP(hipMemset(this->Height.get(), 0, pixel * sizeof(float)));
 	{
@@ -2303,6 +2322,19 @@ std::array<unsigned long long, 4> STPHeightfieldBatch::counters(const hipStream_
 	STP_CHECK_HIP(hipGetLastError());
 	std::array<unsigned long long, 4> out {};
 	STP_CHECK_HIP(hipMemcpyAsync(out.data(), this->Counter.get(), sizeof(out), hipMemcpyDeviceToHost, stream));
+	STP_CHECK_HIP(hipStreamSynchronize(stream));
+	return out;
+}
+
+unsigned long long STPHeightfieldBatch::binTotal(const hipStream_t stream) {
+	zero64KERNEL<<<1, 64, 0, stream>>>(this->Counter.get());
+	if (!this->Record.empty()) {
+		binTotalKERNEL<<<dim3(16u, this->chunkCount()), 256, 0, stream>>>(this->ChunkRecord, this->BinCount.get(),
+			this->Counter.get() + 4);
+	}
+	STP_CHECK_HIP(hipGetLastError());
+	unsigned long long out = 0ull;
+	STP_CHECK_HIP(hipMemcpyAsync(&out, this->Counter.get() + 4, sizeof(out), hipMemcpyDeviceToHost, stream));
 	STP_CHECK_HIP(hipStreamSynchronize(stream));
 	return out;
 }

@@ -118,6 +118,8 @@ namespace SuperTerrainPlus {
 		const std::vector<double>& latency() const noexcept { return this->Latency; }
 		// {steps, erode steps, deposit steps, killed drops} of the last run (synchronises).
 		std::array<unsigned long long, 4> counters(hipStream_t stream);
+		// Sum over the bound chunks' pixels of the last run's histogram bin counts (synchronises)
+		unsigned long long binTotal(hipStream_t stream);
 		// Throws STPCapacityError if a histogram overflowed its bin capacity (synchronises).
 		void check(hipStream_t stream);
 		void readChunk(unsigned int i, float* height, unsigned short* fixed, unsigned short* halo, float* ledger) const;

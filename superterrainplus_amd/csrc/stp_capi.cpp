@@ -834,6 +834,14 @@ int stp_batch_counters(stp_heightfield_batch* const batch, uint64_t h_out[6], vo
 	});
 }
 
+int stp_batch_bin_total(stp_heightfield_batch* const batch, uint64_t* const h_out, void* const stream) {
+	return guarded([&] {
+		require(batch && h_out, "null argument");
+		batch->Batch->check(toStream(stream));
+		*h_out = batch->Batch->binTotal(toStream(stream));
+	});
+}
+
 int stp_batch_set_tuning(stp_heightfield_batch* const batch, const stp_batch_tuning* const tuning) {
 	return guarded([&] {
 		require(batch && tuning, "null argument");

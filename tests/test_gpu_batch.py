@@ -251,6 +251,24 @@ def test_batch_negative_heights_use_global_erosion(gpu):
     b.run(0)
 
 
+def test_batch_bin_total_matches_oracle(gpu, oracle):
+    """stp_batch_bin_total (a measurement aid: the blend's (pixel, biome) noise evaluations) equals
+    the oracle histograms' bin count summed over every chunk's pixels; an odd side leaves padding
+    pixels after its chunk, which must not count."""
+    sides, origins = [61, 96, 64], [(7, -3), (100, 200), (-640, 320)]
+    b = Batch(0.0, sides=sides, origins=origins)
+    b.run(0)
+    total = C.c_uint64()
+    capi.call("stp_batch_bin_total", C.c_void_p(b.h), C.byref(total), None)
+    expect = 0
+    for s, o in zip(sides, origins):
+        r = RADIUS
+        halo = oracle.voronoi_map(SEED, COUNT, CELL, (o[0] - r, o[1] - r), s + 2 * r, s + 2 * r)
+        _, _, off = oracle.shf_window(halo, (r, r), (s, s), r)
+        expect += int(np.asarray(off)[-1])
+    assert total.value == expect
+
+
 def test_batch_plain_matches_graph_statistically(gpu, oracle):
     """The plain per-chunk run and the graph run erode with the same drops; only the order of
     concurrent atomic updates differs. Yardstick as above: two serialized runs with different
