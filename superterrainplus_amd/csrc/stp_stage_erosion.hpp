@@ -283,7 +283,7 @@ namespace SuperTerrainPlus::STPStage {
 	}
 
 	// The erode update of the brush of radius R around `cell` (LDS, row stride rw), in fixed point.
-	// Per tap: want = round(amount * w) and one returning atomic subtraction; the old height it
+	// Per tap: want = round(amount * w) and one returning atomic add of -want; the old height it
 	// returns gives min(old, want), and a tap with old < want gets the excess back, so the cell ends
 	// at old - min(old, want), the reference's erode update (STPRainDrop.cu:150-155). Within a row
 	// the taps are consecutive words (one address, immediate offsets).
