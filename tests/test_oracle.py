@@ -13,6 +13,11 @@ What pins each oracle function (beyond restating it):
   single histogram       reference KAT (SuperTest+/SuperAlgorithm+/STPTestHistogram.cpp:44-104:
                          items AND order at pixels 0/8/15), reference domain errors, brute-force
                          counts, and an independent derivation of the bin order (chain starts).
+  multi-biome blend      identities against the fractal map (pinned above): one bin of weight 1
+                         gives fractal * Variation + Depth; Variation 0 gives the bin-order sum
+                         of weight * Depth; any histogram gives the bin-order FP32 sum of
+                         weight * per-biome map; a coarse level of detail equals the fine map
+                         at the same world samples; FP32 vs FP64 <= 1e-5.
   free-slip mapping      reference KAT (SuperTest+/SuperTerrain+/STPTestNearestNeighbour.cpp:50-91).
   chunk math             reference KAT (SuperTest+/SuperTerrain+/STPTest2DChunk.cpp:30-89).
   format                 closed form with numpy's IEEE round-half-even.
@@ -403,3 +408,135 @@ def test_voronoi_golden_and_nearest_site(oracle):
     for x, y in rng.integers(0, mb.side, (300, 2)):
         d = (s[:, 0] - x) ** 2 + (s[:, 1] - y) ** 2
         assert vm[y, x] == s[np.argmin(d), 2] or np.sum(d == d.min()) > 1
+
+
+# ------------------------------------------------------------------ multi-biome blend
+# SuperDemo+/Script/STPMultiHeightGenerator.cu:35-71: a pixel's height is the sum, in bin
+# order and in FP32, of weight * (fractal(x, y; biome) * Variation + Depth), the fractal sampled
+# with Offset = the chunk's offset and HalfDimension = Dimension / 2 per axis
+# (SuperDemo+/World/Biomes/STPCommonCompiler.cpp:236). The identities below check orc_multi_biome
+# against the fractal map (pinned by its own tests above), so that each plausible slip fails one
+# of them: the offset or half dimension of the wrong axis (non-square maps), bins summed out of
+# order or at another precision, Depth dropped or scaled by Variation, a biome's parameters read
+# from the wrong table entry, the level-of-detail step applied to the wrong term.
+
+# (Scale, Octave, Persistence, Lacunarity, Depth, Variation): every field differs between biomes
+BLEND_BIOMES = [(173.5, 4, 0.5, 2.0, 0.125, 0.75), (311.25, 7, 0.45, 2.1, 0.3, 0.5),
+                (97.75, 6, 0.55, 1.9, 0.05, 0.875), (402.0, 8, 0.5, 2.0, 0.6, 0.25),
+                (230.0, 5, 0.6, 2.2, 0.0, 1.0)]
+
+
+def blend_histograms(rng, pixels, biomes, max_bins):
+    """Random jagged histograms: 1..max_bins bins per pixel, random items (repeats allowed, as the
+    blend does not care) and weights."""
+    counts = rng.integers(1, max_bins + 1, pixels)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+    item = rng.integers(0, biomes, int(off[-1])).astype(np.uint16)
+    weight = rng.uniform(0.01, 1.0, int(off[-1])).astype(np.float32)
+    return item, weight, off, counts
+
+
+def biome_fractal(oracle, perm, grad, bp, mx, my, offset, f64=False):
+    d = oracle.FractalDesc()
+    d.Scale, d.Octave, d.Persistence, d.Lacunarity = bp[0], bp[1], bp[2], bp[3]
+    d.OffsetX, d.OffsetY = offset
+    d.HalfX, d.HalfY = mx / 2.0, my / 2.0  # STPCommonCompiler.cpp:236: vec2(Dimension) / 2
+    d.Amplitude = d.Frequency = 1.0
+    return oracle.fractal_map(perm, grad, d, mx, my, f64)
+
+
+def bin_order_sum(item, weight, off, counts, per_biome, dt):
+    """sum over slots k = 0, 1, ... of weight * per_biome[item] at precision dt, slot by slot
+    (the order of STPSingleHistogramWrapper::iterate)."""
+    flat = [np.asarray(m, dt).ravel() for m in per_biome]
+    h = np.zeros(counts.size, dt)
+    first = off[:-1].astype(np.int64)
+    for k in range(int(counts.max())):
+        px = np.nonzero(counts > k)[0]
+        b = first[px] + k
+        n = np.array([flat[i][p] for i, p in zip(item[b], px)], dt)
+        h[px] = h[px] + np.asarray(weight[b], dt) * n
+    return h
+
+
+@pytest.mark.parametrize("f64", [False, True])
+def test_blend_single_bin_is_biome_fractal(oracle, f64):
+    perm, g32, g64 = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+    grad, dt = (g64, np.float64) if f64 else (g32, np.float32)
+    mx, my, offset = 37, 23, (-1234.5, 987.25)
+    for b, bp in enumerate(BLEND_BIOMES):
+        n = mx * my
+        item = np.full(n, b, np.uint16)
+        weight = np.ones(n, np.float32)
+        off = np.arange(n + 1, dtype=np.uint32)
+        got = oracle.multi_biome(perm, grad, BLEND_BIOMES, item, weight, off, (mx, my), offset, f64=f64)
+        f = biome_fractal(oracle, perm, grad, bp, mx, my, offset, f64)
+        want = f * dt(np.float32(bp[5])) + dt(np.float32(bp[4]))
+        assert got.dtype == dt and got.tobytes() == want.astype(dt).tobytes(), b
+
+
+def test_blend_zero_variation_is_weighted_depth(oracle):
+    perm, g32, _ = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+    biomes = [(bp[0], bp[1], bp[2], bp[3], bp[4] + 0.01 * i, 0.0) for i, bp in enumerate(BLEND_BIOMES)]
+    rng = np.random.default_rng(21)
+    mx, my = 29, 41
+    item, weight, off, counts = blend_histograms(rng, mx * my, len(biomes), 6)
+    got = oracle.multi_biome(perm, g32, biomes, item, weight, off, (mx, my), (511.0, -77.5))
+    depth = [np.full(mx * my, np.float32(bp[4])) for bp in biomes]
+    want = bin_order_sum(item, weight, off, counts, depth, np.float32)
+    assert got.ravel().tobytes() == want.tobytes()
+
+
+@pytest.mark.parametrize("f64", [False, True])
+def test_blend_is_bin_order_sum_of_biome_maps(oracle, f64):
+    perm, g32, g64 = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+    grad, dt = (g64, np.float64) if f64 else (g32, np.float32)
+    rng = np.random.default_rng(22)
+    mx, my, offset = 48, 31, (3000.25, -2047.5)
+    item, weight, off, counts = blend_histograms(rng, mx * my, len(BLEND_BIOMES), 8)
+    got = oracle.multi_biome(perm, grad, BLEND_BIOMES, item, weight, off, (mx, my), offset, f64=f64)
+    maps = [biome_fractal(oracle, perm, grad, bp, mx, my, offset, f64) * dt(np.float32(bp[5]))
+            + dt(np.float32(bp[4])) for bp in BLEND_BIOMES]
+    want = bin_order_sum(item, weight, off, counts, maps, dt)
+    assert got.ravel().tobytes() == want.tobytes()
+    # the order matters: summed in reverse slot order some pixels round differently
+    rev = np.zeros_like(want)
+    first = off[:-1].astype(np.int64)
+    for k in range(int(counts.max()) - 1, -1, -1):
+        px = np.nonzero(counts > k)[0]
+        b = first[px] + k
+        n = np.array([maps[i].ravel()[p] for i, p in zip(item[b], px)], dt)
+        rev[px] = rev[px] + np.asarray(weight[b], dt) * n
+    assert (rev != want).any()
+
+
+@pytest.mark.parametrize("step", [2, 3])
+def test_blend_level_of_detail_samples_the_fine_map(oracle, step):
+    """At `step` world units per pixel, coarse pixel (x, y) samples world (x * step, y * step) with
+    the half dimension of the covered area: it equals the fine (step 1) map over that area at
+    pixel (x * step, y * step), given the same histogram."""
+    perm, g32, _ = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+    rng = np.random.default_rng(23 + step)
+    mx, my, offset = 20, 13, (-640.0, 1280.5)
+    item, weight, off, counts = blend_histograms(rng, mx * my, len(BLEND_BIOMES), 4)
+    coarse = oracle.multi_biome(perm, g32, BLEND_BIOMES, item, weight, off, (mx, my), offset, step=step)
+    fx, fy = mx * step, my * step
+    src = ((np.arange(fy) // step)[:, None] * mx + (np.arange(fx) // step)[None, :]).ravel()
+    fcounts = counts[src]
+    foff = np.concatenate([[0], np.cumsum(fcounts)]).astype(np.uint32)
+    take = np.concatenate([np.arange(off[p], off[p + 1]) for p in src])
+    fine = oracle.multi_biome(perm, g32, BLEND_BIOMES, item[take], weight[take], foff, (fx, fy), offset)
+    assert coarse.tobytes() == np.ascontiguousarray(fine[::step, ::step]).tobytes()
+
+
+def test_blend_fp32_vs_fp64_on_filtered_biomes(oracle):
+    """The blend of a real filter output (a Voronoi map through the filter at r = 8) in FP32 and FP64
+    agrees to 1e-5 max-abs."""
+    perm, g32, g64 = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+    s, r = 96, 8
+    halo = oracle.voronoi_map(7, len(BLEND_BIOMES), 48, (-1000 - r, 600 - r), s + 2 * r, s + 2 * r)
+    item, weight, off = oracle.shf_window(halo, (r, r), (s, s), r)
+    assert int(np.diff(off).max()) > 1
+    a = oracle.multi_biome(perm, g32, BLEND_BIOMES, item, weight, off, (s, s), (-1000 + s / 2, 600 + s / 2))
+    b = oracle.multi_biome(perm, g64, BLEND_BIOMES, item, weight, off, (s, s), (-1000 + s / 2, 600 + s / 2), f64=True)
+    assert np.abs(a - b).max() <= 1e-5
