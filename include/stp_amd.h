@@ -364,6 +364,12 @@ int stp_batch_chunk_latencies(stp_heightfield_batch* batch, double* h_ms, uint32
 /* h_out = {pixels, drops, drop steps, erode steps, deposit steps, killed drops} of the last run
  * (synchronises `stream`); fails with STP_ERROR_CAPACITY if a histogram overflowed. */
 int stp_batch_counters(stp_heightfield_batch* batch, uint64_t h_out[6], void* stream);
+/* No reference counterpart: *h_status = the status bits of the last run (synchronises `stream`):
+ * 1 a pixel histogram held more distinct biomes than the bin capacity, 2 a cell of the LDS-tiled
+ * erosion's fixed-point window ended outside [-4, 7.5] (heights piled up towards the window's
+ * range of [-8, 8)); the results of such a run are not to be used. stp_batch_counters and
+ * stp_batch_bin_total return STP_ERROR_CAPACITY for either. */
+int stp_batch_status(stp_heightfield_batch* batch, uint32_t* h_status, void* stream);
 /* No reference counterpart (a measurement aid): *h_out = the sum over every pixel of the bound
  * chunks of its histogram bin count in the last run, i.e. the (pixel, biome) noise evaluations of
  * the blend (synchronises `stream`; STP_ERROR_CAPACITY if a histogram overflowed). */
@@ -631,14 +637,15 @@ typedef struct {
 	uint64_t Bytes;             /* of the whole record */
 } stp_chunk_record;
 
-/* Bytes of one record of a width x height chunk with `maps`. */
+/* Bytes of one record of a width x height chunk with `maps` (width and height in [1, 65536]). */
 int stp_chunk_record_bytes(uint32_t width, uint32_t height, uint32_t maps, uint64_t* bytes);
 /* Records of the listed chunks (count (X, Y) pairs, each with a map) one after another into h_out
  * (capacity bytes); *written = count * the record size. Synchronous. */
 int stp_world_export(stp_heightfield_world* world, const int32_t* h_chunks, uint32_t count, uint32_t maps, void* h_out,
 	uint64_t capacity, uint64_t* written);
-/* Validates the record at `data` (size bytes available: magic, version, sizes, check) and points
- * *record into it; STP_ERROR_INVALID_ARGUMENT on any mismatch. Host only. */
+/* Validates the record at `data` (size bytes available: magic, version, width and height in
+ * [1, 65536], payload size, check) and points *record into it; STP_ERROR_INVALID_ARGUMENT on any
+ * mismatch. Host only. */
 int stp_chunk_record_read(const void* data, uint64_t size, stp_chunk_record* record);
 
 /* The conflict-free schedule the world uses (host only): h_batch[i] = batch of centre i, first

@@ -252,6 +252,7 @@ _SIGNATURES = {
     "stp_batch_chunk_latencies": (_I, [_P, _P, _U32]),
     "stp_batch_counters": (_I, [_P, _P, _P]),
     "stp_batch_bin_total": (_I, [_P, _P, _P]),
+    "stp_batch_status": (_I, [_P, _P, _P]),
     "stp_batch_read_chunk": (_I, [_P, _U32, _P, _P, _P, _P]),
     "stp_diversity_runtime_create": (_I, [_P, _P, _U32, _P, _U32, _U32, C.c_char_p, _P, _U32, _P]),
     "stp_rtc_compile": (_I, [C.c_char_p, C.c_char_p, _P, _U32, _P, _U32, _P, _P, _U32, C.c_int, _P]),

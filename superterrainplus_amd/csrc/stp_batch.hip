@@ -1289,7 +1289,8 @@ namespace SuperTerrainPlus::STPBatch {
 	__global__ void __launch_bounds__(Threads) erodeTileKERNEL(const uint2* const tiles, const Chunk* const chunks,
 		const ErosionArgs args, const unsigned int tile, const unsigned int halo, const int2* const brush_offset,
 		const float* const brush_weight, const unsigned int* const tile_drops, const unsigned int* const tile_first,
-		const unsigned int* const bin, float* const height, float* const ledger, uint2* const drop_info) {
+		const unsigned int* const bin, float* const height, float* const ledger, uint2* const drop_info,
+		unsigned int* const status) {
 		extern __shared__ int window[];
 		__shared__ int2 brush[MaxTiledTaps];
 		const uint2 t = tiles[blockIdx.x];
@@ -1333,6 +1334,9 @@ namespace SuperTerrainPlus::STPBatch {
 			const int q = window[i];
 			if (q != toFixedPoint(*p)) {
 				*p = fromFixedPoint(q);
+			}
+			if (q < WindowLowest || q > WindowHighest) {
+				*status = 1u;
 			}
 		}
 	}
@@ -2125,6 +2129,10 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 		break;
 	}
 	case 4: {
+		if (clear_flag) {
+			// the erosion window status (a caller running stages 4..6 only, on heights it wrote)
+			zeroKERNEL<<<1, 64, 0, stream>>>(this->Flag.get() + 1, 1u);
+		}
 		// tiled erosion: bin the drops by spawn tile
 		const unsigned int tile = this->Tuning.ErosionTile;
 		const auto [t, n] = range(4);
@@ -2171,7 +2179,7 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 				kernel<<<nt, threads, this->ErosionLds, stream>>>(tt, this->ChunkRecord, args, tile,
 					this->ErosionHalo, this->BrushOffset.get(), this->BrushWeight.get(), this->TileDrops.get(),
 					this->TileFirst.get(), this->DropBin.get(), this->Height.get(), this->Ledger.get(),
-					this->DropInfo.get());
+					this->DropInfo.get(), this->Flag.get() + 1);
 			};
 			switch (this->Tuning.ErosionThreads) {
 			case 256u:
@@ -2340,12 +2348,21 @@ unsigned long long STPHeightfieldBatch::binTotal(const hipStream_t stream) {
 }
 
 void STPHeightfieldBatch::check(const hipStream_t stream) {
-	unsigned int flag = 0u;
-	STP_CHECK_HIP(hipMemcpyAsync(&flag, this->Flag.get(), sizeof(flag), hipMemcpyDeviceToHost, stream));
-	STP_CHECK_HIP(hipStreamSynchronize(stream));
-	if (flag != 0u) {
+	const unsigned int flag = this->status(stream);
+	if (flag & 1u) {
 		STP_THROW(STPCapacityError, "a pixel histogram holds more distinct biomes than the bin capacity");
 	}
+	if (flag & 2u) {
+		STP_THROW(STPCapacityError, "an eroded height left the range of the LDS-tiled erosion window ([-4, 7.5]); "
+			"erode these chunks with the global-atomic kernel (erosion tile 0)");
+	}
+}
+
+unsigned int STPHeightfieldBatch::status(const hipStream_t stream) {
+	unsigned int flag[2] {};
+	STP_CHECK_HIP(hipMemcpyAsync(flag, this->Flag.get(), sizeof(flag), hipMemcpyDeviceToHost, stream));
+	STP_CHECK_HIP(hipStreamSynchronize(stream));
+	return (flag[0] != 0u ? 1u : 0u) | (flag[1] != 0u ? 2u : 0u);
 }
 
 void STPHeightfieldBatch::readChunk(const unsigned int i, float* const height, unsigned short* const fixed,

@@ -120,8 +120,13 @@ namespace SuperTerrainPlus {
 		std::array<unsigned long long, 4> counters(hipStream_t stream);
 		// Sum over the bound chunks' pixels of the last run's histogram bin counts (synchronises)
 		unsigned long long binTotal(hipStream_t stream);
-		// Throws STPCapacityError if a histogram overflowed its bin capacity (synchronises).
+		// Throws STPCapacityError if a histogram overflowed its bin capacity or a tiled erosion
+		// window cell left its range (synchronises).
 		void check(hipStream_t stream);
+		// Status bits of the runs since the flags were last cleared (every run of stage 0 clears
+		// them; runStages without stage 0 keeps them): 1 a histogram overflowed its bin capacity, 2 a
+		// tiled erosion window cell left [WindowLowest, WindowHighest] (synchronises).
+		unsigned int status(hipStream_t stream);
 		void readChunk(unsigned int i, float* height, unsigned short* fixed, unsigned short* halo, float* ledger) const;
 
 		// Switches the erosion launch shape (throws STPInvalidArgument if it cannot run here: the

@@ -834,6 +834,13 @@ int stp_batch_counters(stp_heightfield_batch* const batch, uint64_t h_out[6], vo
 	});
 }
 
+int stp_batch_status(stp_heightfield_batch* const batch, uint32_t* const h_status, void* const stream) {
+	return guarded([&] {
+		require(batch && h_status, "null argument");
+		*h_status = batch->Batch->status(toStream(stream));
+	});
+}
+
 int stp_batch_bin_total(stp_heightfield_batch* const batch, uint64_t* const h_out, void* const stream) {
 	return guarded([&] {
 		require(batch && h_out, "null argument");
@@ -1219,8 +1226,9 @@ int stp_layer_factory_status(stp_layer_factory* const factory, uint32_t* const s
 
 int stp_chunk_record_bytes(const uint32_t width, const uint32_t height, const uint32_t maps, uint64_t* const bytes) {
 	return guarded([&] {
-		require(bytes && width > 0u && height > 0u && maps != 0u && (maps & ~STPChunkRecord::MapAll) == 0u,
-			"positive size, maps a non-empty combination of 1, 2, 4");
+		require(bytes && width > 0u && height > 0u && width <= STPChunkRecord::MaxSide
+			&& height <= STPChunkRecord::MaxSide && maps != 0u && (maps & ~STPChunkRecord::MapAll) == 0u,
+			"size in [1, 65536] per side, maps a non-empty combination of 1, 2, 4");
 		*bytes = STPChunkRecord::Layout(width, height, maps).record();
 	});
 }

@@ -134,7 +134,8 @@ STPChunkRecord::View STPChunkRecord::read(const unsigned char* const data, const
 	v.Maps = get<unsigned int>(data + 36u);
 	const unsigned long long payload = get<unsigned long long>(data + 40u);
 	v.Check = get<unsigned int>(data + 48u);
-	if (v.Maps == 0u || (v.Maps & ~MapAll) != 0u || v.Completeness > 3u || v.Width == 0u || v.Height == 0u) {
+	if (v.Maps == 0u || (v.Maps & ~MapAll) != 0u || v.Completeness > 3u || v.Width == 0u || v.Height == 0u
+		|| v.Width > MaxSide || v.Height > MaxSide) {
 		STP_THROW(STPInvalidArgument, "chunk record header: maps, completeness or size out of range");
 	}
 	const Layout l(v.Width, v.Height, v.Maps);

@@ -26,12 +26,16 @@ namespace SuperTerrainPlus::STPChunkRecord {
 
 	constexpr unsigned int HeaderBytes = 64u, Version = 1u;
 	constexpr unsigned int MapBiome = 1u, MapHeight = 2u, MapFixed = 4u, MapAll = 7u;
+	// Largest width and height of a record: with both at most 2^16 every map and payload size
+	// (at most 8 bytes per pixel, 2^35 bytes) is exact in 64 bits.
+	constexpr unsigned int MaxSide = 65536u;
 
 	inline unsigned long long padded(const unsigned long long bytes) noexcept {
 		return (bytes + 15ull) & ~15ull;
 	}
 
-	// Byte offsets of the maps inside the payload and the payload size.
+	// Byte offsets of the maps inside the payload and the payload size (width and height at most
+	// MaxSide; callers check).
 	struct Layout {
 		unsigned long long Biome = 0u, Height = 0u, Fixed = 0u, Payload = 0u;
 

@@ -109,7 +109,7 @@ namespace {
 					ledger[drop] = lost + d.sediment;
 				}
 				if (drop_info) {
-					drop_info[drop] = make_uint2(steps | (erodeSteps << 16), depositSteps | (killed << 16));
+					drop_info[drop] = packDropInfo(steps, erodeSteps, depositSteps, killed);
 				}
 			}
 		}

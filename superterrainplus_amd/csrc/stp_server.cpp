@@ -254,6 +254,7 @@ void STPHeightfieldServer::launch() {
 			this->Cache.emplace(p.K, std::move(e));
 
 			const unsigned int step = this->Setting.ChunkWorldSize / p.K.Side;
+			// submit() keeps |X|, |Y| below 2^30 / ChunkWorldSize, so the origins fit in 32 bits
 			const long long world = static_cast<long long>(this->Setting.ChunkWorldSize);
 			chunks.push_back(STPBatchChunk {p.K.Side, static_cast<int>(p.K.X * world), static_cast<int>(p.K.Y * world), step});
 			streamId.push_back(streamWord(p.K.X, p.K.Y, p.K.Side));

@@ -130,6 +130,15 @@ namespace SuperTerrainPlus::STPStage {
 		d.volume *= (1.0f - a.Rain.EvaporateSpeed);
 	}
 
+	// A drop's counters as two words: {steps | erode steps << 16, deposit steps | killed << 16}.
+	// STPRainDropSetting::validate bounds every drop's lifetime by 65535 steps; each count saturates
+	// at 0xFFFF all the same, so that no count can carry into its neighbour's field.
+	__device__ inline uint2 packDropInfo(const unsigned int steps, const unsigned int erode_steps,
+		const unsigned int deposit_steps, const unsigned int killed) {
+		return make_uint2(min(steps, 0xFFFFu) | (min(erode_steps, 0xFFFFu) << 16),
+			min(deposit_steps, 0xFFFFu) | (killed << 16));
+	}
+
 	__device__ inline bool alive(const ErosionArgs& a, const DropState& d, const unsigned int life) {
 		return (a.Rain.MaxLifetime == 0u || life < a.Rain.MaxLifetime) && d.volume >= a.Rain.minWaterVolume;
 	}
@@ -207,6 +216,11 @@ namespace SuperTerrainPlus::STPStage {
 	// the reference's update takes negative amounts and feeds them back as negative deposits
 	// without bound: FP32 maps follow that, a fixed-point window would wrap.
 	constexpr float WindowHeightLimit = 4.0f;
+	// A window cell whose final value lies outside [WindowLowest, WindowHighest] (fixed point) has
+	// left, or nearly left, the range the window can hold: heights start in [0, 4] and end >= 0 up
+	// to rounding, and a cell piled up past 8 wraps to about -8. The tiled kernel flags such a cell
+	// in the batch status rather than pass a wrapped height on silently.
+	constexpr int WindowLowest = -(4 << FixedBits), WindowHighest = (15 << FixedBits) / 2;
 
 	__device__ inline int toFixedPoint(const float h) { return __float2int_rn(h * FixedScale); }
 	__device__ inline float fromFixedPoint(const int q) { return static_cast<float>(q) * FixedStep; }
@@ -569,7 +583,7 @@ namespace SuperTerrainPlus::STPStage {
 			ledger[drop] = lost + d.sediment;
 		}
 		if (drop_info) {
-			drop_info[drop] = make_uint2(steps | (erodeSteps << 16), depositSteps | (killed << 16));
+			drop_info[drop] = packDropInfo(steps, erodeSteps, depositSteps, killed);
 		}
 	}
 

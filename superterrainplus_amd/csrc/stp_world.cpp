@@ -226,6 +226,8 @@ void STPHeightfieldWorld::erode(const std::vector<Key>& centres) {
 	// erosion binning, erosion and format of the merged maps
 	this->Eroder->runStages(4, STPHeightfieldBatch::StageCount, this->Stream);
 	pass(neighbourhoodKERNEL<false>);
+	// a re-eroded chunk can pile heights up: a window cell out of range is an error, not a result
+	this->Eroder->check(this->Stream);
 	for (const Key& c : centres) {
 		this->Chunk.at(c).State = STPChunkState::Complete;
 	}
@@ -233,6 +235,15 @@ void STPHeightfieldWorld::erode(const std::vector<Key>& centres) {
 
 std::vector<int2> STPHeightfieldWorld::request(const std::vector<int2>& chunks) {
 	const int h = static_cast<int>((this->Setting.ErosionNeighbour - 1u) / 2u);
+	// every chunk of a neighbourhood (and its filter halo) keeps its world pixels within +-2^30, so
+	// the 32-bit world origins of generate() and erode() cannot overflow
+	const long long reach = (1ll << 30) / static_cast<long long>(this->Setting.MapSize);
+	for (const int2 c : chunks) {
+		if (std::llabs(static_cast<long long>(c.x)) + h >= reach || std::llabs(static_cast<long long>(c.y)) + h >= reach) {
+			STP_THROW(STPInvalidArgument, "chunk (" + std::to_string(c.x) + ", " + std::to_string(c.y)
+				+ ") or its erosion neighbourhood is out of the 32-bit world range");
+		}
+	}
 	std::vector<Key> centres;
 	std::unordered_set<Key, KeyHash> seen;
 	for (const int2 c : chunks) {
@@ -354,6 +365,9 @@ unsigned long long STPHeightfieldWorld::exportRecords(const std::vector<int2>& c
 		STP_THROW(STPInvalidArgument, "maps: a non-empty combination of biome (1), FP32 height (2), UINT16 height (4)");
 	}
 	const unsigned int m = this->Setting.MapSize;
+	if (m > R::MaxSide) {
+		STP_THROW(STPInvalidArgument, "chunk records hold maps of at most 65536 pixels per side");
+	}
 	const R::Layout l(m, m, maps);
 	const size_t n = chunks.size();
 	const unsigned long long record = l.record(), total = record * n;

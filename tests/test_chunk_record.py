@@ -92,3 +92,18 @@ def test_reader_refuses_damaged_records():
         capi.read_chunk_records(b"STPX" + good[4:])  # magic
     with pytest.raises(capi.STPError):
         capi.read_chunk_records(build(1, 2, 8, 8, 2, height=f, check=12345))
+
+
+def test_reader_refuses_sizes_beyond_the_side_limit():
+    """A header whose width x height x 4 wraps 64 bits (2^31 x 2^31, FP32 and biome maps: the wrapped
+    payload size is 0) must not pass as a record of no pixels; sides above 65536 are refused, by
+    the reader and by the size query."""
+    header = b"STPC" + struct.pack("<HHiiiiIIIIQI12x", 1, 64, 0, 0, 0, 0, 1 << 31, 1 << 31, 3, 3, 0, 0)
+    with pytest.raises(capi.STPError):
+        capi.read_chunk_records(header)
+    header = b"STPC" + struct.pack("<HHiiiiIIIIQI12x", 1, 64, 0, 0, 0, 0, 65537, 1, 3, 4, pad(65537 * 2), 0)
+    with pytest.raises(capi.STPError):
+        capi.read_chunk_records(header + b"\0" * pad(65537 * 2))
+    with pytest.raises(capi.STPError):
+        record_bytes(65537, 1, 1)
+    assert record_bytes(65536, 1, 4) == 64 + 65536 * 2

@@ -267,6 +267,20 @@ def test_batch_bin_total_matches_oracle(gpu, oracle):
         _, _, off = oracle.shf_window(halo, (r, r), (s, s), r)
         expect += int(np.asarray(off)[-1])
     assert total.value == expect
+    status = C.c_uint32(7)
+    capi.call("stp_batch_status", C.c_void_p(b.h), C.byref(status), None)
+    assert status.value == 0
+
+
+def test_batch_status_clear_after_tiled_erosion(gpu):
+    """The tiled erosion's window status stays clear on the demo's height range (it flags a window
+    cell that ends outside [-4, 7.5], where the fixed-point window would wrap)."""
+    b = Batch(1.0)
+    assert b.tuning()[0] > 0
+    b.run(0)
+    status = C.c_uint32(7)
+    capi.call("stp_batch_status", C.c_void_p(b.h), C.byref(status), None)
+    assert status.value == 0
 
 
 def test_batch_plain_matches_graph_statistically(gpu, oracle):
