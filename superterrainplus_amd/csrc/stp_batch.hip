@@ -1539,6 +1539,8 @@ void STPHeightfieldBatch::init(const STPEnvironment::STPRainDropSetting& rain) {
 	this->Ledger = STPSmartDeviceMemory::makeDevice<float>(drops);
 	this->DropInfo = STPSmartDeviceMemory::makeDevice<uint2>(drops);
 	this->Flag = STPSmartDeviceMemory::makeDevice<unsigned int>(4u);
+	// a batch that only ever runs some stages (an eroder: stages 4..6) never clears the others' words
+	STP_CHECK_HIP(hipMemset(this->Flag.get(), 0, 4u * sizeof(unsigned int)));
 	// 0..3 the drop counters, 4 the bin total
 	this->Counter = STPSmartDeviceMemory::makeDevice<unsigned long long>(5u);
 	STP_CHECK_HIP(hipMemset(this->Fixed.get(), 0, pixel * sizeof(unsigned short)));

@@ -123,8 +123,8 @@ namespace SuperTerrainPlus {
 		// Throws STPCapacityError if a histogram overflowed its bin capacity or a tiled erosion
 		// window cell left its range (synchronises).
 		void check(hipStream_t stream);
-		// Status bits of the runs since the flags were last cleared (every run of stage 0 clears
-		// them; runStages without stage 0 keeps them): 1 a histogram overflowed its bin capacity, 2 a
+		// Status bits of the runs since the flags were last cleared (construction and every run of
+		// stage 0 clear both; a run of stage 4 clears bit 2): 1 a histogram overflowed its bin capacity, 2 a
 		// tiled erosion window cell left [WindowLowest, WindowHighest] (synchronises).
 		unsigned int status(hipStream_t stream);
 		void readChunk(unsigned int i, float* height, unsigned short* fixed, unsigned short* halo, float* ledger) const;

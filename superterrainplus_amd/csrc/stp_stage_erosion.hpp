@@ -220,7 +220,10 @@ namespace SuperTerrainPlus::STPStage {
 	// left, or nearly left, the range the window can hold: heights start in [0, 4] and end >= 0 up
 	// to rounding, and a cell piled up past 8 wraps to about -8. The tiled kernel flags such a cell
 	// in the batch status rather than pass a wrapped height on silently.
-	constexpr int WindowLowest = -(4 << FixedBits), WindowHighest = (15 << FixedBits) / 2;
+	constexpr int WindowLowest = -(4 << FixedBits), WindowHighest = 15 << (FixedBits - 1);
+	// 7.5 in fixed point is 15 * 2^27: below INT_MAX (15 << FixedBits would not be)
+	static_assert(static_cast<long long>(WindowHighest) == 15ll << (FixedBits - 1), "window bound overflows");
+	static_assert(WindowHighest > (7 << FixedBits) && WindowLowest < 0, "window bounds out of order");
 
 	__device__ inline int toFixedPoint(const float h) { return __float2int_rn(h * FixedScale); }
 	__device__ inline float fromFixedPoint(const int q) { return static_cast<float>(q) * FixedStep; }
