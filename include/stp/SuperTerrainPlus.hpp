@@ -326,6 +326,18 @@ namespace SuperTerrainPlus {
 			hipStream_t stream);
 		void erodeDevice(STPHeightFloat_t* const* heightfield_original, STPHeightFixed_t* const* heightfield_low,
 			hipStream_t stream);
+		// No reference counterpart; for callers that cannot allocate (code under graph capture,
+		// serving loops, harnesses that forbid allocation). erodeDevice with the merged map held in
+		// the library's fixed device scratch instead of the generator's memory pool: no allocation
+		// and no free. The merged map may hold at most ErosionScratchPixels pixels (STPCapacityError
+		// otherwise). Every call on one device shares that scratch: calls that overlap on one
+		// device race.
+		void erodeDeviceNoAlloc(STPHeightFloat_t* const* heightfield_original, STPHeightFixed_t* const* heightfield_low,
+			hipStream_t stream);
+		static constexpr unsigned long long ErosionScratchPixels = 4ull << 20;
+		// Looks up the scratch's address on the current device (once per device; the library's
+		// initialisation calls it, and the first allocation-free call on a device does if needed).
+		static void prepareScratch();
 
 		void setErosionMode(STPErosionMode mode) noexcept;
 		const STPNearestNeighbourInformation& erosionNeighbour() const noexcept;

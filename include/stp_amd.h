@@ -280,6 +280,16 @@ int stp_heightfield_generate_device(stp_heightfield_generator* generator, float*
 	const uint16_t* const* d_biomemap, float offset_x, float offset_y, void* stream);
 int stp_heightfield_erode_device(stp_heightfield_generator* generator, float* const* d_heightfield,
 	uint16_t* const* d_fixed, void* stream);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation):
+ * stp_heightfield_erode_device with the merged free-slip map in the library's fixed device scratch
+ * (STP_EROSION_SCRATCH_PIXELS FP32 pixels, 16 MiB per device, held until the process ends) instead
+ * of the generator's memory pool. It calls no allocation or free. A merged map larger than the
+ * scratch gives STP_ERROR_CAPACITY. Every call on one device shares that device's scratch and
+ * leaves it holding its merged map: calls that overlap on one device race. */
+#define STP_EROSION_SCRATCH_PIXELS (4u << 20)
+int stp_heightfield_erode_device_noalloc(stp_heightfield_generator* generator, float* const* d_heightfield,
+	uint16_t* const* d_fixed, void* stream);
 
 /* ------------------------------------------------------------------ runtime compilation (hipRTC)
  * STPDeviceRuntimeBinary::compile (SuperTerrain+/SuperTerrain+/Public/SuperTerrain+/GPGPU/

@@ -243,6 +243,7 @@ _SIGNATURES = {
     "stp_heightfield_erode": (_I, [_P, _P, _P]),
     "stp_heightfield_generate_device": (_I, [_P, _P, _P, _F, _F, _P]),
     "stp_heightfield_erode_device": (_I, [_P, _P, _P, _P]),
+    "stp_heightfield_erode_device_noalloc": (_I, [_P, _P, _P, _P]),
     "stp_batch_create": (_I, [C.POINTER(SimplexNoiseSetting), _P, _U32, C.POINTER(RainDropSetting),
                               C.POINTER(BatchSetting), _P, _U32, C.POINTER(C.c_void_p)]),
     "stp_batch_destroy": (_I, [_P]),

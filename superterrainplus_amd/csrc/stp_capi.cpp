@@ -223,6 +223,8 @@ int stp_engine_initialise(const int device) {
 	return guarded([&] {
 		STP_CHECK_HIP(hipSetDevice(device));
 		STP_CHECK_HIP(hipFree(nullptr));
+		// the allocation-free erosion's scratch address on this device
+		STPHeightfieldGenerator::prepareScratch();
 	});
 }
 
@@ -634,6 +636,14 @@ int stp_heightfield_erode_device(stp_heightfield_generator* const generator, flo
 	return guarded([&] {
 		require(generator && d_heightfield && d_fixed, "null argument");
 		generator->Generator->erodeDevice(d_heightfield, d_fixed, toStream(stream));
+	});
+}
+
+int stp_heightfield_erode_device_noalloc(stp_heightfield_generator* const generator, float* const* const d_heightfield,
+	uint16_t* const* const d_fixed, void* const stream) {
+	return guarded([&] {
+		require(generator && d_heightfield && d_fixed, "null argument");
+		generator->Generator->erodeDeviceNoAlloc(d_heightfield, d_fixed, toStream(stream));
 	});
 }
 

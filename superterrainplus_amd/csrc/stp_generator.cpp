@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 using namespace SuperTerrainPlus;
 using namespace SuperTerrainPlus::STPAlgorithm;
@@ -430,6 +432,42 @@ void STPHeightfieldGenerator::generateDevice(STPHeightFloat_t* const heightfield
 	g.Diversity(heightmap_buffer, samplemap_buffer, offset);
 }
 
+// ==================================================================== allocation-free erosion
+// The merged FP32 map of erodeDeviceNoAlloc lives in a file-scope device array. In host code the
+// array's name is not a device address: its address on each device comes from
+// hipGetSymbolAddress, once per device with that device current, and only that address is passed
+// to kernels.
+static __device__ __align__(256) float scratch[STPHeightfieldGenerator::ErosionScratchPixels];
+
+namespace {
+	constexpr int ScratchDevices = 64;
+	std::mutex ScratchLock;
+	float* ScratchAddress[ScratchDevices] = {};
+
+	// the scratch's address on the current device
+	float* scratchOnDevice() {
+		int device = 0;
+		STP_CHECK_HIP(hipGetDevice(&device));
+		if (device < 0 || device >= ScratchDevices) {
+			STP_THROW(STPInvalidArgument, "device ordinal beyond the scratch table");
+		}
+		std::lock_guard<std::mutex> guard(ScratchLock);
+		if (!ScratchAddress[device]) {
+			void* p = nullptr;
+			STP_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+			if (!p) {
+				STP_THROW(STPHIPError, "no device address for the erosion scratch");
+			}
+			ScratchAddress[device] = static_cast<float*>(p);
+		}
+		return ScratchAddress[device];
+	}
+} // namespace
+
+void STPHeightfieldGenerator::prepareScratch() {
+	(void)scratchOnDevice();
+}
+
 namespace {
 	template <class FB, class XB>
 	void runErosion(STPHeightfieldGenerator::STPErosionMode mode, const STPNearestNeighbourInformation& nn,
@@ -474,6 +512,30 @@ void STPHeightfieldGenerator::erodeDevice(STPHeightFloat_t* const* const heightf
 	const XB xbuf(heightfield_low, g.ErosionNeighbour, {g.Pool, stream}, XB::STPChunkLocation::Device);
 	runErosion(static_cast<STPErosionMode>(g.Mode.load()), g.ErosionNeighbour, g.Rain, g.BrushIndex.get(),
 		g.BrushWeight.get(), g.BrushSize, g.ErodeCounter++, fbuf, xbuf);
+}
+
+void STPHeightfieldGenerator::erodeDeviceNoAlloc(STPHeightFloat_t* const* const heightfield_original,
+	STPHeightFixed_t* const* const heightfield_low, const hipStream_t stream) {
+	STPImpl& g = *this->Impl;
+	const STPNearestNeighbourInformation& nn = g.ErosionNeighbour;
+	if (!heightfield_original || !heightfield_low) {
+		STP_THROW(STPInvalidArgument, "null chunk pointer array");
+	}
+	const unsigned long long pixels = static_cast<unsigned long long>(nn.TotalMapSize.x) * nn.TotalMapSize.y;
+	if (pixels > ErosionScratchPixels) {
+		STP_THROW(STPCapacityError, "the merged erosion map holds " + std::to_string(pixels) + " pixels; the "
+			"allocation-free erosion holds at most " + std::to_string(ErosionScratchPixels));
+	}
+	float* const merged = scratchOnDevice();
+	// every merged pixel is overwritten by the merge: nothing in the scratch needs clearing
+	STPHeightfieldKernel::mergeNeighbour(reinterpret_cast<const void* const*>(heightfield_original), merged,
+		sizeof(STPHeightFloat_t), nn, stream);
+	STPHeightfieldKernel::hydraulicErosion(merged, nn, g.Rain, g.BrushIndex.get(), g.BrushWeight.get(), g.BrushSize,
+		g.ErodeCounter++,
+		static_cast<STPErosionMode>(g.Mode.load()) == STPErosionMode::Serialized
+			? STPHeightfieldKernel::STPErosionMode::Serialized : STPHeightfieldKernel::STPErosionMode::Parallel,
+		nullptr, nullptr, stream);
+	STPHeightfieldKernel::unmergeFormatNeighbour(merged, heightfield_original, heightfield_low, nn, stream);
 }
 
 // ==================================================================== built-in diversity generators

@@ -89,6 +89,9 @@ namespace SuperTerrainPlus {
 			const STPNearestNeighbourInformation& nn, hipStream_t stream);
 		void unmergeNeighbour(const void* merged, void* const* chunks, unsigned int element_size,
 			const STPNearestNeighbourInformation& nn, hipStream_t stream);
+		// unmerge of the FP32 map into `height` with each pixel's UINT16 format into `fixed` (one pass)
+		void unmergeFormatNeighbour(const float* merged, float* const* height, STPHeightFixed_t* const* fixed,
+			const STPNearestNeighbourInformation& nn, hipStream_t stream);
 
 		// --- single histogram filter (stp_kernels_shf.hip)
 		struct STPFilterWorkspace {

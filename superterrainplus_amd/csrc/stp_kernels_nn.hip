@@ -8,6 +8,7 @@
 // One workgroup row covers 256 consecutive merged pixels, so reads and writes are coalesced
 // except at chunk seams. Roofline: HBM, 2 x element size bytes per pixel.
 #include "stp_internal.hpp"
+#include "stp_stage_erosion.hpp"
 
 using namespace SuperTerrainPlus;
 
@@ -35,6 +36,23 @@ namespace {
 		} else {
 			c[l] = merged_in[g];
 		}
+	}
+
+	// Unmerge of an FP32 merged map fused with the UINT16 format of each pixel: the same bits as
+	// formatting the merged map and unmerging both, in one pass (4 B read, 6 B written per pixel).
+	__global__ void __launch_bounds__(256) unmergeFormatKERNEL(const ChunkTable height, const ChunkTable fixed,
+		const float* const merged, const STPNearestNeighbourInformation nn) {
+		const unsigned int gx = blockIdx.x * blockDim.x + threadIdx.x, gy = blockIdx.y;
+		if (gx >= nn.TotalMapSize.x) {
+			return;
+		}
+		const unsigned int lx = gx / nn.MapSize.x, px = gx - lx * nn.MapSize.x;
+		const unsigned int ly = gy / nn.MapSize.y, py = gy - ly * nn.MapSize.y;
+		const unsigned int chunk = lx + ly * nn.ChunkNearestNeighbour.x;
+		const size_t l = px + static_cast<size_t>(py) * nn.MapSize.x;
+		const float h = merged[gx + static_cast<size_t>(gy) * nn.TotalMapSize.x];
+		static_cast<float*>(const_cast<void*>(height.Ptr[chunk]))[l] = h;
+		static_cast<STPHeightFixed_t*>(const_cast<void*>(fixed.Ptr[chunk]))[l] = STPStage::toFixed(h);
 	}
 
 	template <bool Merge>
@@ -95,4 +113,16 @@ void STPHeightfieldKernel::unmergeNeighbour(const void* const merged, void* cons
 		STP_THROW(STPInvalidArgument, "null merged buffer");
 	}
 	launch<false>(table(const_cast<const void* const*>(chunks), nn), nullptr, merged, element_size, nn, stream);
+}
+
+void STPHeightfieldKernel::unmergeFormatNeighbour(const float* const merged, float* const* const height,
+	STPHeightFixed_t* const* const fixed, const STPNearestNeighbourInformation& nn, const hipStream_t stream) {
+	if (!merged || !height || !fixed) {
+		STP_THROW(STPInvalidArgument, "null merged buffer or chunk pointer array");
+	}
+	const ChunkTable h = table(reinterpret_cast<const void* const*>(height), nn),
+		f = table(reinterpret_cast<const void* const*>(fixed), nn);
+	const dim3 grid((nn.TotalMapSize.x + 255u) / 256u, nn.TotalMapSize.y);
+	unmergeFormatKERNEL<<<grid, 256, 0, stream>>>(h, f, merged, nn);
+	STP_CHECK_HIP(hipGetLastError());
 }
