@@ -406,7 +406,9 @@ int stp_batch_get_tuning(stp_heightfield_batch* batch, stp_batch_tuning* out);
  * it to `out` (optional). Synchronous. */
 int stp_batch_autotune(stp_heightfield_batch* batch, uint32_t repeat, void* stream, stp_batch_tuning* out);
 /* Copies chunk `index`'s outputs to host memory (each pointer optional): heights and UINT16
- * (side^2), halo biome map ((side + 2 * Radius)^2), per-drop final sediment (drops). */
+ * (side^2), halo biome map ((side + 2 * Radius)^2), per-drop final sediment (drops; one value per
+ * drop of the chunk, in the order the erosion ran them: drop index for the global-atomic kernel,
+ * tile order for the LDS-tiled one). */
 int stp_batch_read_chunk(stp_heightfield_batch* batch, uint32_t index, float* h_height, uint16_t* h_fixed,
 	uint16_t* h_biome_halo, float* h_ledger);
 

@@ -1133,7 +1133,7 @@ namespace SuperTerrainPlus::STPBatch {
 		if (drop >= c.Drops) {
 			return;
 		}
-		erodeDropAnyBrush(GlobalSurface {height + c.Pixel, c.Side}, chunkArgs(args, c), brush, drop, ledger + c.DropBase,
+		erodeDropAnyBrush(GlobalSurface {height + c.Pixel, c.Side}, chunkArgs(args, c), brush, drop, drop, ledger + c.DropBase,
 			drop_info + c.DropBase);
 	}
 
@@ -1285,6 +1285,29 @@ namespace SuperTerrainPlus::STPBatch {
 		bin[tile_first[c.TileBase + dropTile(d, c.Side, tile)] + rank[c.DropBase + drop]] = drop;
 	}
 
+	// Window cells in flight per thread in the tiled erosion's load and store-back loops.
+	constexpr unsigned int WindowBatch = 16u;
+	// (row, column) of a window cell, stepped by a fixed number of cells along the row-major window.
+	struct WindowCursor {
+		unsigned int X, Y;
+	};
+	struct WindowWalk {
+		unsigned int Width, StepX, StepY;
+		// width and step are the same on every lane: the quotient and remainder are kept scalar
+		__device__ WindowWalk(const unsigned int width, const unsigned int step) :
+			Width(width), StepX(__builtin_amdgcn_readfirstlane(step % width)),
+			StepY(__builtin_amdgcn_readfirstlane(step / width)) {}
+		__device__ WindowCursor start(const unsigned int i) const {
+			const unsigned int y = i / this->Width;
+			return WindowCursor {i - y * this->Width, y};
+		}
+		__device__ WindowCursor next(const WindowCursor c) const {
+			const unsigned int x = c.X + this->StepX;
+			const bool wrap = x >= this->Width;
+			return WindowCursor {wrap ? x - this->Width : x, c.Y + this->StepY + (wrap ? 1u : 0u)};
+		}
+	};
+
 	template <unsigned int Threads>
 	__global__ void __launch_bounds__(Threads) erodeTileKERNEL(const uint2* const tiles, const Chunk* const chunks,
 		const ErosionArgs args, const unsigned int tile, const unsigned int halo, const int2* const brush_offset,
@@ -1310,9 +1333,29 @@ namespace SuperTerrainPlus::STPBatch {
 		s.RH = min(y0 + tile + halo, side) - s.Y0;
 		float* const map = height + c.Pixel;
 		const unsigned int cells = s.RW * s.RH;
-		for (unsigned int i = threadIdx.x; i < cells; i += Threads) {
-			const unsigned int wy = i / s.RW, wx = i - wy * s.RW;
-			window[i] = toFixedPoint(map[(s.Y0 + wy) * side + s.X0 + wx]);
+		// A thread's window cells i = threadIdx.x + j Threads, walked as (row, column) with the
+		// row and column steps of Threads cells: no division per cell. Each batch has WindowBatch
+		// loads in flight before the first is used (a load-then-use loop waits out one HBM round
+		// trip per cell; the workgroup holds its CU alone, so nothing else hides that wait).
+		const WindowWalk walk(s.RW, Threads);
+		const WindowCursor last = walk.start(cells - 1u);
+		for (unsigned int base = threadIdx.x; base < cells; base += WindowBatch * Threads) {
+			float v[WindowBatch];
+			WindowCursor at = walk.start(base);
+#pragma unroll
+			for (unsigned int j = 0u; j < WindowBatch; j++) {
+				// past the last cell, reload the last cell (in bounds, never stored)
+				const WindowCursor k = base + j * Threads < cells ? at : last;
+				v[j] = map[(s.Y0 + k.Y) * side + s.X0 + k.X];
+				at = walk.next(at);
+			}
+#pragma unroll
+			for (unsigned int j = 0u; j < WindowBatch; j++) {
+				const unsigned int i = base + j * Threads;
+				if (i < cells) {
+					window[i] = toFixedPoint(v[j]);
+				}
+			}
 		}
 		for (unsigned int k = threadIdx.x; k < args.BrushSize; k += Threads) {
 			const int2 o = brush_offset[k];
@@ -1322,22 +1365,39 @@ namespace SuperTerrainPlus::STPBatch {
 		const ErosionArgs a = chunkArgs(args, c);
 		const unsigned int first = tile_first[g];
 		for (unsigned int k = threadIdx.x; k < n; k += Threads) {
-			erodeDropAnyBrush(s, a, brush, bin[first + k], ledger + c.DropBase, drop_info + c.DropBase);
+			// results in tile order (consecutive lanes write consecutive words), within the chunk's range
+			erodeDropAnyBrush(s, a, brush, bin[first + k], first + k, ledger, drop_info);
 		}
 		__syncthreads();
 		// No other workgroup of this colour batch touches the window, so the heights in HBM are
 		// still the ones the window was made from: a cell whose fixed-point value is unchanged
 		// keeps its exact FP32 height.
-		for (unsigned int i = threadIdx.x; i < cells; i += Threads) {
-			const unsigned int wy = i / s.RW, wx = i - wy * s.RW;
-			float* const p = map + (s.Y0 + wy) * side + s.X0 + wx;
-			const int q = window[i];
-			if (q != toFixedPoint(*p)) {
-				*p = fromFixedPoint(q);
+		bool outside = false;
+		for (unsigned int base = threadIdx.x; base < cells; base += WindowBatch * Threads) {
+			float old[WindowBatch];
+			unsigned int at[WindowBatch];
+			WindowCursor k = walk.start(base);
+#pragma unroll
+			for (unsigned int j = 0u; j < WindowBatch; j++) {
+				const WindowCursor w = base + j * Threads < cells ? k : last;
+				at[j] = (s.Y0 + w.Y) * side + s.X0 + w.X;
+				old[j] = map[at[j]];
+				k = walk.next(k);
 			}
-			if (q < WindowLowest || q > WindowHighest) {
-				*status = 1u;
+#pragma unroll
+			for (unsigned int j = 0u; j < WindowBatch; j++) {
+				const unsigned int i = base + j * Threads;
+				if (i < cells) {
+					const int q = window[i];
+					if (q != toFixedPoint(old[j])) {
+						map[at[j]] = fromFixedPoint(q);
+					}
+					outside |= q < WindowLowest || q > WindowHighest;
+				}
 			}
+		}
+		if (outside) {
+			*status = 1u;
 		}
 	}
 

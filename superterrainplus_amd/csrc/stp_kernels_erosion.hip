@@ -35,7 +35,7 @@ namespace {
 		__syncthreads();
 		const unsigned int drop = blockIdx.x * blockDim.x + threadIdx.x;
 		if (drop < a.Rain.RainDropCount) {
-			erodeDropAnyBrush(GlobalSurface {map, a.Total.x}, a, brush, drop, ledger, drop_info);
+			erodeDropAnyBrush(GlobalSurface {map, a.Total.x}, a, brush, drop, drop, ledger, drop_info);
 		}
 	}
 

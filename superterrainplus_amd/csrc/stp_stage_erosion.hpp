@@ -312,7 +312,9 @@ namespace SuperTerrainPlus::STPStage {
 		constexpr int H = brushRowHalf(R, Dy), N = 2 * H + 1;
 		// the row's first tap; opaque to the optimiser, so that it keeps one address per row with
 		// non-negative immediate offsets instead of one address per tap left of the centre
-		int first = Dy * rw - H;
+		// (rw is the window's row stride, the same on every lane: read it as a scalar, whatever
+		// register the surrounding code left it in)
+		int first = Dy * __builtin_amdgcn_readfirstlane(rw) - H;
 		asm("" : "+s"(first));
 		int* const p = cell + first;
 #pragma unroll
@@ -516,10 +518,11 @@ namespace SuperTerrainPlus::STPStage {
 
 	// One drop in parallel mode on `s`. `brush` holds (linear tap offset for s.stride(), weight
 	// bits). Taps > 0 is the compile-time brush size, Taps == 0 a loop over a.BrushSize taps; the
-	// surface's `erode` does a step's erode update.
+	// surface's `erode` does a step's erode update. The drop's final sediment and counters go to
+	// ledger[slot] and drop_info[slot].
 	template <class Surface, unsigned int Taps>
 	__device__ inline void erodeDrop(const Surface& s, const ErosionArgs& a, const int2* const brush,
-		const unsigned int drop, float* const ledger, uint2* const drop_info) {
+		const unsigned int drop, const unsigned int slot, float* const ledger, uint2* const drop_info) {
 		DropState d = spawn(a, drop);
 		float lost = 0.0f;
 		unsigned int steps = 0u, erodeSteps = 0u, depositSteps = 0u, killed = 0u;
@@ -583,29 +586,29 @@ namespace SuperTerrainPlus::STPStage {
 			c = next;
 		}
 		if (ledger) {
-			ledger[drop] = lost + d.sediment;
+			ledger[slot] = lost + d.sediment;
 		}
 		if (drop_info) {
-			drop_info[drop] = packDropInfo(steps, erodeSteps, depositSteps, killed);
+			drop_info[slot] = packDropInfo(steps, erodeSteps, depositSteps, killed);
 		}
 	}
 
 	// Brush sizes with a specialised (straight-line) drop: r = 2, 3, 4.
 	template <class Surface>
 	__device__ inline void erodeDropAnyBrush(const Surface& s, const ErosionArgs& a, const int2* const brush,
-		const unsigned int drop, float* const ledger, uint2* const drop_info) {
+		const unsigned int drop, const unsigned int slot, float* const ledger, uint2* const drop_info) {
 		switch (a.BrushSize) {
 		case 9u:
-			erodeDrop<Surface, 9u>(s, a, brush, drop, ledger, drop_info);
+			erodeDrop<Surface, 9u>(s, a, brush, drop, slot, ledger, drop_info);
 			break;
 		case 25u:
-			erodeDrop<Surface, 25u>(s, a, brush, drop, ledger, drop_info);
+			erodeDrop<Surface, 25u>(s, a, brush, drop, slot, ledger, drop_info);
 			break;
 		case 45u:
-			erodeDrop<Surface, 45u>(s, a, brush, drop, ledger, drop_info);
+			erodeDrop<Surface, 45u>(s, a, brush, drop, slot, ledger, drop_info);
 			break;
 		default:
-			erodeDrop<Surface, 0u>(s, a, brush, drop, ledger, drop_info);
+			erodeDrop<Surface, 0u>(s, a, brush, drop, slot, ledger, drop_info);
 			break;
 		}
 	}
