@@ -1,0 +1,186 @@
+"""The multi-biome chunk end to end at its full size: a 2048^2 chunk whose biome map is a seeded
+Voronoi map of 8 biomes over 64 jittered sites (seed 7, one site per 256^2 cell), each biome with
+its own 4-8 octave simplex parameters, blended through the single histogram filter of radius 32,
+eroded by 1,048,576 parallel drops and formatted to UINT16.
+
+Reference chain: SuperDemo+/World/Biomes/STPBiomefieldGenerator.cpp:79-152 (filter + blend plugin)
+-> SuperTerrain+/SuperTerrain+/Private/World/Chunk/STPHeightfieldGenerator.cpp:158-197 (generate,
+erode) -> SuperTerrain+/SuperTerrain+/Private/GPGPU/STPHeightfieldKernel.cu:37-57,80-122.
+
+Bars:
+  * filter + blend: bit-exact against the FP32 oracle, <= 1e-5 max-abs against the FP64 oracle;
+  * erosion (parallel, judged as every parallel erosion here): mass balance
+    sum(h_after - h_before) = -sum(ledger) to 2e-4 relative; sediment moved within
+    4/sqrt(drops) + 0.12 of the serialized FP32 oracle run with the same drops (0.12: the
+    concurrency deficit of parallel drops, which all start on the unmodified terrain); RMS against
+    that run <= 1.5 x the RMS between two serialized runs with different drops;
+  * UINT16: bit-exact format of the eroded map.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from superterrainplus_amd import capi, workloads as W
+
+from helpers import gpu_erode, log, oracle_rain
+
+pytestmark = pytest.mark.gpu
+
+MB = W.multi_biome()
+NOISE = capi.SimplexNoiseSetting(MB.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, MB.engine)
+
+
+def chunk_setting(side):
+    c = capi.ChunkSetting()
+    c.ChunkSize[:] = [side, side]
+    c.MapSize[:] = [side, side]
+    c.ChunkScale[:] = [1.0, 1.0]
+    c.DiversityNearestNeighbour[:] = [3, 3]
+    c.ErosionNearestNeighbour[:] = [1, 1]
+    c.SplatNearestNeighbour[:] = [1, 1]
+    c.RenderDistance[:] = [3, 3]
+    return c
+
+
+def oracle_pre_erosion(oracle, origin, offset):
+    """Halo biome map, FP32 and FP64 blends of the chunk at world pixel `origin` whose noise is
+    sampled at `offset` (the blend's HalfDimension is side / 2)."""
+    side, r = MB.side, MB.radius
+    halo = oracle.voronoi_map(MB.biome_seed, MB.biome_count, MB.cell, (origin[0] - r, origin[1] - r),
+                              side + 2 * r, side + 2 * r)
+    item, weight, off = oracle.shf_window(halo, (r, r), (side, side), r, cap_per_pixel=8)
+    perm, g32, g64 = oracle.permutation(MB.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, MB.engine)
+    h32 = oracle.multi_biome(perm, g32, MB.table(), item, weight, off, (side, side), offset)
+    h64 = oracle.multi_biome(perm, g64, MB.table(), item, weight, off, (side, side), offset, f64=True)
+    return halo, h32, h64
+
+
+def serialized(oracle, h, drops, seed, stream_id=0):
+    e = h.copy()
+    s = W.erosion_setting(drops, seed=seed)
+    ledger, _ = oracle.erode(e, (1, 1), e.shape[::-1], oracle_rain(oracle, s), drops, seed, stream_id=stream_id)
+    return e, float(ledger.astype(np.float64).sum())
+
+
+def check_erosion(oracle, pre, eroded, ledger, drops, seed, stream_id=0, what=""):
+    moved = float(ledger.astype(np.float64).sum())
+    dm = float(eroded.astype(np.float64).sum() - pre.astype(np.float64).sum())
+    ser, ser_moved = serialized(oracle, pre, drops, seed, stream_id)
+    other, _ = serialized(oracle, pre, drops, seed + 1000, stream_id)
+    rms = float(np.sqrt(np.mean((eroded.astype(np.float64) - ser) ** 2)))
+    floor = float(np.sqrt(np.mean((other.astype(np.float64) - ser) ** 2)))
+    print(f"[{what}] moved {moved:.3f} (serialized {ser_moved:.3f}, ratio {moved / ser_moved:.4f}), "
+          f"mass residual {abs(dm + moved) / moved:.2e}, rms {rms:.3e} vs floor {floor:.3e}", flush=True)
+    assert moved > 0.0
+    assert abs(dm + moved) <= 2e-4 * moved, (dm, moved)
+    assert abs(moved / ser_moved - 1.0) <= 4.0 / np.sqrt(drops) + 0.12, (moved, ser_moved)
+    assert rms <= 1.5 * floor, (rms, floor)
+
+
+def test_multi_biome_chunk_through_the_generator(gpu, oracle):
+    """The drop-in path: the biomefield plugin over a 3x3 diversity neighbourhood of device biome
+    maps (STPHeightfieldGenerator::generate), then erosion of the 2048^2 chunk and its format."""
+    side, r = MB.side, MB.radius
+    log("multi-biome 2048^2 chunk: generator generate")
+    props = MB.table()
+    d = C.c_void_p()
+    capi.call("stp_diversity_biomefield_create", C.byref(NOISE), capi.biome_array(props), len(props),
+              (C.c_uint32 * 2)(side, side), r, 8, C.byref(d))
+    plugin = d.value
+    gen = C.c_void_p()
+    capi.call("stp_heightfield_generator_create", C.byref(chunk_setting(side)),
+              C.byref(W.erosion_setting(MB.drops, seed=MB.erosion_seed)), C.c_void_p(plugin), C.byref(gen))
+    try:
+        biomes = []
+        for cy in (-1, 0, 1):
+            for cx in (-1, 0, 1):
+                b = capi.DeviceArray((side, side), np.uint16)
+                capi.call("stp_voronoi_biomemap", MB.biome_seed, MB.biome_count, MB.cell, cx * side, cy * side,
+                          b.vp, side, side, side, None)
+                biomes.append(b)
+        d_h = capi.DeviceArray((side, side), np.float32)
+        capi.call("stp_heightfield_generate_device", gen, d_h.vp, (C.c_void_p * 9)(*[b.ptr for b in biomes]),
+                  0.0, 0.0, None)
+        capi.synchronize()
+        got = d_h.download()
+        halo, h32, h64 = oracle_pre_erosion(oracle, (0, 0), (0.0, 0.0))
+        # the device Voronoi maps are the oracle's: the centre chunk and the halo's corner
+        assert (biomes[4].download() == halo[r:r + side, r:r + side]).all()
+        assert (biomes[0].download()[side - r:, side - r:] == halo[:r, :r]).all()
+        assert got.tobytes() == h32.tobytes()
+        assert float(np.abs(got.astype(np.float64) - h64).max()) <= 1e-5
+
+        log("multi-biome 2048^2 chunk: 1,048,576-drop erosion with ledger")
+        d_e = capi.DeviceArray.from_host(got)
+        led, _ = gpu_erode(d_e, (side, side), (1, 1), W.erosion_setting(MB.drops, seed=MB.erosion_seed), mode=0,
+                           info=False)
+        eroded = d_e.download()
+        check_erosion(oracle, got, eroded, led, MB.drops, MB.erosion_seed, what="per-chunk kernel")
+        d_u = capi.DeviceArray((side, side), np.uint16)
+        capi.call("stp_format_heightmap", d_e.vp, d_u.vp, side, side, None)
+        capi.synchronize()
+        assert (d_u.download() == oracle.format_heightmap(eroded)).all()
+
+        log("multi-biome 2048^2 chunk: generator erode")
+        d_f = capi.DeviceArray((side, side), np.uint16)
+        capi.call("stp_heightfield_erode_device", gen, (C.c_void_p * 1)(d_h.ptr), (C.c_void_p * 1)(d_f.ptr), None)
+        capi.synchronize()
+        ge = d_h.download()
+        assert (d_f.download() == oracle.format_heightmap(ge)).all()
+        # the same drops as above: the same sediment budget up to the concurrency of the run
+        assert not np.array_equal(ge, got)
+        assert abs(float(ge.astype(np.float64).sum() - eroded.astype(np.float64).sum())) <= 0.05 * float(led.sum())
+    finally:
+        capi.lib().stp_heightfield_generator_destroy(gen)
+        capi.call("stp_diversity_destroy", C.c_void_p(plugin))
+
+
+def test_multi_biome_chunk_through_the_batch(gpu, oracle):
+    """The same chunk as a one-chunk batch (biome map, filter, blend, LDS-tiled erosion, format on
+    the device): bit-exact before erosion, then the erosion bars (drops keyed by stream id 0)."""
+    side, r = MB.side, MB.radius
+    origin = (0, 0)
+    props = MB.table()
+
+    def batch(dpp):
+        setting = capi.BatchSetting(MB.biome_seed, MB.biome_count, MB.cell, r, 8, dpp)
+        chunks = (capi.BatchChunk * 1)(capi.BatchChunk(side, (C.c_int32 * 2)(*origin)))
+        h = C.c_void_p()
+        capi.call("stp_batch_create", C.byref(NOISE), capi.biome_array(props), len(props),
+                  C.byref(W.erosion_setting(1, seed=MB.erosion_seed)), C.byref(setting), chunks, 1, C.byref(h))
+        return h
+
+    def read(h, drops):
+        out = np.zeros((side, side), np.float32), np.zeros((side, side), np.uint16)
+        led = np.zeros(max(drops, 1), np.float32)
+        capi.call("stp_batch_read_chunk", h, 0, capi.ptr(out[0]), capi.ptr(out[1]), None, capi.ptr(led))
+        return out[0], out[1], led[:drops]
+
+    log("multi-biome 2048^2 chunk: batch without erosion")
+    pre_b = batch(0.0)
+    try:
+        capi.call("stp_batch_run", pre_b, 0, None)
+        capi.synchronize()
+        pre, pre_fixed, _ = read(pre_b, 0)
+    finally:
+        capi.lib().stp_batch_destroy(pre_b)
+    halo, h32, h64 = oracle_pre_erosion(oracle, origin, (origin[0] + side / 2, origin[1] + side / 2))
+    assert pre.tobytes() == h32.tobytes()
+    assert (pre_fixed == oracle.format_heightmap(h32)).all()
+    assert float(np.abs(pre.astype(np.float64) - h64).max()) <= 1e-5
+
+    log("multi-biome 2048^2 chunk: batch with 1,048,576 drops")
+    dpp = MB.drops / float(side * side)
+    b = batch(dpp)
+    try:
+        capi.call("stp_batch_run", b, 0, None)
+        capi.synchronize()
+        status = C.c_uint32()
+        capi.call("stp_batch_status", b, C.byref(status), None)
+        assert status.value == 0
+        eroded, fixed, led = read(b, MB.drops)
+    finally:
+        capi.lib().stp_batch_destroy(b)
+    check_erosion(oracle, pre, eroded, led, MB.drops, MB.erosion_seed, stream_id=0, what="batch")
+    assert (fixed == oracle.format_heightmap(eroded)).all()
