@@ -266,6 +266,8 @@ namespace SuperTerrainPlus {
 			const STPNearestNeighbourTextureBuffer& Main;
 			MutableT* HostMem = nullptr;
 			MutableT* DeviceMem = nullptr;
+			// host chunks of a device buffer: each chunk's contiguous copy in device memory
+			MutableT* Staging = nullptr;
 			void copy(bool pack);
 		};
 

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 using namespace SuperTerrainPlus;
 using namespace SuperTerrainPlus::STPAlgorithm;
@@ -40,18 +41,24 @@ STPNearestNeighbourTextureBuffer<T, MM>::STPMergedBuffer::STPMergedBuffer(const 
 	const auto [pool, stream] = this->Main.DeviceMemInfo;
 	const bool aliased = this->Location == STPMemoryLocation::DeviceMemory && this->Main.ChunkLocation == STPChunkLocation::Device
 		&& neighbourCount<T>(nn) == 1u;
+	const auto deviceBuffer = [&]() {
+		void* p = nullptr;
+		if (pool) {
+			STP_CHECK_HIP(hipMallocFromPoolAsync(&p, bytes, pool, stream));
+		} else {
+			STP_CHECK_HIP(hipMallocAsync(&p, bytes, stream));
+		}
+		return static_cast<MutableT*>(p);
+	};
 	if (this->Location == STPMemoryLocation::HostMemory) {
 		this->HostMem = static_cast<MutableT*>(STPSmartDeviceMemory::allocatePinned(bytes));
 	} else if (aliased) {
 		this->DeviceMem = const_cast<MutableT*>(this->Main.NeighbourTexture[0]);
-	} else if (pool) {
-		void* p = nullptr;
-		STP_CHECK_HIP(hipMallocFromPoolAsync(&p, bytes, pool, stream));
-		this->DeviceMem = static_cast<MutableT*>(p);
 	} else {
-		void* p = nullptr;
-		STP_CHECK_HIP(hipMallocAsync(&p, bytes, stream));
-		this->DeviceMem = static_cast<MutableT*>(p);
+		this->DeviceMem = deviceBuffer();
+		if (this->Main.ChunkLocation == STPChunkLocation::Host) {
+			this->Staging = deviceBuffer();
+		}
 	}
 	if constexpr (MM != STPNearestNeighbourTextureBufferMemoryMode::WriteOnly) {
 		if (!aliased) {
@@ -85,6 +92,10 @@ STPNearestNeighbourTextureBuffer<T, MM>::STPMergedBuffer::~STPMergedBuffer() noe
 			const hipError_t e = hipFreeAsync(this->DeviceMem, stream);
 			err = err == hipSuccess ? e : err;
 		}
+		if (this->Staging) {
+			const hipError_t e = hipFreeAsync(this->Staging, stream);
+			err = err == hipSuccess ? e : err;
+		}
 		if (this->Main.ChunkLocation == STPChunkLocation::Host) {
 			// host chunks must hold the result when the buffer dies (reference semantics)
 			const hipError_t e = hipStreamSynchronize(stream);
@@ -116,7 +127,31 @@ void STPNearestNeighbourTextureBuffer<T, MM>::STPMergedBuffer::copy(const bool p
 		}
 		return;
 	}
-	MutableT* const merged = this->Location == STPMemoryLocation::HostMemory ? this->HostMem : this->DeviceMem;
+	if (this->Location == STPMemoryLocation::DeviceMemory) {
+		// host chunks: one contiguous copy per chunk between the chunk and its slot in the staging
+		// buffer, and the device gather / scatter between the staging buffer and the merged map (a
+		// pitched copy from pageable host memory moves row by row)
+		const size_t elements = static_cast<size_t>(nn.MapSize.x) * nn.MapSize.y;
+		std::vector<void*> slot(count);
+		for (unsigned int i = 0u; i < count; i++) {
+			slot[i] = this->Staging + i * elements;
+		}
+		if (pack) {
+			for (unsigned int i = 0u; i < count; i++) {
+				STP_CHECK_HIP(hipMemcpyAsync(slot[i], this->Main.NeighbourTexture[i], elements * sizeof(T),
+					hipMemcpyHostToDevice, stream));
+			}
+			STPHeightfieldKernel::mergeNeighbour(slot.data(), this->DeviceMem, sizeof(T), nn, stream);
+		} else {
+			STPHeightfieldKernel::unmergeNeighbour(this->DeviceMem, slot.data(), sizeof(T), nn, stream);
+			for (unsigned int i = 0u; i < count; i++) {
+				STP_CHECK_HIP(hipMemcpyAsync(const_cast<MutableT*>(this->Main.NeighbourTexture[i]), slot[i],
+					elements * sizeof(T), hipMemcpyDeviceToHost, stream));
+			}
+		}
+		return;
+	}
+	MutableT* const merged = this->HostMem;
 	const size_t row = nn.MapSize.x * sizeof(T), total_row = row * nn.ChunkNearestNeighbour.x;
 	for (unsigned int i = 0u; i < count; i++) {
 		const uint2 local = STPChunk::calcLocalChunkCoordinate(i, nn.ChunkNearestNeighbour);
