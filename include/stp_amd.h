@@ -442,6 +442,12 @@ typedef struct {
 	uint64_t Requests, CacheHits, Batches, Evictions;
 	double MeanBatchChunks;
 	uint64_t PeakResidentBytes;
+	/* the requests whose chunk was generated for them (neither resident nor already in
+	 * production when they were admitted): count, latency percentiles (ms) and their pixels per
+	 * second of replay wall time; the fields above cover every request, cache hits included */
+	uint64_t Generated;
+	double GeneratedP50, GeneratedP99, GeneratedP999, GeneratedMax;
+	double GeneratedMpixelPerSecond;
 } stp_server_stats;
 
 typedef struct stp_heightfield_server stp_heightfield_server;
@@ -469,6 +475,12 @@ int stp_server_memory(stp_heightfield_server* server, uint64_t* engine_bytes, ui
  * (non-decreasing). h_latency_ms[i] receives each request's latency. Synchronous. */
 int stp_server_replay(stp_heightfield_server* server, const stp_chunk_request* h_requests, const double* h_arrival_s,
 	uint32_t count, double deadline_ms, double* h_latency_ms, stp_server_stats* out);
+/* No reference counterpart (a test aid): the Philox stream word of the drops of chunk (X, Y) at
+ * Side pixels, the same in every batch. Every served chunk is produced with a free-slip halo of
+ * MaxLifetime + max(r, 2) + 1 pixels of its neighbours' terrain at its level of detail and its
+ * drops spawn in the chunk, so it erodes as the centre of its 3 x 3 neighbourhood of uneroded
+ * chunks does (no drop reaches beyond the halo); the halo is discarded. */
+int stp_server_stream_word(int32_t x, int32_t y, uint32_t side, uint32_t* word);
 
 /* ------------------------------------------------------------------ splatmap
  * The texture region of every pixel of the rendered area's chunks (STPWorldPipeline.cpp:671-707
@@ -630,6 +642,19 @@ int stp_world_state(stp_heightfield_world* world, int32_t x, int32_t y, uint32_t
  * the input of stp_splatmap_generate). Synchronous. */
 int stp_world_render_area(stp_heightfield_world* world, int32_t x, int32_t y, uint32_t radius, uint16_t* d_biome,
 	uint16_t* d_height);
+/* STPWorldPipeline::load (SuperTerrain+/SuperTerrain+/Private/World/STPWorldPipeline.cpp:1504-1633,
+ * STPWorldPipeline.h:51-60,139): the rendered area around chunk (x, y) ((2 radius + 1)^2 chunks, made
+ * complete, as stp_world_render_area assembles it) is loaded on a worker thread into the back buffer
+ * while the caller renders from the front buffer. *status: 0 unchanged (the same centre, nothing
+ * finished since), 1 back buffer busy (a load is running and this call is ignored, or a changed
+ * centre's load has just started, discarding a finished load that was not swapped), 2 swapped (the
+ * load of this centre finished; front and back buffers were swapped). Returns at once; every other
+ * stp_world_* call on this world waits for a running load. */
+int stp_world_load(stp_heightfield_world* world, int32_t x, int32_t y, uint32_t radius, uint32_t* status);
+/* The front buffer: device biome and UINT16 height maps of the last swapped load, *side =
+ * (2 radius + 1) * MapSize pixels a side, row-major (NULL and 0 before the first swap); valid until
+ * the next load that returns 2. Any pointer may be NULL. */
+int stp_world_front(stp_heightfield_world* world, const uint16_t** d_biome, const uint16_t** d_height, uint32_t* side);
 /* ------------------------------------------------------------------ chunk records
  * The byte format in which chunk maps leave the device (files, sockets): the reference's map pixel
  * formats (STPWorldMapPixelFormat.hpp:12-21: biome u16, height f32 and u16, row-major per chunk as

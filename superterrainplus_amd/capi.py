@@ -166,7 +166,10 @@ class ServerStats(C.Structure):
     _fields_ = [("P50", C.c_double), ("P99", C.c_double), ("P999", C.c_double), ("Mean", C.c_double),
                 ("Max", C.c_double), ("MpixelPerSecond", C.c_double), ("DeadlineMissFraction", C.c_double),
                 ("Requests", C.c_uint64), ("CacheHits", C.c_uint64), ("Batches", C.c_uint64),
-                ("Evictions", C.c_uint64), ("MeanBatchChunks", C.c_double), ("PeakResidentBytes", C.c_uint64)]
+                ("Evictions", C.c_uint64), ("MeanBatchChunks", C.c_double), ("PeakResidentBytes", C.c_uint64),
+                ("Generated", C.c_uint64), ("GeneratedP50", C.c_double), ("GeneratedP99", C.c_double),
+                ("GeneratedP999", C.c_double), ("GeneratedMax", C.c_double),
+                ("GeneratedMpixelPerSecond", C.c_double)]
 
 
 class BatchTuning(C.Structure):
@@ -287,6 +290,9 @@ _SIGNATURES = {
     "stp_world_schedule": (_I, [_P, _U32, _U32, _P]),
     "stp_world_render_area": (_I, [_P, C.c_int32, C.c_int32, _U32, _P, _P]),
     "stp_world_stream_word": (_I, [C.c_int32, C.c_int32, _P]),
+    "stp_world_load": (_I, [_P, C.c_int32, C.c_int32, _U32, _P]),
+    "stp_world_front": (_I, [_P, _P, _P, _P]),
+    "stp_server_stream_word": (_I, [C.c_int32, C.c_int32, _U32, _P]),
     "stp_layer_program_demo": (_I, [_P, _U32, _P]),
     "stp_biome_registry_demo": (_I, [_P]),
     "stp_layer_factory_create": (_I, [_P, _U32, _U64, _P, _U32, _U32, _P]),

@@ -550,10 +550,49 @@ namespace SuperTerrainPlus::STPBatch {
 	// blocks, a fold of each line's summaries in order (segmentCarryKERNEL) and a resumed filter
 	// per segment give the unsplit line's result exactly.
 
-	// outputs [x, y) of segment `seg` when `outputs` outputs are split into `segs`
-	__device__ inline uint2 segmentRange(const unsigned int outputs, const unsigned int segs, const unsigned int seg) {
-		const unsigned int len = (outputs + segs - 1u) / segs, a = min(seg * len, outputs);
-		return make_uint2(a, min(a + len, outputs));
+	// Outputs [x, y) of segment `seg` of chunk c's lines, segments of at most SegLen outputs. A chunk
+	// with a free-slip halo (Crop > 0) splits its regions [0, Crop), [Crop, Side - Crop) and
+	// [Side - Crop, Side) apart, and each region's first segment starts from an empty state, as a
+	// line of its own (segmentRestarts): its centre's lines begin where the lines of the chunk
+	// generated alone begin, so the filter's bin order, and the blend, are the same there.
+	__device__ inline uint3 segmentRegions(const Chunk& c) {
+		const unsigned int len = c.SegLen;
+		return make_uint3(len, (c.Crop + len - 1u) / len, (c.Side - 2u * c.Crop + len - 1u) / len);
+	}
+	__device__ inline uint2 segmentRange(const Chunk& c, const unsigned int seg) {
+		const unsigned int len = c.SegLen;
+		if (c.Crop == 0u) {
+			const unsigned int a = min(seg * len, c.Side);
+			return make_uint2(a, min(a + len, c.Side));
+		}
+		const uint3 r = segmentRegions(c);
+		unsigned int base, end, k;
+		if (seg < r.y) {
+			base = 0u;
+			end = c.Crop;
+			k = seg;
+		} else if (seg < r.y + r.z) {
+			base = c.Crop;
+			end = c.Side - c.Crop;
+			k = seg - r.y;
+		} else {
+			base = c.Side - c.Crop;
+			end = c.Side;
+			k = seg - r.y - r.z;
+		}
+		const unsigned int a = min(base + k * len, end);
+		return make_uint2(a, min(a + len, end));
+	}
+	// a segment whose carried state is empty: every line's first, and each region's first in a halo chunk
+	__device__ inline bool segmentRestarts(const Chunk& c, const unsigned int seg) {
+		if (seg == 0u) {
+			return true;
+		}
+		if (c.Crop == 0u) {
+			return false;
+		}
+		const uint3 r = segmentRegions(c);
+		return seg == r.y || seg == r.y + r.z;
 	}
 
 	// word of (segment, field, biome, line) in a chunk's part of the segment-state arena; fields
@@ -634,6 +673,13 @@ namespace SuperTerrainPlus::STPBatch {
 			key[i] = NoRun;
 		}
 		for (unsigned int g = 0u; g < c.Segs; g++) {
+			if (segmentRestarts(c, g)) {
+#pragma unroll
+				for (unsigned int i = 0u; i < MaxB; i++) {
+					last[i] = -0x40000000;
+					key[i] = NoRun;
+				}
+			}
 #pragma unroll
 			for (unsigned int i = 0u; i < MaxB; i++) {
 				const size_t w0 = segmentWord<MaxB>(c, lines, g, 0u, i, line), w1 = segmentWord<MaxB>(c, lines, g, 1u, i, line);
@@ -669,7 +715,7 @@ namespace SuperTerrainPlus::STPBatch {
 		if (seg + 1u >= c.Segs) {
 			return;
 		}
-		const uint2 yr = segmentRange(side, c.Segs, seg);
+		const uint2 yr = segmentRange(c, seg);
 		const bool active = col < ncols;
 		const unsigned short* const colc = biome + c.Halo + min(col, ncols - 1u);
 		RunSummary<MaxB> sum;
@@ -704,7 +750,7 @@ namespace SuperTerrainPlus::STPBatch {
 		if (seg + 1u >= c.Segs) {
 			return;
 		}
-		const uint2 xr = segmentRange(side, c.Segs, seg);
+		const uint2 xr = segmentRange(c, seg);
 		const bool active = y < side;
 		const unsigned int yc = min(y, side - 1u);
 		const unsigned short* const inBin = vbin + c.Line * vcap;
@@ -754,7 +800,7 @@ namespace SuperTerrainPlus::STPBatch {
 		const unsigned int col = (t.y % nb) * ShfLineBlock + lane;
 		const bool active = col < ncols;
 		// this block's segment of the outputs; an unsplit line is the one segment [0, side)
-		const uint2 yr = segmentRange(side, c.Segs, seg);
+		const uint2 yr = segmentRange(c, seg);
 		const size_t lastRow = side + 2u * radius - 1u;
 		const unsigned short* const colc = biome + c.Halo + min(col, ncols - 1u);
 		const auto fetch = [&](unsigned int (&v)[VPrefetch], const long long row0) {
@@ -908,7 +954,7 @@ namespace SuperTerrainPlus::STPBatch {
 		const unsigned int y0 = (t.y % nb) * ShfLineBlock, y = y0 + lane;
 		const bool active = y < side;
 		// this block's segment of the outputs; an unsplit line is the one segment [0, side)
-		const uint2 xr = segmentRange(side, c.Segs, seg);
+		const uint2 xr = segmentRange(c, seg);
 		const int xBegin = static_cast<int>(xr.x), xEnd = static_cast<int>(xr.y);
 		const unsigned short* const inBin = vbin + c.Line * vcap;
 		const unsigned char* const inN = vcount + c.Line;
@@ -1106,12 +1152,10 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 	}
 
-	// the erosion arguments of chunk c: a map of Side / NN per chunk, merged NN x NN (Side), drops
-	// spawned in the centre map
+	// the erosion arguments of chunk c: drops spawned in its spawn map, a map of Side
 	__device__ inline ErosionArgs chunkArgs(ErosionArgs a, const Chunk& c) {
-		const unsigned int map = c.Side / c.NN;
-		a.MapSize = make_uint2(map, map);
-		a.NN = make_uint2(c.NN, c.NN);
+		a.MapSize = make_uint2(c.SpawnSide, c.SpawnSide);
+		a.SpawnOrigin = make_uint2(c.SpawnOrigin, c.SpawnOrigin);
 		a.Total = make_uint2(c.Side, c.Side);
 		a.StreamId = c.StreamId;
 		a.Rain.RainDropCount = c.Drops;
@@ -1401,16 +1445,20 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 	}
 
-	// chunk blockIdx.y's outputs -> out[chunk] (heights), out[chunks + chunk] (UINT16)
+	// chunk blockIdx.y's outputs -> out[chunk] (heights), out[chunks + chunk] (UINT16): its map
+	// without the Crop pixels of free-slip halo on every side, (Side - 2 Crop)^2 row-major
 	__global__ void __launch_bounds__(256) copyOutKERNEL(const Chunk* const chunks, const unsigned int count,
 		const float* const height, const unsigned short* const fixed, void* const* const out) {
 		const Chunk c = chunks[blockIdx.y];
 		float* const h = static_cast<float*>(out[blockIdx.y]);
 		unsigned short* const f = static_cast<unsigned short*>(out[count + blockIdx.y]);
-		const unsigned long long n = static_cast<unsigned long long>(c.Side) * c.Side;
+		const unsigned int side = c.Side - 2u * c.Crop;
+		const unsigned long long n = static_cast<unsigned long long>(side) * side;
 		for (unsigned long long i = blockIdx.x * 256ull + threadIdx.x; i < n; i += 256ull * gridDim.x) {
-			h[i] = height[c.Pixel + i];
-			f[i] = fixed[c.Pixel + i];
+			const unsigned long long y = i / side, x = i - y * side;
+			const unsigned long long from = c.Pixel + (y + c.Crop) * c.Side + x + c.Crop;
+			h[i] = height[from];
+			f[i] = fixed[from];
 		}
 	}
 
@@ -1589,7 +1637,9 @@ void STPHeightfieldBatch::init(const STPEnvironment::STPRainDropSetting& rain) {
 	this->VCount = STPSmartDeviceMemory::makeDevice<unsigned char>(line);
 	// slots of 3 words for each of at most 8 biomes: a few-line binding holds at most
 	// SegmentTarget slots, the long-line split at most sum (side / LongSegment + 1) (side + 2r)
-	this->SegCapacity = SegmentTarget + pixel / LongSegment + sideSum + sideSum * r2 / LongSegment + r2 * cap.Chunks + 2ull;
+	// (a chunk with a free-slip halo splits its lines into three regions: at most 2 more segments)
+	this->SegCapacity = SegmentTarget + pixel / LongSegment + sideSum + sideSum * r2 / LongSegment + r2 * cap.Chunks + 2ull
+		+ 2ull * (sideSum + r2 * cap.Chunks);
 	this->SegState = STPSmartDeviceMemory::makeDevice<unsigned int>(this->SegCapacity * 3ull * 8ull);
 	this->BinCount = STPSmartDeviceMemory::makeDevice<unsigned char>(pixel);
 	this->BinItem = STPSmartDeviceMemory::makeDevice<unsigned short>(pixel * binCap);
@@ -1656,13 +1706,22 @@ void STPHeightfieldBatch::bind(const std::vector<STPBatchChunk>& chunks, const s
 		if ((nn & 1u) == 0u || in.Side % nn != 0u || in.Side / nn < 2u * this->Rain.ErosionBrushRadius + 2u) {
 			STP_THROW(STPInvalidArgument, "a neighbourhood is odd, divides the side and keeps maps of at least 2r + 2");
 		}
+		const unsigned int slip = in.ErosionHalo;
+		if (slip != 0u && (nn != 1u || in.Side <= 2u * slip
+			|| in.Side - 2u * slip < 2u * this->Rain.ErosionBrushRadius + 2u)) {
+			STP_THROW(STPInvalidArgument, "a free-slip halo needs Neighbour 1 and a centre map of at least 2r + 2");
+		}
 		Chunk c;
 		c.Side = in.Side;
 		c.OriginX = in.OriginX;
 		c.OriginY = in.OriginY;
-		c.NN = nn;
+		// a neighbourhood spawns in its centre map (STPHeightfieldKernel.cu:93-102), a halo chunk in
+		// its centre, a plain chunk everywhere
+		c.SpawnSide = slip != 0u ? in.Side - 2u * slip : in.Side / nn;
+		c.SpawnOrigin = slip != 0u ? slip : (nn / 2u) * c.SpawnSide;
+		c.Crop = slip;
 		const unsigned long long px = static_cast<unsigned long long>(in.Side) * in.Side,
-			spawnPx = static_cast<unsigned long long>(in.Side / nn) * (in.Side / nn);
+			spawnPx = static_cast<unsigned long long>(c.SpawnSide) * c.SpawnSide;
 		c.Drops = static_cast<unsigned int>(static_cast<double>(spawnPx) * this->Setting.DropsPerPixel);
 		c.Pixel = pixel;
 		c.Halo = halo;
@@ -1696,6 +1755,16 @@ void STPHeightfieldBatch::bind(const std::vector<STPBatchChunk>& chunks, const s
 	for (Chunk& c : rec) {
 		const unsigned int few = factor > 1u ? std::max(1u, std::min(factor, c.Side / MinSegment)) : 1u;
 		c.Segs = this->itemPath() ? std::max(few, (c.Side + LongSegment - 1u) / LongSegment) : 1u;
+		c.SegLen = (c.Side + c.Segs - 1u) / c.Segs;
+		if (c.Crop != 0u) {
+			// halo, centre and halo split apart (segmentRange)
+			if (!this->itemPath()) {
+				STP_THROW(STPInvalidArgument, "a free-slip halo needs the filter's item path (at most 8 biomes, a bin "
+					"capacity of at most 8, a radius of at most 127)");
+			}
+			const unsigned int len = c.SegLen;
+			c.Segs = 2u * ((c.Crop + len - 1u) / len) + (c.Side - 2u * c.Crop + len - 1u) / len;
+		}
 		c.SegLine = slots;
 		if (c.Segs > 1u) {
 			slots += static_cast<unsigned long long>(c.Segs) * (c.Side + 2u * radius);
@@ -1823,7 +1892,7 @@ ErosionArgs STPHeightfieldBatch::erosionArgs() const {
 	ErosionArgs args;
 	args.Rain = this->Rain;
 	args.MapSize = make_uint2(0u, 0u);
-	args.NN = make_uint2(1u, 1u);
+	args.SpawnOrigin = make_uint2(0u, 0u);
 	args.Total = args.MapSize;
 	args.BrushSize = this->BrushSize;
 	args.StreamId = 0u;

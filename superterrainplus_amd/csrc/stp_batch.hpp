@@ -32,6 +32,14 @@ namespace SuperTerrainPlus {
 		// of side Side / Neighbour, and its drops spawn in the centre one, as the reference's
 		// erosion of a neighbourhood (STPHeightfieldKernel.cu:93-102). 1 = a chunk on its own.
 		unsigned int Neighbour = 1u;
+		// Free-slip halo (Neighbour 1 only): the chunk is a centre map of side Side - 2 ErosionHalo
+		// with ErosionHalo pixels of its neighbours' terrain on every side, and its drops spawn in
+		// the centre. With ErosionHalo >= MaxLifetime + max(r, 2) + 1 no drop reaches the halo's
+		// border, so the centre erodes as the centre of a neighbourhood of uneroded chunks would.
+		// copyOut copies the centre map. The filter lines restart at the centre's edges, so before
+		// erosion the centre is bit-exact with the chunk generated alone (needs the filter's item
+		// path: at most 8 biomes and a bin capacity of at most 8). 0 = no halo.
+		unsigned int ErosionHalo = 0u;
 	};
 
 	// Arena sizes of a batch that is bound to different chunk lists over its lifetime.
@@ -65,8 +73,11 @@ namespace SuperTerrainPlus {
 			unsigned int StreamId;     // Philox counter word of this chunk's drops
 			unsigned int TileBase;     // first erosion tile of the chunk in the tile-bin arenas
 			unsigned int Step;         // world units per pixel
-			unsigned int NN;           // free-slip neighbourhood: Side = NN * map side, drops spawn in the centre map
+			unsigned int SpawnSide;    // drops spawn in the square [SpawnOrigin, SpawnOrigin + SpawnSide)^2
+			unsigned int SpawnOrigin;  //   (the centre map of a neighbourhood, the centre of a halo chunk)
+			unsigned int Crop;         // pixels copyOut trims on every side (the free-slip halo)
 			unsigned int Segs;         // filter lines are split into Segs segments of outputs
+			unsigned int SegLen;       //   of at most SegLen outputs (a halo chunk splits halo, centre, halo apart)
 			unsigned long long SegLine; // first slot of the chunk in the segment-state arena (Segs * (Side + 2r) slots)
 		};
 	}

@@ -201,7 +201,7 @@ struct stp_heightfield_world {
 
 struct stp_heightfield_server {
 	std::unique_ptr<STPHeightfieldServer> Server;
-	std::vector<std::pair<unsigned long long, double>> Done; // completed, not yet returned by poll
+	std::vector<STPServerCompletion> Done; // completed, not yet returned by poll
 };
 
 struct stp_heightfield_batch {
@@ -938,8 +938,8 @@ int stp_server_poll(stp_heightfield_server* const server, uint64_t* const h_tick
 		}
 		const uint32_t n = static_cast<uint32_t>(std::min<size_t>(capacity, server->Done.size()));
 		for (uint32_t i = 0u; i < n; i++) {
-			h_ticket[i] = server->Done[i].first;
-			h_latency_ms[i] = server->Done[i].second;
+			h_ticket[i] = server->Done[i].Ticket;
+			h_latency_ms[i] = server->Done[i].Ms;
 		}
 		server->Done.erase(server->Done.begin(), server->Done.begin() + n);
 		*count = n;
@@ -1004,6 +1004,12 @@ int stp_server_replay(stp_heightfield_server* const server, const stp_chunk_requ
 			out->Evictions = st.Evictions;
 			out->MeanBatchChunks = st.MeanBatchChunks;
 			out->PeakResidentBytes = st.PeakResidentBytes;
+			out->Generated = st.Generated;
+			out->GeneratedP50 = st.GeneratedP50;
+			out->GeneratedP99 = st.GeneratedP99;
+			out->GeneratedP999 = st.GeneratedP999;
+			out->GeneratedMax = st.GeneratedMax;
+			out->GeneratedMpixelPerSecond = st.GeneratedMpixelPerSecond;
 		}
 	});
 }
@@ -1111,6 +1117,22 @@ int stp_world_render_area(stp_heightfield_world* const world, const int32_t x, c
 	});
 }
 
+int stp_world_load(stp_heightfield_world* const world, const int32_t x, const int32_t y, const uint32_t radius,
+	uint32_t* const status) {
+	return guarded([&] {
+		require(world && status, "null argument");
+		*status = static_cast<uint32_t>(world->World->load(make_int2(x, y), radius));
+	});
+}
+
+int stp_world_front(stp_heightfield_world* const world, const uint16_t** const d_biome, const uint16_t** const d_height,
+	uint32_t* const side) {
+	return guarded([&] {
+		require(world, "null world");
+		world->World->front(d_biome, d_height, side);
+	});
+}
+
 int stp_world_schedule(const int32_t* const h_centres, const uint32_t count, const uint32_t neighbour,
 	uint32_t* const h_batch) {
 	return guarded([&] {
@@ -1128,6 +1150,13 @@ int stp_world_stream_word(const int32_t x, const int32_t y, uint32_t* const word
 	return guarded([&] {
 		require(word, "null argument");
 		*word = STPHeightfieldWorld::streamWord(make_int2(x, y));
+	});
+}
+
+int stp_server_stream_word(const int32_t x, const int32_t y, const uint32_t side, uint32_t* const word) {
+	return guarded([&] {
+		require(word, "null argument");
+		*word = STPHeightfieldServer::streamWord(x, y, side);
 	});
 }
 

@@ -153,7 +153,7 @@ void STPHeightfieldKernel::hydraulicErosion(float* const height, const STPNeares
 	ErosionArgs a;
 	a.Rain = rd;
 	a.MapSize = nn.MapSize;
-	a.NN = nn.ChunkNearestNeighbour;
+	a.SpawnOrigin = make_uint2((nn.ChunkNearestNeighbour.x / 2u) * nn.MapSize.x, (nn.ChunkNearestNeighbour.y / 2u) * nn.MapSize.y);
 	a.Total = nn.TotalMapSize;
 	a.BrushSize = brush_size;
 	a.StreamId = stream_id;

@@ -25,18 +25,17 @@ size_t STPHeightfieldServer::KeyHash::operator()(const Key& k) const noexcept {
 	return static_cast<size_t>(h);
 }
 
-namespace {
-	// Philox stream word of a chunk: a hash of (X, Y, Side), independent of the batch it runs in.
-	unsigned int streamWord(const int x, const int y, const unsigned int side) {
-		unsigned int h = static_cast<unsigned int>(x) * 0x85EBCA6Bu;
-		h ^= static_cast<unsigned int>(y) * 0xC2B2AE35u + 0x27D4EB2Fu + (h << 6) + (h >> 2);
-		h ^= side * 0x165667B1u + (h << 6) + (h >> 2);
-		h ^= h >> 16;
-		h *= 0x7FEB352Du;
-		h ^= h >> 15;
-		return h;
-	}
+unsigned int STPHeightfieldServer::streamWord(const int x, const int y, const unsigned int side) noexcept {
+	unsigned int h = static_cast<unsigned int>(x) * 0x85EBCA6Bu;
+	h ^= static_cast<unsigned int>(y) * 0xC2B2AE35u + 0x27D4EB2Fu + (h << 6) + (h >> 2);
+	h ^= side * 0x165667B1u + (h << 6) + (h >> 2);
+	h ^= h >> 16;
+	h *= 0x7FEB352Du;
+	h ^= h >> 15;
+	return h;
+}
 
+namespace {
 	bool powerOfTwo(const unsigned int v) { return v != 0u && (v & (v - 1u)) == 0u; }
 } // namespace
 
@@ -62,11 +61,20 @@ STPHeightfieldServer::STPHeightfieldServer(const STPEnvironment::STPSimplexNoise
 	if (batch.BinCapacity < batch.BiomeCount) {
 		STP_THROW(STPInvalidArgument, "the server needs BinCapacity >= BiomeCount");
 	}
+	// every chunk is produced with a free-slip halo the drops cannot cross within their lifetime
+	// (no halo without a lifetime bound)
+	this->ErosionHalo = rain.MaxLifetime > 0u ? rain.MaxLifetime + std::max(rain.ErosionBrushRadius, 2u) + 1u : 0u;
+	if (static_cast<unsigned long long>(this->ErosionHalo) * setting.ChunkWorldSize >= (1ull << 30)) {
+		STP_THROW(STPInvalidArgument, "the free-slip halo (MaxLifetime + max(r, 2) + 1 pixels) spans more than 2^30 "
+			"world units at the coarsest level of detail");
+	}
+	const unsigned long long largest = static_cast<unsigned long long>(setting.MaxSide + 2u * this->ErosionHalo)
+		* (setting.MaxSide + 2u * this->ErosionHalo);
 	STPBatchCapacity cap;
-	cap.Pixels = setting.BatchPixels + 4ull * setting.BatchChunks;
+	cap.Pixels = std::max(setting.BatchPixels, largest) + 4ull * setting.BatchChunks;
 	cap.Drops = static_cast<unsigned long long>(static_cast<double>(setting.BatchPixels) * batch.DropsPerPixel) + setting.BatchChunks;
 	cap.Chunks = setting.BatchChunks;
-	cap.MaxSide = setting.MaxSide;
+	cap.MaxSide = setting.MaxSide + 2u * this->ErosionHalo;
 
 	size_t freeBefore = 0u, freeAfter = 0u, total = 0u;
 	STP_CHECK_HIP(hipMemGetInfo(&freeBefore, &total));
@@ -140,8 +148,9 @@ unsigned long long STPHeightfieldServer::submit(const STPChunkRequest& request) 
 }
 
 void STPHeightfieldServer::complete(Entry& e, const Clock::time_point now) {
-	for (const auto& [ticket, arrival] : e.Waiting) {
-		this->Completed.emplace_back(ticket, std::chrono::duration<double, std::milli>(now - arrival).count());
+	for (const Entry::Waiter& w : e.Waiting) {
+		this->Completed.push_back(
+			STPServerCompletion {w.Ticket, std::chrono::duration<double, std::milli>(now - w.Arrival).count(), w.Generated});
 	}
 	e.Waiting.clear();
 }
@@ -201,7 +210,7 @@ void STPHeightfieldServer::launch() {
 		}
 		Entry& e = found->second;
 		this->Lru.splice(this->Lru.begin(), this->Lru, e.Lru);
-		e.Waiting.emplace_back(it->Ticket, it->Arrival);
+		e.Waiting.push_back(Entry::Waiter {it->Ticket, it->Arrival, false});
 		if (e.Ready) {
 			this->Hits++;
 			this->complete(e, now);
@@ -224,12 +233,16 @@ void STPHeightfieldServer::launch() {
 			const auto found = this->Cache.find(p.K);
 			if (found != this->Cache.end()) {
 				// the same chunk asked twice in this batch
-				found->second.Waiting.emplace_back(p.Ticket, p.Arrival);
+				found->second.Waiting.push_back(Entry::Waiter {p.Ticket, p.Arrival, false});
 				this->Queue.pop_front();
 				continue;
 			}
-			const unsigned long long px = static_cast<unsigned long long>(p.K.Side) * p.K.Side;
-			const unsigned long long d = static_cast<unsigned long long>(static_cast<double>(px) * this->BatchSetting.DropsPerPixel);
+			// the batch produces the chunk with its halo (`produced` pixels per side); the resident
+			// entry holds the chunk alone (`centre` pixels), as bytesOf lays it out
+			const unsigned int halo = this->ErosionHalo, produced = p.K.Side + 2u * halo;
+			const unsigned long long px = static_cast<unsigned long long>(produced) * produced,
+				centre = static_cast<unsigned long long>(p.K.Side) * p.K.Side;
+			const unsigned long long d = static_cast<unsigned long long>(static_cast<double>(centre) * this->BatchSetting.DropsPerPixel);
 			if (pixels + ((px + 3ull) & ~3ull) > cap.Pixels || drops + d > cap.Drops) {
 				break;
 			}
@@ -242,9 +255,9 @@ void STPHeightfieldServer::launch() {
 			Entry e;
 			e.Height = static_cast<float*>(mem);
 			e.Fixed = reinterpret_cast<unsigned short*>(static_cast<unsigned char*>(mem)
-				+ ((px * sizeof(float) + 255ull) & ~255ull));
+				+ ((centre * sizeof(float) + 255ull) & ~255ull));
 			e.Bytes = bytes;
-			e.Waiting.emplace_back(p.Ticket, p.Arrival);
+			e.Waiting.push_back(Entry::Waiter {p.Ticket, p.Arrival, true});
 			this->Lru.push_front(p.K);
 			e.Lru = this->Lru.begin();
 			this->Resident += bytes;
@@ -254,9 +267,14 @@ void STPHeightfieldServer::launch() {
 			this->Cache.emplace(p.K, std::move(e));
 
 			const unsigned int step = this->Setting.ChunkWorldSize / p.K.Side;
-			// submit() keeps |X|, |Y| below 2^30 / ChunkWorldSize, so the origins fit in 32 bits
-			const long long world = static_cast<long long>(this->Setting.ChunkWorldSize);
-			chunks.push_back(STPBatchChunk {p.K.Side, static_cast<int>(p.K.X * world), static_cast<int>(p.K.Y * world), step});
+			// submit() keeps |X|, |Y| below 2^30 / ChunkWorldSize and the constructor the halo's world
+			// units (halo * step <= halo * ChunkWorldSize) below 2^30, so the origins fit in 32 bits
+			const long long world = static_cast<long long>(this->Setting.ChunkWorldSize),
+				back = static_cast<long long>(halo) * step;
+			// the chunk with its halo: the centre is bit-exact with the chunk alone before erosion
+			// (same sample coordinates), and erodes as the centre of its uneroded neighbours
+			chunks.push_back(STPBatchChunk {produced, static_cast<int>(p.K.X * world - back),
+				static_cast<int>(p.K.Y * world - back), step, 1u, halo});
 			streamId.push_back(streamWord(p.K.X, p.K.Y, p.K.Side));
 			engine.Chunks.push_back(p.K);
 			pixels += (px + 3ull) & ~3ull;
@@ -281,8 +299,8 @@ void STPHeightfieldServer::poll() {
 	this->launch();
 }
 
-std::vector<std::pair<unsigned long long, double>> STPHeightfieldServer::drain() {
-	std::vector<std::pair<unsigned long long, double>> out;
+std::vector<STPServerCompletion> STPHeightfieldServer::drain() {
+	std::vector<STPServerCompletion> out;
 	out.swap(this->Completed);
 	return out;
 }
@@ -328,13 +346,14 @@ STPServerStats STPHeightfieldServer::replay(const std::vector<STPChunkRequest>& 
 	}
 	const size_t n = request.size();
 	latency_ms.assign(n, -1.0);
+	std::vector<char> generated(n, 0);
 	std::unordered_map<unsigned long long, size_t> index;
 	const unsigned long long hits0 = this->Hits, batches0 = this->Batches, chunks0 = this->BatchedChunks,
 		evict0 = this->Evictions;
 	this->PeakResident = this->Resident;
 	const Clock::time_point t0 = Clock::now();
 	size_t next = 0u, done = 0u;
-	unsigned long long pixels = 0ull;
+	unsigned long long pixels = 0ull, generatedPixels = 0ull;
 	Clock::time_point last = t0;
 	while (done < n) {
 		const Clock::time_point now = Clock::now();
@@ -349,13 +368,18 @@ STPServerStats STPHeightfieldServer::replay(const std::vector<STPChunkRequest>& 
 			next++;
 		}
 		this->poll();
-		for (const auto& [ticket, ms] : this->drain()) {
-			const auto it = index.find(ticket);
+		for (const STPServerCompletion& c : this->drain()) {
+			const auto it = index.find(c.Ticket);
 			if (it == index.end()) {
 				continue; // submitted outside this replay
 			}
-			latency_ms[it->second] = ms;
-			pixels += static_cast<unsigned long long>(request[it->second].Side) * request[it->second].Side;
+			latency_ms[it->second] = c.Ms;
+			const unsigned long long px = static_cast<unsigned long long>(request[it->second].Side) * request[it->second].Side;
+			pixels += px;
+			if (c.Generated) {
+				generated[it->second] = 1;
+				generatedPixels += px;
+			}
 			done++;
 			last = Clock::now();
 		}
@@ -364,18 +388,32 @@ STPServerStats STPHeightfieldServer::replay(const std::vector<STPChunkRequest>& 
 		}
 	}
 	STPServerStats s {};
-	std::vector<double> sorted(latency_ms);
+	std::vector<double> sorted(latency_ms), sortedGenerated;
+	for (size_t i = 0u; i < n; i++) {
+		if (generated[i]) {
+			sortedGenerated.push_back(latency_ms[i]);
+		}
+	}
 	std::sort(sorted.begin(), sorted.end());
-	const auto pct = [&sorted](const double q) {
-		const double pos = q * static_cast<double>(sorted.size() - 1u);
+	std::sort(sortedGenerated.begin(), sortedGenerated.end());
+	// linear interpolation between closest ranks
+	const auto pct = [](const std::vector<double>& v, const double q) {
+		const double pos = q * static_cast<double>(v.size() - 1u);
 		const size_t lo = static_cast<size_t>(pos);
-		const size_t hi = std::min(lo + 1u, sorted.size() - 1u);
-		return sorted[lo] + (sorted[hi] - sorted[lo]) * (pos - static_cast<double>(lo));
+		const size_t hi = std::min(lo + 1u, v.size() - 1u);
+		return v[lo] + (v[hi] - v[lo]) * (pos - static_cast<double>(lo));
 	};
+	if (!sortedGenerated.empty()) {
+		s.GeneratedP50 = pct(sortedGenerated, 0.5);
+		s.GeneratedP99 = pct(sortedGenerated, 0.99);
+		s.GeneratedP999 = pct(sortedGenerated, 0.999);
+		s.GeneratedMax = sortedGenerated.back();
+	}
+	s.Generated = sortedGenerated.size();
 	if (!sorted.empty()) {
-		s.P50 = pct(0.5);
-		s.P99 = pct(0.99);
-		s.P999 = pct(0.999);
+		s.P50 = pct(sorted, 0.5);
+		s.P99 = pct(sorted, 0.99);
+		s.P999 = pct(sorted, 0.999);
 		s.Max = sorted.back();
 		double sum = 0.0;
 		size_t miss = 0u;
@@ -388,6 +426,7 @@ STPServerStats STPHeightfieldServer::replay(const std::vector<STPChunkRequest>& 
 	}
 	const double wall = std::chrono::duration<double>(last - t0).count();
 	s.MpixelPerSecond = wall > 0.0 ? static_cast<double>(pixels) / wall / 1e6 : 0.0;
+	s.GeneratedMpixelPerSecond = wall > 0.0 ? static_cast<double>(generatedPixels) / wall / 1e6 : 0.0;
 	s.Requests = n;
 	s.CacheHits = this->Hits - hits0;
 	s.Batches = this->Batches - batches0;

@@ -44,6 +44,17 @@ namespace SuperTerrainPlus {
 		unsigned long long Requests, CacheHits, Batches, Evictions;
 		double MeanBatchChunks;
 		unsigned long long PeakResidentBytes;
+		// the requests whose chunk was generated for them (not resident, not already in production)
+		unsigned long long Generated;
+		double GeneratedP50, GeneratedP99, GeneratedP999, GeneratedMax; // their latency, milliseconds
+		double GeneratedMpixelPerSecond;  // generated pixels / wall time of the replay
+	};
+
+	// A completed request: its ticket, latency and whether its chunk was generated for it.
+	struct STPServerCompletion {
+		unsigned long long Ticket;
+		double Ms;
+		bool Generated;
 	};
 
 	class STPHeightfieldServer {
@@ -62,7 +73,7 @@ namespace SuperTerrainPlus {
 		// on idle engines.
 		void poll();
 		// Tickets completed since the last call, with their latency in milliseconds.
-		std::vector<std::pair<unsigned long long, double>> drain();
+		std::vector<STPServerCompletion> drain();
 		// Copies a resident chunk to host memory (either pointer may be null); false if the chunk is
 		// not resident. Synchronous.
 		bool read(const STPChunkRequest& request, float* height, unsigned short* fixed);
@@ -72,6 +83,10 @@ namespace SuperTerrainPlus {
 		// latency_ms[i] receives each request's latency.
 		STPServerStats replay(const std::vector<STPChunkRequest>& request, const std::vector<double>& arrival_s,
 			double deadline_ms, std::vector<double>& latency_ms);
+
+		// Philox stream word of a chunk's drops: a hash of (X, Y, Side), independent of the batch it
+		// runs in.
+		static unsigned int streamWord(int x, int y, unsigned int side) noexcept;
 
 		unsigned long long residentBytes() const noexcept { return this->Resident; }
 		unsigned long long engineBytes() const noexcept { return this->EngineBytes; }
@@ -92,7 +107,12 @@ namespace SuperTerrainPlus {
 			unsigned long long Bytes = 0ull;
 			bool Ready = false;
 			std::list<Key>::iterator Lru;
-			std::vector<std::pair<unsigned long long, Clock::time_point>> Waiting; // tickets, arrival
+			struct Waiter {
+				unsigned long long Ticket;
+				Clock::time_point Arrival;
+				bool Generated; // the request this chunk is produced for
+			};
+			std::vector<Waiter> Waiting;
 		};
 		struct Engine {
 			std::unique_ptr<STPHeightfieldBatch> Batch;
@@ -113,10 +133,13 @@ namespace SuperTerrainPlus {
 		std::unordered_map<Key, Entry, KeyHash> Cache;
 		std::list<Key> Lru; // front = most recently used
 		std::deque<Pending> Queue;
-		std::vector<std::pair<unsigned long long, double>> Completed;
+		std::vector<STPServerCompletion> Completed;
 		hipMemPool_t Pool = nullptr;
 		unsigned long long NextTicket = 0ull, Resident = 0ull, PeakResident = 0ull, EngineBytes = 0ull;
 		unsigned long long Hits = 0ull, Batches = 0ull, Evictions = 0ull, BatchedChunks = 0ull;
+		// pixels of free-slip halo every chunk is produced with (MaxLifetime + max(r, 2) + 1; 0 without
+		// a lifetime bound)
+		unsigned int ErosionHalo = 0u;
 
 		static unsigned long long bytesOf(unsigned int side) noexcept;
 		void complete(Entry& e, Clock::time_point now);

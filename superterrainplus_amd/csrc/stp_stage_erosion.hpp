@@ -16,9 +16,9 @@ namespace SuperTerrainPlus::STPStage {
 
 	struct ErosionArgs {
 		STPRainDropParameter Rain;
-		uint2 MapSize;  // one chunk
-		uint2 NN;       // neighbourhood in chunks
-		uint2 Total;    // merged map = MapSize * NN
+		uint2 MapSize;     // the map drops spawn in (one chunk)
+		uint2 SpawnOrigin; // its first cell in the merged map: (NN / 2) * MapSize for a neighbourhood
+		uint2 Total;       // merged map
 		unsigned int BrushSize;
 		unsigned int StreamId;
 	};
@@ -57,14 +57,13 @@ namespace SuperTerrainPlus::STPStage {
 	};
 
 	__device__ inline DropState spawn(const ErosionArgs& a, const unsigned int drop) {
-		// STPHeightfieldKernel.cu:93-102: centre chunk only, u * (Map - 1) + (NN / 2) * Map
+		// STPHeightfieldKernel.cu:93-102: centre chunk only, u * (Map - 1) + (NN / 2) * Map, the
+		// second term being the spawn map's origin
 		const STPRandom::Philox4 r = STPRandom::philox4x32_10(drop, a.StreamId, 0u, 0u,
 			static_cast<unsigned int>(a.Rain.Seed), static_cast<unsigned int>(a.Rain.Seed >> 32));
 		DropState d;
-		d.px = STPRandom::unitUniform(r.v[0]) * static_cast<float>(a.MapSize.x - 1u)
-			+ static_cast<float>((a.NN.x / 2u) * a.MapSize.x);
-		d.py = STPRandom::unitUniform(r.v[1]) * static_cast<float>(a.MapSize.y - 1u)
-			+ static_cast<float>((a.NN.y / 2u) * a.MapSize.y);
+		d.px = STPRandom::unitUniform(r.v[0]) * static_cast<float>(a.MapSize.x - 1u) + static_cast<float>(a.SpawnOrigin.x);
+		d.py = STPRandom::unitUniform(r.v[1]) * static_cast<float>(a.MapSize.y - 1u) + static_cast<float>(a.SpawnOrigin.y);
 		d.dx = 0.0f;
 		d.dy = 0.0f;
 		d.speed = a.Rain.initSpeed;

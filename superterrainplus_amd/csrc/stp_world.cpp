@@ -133,6 +133,7 @@ STPHeightfieldWorld::STPHeightfieldWorld(const STPEnvironment::STPSimplexNoiseSe
 	STP_CHECK_HIP(hipStreamCreateWithFlags(&this->Stream, hipStreamNonBlocking));
 	int device = 0;
 	STP_CHECK_HIP(hipGetDevice(&device));
+	this->Device = device;
 	hipMemPoolProps props {};
 	props.allocType = hipMemAllocationTypePinned;
 	props.location.type = hipMemLocationTypeDevice;
@@ -141,6 +142,9 @@ STPHeightfieldWorld::STPHeightfieldWorld(const STPEnvironment::STPSimplexNoiseSe
 }
 
 STPHeightfieldWorld::~STPHeightfieldWorld() {
+	if (this->Loader.joinable()) {
+		this->Loader.join();
+	}
 	if (this->Stream) {
 		(void)hipStreamSynchronize(this->Stream);
 		for (auto& [k, e] : this->Chunk) {
@@ -155,6 +159,7 @@ STPHeightfieldWorld::~STPHeightfieldWorld() {
 }
 
 STPHeightfieldWorld::STPChunkState STPHeightfieldWorld::state(const int2 chunk) const {
+	const std::lock_guard<std::recursive_mutex> guard(this->Lock);
 	const auto it = this->Chunk.find(Key {chunk.x, chunk.y});
 	return it == this->Chunk.end() ? STPChunkState::None : it->second.State;
 }
@@ -234,6 +239,7 @@ void STPHeightfieldWorld::erode(const std::vector<Key>& centres) {
 }
 
 std::vector<int2> STPHeightfieldWorld::request(const std::vector<int2>& chunks) {
+	const std::lock_guard<std::recursive_mutex> guard(this->Lock);
 	const int h = static_cast<int>((this->Setting.ErosionNeighbour - 1u) / 2u);
 	// every chunk of a neighbourhood (and its filter halo) keeps its world pixels within +-2^30, so
 	// the 32-bit world origins of generate() and erode() cannot overflow
@@ -300,6 +306,7 @@ std::vector<int2> STPHeightfieldWorld::request(const std::vector<int2>& chunks) 
 }
 
 bool STPHeightfieldWorld::read(const int2 chunk, float* const height, unsigned short* const fixed) {
+	const std::lock_guard<std::recursive_mutex> guard(this->Lock);
 	const auto it = this->Chunk.find(Key {chunk.x, chunk.y});
 	if (it == this->Chunk.end()) {
 		return false;
@@ -317,6 +324,7 @@ bool STPHeightfieldWorld::read(const int2 chunk, float* const height, unsigned s
 
 void STPHeightfieldWorld::renderArea(const int2 centre, const unsigned int radius, unsigned short* const biome,
 	unsigned short* const height) {
+	const std::lock_guard<std::recursive_mutex> guard(this->Lock);
 	if (!biome || !height) {
 		STP_THROW(STPInvalidArgument, "null area buffer");
 	}
@@ -358,8 +366,87 @@ void STPHeightfieldWorld::renderArea(const int2 centre, const unsigned int radiu
 }
 
 
+void STPHeightfieldWorld::joinLoader() {
+	if (this->Loader.joinable()) {
+		this->Loader.join();
+	}
+	if (this->LoaderError) {
+		std::exception_ptr e = this->LoaderError;
+		this->LoaderError = nullptr;
+		std::rethrow_exception(e);
+	}
+}
+
+STPHeightfieldWorld::STPWorldLoadStatus STPHeightfieldWorld::load(const int2 centre, const unsigned int radius) {
+	if (this->LoaderState.load() == 1u) {
+		// STPWorldPipeline.cpp:1510-1514: the loader works on the back buffer, the request is ignored
+		return STPWorldLoadStatus::BackBufferBusy;
+	}
+	const std::lock_guard<std::recursive_mutex> guard(this->Lock);
+	if (this->Pending && centre.x == this->PendingCentre.x && centre.y == this->PendingCentre.y
+		&& radius == this->PendingRadius) {
+		// :1529-1567: the same centre; a finished load is swapped in
+		if (this->LoaderState.load() == 2u) {
+			this->LoaderState = 0u;
+			this->joinLoader();
+			std::swap(this->Front, this->Back);
+			return STPWorldLoadStatus::Swapped;
+		}
+		return STPWorldLoadStatus::Unchanged;
+	}
+	// :1519-1528, 1618-1633: a changed centre; a finished load that was not swapped is discarded
+	if (this->LoaderState.load() == 2u) {
+		this->LoaderState = 0u;
+		this->joinLoader();
+	}
+	const unsigned int map = this->Setting.MapSize, n = 2u * radius + 1u;
+	if (radius > 64u || static_cast<unsigned long long>(n) * map > 65536ull) {
+		STP_THROW(STPInvalidArgument, "the rendered area is at most 129 chunks and 65536 pixels wide");
+	}
+	const unsigned int side = n * map;
+	if (this->Back.Side != side) {
+		const size_t px = static_cast<size_t>(side) * side;
+		this->Back.Biome = STPSmartDeviceMemory::makeDevice<unsigned short>(px);
+		this->Back.Height = STPSmartDeviceMemory::makeDevice<unsigned short>(px);
+		this->Back.Side = side;
+	}
+	this->Pending = true;
+	this->PendingCentre = centre;
+	this->PendingRadius = radius;
+	this->LoaderState = 1u;
+	// the worker takes the lock once this call has released it, and holds it for the whole load
+	this->Loader = std::thread([this, centre, radius]() {
+		{
+			const std::lock_guard<std::recursive_mutex> hold(this->Lock);
+			try {
+				STP_CHECK_HIP(hipSetDevice(this->Device));
+				this->renderArea(centre, radius, this->Back.Biome.get(), this->Back.Height.get());
+			} catch (...) {
+				this->LoaderError = std::current_exception();
+			}
+		}
+		this->LoaderState = 2u;
+	});
+	return STPWorldLoadStatus::BackBufferBusy;
+}
+
+void STPHeightfieldWorld::front(const unsigned short** const biome, const unsigned short** const height,
+	unsigned int* const side) const {
+	const std::lock_guard<std::recursive_mutex> guard(this->Lock);
+	if (biome) {
+		*biome = this->Front.Biome.get();
+	}
+	if (height) {
+		*height = this->Front.Height.get();
+	}
+	if (side) {
+		*side = this->Front.Side;
+	}
+}
+
 unsigned long long STPHeightfieldWorld::exportRecords(const std::vector<int2>& chunks, const unsigned int maps,
 	unsigned char* const out, const unsigned long long capacity) {
+	const std::lock_guard<std::recursive_mutex> guard(this->Lock);
 	namespace R = STPChunkRecord;
 	if (maps == 0u || (maps & ~R::MapAll) != 0u) {
 		STP_THROW(STPInvalidArgument, "maps: a non-empty combination of biome (1), FP32 height (2), UINT16 height (4)");

@@ -15,7 +15,11 @@
 
 #include "stp_batch.hpp"
 
+#include <atomic>
+#include <exception>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -58,6 +62,20 @@ namespace SuperTerrainPlus {
 		unsigned long long exportRecords(const std::vector<int2>& chunks, unsigned int maps, unsigned char* out,
 			unsigned long long capacity);
 
+		// STPWorldPipeline::load (STPWorldPipeline.cpp:1504-1633): the rendered area around `centre`
+		// ((2 radius + 1)^2 chunks, made complete) is assembled on a worker thread into the back
+		// buffer while the caller renders from the front buffer. BackBufferBusy: the worker is still
+		// loading (this call is ignored), or a changed centre's load has just started (a finished
+		// load that was not swapped is discarded); Swapped: the load of this centre finished and the
+		// buffers were swapped; Unchanged: the same centre and nothing finished since. Every other
+		// member waits for a running load.
+		enum class STPWorldLoadStatus : unsigned int { Unchanged = 0u, BackBufferBusy = 1u, Swapped = 2u };
+		STPWorldLoadStatus load(int2 centre, unsigned int radius);
+		// The front buffer: device biome and UINT16 height maps of the last swapped load, `side` =
+		// (2 radius + 1) MapSize pixels a side, row-major (null and 0 before the first swap); valid
+		// until the next Swapped load.
+		void front(const unsigned short** biome, const unsigned short** height, unsigned int* side) const;
+
 		// Conflict-free schedule: batch number of each centre, first fit in the given order; two
 		// centres share a batch only if their neighbourhoods (nn x nn chunks) are disjoint.
 		static std::vector<unsigned int> schedule(const std::vector<int2>& centres, unsigned int nn);
@@ -93,8 +111,25 @@ namespace SuperTerrainPlus {
 		size_t ExportTableCapacity = 0u;
 		std::unique_ptr<STPDiversity::STPLayerPlan> ExportPlan;
 
+		// the asynchronous loader: the worker holds Lock for a whole load, every member takes it
+		struct STPAreaBuffer {
+			STPSmartDeviceMemory::STPDevice<unsigned short> Biome, Height;
+			unsigned int Side = 0u;
+		};
+		mutable std::recursive_mutex Lock;
+		std::thread Loader;
+		std::atomic<unsigned int> LoaderState {0u}; // 0 idle, 1 loading, 2 finished and not swapped
+		std::exception_ptr LoaderError;
+		int Device = 0;
+		bool Pending = false; // a load was started (of PendingCentre, PendingRadius)
+		int2 PendingCentre {0, 0};
+		unsigned int PendingRadius = 0u;
+		STPAreaBuffer Front, Back;
+
 		void generate(const std::vector<Key>& chunks);
 		void erode(const std::vector<Key>& centres);
+		// joins a finished loader thread; rethrows its error
+		void joinLoader();
 	};
 
 } // namespace SuperTerrainPlus

@@ -9,7 +9,8 @@ oracle checks all build the same inputs:
 * ``multi_biome``      -- a 2048x2048 Voronoi biome map (64 jittered sites, seed 7, 8 biomes),
                           per-biome 4-8 octave parameters, filter radius 32, 1,048,576 drops.
 * ``batched``          -- 256 chunks with sides from {128, 256, 512, 1024} (p = .4/.3/.2/.1,
-                          seed 11), 4-biome Voronoi blend with radius 16, 0.25 drops per pixel.
+                          seed 11), each covering a 1024-unit world slot (level of detail
+                          1024 / side), 4-biome Voronoi blend with radius 16, 0.25 drops per pixel.
 
 Randomness that decides workload shapes uses Philox4x32-10 (``philox``) so it does not depend
 on a library's generator implementation.
@@ -158,6 +159,8 @@ class Batched:
     fractal: Fractal = field(default_factory=Fractal)
     drops_per_pixel: float = 0.25
     erosion_seed: int = 42
+    slot: int = 1024
+    lod: bool = True
 
     def chunk_sides(self):
         cum = np.cumsum(self.probs)
@@ -168,10 +171,15 @@ class Batched:
         return out
 
     def chunk_origins(self):
-        """World origin of chunk k: a 16 x 16 grid of 1024-pixel slots centred on the origin,
+        """World origin of chunk k: a 16 x 16 grid of 1024-unit slots centred on the origin,
         so no chunks overlap and world coordinates stay within +-8192 (FP32 noise precision
         degrades linearly with |coordinate|)."""
-        return [((k % 16 - 8) * 1024, (k // 16 - 8) * 1024) for k in range(self.chunks)]
+        return [((k % 16 - 8) * self.slot, (k // 16 - 8) * self.slot) for k in range(self.chunks)]
+
+    def chunk_steps(self):
+        """Level of detail of chunk k: it covers its whole 1024-unit slot, so a chunk of side s
+        samples every 1024 / s world units (1, 2, 4 or 8); lod=False gives every chunk step 1."""
+        return [self.slot // s if self.lod else 1 for s in self.chunk_sides()]
 
     def table(self):
         return biome_table(self.biome_count, self.biome_seed)
@@ -199,6 +207,9 @@ class Served:
     world: int = 2048
     memory_budget: int = 200 << 30
     batch: Batched = field(default_factory=Batched)
+    fly_step: float = 2.0
+    flythrough_rates: tuple = (1000, 4000, 8000, 16000)
+    fly_duration_s: float = 1.0
 
     @staticmethod
     def side_at(distance):
@@ -235,8 +246,40 @@ class Served:
         """(requests, arrival seconds) at `rate` requests per second."""
         n = int(count if count is not None else rate * self.duration_s)
         req = self.walk(n)
+        return req, self.arrivals(rate, n)
+
+    def arrivals(self, rate, n):
         gaps = np.random.default_rng(self.seed + int(rate)).exponential(1.0 / rate, n)
-        return req, np.cumsum(gaps) - gaps[0]
+        return np.cumsum(gaps) - gaps[0]
+
+    def flythrough(self, count):
+        """The first `count` requests of a straight flight east at `fly_step` chunks per frame
+        (y = 0.5): a chunk is requested when it enters the view or when its level of detail gets
+        finer than the one it holds (a renderer keeps the finer map it has while the camera moves
+        away), so no request repeats a (chunk, side) pair."""
+        cam = np.array([0.0, 0.5])
+        have = {}
+        out = []
+        r = int(np.ceil(self.view_radius))
+        while len(out) < count:
+            cx, cy = int(np.floor(cam[0])), int(np.floor(cam[1]))
+            view = []
+            for y in range(cy - r - 1, cy + r + 2):
+                for x in range(cx - r - 1, cx + r + 2):
+                    d = float(np.hypot(x + 0.5 - cam[0], y + 0.5 - cam[1]))
+                    if d <= self.view_radius:
+                        view.append((d, x, y))
+            for d, x, y in sorted(view):
+                side = self.side_at(d)
+                if have.get((x, y), 0) < side:
+                    have[(x, y)] = side
+                    out.append((x, y, side))
+            cam = cam + np.array([self.fly_step, 0.0])
+        return out[:count]
+
+    def flythrough_trace(self, rate, count=None):
+        n = int(count if count is not None else rate * self.duration_s)
+        return self.flythrough(n), self.arrivals(rate, n)
 
 
 def served():

@@ -1,7 +1,9 @@
 """Serving loop: results of a chunk are the same alone and in a mix of other requests (bit-exact
 without erosion, and equal to the oracle at every level of detail; within the parallel-erosion
-tolerance with erosion), cache hits, requests for chunks in production, least-recently-used
-eviction under a small memory budget, and a real-time replay of the camera-walk trace."""
+tolerance with erosion), free-slip erosion (a served chunk erodes as the centre of its
+neighbourhood), cache hits, requests for chunks in production, least-recently-used eviction under
+a small memory budget, and real-time replays of the camera-walk and flythrough traces (generated
+requests reported apart from cache hits; a miss-heavy stream batches several chunks)."""
 import ctypes as C
 
 import numpy as np
@@ -125,6 +127,60 @@ def test_server_erosion_alone_and_in_mix(gpu, oracle):
         assert np.sqrt(np.mean((alone.astype(np.float64) - mix) ** 2)) <= 1.5 * floor, (x, y, side)
 
 
+def server_word(x, y, side):
+    w = C.c_uint32()
+    capi.call("stp_server_stream_word", x, y, side, C.byref(w))
+    return w.value
+
+
+@pytest.mark.parametrize("x,y,side", [(0, 0, 128), (-2, 1, 256)])
+def test_server_erosion_is_free_slip(gpu, oracle, x, y, side):
+    """A served chunk erodes as the centre of its 3 x 3 neighbourhood of uneroded chunks at its
+    level of detail (the chunk is produced with a halo its drops cannot cross), not as a chunk
+    alone whose edges kill drops. Yardstick: the serialized oracle erosion of that neighbourhood
+    with the chunk's drops (its stream word), cut to the centre:
+      * the total absolute height change within 4/sqrt(drops) + 0.12 of the oracle's (the 0.12:
+        the concurrency deficit of parallel drops);
+      * RMS(served - serialized) <= 1.5 x RMS(serialized - serialized with other drops);
+      * within 8 pixels of the chunk's edges the served chunk is closer to that free-slip erosion
+        than to the serialized erosion of the chunk alone with the same drops, whose edges kill
+        drops (on the oracle, alone and free slip differ there by 1.5-2x that band's floor)."""
+    log(f"server free slip {x} {y} {side}")
+    dpp = 0.25
+    setting = W.erosion_setting(1)
+    s = Server(dpp)
+    s.submit(x, y, side)
+    s.flush()
+    served = s.read(x, y, side)[0].astype(np.float64)
+    del s
+    nb = np.ascontiguousarray(np.concatenate(
+        [np.concatenate([oracle_chunk(oracle, x + dx, y + dy, side) for dx in (-1, 0, 1)], axis=1)
+         for dy in (-1, 0, 1)], axis=0))
+    before = nb[side:2 * side, side:2 * side].astype(np.float64)
+    drops, word = int(side * side * dpp), server_word(x, y, side)
+    ser, other = nb.copy(), nb.copy()
+    oracle.erode(ser, (3, 3), (side, side), oracle_rain(oracle, setting), drops, setting.Seed, stream_id=word)
+    oracle.erode(other, (3, 3), (side, side), oracle_rain(oracle, setting), drops, setting.Seed,
+                 stream_id=word ^ 0x5A5A5A5A)
+    ser = ser[side:2 * side, side:2 * side].astype(np.float64)
+    other = other[side:2 * side, side:2 * side].astype(np.float64)
+    alone = before.astype(np.float32).copy()
+    oracle.erode(alone, (1, 1), (side, side), oracle_rain(oracle, setting), drops, setting.Seed, stream_id=word)
+    alone = alone.astype(np.float64)
+    band = np.zeros((side, side), bool)
+    band[:8, :] = band[-8:, :] = band[:, :8] = band[:, -8:] = True
+    near_slip = np.sqrt(np.mean((served - ser)[band] ** 2))
+    near_alone = np.sqrt(np.mean((served - alone)[band] ** 2))
+    print(f"edge band: rms to free slip {near_slip:.3e}, to alone {near_alone:.3e}", flush=True)
+    assert near_slip < near_alone, (near_slip, near_alone)
+    change, want = np.abs(served - before).sum(), np.abs(ser - before).sum()
+    assert abs(change - want) <= (4.0 / np.sqrt(drops) + 0.12) * want, (change, want)
+    floor = np.sqrt(np.mean((ser - other) ** 2))
+    rms = np.sqrt(np.mean((served - ser) ** 2))
+    print(f"free slip {side}: change {change:.4f} vs {want:.4f}, rms {rms:.3e} vs floor {floor:.3e}", flush=True)
+    assert rms <= 1.5 * floor, (rms, floor)
+
+
 def test_server_cache_and_in_flight(gpu):
     log("server cache")
     s = Server(0.0)
@@ -186,6 +242,30 @@ def test_server_replay_camera_walk(gpu):
     assert stats.MpixelPerSecond > 0
     log(f"server replay p50 {stats.P50:.3f} p99 {stats.P99:.3f} ms, {stats.MpixelPerSecond:.1f} Mpx/s, "
         f"{stats.Batches} batches, {stats.MeanBatchChunks:.1f} chunks/batch")
+
+
+def test_server_replay_flythrough_batches_misses(gpu):
+    """A flythrough at 16000 requests/s: every request is a chunk not generated before (here up to
+    the sides capped at 1024, which repeat a chunk when its level of detail passes 1024), requests
+    queue behind running batches and several chunks share a batch; the generated statistics cover
+    exactly the distinct chunks."""
+    log("server flythrough replay")
+    sv = W.served()
+    req, arr = sv.flythrough_trace(16000, count=600)
+    req = [(x, y, min(side, 1024)) for x, y, side in req]
+    s = Server(0.25, batch_pixels=1 << 23, max_side=1024)
+    r = (capi.ChunkRequest * len(req))(*[capi.ChunkRequest(x, y, side) for x, y, side in req])
+    a = (C.c_double * len(req))(*arr)
+    lat = (C.c_double * len(req))()
+    st = capi.ServerStats()
+    capi.call("stp_server_replay", C.c_void_p(s.h), r, a, len(req), sv.deadline_ms, lat, C.byref(st))
+    assert st.Requests == len(req) and st.Generated == len(set(req))
+    assert st.CacheHits <= len(req) - len(set(req))
+    assert st.MeanBatchChunks > 1.0
+    assert st.GeneratedP50 <= st.GeneratedP99 <= st.GeneratedP999 <= st.GeneratedMax + 1e-9
+    assert st.GeneratedMpixelPerSecond > 0
+    log(f"server flythrough generated p50 {st.GeneratedP50:.3f} p99 {st.GeneratedP99:.3f} ms, "
+        f"{st.MeanBatchChunks:.1f} chunks/batch")
 
 
 def test_server_rejects_bad_requests(gpu):

@@ -261,3 +261,50 @@ def test_world_rejects_bad_settings(gpu):
     w = World(0.0)
     with pytest.raises(capi.STPError):
         w.read(99, 99)
+
+
+def test_world_load_swaps_buffers(gpu):
+    """STPWorldPipeline::load on a worker thread (STPWorldPipeline.cpp:1504-1633): a new centre
+    starts a load (back buffer busy); calls while it runs are ignored; once it finished, a load of
+    the same centre swaps the buffers and the front buffer holds the rendered area exactly as the
+    synchronous stp_world_render_area makes it (no erosion: bit-exact); a load of the same centre
+    then changes nothing; a new centre starts a new load while the front buffer stays usable."""
+    import time
+    log("world asynchronous load")
+    w, ref = World(0.0), World(0.0)
+    radius, side = 1, 3 * MAP
+
+    def load(x, y):
+        st = C.c_uint32()
+        capi.call("stp_world_load", C.c_void_p(w.h), x, y, radius, C.byref(st))
+        return st.value
+
+    def front():
+        b, h, n = C.c_void_p(), C.c_void_p(), C.c_uint32()
+        capi.call("stp_world_front", C.c_void_p(w.h), C.byref(b), C.byref(h), C.byref(n))
+        return b.value, h.value, n.value
+
+    def download(ptr):
+        out = np.zeros((side, side), np.uint16)
+        capi.call("stp_memcpy_async", capi.ptr(out), C.c_void_p(ptr), out.nbytes, None)
+        capi.synchronize()
+        return out
+
+    assert front() == (None, None, 0)
+    for centre in [(2, -1), (9, 4)]:
+        assert load(*centre) == 1
+        deadline, status = time.time() + 60.0, 1
+        while status == 1 and time.time() < deadline:
+            time.sleep(0.002)
+            status = load(*centre)
+        assert status == 2, status
+        assert load(*centre) == 0
+        b, h, n = front()
+        assert n == side and b and h
+        rb, rh = capi.DeviceArray((side, side), np.uint16), capi.DeviceArray((side, side), np.uint16)
+        capi.call("stp_world_render_area", C.c_void_p(ref.h), centre[0], centre[1], radius, rb.vp, rh.vp)
+        capi.synchronize()
+        assert (download(b) == rb.download()).all() and (download(h) == rh.download()).all()
+        # the chunks of the area are complete in the world itself
+        assert w.state(*centre) == 2
+
