@@ -1,7 +1,8 @@
 """Benchmark of the heightfield path on one MI355X.
 
 Workload (the batched configuration): 256 square chunks with sides drawn from {128, 256, 512,
-1024} (p = 0.4/0.3/0.2/0.1, seed 11), each through the whole path -- 4-biome Voronoi biome map
+1024} (p = 0.4/0.3/0.2/0.1, seed 11), each covering a 1024-unit world slot (level of detail
+1024 / side world units per pixel), each through the whole path -- 4-biome Voronoi biome map
 with a radius-16 halo, single histogram filter (r = 16), 4-biome 8-octave simplex blend,
 parallel erosion at 0.25 drops per pixel, UINT16 format -- all chunks resident in HBM.
 
@@ -9,8 +10,15 @@ parallel erosion at 0.25 drops per pixel, UINT16 format -- all chunks resident i
   stage for all chunks).
 * plain: the same kernels with one chunk per launch sequence and a host synchronisation after
   every chunk (the reference's generate()/erode() calling pattern); chunk latencies p50/p99.
-* roofline: the dominant kernel (erosion), algorithmic bytes per launch from the per-drop step
-  counters, divided by its HIP-event duration inside the timed graph replays.
+* roofline: the dominant kernel (the LDS-tiled erosion) against the bound that limits it, the
+  issue rate of VALU instructions: its VALU instructions per 64 drop steps (SQ_INSTS_VALU over a
+  rocprofv3 run of this workload) times this run's drop steps, over its HIP-event duration
+  inside the timed graph replays. Beside it the LDS view (bank accesses of the drop steps) and
+  the HBM view (counter traffic against the compulsory bytes).
+* accuracy: a chunk of every size class of the timed run checked against the oracle.
+* configs: the single chunk (512^2, serialized), the free-slip neighbourhood (3 x 3 x 512^2) and
+  the multi-biome chunk (2048^2) through the drop-in STPHeightfieldGenerator API, and 64 chunks
+  of 512^2 batched against plain.
 * cpu_baseline: the FP32 oracle running the same per-chunk pipeline on the host cores, on a
   time-bounded sample of the same chunks.
 
@@ -30,70 +38,117 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "eroded heightfield Mpixel/s + p50/p99 chunk latency, batched vs plain, 1 GPU"
-HBM_PEAK_GBS = 8000.0
 STAGES = ["biome", "filter_vertical", "filter_horizontal", "blend", "erosion_bin", "erosion", "format"]
+# MI355X: 256 CUs at 2.4 GHz; HBM3E 8 TB/s
+CUS, CLOCK_GHZ, HBM_PEAK_GBS = 256, 2.4, 8000.0
+# VALU issue: one wave64 FP32 instruction per cycle per CU (four SIMD16 units, 4 cycles each): 614.4 G
+# instructions per second for the chip, the published 78.6 TFLOP/s non-packed FP32 FMA rate (a probe of
+# dependent-free v_mul/v_add/v_fma_f32 streams reached 0.98 per CU per cycle; int32 v_add/v_xor streams
+# issue up to 1.9, so for a kernel with integer instructions this peak is a lower bound of its issue limit)
+VALU_PEAK_GINSTR = CUS * CLOCK_GHZ
+# LDS: 32 banks of 4 bytes per CU, one access per bank per cycle: 128 B per cycle per CU
+LDS_PEAK_GBS = CUS * 128 * CLOCK_GHZ
 # algorithmic bytes of one drop step (r = 3 brush, 25 taps): two 2x2 bilinear height reads;
 # an erode step adds 25 x (4 B read + 8 B atomic read-modify-write); a deposit step 4 x 8 B.
 BYTES_STEP, BYTES_ERODE, BYTES_DEPOSIT = 32, 25 * 12, 4 * 8
-# LDS integer atomics of the tiled erosion: 25 per erode step (one per brush tap), 4 per deposit step
-ATOMICS_ERODE, ATOMICS_DEPOSIT = 25, 4
+# LDS bank accesses of the tiled erosion (4 B each): 8 height reads per step, one atomic per brush
+# tap of an erode step (25), one per cell of a deposit step (4)
+LDS_ACCESS_STEP, LDS_ACCESS_ERODE, LDS_ACCESS_DEPOSIT = 8, 25, 4
 # algorithmic FLOP of the blend (STPSimplexNoise.cu:5-110 as written, FMA = 2, division = 1): 53 per
 # simplex2D sample + 13 per octave of the fractal loop, and per histogram bin 3 more (the weight
 # product and the sum); one bin of a pixel costs octave * 66 + 3
 FLOP_PER_OCTAVE, FLOP_PER_BIN = 66, 3
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per launch of the tiled erosion kernel (tile 128, 1024 threads) on this workload, from
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a kernel of known size (profiles/erosion_hbm_traffic.md)
-TILED_EROSION_TRAFFIC = 493.0e6
+# Measured on this workload with rocprofv3 for the LDS-tiled erosion kernel at tile 128 x 1024
+# threads (profiles/erosion_sq_counters.md, profiles/erosion_hbm_traffic.md): VALU wave64
+# instructions per 64 drop steps (SQ_INSTS_VALU / (drop steps / 64)), and HBM bytes per launch
+# (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on the format kernel).
+EROSION_PROFILE_SHAPE = (128, 1024)
+EROSION_VALU_PER_WAVE_STEP = 186.9
+EROSION_HBM_BYTES_PER_LAUNCH = 359.5e6
 # serving loop: two engines of 8 Mpx / 512 chunks each
 SERVER_BATCH_PIXELS, SERVER_BATCH_CHUNKS, SERVER_ENGINES = 8 << 20, 512, 2
+
+
+def host_threads():
+    """Threads for the CPU baseline: the cores this process may run on, at most 16 (one GPU's
+    share of the host)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail)), avail
 
 
 def workload():
     from superterrainplus_amd import workloads as W
     cfg = W.batched()
-    sides = cfg.chunk_sides()
-    origins = cfg.chunk_origins()
     props = [(p[0], cfg.fractal.octave, p[2], p[3], p[4], p[5]) for p in cfg.table()]
-    return cfg, sides, origins, props
+    return cfg, cfg.chunk_sides(), cfg.chunk_origins(), cfg.chunk_steps(), props
 
 
-def make_batch(cfg, sides, origins, props, layers=None):
+def make_batch(cfg, sides, origins, steps, props, layers=None, dpp=None):
     from superterrainplus_amd import capi, workloads as W
     noise = capi.SimplexNoiseSetting(cfg.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, cfg.engine)
     # a pixel's histogram can hold at most biome_count distinct biomes
     setting = capi.BatchSetting(cfg.biome_seed, cfg.biome_count, cfg.cell, cfg.radius, cfg.biome_count,
-                                cfg.drops_per_pixel, layers)
-    chunks = (capi.BatchChunk * len(sides))(*[capi.BatchChunk(s, (C.c_int32 * 2)(*o)) for s, o in zip(sides, origins)])
+                                cfg.drops_per_pixel if dpp is None else dpp, layers)
+    chunks = (capi.BatchChunk * len(sides))(*[capi.BatchChunk(s, (C.c_int32 * 2)(*o), st)
+                                              for s, o, st in zip(sides, origins, steps)])
     h = C.c_void_p()
     capi.call("stp_batch_create", C.byref(noise), capi.biome_array(props), len(props),
               C.byref(W.erosion_setting(1, seed=cfg.erosion_seed)), C.byref(setting), chunks, len(sides), C.byref(h))
     return h
 
 
-def cpu_baseline(cfg, sides, origins, props, seconds, threads):
-    """FP32 oracle pipeline per chunk on `threads` host threads, chunks in workload order until
-    `seconds` of wall time have passed."""
+def oracle_module():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    from superterrainplus_amd import workloads as W
-    perm, g32, _ = O.permutation(cfg.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, cfg.engine)
-    s = W.erosion_setting(1, seed=cfg.erosion_seed)
+    return O
+
+
+def oracle_rain(O, s):
     rain = O.RainDropSetting()
     for f, _ in O.RainDropSetting._fields_:
         setattr(rain, f, getattr(s, f))
+    return rain
+
+
+def oracle_chunk(O, cfg, props, side, origin, step, grad, f64=False):
+    """Pre-erosion heights of a batch chunk (the oracle's biome map, filter and blend at the
+    chunk's level of detail)."""
+    from superterrainplus_amd import workloads as W
+    perm = O.permutation(cfg.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, cfg.engine)[0]
     r = cfg.radius
+    halo = O.voronoi_map(cfg.biome_seed, cfg.biome_count, cfg.cell, (origin[0] - r * step, origin[1] - r * step),
+                         side + 2 * r, side + 2 * r, step=step)
+    item, weight, off = O.shf_window(halo, (r, r), (side, side), r, cap_per_pixel=8)
+    half = side * step / 2.0
+    return O.multi_biome(perm, grad, props, item, weight, off, (side, side), (origin[0] + half, origin[1] + half),
+                         f64=f64, step=step)
+
+
+def rms(a, b):
+    return float(np.sqrt(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2)))
+
+
+def cpu_baseline(cfg, sides, origins, steps, props, seconds, threads, f64=False):
+    """The oracle pipeline per chunk (biome map, filter, blend, serialized erosion, format) on
+    `threads` host threads, chunks in workload order until `seconds` of wall time have passed;
+    FP32 (f64=False) or FP64."""
+    O = oracle_module()
+    from superterrainplus_amd import workloads as W
+    _, g32, g64 = O.permutation(cfg.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, cfg.engine)
+    s = W.erosion_setting(1, seed=cfg.erosion_seed)
+    rain = oracle_rain(O, s)
     deadline = time.perf_counter() + seconds
 
     def one(i):
         if time.perf_counter() > deadline:
             return 0
-        side, o = sides[i], origins[i]
-        halo = O.voronoi_map(cfg.biome_seed, cfg.biome_count, cfg.cell, (o[0] - r, o[1] - r), side + 2 * r, side + 2 * r)
-        item, weight, off = O.shf_window(halo, (r, r), (side, side), r, cap_per_pixel=8)
-        h = O.multi_biome(perm, g32, props, item, weight, off, (side, side), (o[0] + side / 2, o[1] + side / 2))
-        drops = int(side * side * cfg.drops_per_pixel)
-        O.erode(h, (1, 1), (side, side), rain, drops, s.Seed, stream_id=i)
+        side = sides[i]
+        h = oracle_chunk(O, cfg, props, side, origins[i], steps[i], g64 if f64 else g32, f64=f64)
+        O.erode(h, (1, 1), (side, side), rain, int(side * side * cfg.drops_per_pixel), s.Seed, stream_id=i, f64=f64)
         O.format_heightmap(h)
         return side * side
 
@@ -103,43 +158,379 @@ def cpu_baseline(cfg, sides, origins, props, seconds, threads):
     wall = time.perf_counter() - t0
     px = sum(done)
     n = sum(1 for d in done if d)
-    return {"value": px / wall / 1e6, "unit": "Mpixel/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} of {len(sides)} chunks ({px / 1e6:.2f} Mpixel) of the same workload, FP32 oracle "
-                      f"(noise, filter, blend, serialized erosion, format), {wall:.1f} s wall"}
+    return {"value": round(px / wall / 1e6, 3), "unit": "Mpixel/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} of {len(sides)} chunks ({px / 1e6:.2f} Mpixel) of the same workload, "
+                      f"{'FP64' if f64 else 'FP32'} oracle (biome map, filter, blend, serialized erosion, format) "
+                      f"on {threads} thread{'s' if threads > 1 else ''}, {wall:.1f} s wall"}
 
 
-def cpu_baseline_fp64(cfg, sides, origins, props, seconds):
-    """The FP64 oracle pipeline on one thread (noise, filter and blend in FP64, serialized FP64
-    erosion, format), chunks in workload order until `seconds` of wall time have passed."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    from superterrainplus_amd import workloads as W
-    perm, _, g64 = O.permutation(cfg.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, cfg.engine)
-    s = W.erosion_setting(1, seed=cfg.erosion_seed)
-    rain = O.RainDropSetting()
-    for f, _ in O.RainDropSetting._fields_:
-        setattr(rain, f, getattr(s, f))
-    r = cfg.radius
-    t0, px, n = time.perf_counter(), 0, 0
-    for i, (side, o) in enumerate(zip(sides, origins)):
-        if time.perf_counter() - t0 > seconds:
-            break
-        halo = O.voronoi_map(cfg.biome_seed, cfg.biome_count, cfg.cell, (o[0] - r, o[1] - r), side + 2 * r, side + 2 * r)
-        item, weight, off = O.shf_window(halo, (r, r), (side, side), r, cap_per_pixel=8)
-        h = O.multi_biome(perm, g64, props, item, weight, off, (side, side), (o[0] + side / 2, o[1] + side / 2), f64=True)
-        O.erode(h, (1, 1), (side, side), rain, int(side * side * cfg.drops_per_pixel), s.Seed, stream_id=i, f64=True)
-        O.format_heightmap(h)
-        px += side * side
-        n += 1
-    wall = time.perf_counter() - t0
-    return {"value": round(px / wall / 1e6, 3), "unit": "Mpixel/s", "cores": 1, "kind": "port",
-            "sample": f"first {n} of {len(sides)} chunks ({px / 1e6:.2f} Mpixel) of the same workload, FP64 oracle on one "
-                      f"thread, {wall:.1f} s wall"}
+def validate(cfg, sides, origins, steps, props, batch, threads):
+    """The timed run's chunks against the oracle, one chunk of every size class:
+    * before erosion (the same chunks in a batch without drops): bit-exact against the FP32
+      oracle, max-abs and RMS against the FP64 oracle;
+    * eroded (the timed batch's own output): RMS against the serialized FP32 oracle erosion with
+      the same drops, the floor it is judged by (RMS between serialized runs with different drops),
+      RMS against the serialized FP64 erosion of the FP64 map, the serialized FP32 run's own RMS
+      against that FP64 run, the mass-ledger residual and the sediment moved against serialized."""
+    from superterrainplus_amd import capi, workloads as W
+    O = oracle_module()
+    _, g32, g64 = O.permutation(cfg.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, cfg.engine)
+    first = {}
+    for i, s in enumerate(sides):
+        first.setdefault(s, i)
+    idx = [first[s] for s in sorted(first)]
+    pre_b = make_batch(cfg, [sides[i] for i in idx], [origins[i] for i in idx], [steps[i] for i in idx], props,
+                       dpp=0.0)
+    pre, got = [], []
+    try:
+        capi.call("stp_batch_run", pre_b, 0, None)
+        capi.synchronize()
+        for k, i in enumerate(idx):
+            s = sides[i]
+            h = np.zeros((s, s), np.float32)
+            capi.call("stp_batch_read_chunk", pre_b, k, capi.ptr(h), None, None, None)
+            pre.append(h)
+    finally:
+        capi.lib().stp_batch_destroy(pre_b)
+    for i in idx:
+        s = sides[i]
+        drops = int(s * s * cfg.drops_per_pixel)
+        h, f = np.zeros((s, s), np.float32), np.zeros((s, s), np.uint16)
+        led = np.zeros(max(drops, 1), np.float32)
+        capi.call("stp_batch_read_chunk", batch, i, capi.ptr(h), capi.ptr(f), None, capi.ptr(led))
+        got.append((h, f, led[:drops]))
+    setting = W.erosion_setting(1, seed=cfg.erosion_seed)
+    rain = oracle_rain(O, setting)
+
+    def check(k):
+        i = idx[k]
+        s, st, o = sides[i], steps[i], origins[i]
+        drops = int(s * s * cfg.drops_per_pixel)
+        h32 = oracle_chunk(O, cfg, props, s, o, st, g32)
+        h64 = oracle_chunk(O, cfg, props, s, o, st, g64, f64=True)
+        eroded, fixed, led = got[k]
+        ser = pre[k].copy()
+        moved_ser = float(O.erode(ser, (1, 1), (s, s), rain, drops, setting.Seed, stream_id=i)[0].astype(np.float64).sum())
+        other = pre[k].copy()
+        O.erode(other, (1, 1), (s, s), rain, drops, setting.Seed + 1000, stream_id=i)
+        ser64 = h64.copy()
+        O.erode(ser64, (1, 1), (s, s), rain, drops, setting.Seed, stream_id=i, f64=True)
+        moved = float(led.astype(np.float64).sum())
+        dm = float(eroded.astype(np.float64).sum() - pre[k].astype(np.float64).sum())
+        return {"chunk": i, "side": s, "step": st, "drops": drops,
+                "pre_erosion_bit_exact_fp32": bool(pre[k].tobytes() == h32.tobytes()),
+                "pre_erosion_max_abs_vs_fp64": float(np.abs(pre[k].astype(np.float64) - h64).max()),
+                "pre_erosion_rms_vs_fp64": rms(pre[k], h64),
+                "uint16_bit_exact": bool((fixed == O.format_heightmap(eroded)).all()),
+                "rms_vs_serialized_fp32": rms(eroded, ser),
+                "rms_serialized_other_drops": rms(other, ser),
+                "rms_vs_fp64": rms(eroded, ser64),
+                "rms_serialized_fp32_vs_fp64": rms(ser, ser64),
+                "mass_ledger_residual": abs(dm + moved) / moved if moved > 0 else None,
+                "sediment_moved_ratio": moved / moved_ser if moved_ser > 0 else None}
+
+    with cf.ThreadPoolExecutor(min(threads, len(idx))) as ex:
+        rows = list(ex.map(check, range(len(idx))))
+    for r in rows:
+        for k, v in list(r.items()):
+            if isinstance(v, float):
+                r[k] = float(f"{v:.4g}")
+    return {"chunks": rows,
+            "note": "pre-erosion heights are deterministic and meet 1e-4 RMS against FP64 with room to spare; "
+                    "eroded heights cannot: the serialized FP32 oracle itself differs from the FP64 one by "
+                    "rms_serialized_fp32_vs_fp64 (erosion is chaotic: one rounding moves a drop's path), so "
+                    "parallel erosion is judged against the serialized FP32 run with the same drops, by "
+                    "rms <= 1.5 x rms_serialized_other_drops, the mass ledger and the sediment moved"}
+
+
+def time_batch(batch, steps, stream, mode=0):
+    """Wall milliseconds per run of `mode` (0 graph, 2 plain) over `steps` runs after a warm-up."""
+    from superterrainplus_amd import capi
+    capi.call("stp_batch_run", batch, mode, stream)
+    capi.call("stp_stream_synchronize", stream)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        capi.call("stp_batch_run", batch, mode, stream)
+    capi.call("stp_stream_synchronize", stream)
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
+def chunks_512(stream, count=64, steps=10, plain_steps=2):
+    """`count` chunks of 512^2 (step 1, 8 x 8 grid of 1024-unit slots), otherwise the batched
+    workload: batched graph replays against the plain chunk-by-chunk pattern on the same kernels."""
+    from superterrainplus_amd import capi
+    cfg, _, _, _, props = workload()
+    sides = [512] * count
+    origins = [((k % 8 - 4) * cfg.slot, (k // 8 - 4) * cfg.slot) for k in range(count)]
+    b = make_batch(cfg, sides, origins, [1] * count, props)
+    try:
+        tuning = capi.BatchTuning()
+        capi.call("stp_batch_autotune", b, 2, stream, C.byref(tuning))
+        graph_ms = time_batch(b, steps, stream)
+        plain_ms = time_batch(b, plain_steps, stream, mode=2)
+        lat = (C.c_double * count)()
+        capi.call("stp_batch_chunk_latencies", b, lat, count)
+    finally:
+        capi.lib().stp_batch_destroy(b)
+    px = count * 512 * 512
+    out = {"workload": f"{count} chunks of 512^2, the batched workload's biomes, filter, blend and erosion",
+           "batched_mpixel_s": round(px / graph_ms / 1e3, 1), "plain_mpixel_s": round(px / plain_ms / 1e3, 1),
+           "batched_ms_per_step": round(graph_ms, 3),
+           "plain_chunk_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                      "p99": round(float(np.percentile(lat, 99)), 3)},
+           "erosion_tuning": [tuning.ErosionTile, tuning.ErosionThreads]}
+    out["batched_vs_plain"] = round(out["batched_mpixel_s"] / out["plain_mpixel_s"], 2)
+    print(f"[bench] 512^2 chunks: batched {out['batched_mpixel_s']} Mpixel/s, plain {out['plain_mpixel_s']} "
+          f"({out['batched_vs_plain']}x)", file=sys.stderr, flush=True)
+    return out
+
+
+class Generator:
+    """STPHeightfieldGenerator through the C ABI with a built-in diversity plugin."""
+
+    def __init__(self, plugin, map_size, diversity_nn, erosion_nn, rain, mode):
+        from superterrainplus_amd import capi
+        c = capi.ChunkSetting()
+        c.ChunkSize[:] = [map_size, map_size]
+        c.MapSize[:] = [map_size, map_size]
+        c.ChunkScale[:] = [1.0, 1.0]
+        c.DiversityNearestNeighbour[:] = [diversity_nn, diversity_nn]
+        c.ErosionNearestNeighbour[:] = [erosion_nn, erosion_nn]
+        c.SplatNearestNeighbour[:] = [1, 1]
+        c.RenderDistance[:] = [3, 3]
+        h = C.c_void_p()
+        capi.call("stp_heightfield_generator_create", C.byref(c), C.byref(rain), C.c_void_p(plugin), C.byref(h))
+        self.h = h
+        capi.call("stp_heightfield_generator_set_erosion_mode", h, mode)
+
+    def close(self):
+        from superterrainplus_amd import capi
+        capi.lib().stp_heightfield_generator_destroy(self.h)
+
+
+def generator_configs(reps=3):
+    """The three reference-scale configurations through the drop-in generator API
+    (STPHeightfieldGenerator::generate / erode, STPHeightfieldGenerator.cpp:158-197): host
+    pointers (the reference's calling convention, PCIe copies included, synchronous per call) and
+    device pointers (chunks resident in HBM). Mpixel/s counts the chunks' pixels through generate
+    and erode; rms_vs_fp64 compares the GPU's eroded map with the FP64 oracle (noise, blend and a
+    serialized FP64 erosion with the same drops)."""
+    from superterrainplus_amd import capi, workloads as W
+    O = oracle_module()
+    out = {}
+
+    def timed(fn):
+        fn()
+        capi.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        capi.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    # -- the single chunk: 512^2, 8-octave simplex, 8192 drops
+    pc = W.plain_chunk()
+    noise = capi.SimplexNoiseSetting(pc.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, pc.engine)
+    d = C.c_void_p()
+    capi.call("stp_diversity_simplex_create", C.byref(noise), C.byref(capi.BiomeProperty(*pc.fractal.biome())),
+              (C.c_uint32 * 2)(pc.side, pc.side), C.byref(d))
+    none_biome = (C.c_void_p * 1)(None)
+    perm, g32, g64 = O.permutation(pc.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, pc.engine)
+    rain = W.erosion_setting(pc.drops, seed=pc.erosion_seed)
+    try:
+        for mode, name in ((1, "serialized"), (0, "parallel")):
+            g = Generator(d.value, pc.side, 1, 1, rain, mode)
+            try:
+                h = np.zeros((pc.side, pc.side), np.float32)
+                f = np.zeros((pc.side, pc.side), np.uint16)
+
+                def host():
+                    capi.call("stp_heightfield_generate", g.h, capi.ptr(h), none_biome, 0.0, 0.0)
+                    capi.call("stp_heightfield_erode", g.h, (C.c_void_p * 1)(h.ctypes.data), (C.c_void_p * 1)(f.ctypes.data))
+                host_ms = timed(host)
+                dh, df = capi.DeviceArray((pc.side, pc.side), np.float32), capi.DeviceArray((pc.side, pc.side), np.uint16)
+
+                def device():
+                    capi.call("stp_heightfield_generate_device", g.h, dh.vp, none_biome, 0.0, 0.0, None)
+                    capi.call("stp_heightfield_erode_device", g.h, (C.c_void_p * 1)(dh.ptr), (C.c_void_p * 1)(df.ptr), None)
+                dev_ms = timed(device)
+            finally:
+                g.close()
+            out[f"single_chunk_{name}"] = {"host_api_ms": round(host_ms, 3), "device_api_ms": round(dev_ms, 3),
+                                           "host_api_mpixel_s": round(pc.side ** 2 / host_ms / 1e3, 2),
+                                           "device_api_mpixel_s": round(pc.side ** 2 / dev_ms / 1e3, 2)}
+        # accuracy of the serialized chunk: a fresh generator's first erosion (stream id 0)
+        g = Generator(d.value, pc.side, 1, 1, rain, 1)
+        try:
+            capi.call("stp_heightfield_generate", g.h, capi.ptr(h), none_biome, 0.0, 0.0)
+            capi.call("stp_heightfield_erode", g.h, (C.c_void_p * 1)(h.ctypes.data), (C.c_void_p * 1)(f.ctypes.data))
+        finally:
+            g.close()
+        desc = O.FractalDesc(0.5, 2.0, 8, 200.0, 0.0, 0.0, pc.side / 2, pc.side / 2, 1.0, 1.0)
+        w32, w64 = O.fractal_map(perm, g32, desc, pc.side, pc.side), O.fractal_map(perm, g64, desc, pc.side, pc.side, f64=True)
+        O.erode(w32, (1, 1), (pc.side, pc.side), oracle_rain(O, rain), pc.drops, rain.Seed)
+        O.erode(w64, (1, 1), (pc.side, pc.side), oracle_rain(O, rain), pc.drops, rain.Seed, f64=True)
+        out["single_chunk_serialized"].update({"bit_exact_fp32": bool(h.tobytes() == w32.tobytes()),
+                                               "rms_vs_fp64": float(f"{rms(h, w64):.4g}")})
+    finally:
+        capi.call("stp_diversity_destroy", d)
+    out["single_chunk_serialized"]["workload"] = (f"{pc.side}^2 chunk, 8-octave simplex, {pc.drops} drops, "
+                                                  "serialized erosion (bit-exact mode), UINT16")
+
+    # -- the free-slip neighbourhood: 3 x 3 chunks of 512^2, 65536 parallel drops
+    fs = W.free_slip()
+    noise = capi.SimplexNoiseSetting(fs.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, fs.engine)
+    d = C.c_void_p()
+    capi.call("stp_diversity_simplex_create", C.byref(noise), C.byref(capi.BiomeProperty(*fs.fractal.biome())),
+              (C.c_uint32 * 2)(fs.side, fs.side), C.byref(d))
+    rain = W.erosion_setting(fs.drops, seed=fs.erosion_seed)
+    offs = fs.chunk_offsets()
+    try:
+        g = Generator(d.value, fs.side, 1, 3, rain, 0)
+        try:
+            hs = [np.zeros((fs.side, fs.side), np.float32) for _ in range(9)]
+            fx = [np.zeros((fs.side, fs.side), np.uint16) for _ in range(9)]
+            hp = (C.c_void_p * 9)(*[a.ctypes.data for a in hs])
+            fp = (C.c_void_p * 9)(*[a.ctypes.data for a in fx])
+
+            def host():
+                for a, o in zip(hs, offs):
+                    capi.call("stp_heightfield_generate", g.h, capi.ptr(a), none_biome, o[0], o[1])
+                capi.call("stp_heightfield_erode", g.h, hp, fp)
+            host_ms = timed(host)
+            dh = [capi.DeviceArray((fs.side, fs.side), np.float32) for _ in range(9)]
+            df = [capi.DeviceArray((fs.side, fs.side), np.uint16) for _ in range(9)]
+            dhp = (C.c_void_p * 9)(*[a.ptr for a in dh])
+            dfp = (C.c_void_p * 9)(*[a.ptr for a in df])
+
+            def device():
+                for a, o in zip(dh, offs):
+                    capi.call("stp_heightfield_generate_device", g.h, a.vp, none_biome, o[0], o[1], None)
+                capi.call("stp_heightfield_erode_device", g.h, dhp, dfp, None)
+            dev_ms = timed(device)
+        finally:
+            g.close()
+        # accuracy: a fresh generator's first erosion (stream id 0) of the generated neighbourhood
+        g = Generator(d.value, fs.side, 1, 3, rain, 0)
+        try:
+            for a, o in zip(hs, offs):
+                capi.call("stp_heightfield_generate", g.h, capi.ptr(a), none_biome, o[0], o[1])
+            capi.call("stp_heightfield_erode", g.h, hp, fp)
+        finally:
+            g.close()
+        gpu = O.nn_merge(hs, (3, 3))
+        w64 = O.nn_merge([O.fractal_map(perm, g64, O.FractalDesc(0.5, 2.0, 8, 200.0, o[0], o[1], fs.side / 2,
+                                                                  fs.side / 2, 1.0, 1.0), fs.side, fs.side, f64=True)
+                          for o in offs], (3, 3))
+        O.erode(w64, (3, 3), (fs.side, fs.side), oracle_rain(O, rain), fs.drops, rain.Seed, f64=True)
+    finally:
+        capi.call("stp_diversity_destroy", d)
+    px = 9 * fs.side ** 2
+    out["free_slip"] = {"workload": f"3x3 chunks of {fs.side}^2 at (-1..1)^2: 9 x generate + 1 x erode of the "
+                                    f"1536^2 free-slip map, {fs.drops} parallel drops, UINT16",
+                        "host_api_ms": round(host_ms, 3), "device_api_ms": round(dev_ms, 3),
+                        "host_api_mpixel_s": round(px / host_ms / 1e3, 1), "device_api_mpixel_s": round(px / dev_ms / 1e3, 1),
+                        "rms_vs_fp64": float(f"{rms(gpu, w64):.4g}")}
+
+    # -- the multi-biome chunk: 2048^2, 8 biomes (4-8 octaves), filter radius 32, 1,048,576 drops
+    mb = W.multi_biome()
+    noise = capi.SimplexNoiseSetting(mb.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, mb.engine)
+    props = mb.table()
+    d = C.c_void_p()
+    capi.call("stp_diversity_biomefield_create", C.byref(noise), capi.biome_array(props), len(props),
+              (C.c_uint32 * 2)(mb.side, mb.side), mb.radius, 8, C.byref(d))
+    rain = W.erosion_setting(mb.drops, seed=mb.erosion_seed)
+    try:
+        biomes = []
+        for cy in (-1, 0, 1):
+            for cx in (-1, 0, 1):
+                b = capi.DeviceArray((mb.side, mb.side), np.uint16)
+                capi.call("stp_voronoi_biomemap", mb.biome_seed, mb.biome_count, mb.cell, cx * mb.side, cy * mb.side,
+                          b.vp, mb.side, mb.side, mb.side, None)
+                biomes.append(b)
+        host_biomes = [b.download() for b in biomes]
+        bp = (C.c_void_p * 9)(*[b.ptr for b in biomes])
+        hbp = (C.c_void_p * 9)(*[b.ctypes.data for b in host_biomes])
+        g = Generator(d.value, mb.side, 3, 1, rain, 0)
+        try:
+            dh, df = capi.DeviceArray((mb.side, mb.side), np.float32), capi.DeviceArray((mb.side, mb.side), np.uint16)
+
+            def device():
+                capi.call("stp_heightfield_generate_device", g.h, dh.vp, bp, 0.0, 0.0, None)
+                capi.call("stp_heightfield_erode_device", g.h, (C.c_void_p * 1)(dh.ptr), (C.c_void_p * 1)(df.ptr), None)
+            dev_ms = timed(device)
+            h = np.zeros((mb.side, mb.side), np.float32)
+            f = np.zeros((mb.side, mb.side), np.uint16)
+
+            def host():
+                capi.call("stp_heightfield_generate", g.h, capi.ptr(h), hbp, 0.0, 0.0)
+                capi.call("stp_heightfield_erode", g.h, (C.c_void_p * 1)(h.ctypes.data), (C.c_void_p * 1)(f.ctypes.data))
+            host_ms = timed(host)
+        finally:
+            g.close()
+        g = Generator(d.value, mb.side, 3, 1, rain, 0)
+        try:
+            capi.call("stp_heightfield_generate", g.h, capi.ptr(h), hbp, 0.0, 0.0)
+            pre = h.copy()
+            capi.call("stp_heightfield_erode", g.h, (C.c_void_p * 1)(h.ctypes.data), (C.c_void_p * 1)(f.ctypes.data))
+        finally:
+            g.close()
+    finally:
+        capi.call("stp_diversity_destroy", d)
+    r = mb.radius
+    halo = O.voronoi_map(mb.biome_seed, mb.biome_count, mb.cell, (-r, -r), mb.side + 2 * r, mb.side + 2 * r)
+    item, weight, off = O.shf_window(halo, (r, r), (mb.side, mb.side), r, cap_per_pixel=8)
+    perm, g32, g64 = O.permutation(mb.noise_seed, W.DISTRIBUTION, W.GRADIENT_OFFSET, mb.engine)
+    w64 = O.multi_biome(perm, g64, props, item, weight, off, (mb.side, mb.side), (0.0, 0.0), f64=True)
+    pre_max = float(np.abs(pre.astype(np.float64) - w64).max())
+    O.erode(w64, (1, 1), (mb.side, mb.side), oracle_rain(O, rain), mb.drops, rain.Seed, f64=True)
+    px = mb.side ** 2
+    out["multi_biome"] = {"workload": f"{mb.side}^2 chunk, 8-biome Voronoi (64 sites, seed 7), 4-8 octaves per biome, "
+                                      f"filter r={mb.radius} over a 3x3 diversity neighbourhood, {mb.drops} parallel "
+                                      "drops, UINT16",
+                          "host_api_ms": round(host_ms, 3), "device_api_ms": round(dev_ms, 3),
+                          "host_api_mpixel_s": round(px / host_ms / 1e3, 1),
+                          "device_api_mpixel_s": round(px / dev_ms / 1e3, 1),
+                          "pre_erosion_max_abs_vs_fp64": float(f"{pre_max:.4g}"),
+                          "rms_vs_fp64": float(f"{rms(h, w64):.4g}")}
+    for k, v in out.items():
+        print(f"[bench] {k}: host API {v['host_api_mpixel_s']} Mpixel/s, device API {v['device_api_mpixel_s']} "
+              f"Mpixel/s" + (f", rms vs FP64 {v['rms_vs_fp64']}" if "rms_vs_fp64" in v else ""),
+              file=sys.stderr, flush=True)
+    return out
+
+
+def multi_biome_batch(stream, steps=5):
+    """The multi-biome chunk as a one-chunk batch (device biome map, filter, blend, LDS-tiled
+    erosion with 1,048,576 drops, format), latency tuning."""
+    from superterrainplus_amd import capi, workloads as W
+    mb = W.multi_biome()
+    cfg = W.batched()
+    cfg.biome_seed, cfg.biome_count, cfg.cell, cfg.radius = mb.biome_seed, mb.biome_count, mb.cell, mb.radius
+    cfg.noise_seed, cfg.engine, cfg.erosion_seed = mb.noise_seed, mb.engine, mb.erosion_seed
+    cfg.drops_per_pixel = mb.drops / float(mb.side ** 2)
+    b = make_batch(cfg, [mb.side], [(0, 0)], [1], mb.table())
+    try:
+        tuning = capi.BatchTuning()
+        capi.call("stp_batch_autotune", b, 2, stream, C.byref(tuning))
+        ms = time_batch(b, steps, stream)
+    finally:
+        capi.lib().stp_batch_destroy(b)
+    out = {"workload": f"one {mb.side}^2 chunk as a batch: 8 biomes, filter r={mb.radius}, {mb.drops} drops (LDS-tiled)",
+           "ms": round(ms, 3), "mpixel_s": round(mb.side ** 2 / ms / 1e3, 1),
+           "erosion_tuning": [tuning.ErosionTile, tuning.ErosionThreads]}
+    print(f"[bench] multi-biome chunk as a batch: {out['ms']} ms, {out['mpixel_s']} Mpixel/s", file=sys.stderr,
+          flush=True)
+    return out
 
 
 def served(seconds_scale=1.0):
-    """Camera-walk request stream replayed in real time at each rate on a fresh server; a short
-    warm-up trace far from the walk comes first so that the timed replay finds warm engines."""
+    """Camera-driven request streams replayed in real time at each rate on a fresh server; a short
+    warm-up trace far from the walk comes first so that the timed replay finds warm engines.
+    * walk: the camera random walk (revisits chunks: many requests are cache hits);
+    * flythrough: a straight fast flight (every request is a chunk not yet generated), at rates
+      high enough that requests queue behind running batches and are batched together.
+    Every record carries the statistics of all requests and of the generated ones alone."""
     from superterrainplus_amd import capi, workloads as W
     sv = W.served()
     cfg = sv.batch
@@ -149,34 +540,53 @@ def served(seconds_scale=1.0):
                               cfg.drops_per_pixel)
     setting = capi.ServerSetting(sv.memory_budget, SERVER_BATCH_PIXELS, SERVER_BATCH_CHUNKS, 2048, SERVER_ENGINES,
                                  sv.world)
-    out = {}
-    for rate in sv.rates:
-        h = C.c_void_p()
-        capi.call("stp_server_create", C.byref(noise), capi.biome_array(props), len(props),
-                  C.byref(W.erosion_setting(1, seed=cfg.erosion_seed)), C.byref(batch), C.byref(setting), C.byref(h))
-        try:
-            def replay(req, arr):
-                r = (capi.ChunkRequest * len(req))(*[capi.ChunkRequest(x, y, side) for x, y, side in req])
-                a = (C.c_double * len(req))(*arr)
-                lat = (C.c_double * len(req))()
-                st = capi.ServerStats()
-                capi.call("stp_server_replay", h, r, a, len(req), sv.deadline_ms, lat, C.byref(st))
-                return st
-            wreq, warr = sv.trace(rate, count=min(200, int(rate * sv.duration_s)))
-            replay([(x + 4096, y, side) for x, y, side in wreq], warr)
-            req, arr = sv.trace(rate, count=int(rate * sv.duration_s * seconds_scale))
-            st = replay(req, arr)
-        finally:
-            capi.lib().stp_server_destroy(h)
-        out[str(rate)] = {"requests": int(st.Requests), "p50_ms": round(st.P50, 3), "p99_ms": round(st.P99, 3),
-                          "p99_9_ms": round(st.P999, 3), "max_ms": round(st.Max, 3),
-                          "mpixel_s": round(st.MpixelPerSecond, 1),
-                          "deadline_miss": round(st.DeadlineMissFraction, 4), "batches": int(st.Batches),
-                          "chunks_per_batch": round(st.MeanBatchChunks, 1), "cache_hits": int(st.CacheHits),
-                          "evictions": int(st.Evictions)}
-        print(f"[bench] served {rate}/s: p50 {st.P50:.2f} p99 {st.P99:.2f} p99.9 {st.P999:.2f} ms, "
-              f"{st.MpixelPerSecond:.0f} Mpixel/s, {st.Batches} batches", file=sys.stderr, flush=True)
-    return out
+
+    def run(trace, rates, label, duration_s):
+        out = {}
+        for rate in rates:
+            h = C.c_void_p()
+            capi.call("stp_server_create", C.byref(noise), capi.biome_array(props), len(props),
+                      C.byref(W.erosion_setting(1, seed=cfg.erosion_seed)), C.byref(batch), C.byref(setting), C.byref(h))
+            try:
+                def replay(req, arr):
+                    r = (capi.ChunkRequest * len(req))(*[capi.ChunkRequest(x, y, side) for x, y, side in req])
+                    a = (C.c_double * len(req))(*arr)
+                    lat = (C.c_double * len(req))()
+                    st = capi.ServerStats()
+                    capi.call("stp_server_replay", h, r, a, len(req), sv.deadline_ms, lat, C.byref(st))
+                    return st
+                wreq, warr = trace(rate, min(200, int(rate * duration_s)))
+                replay([(x + 4096, y, side) for x, y, side in wreq], warr)
+                req, arr = trace(rate, int(rate * duration_s * seconds_scale))
+                st = replay(req, arr)
+            finally:
+                capi.lib().stp_server_destroy(h)
+            out[str(rate)] = {"requests": int(st.Requests), "p50_ms": round(st.P50, 3), "p99_ms": round(st.P99, 3),
+                              "p99_9_ms": round(st.P999, 3), "max_ms": round(st.Max, 3),
+                              "mpixel_s": round(st.MpixelPerSecond, 1),
+                              "generated": {"requests": int(st.Generated), "p50_ms": round(st.GeneratedP50, 3),
+                                            "p99_ms": round(st.GeneratedP99, 3), "p99_9_ms": round(st.GeneratedP999, 3),
+                                            "max_ms": round(st.GeneratedMax, 3),
+                                            "mpixel_s": round(st.GeneratedMpixelPerSecond, 1)},
+                              "deadline_miss": round(st.DeadlineMissFraction, 4), "batches": int(st.Batches),
+                              "chunks_per_batch": round(st.MeanBatchChunks, 2), "cache_hits": int(st.CacheHits),
+                              "evictions": int(st.Evictions)}
+            print(f"[bench] served {label} {rate}/s: p50 {st.P50:.2f} p99 {st.P99:.2f} ms; generated "
+                  f"{st.Generated}: p50 {st.GeneratedP50:.2f} p99 {st.GeneratedP99:.2f} p99.9 {st.GeneratedP999:.2f} ms, "
+                  f"{st.GeneratedMpixelPerSecond:.0f} Mpixel/s; {st.MeanBatchChunks:.2f} chunks/batch",
+                  file=sys.stderr, flush=True)
+        return out
+
+    walk = run(lambda rate, n: sv.trace(rate, count=n), sv.rates, "walk", sv.duration_s)
+    fly = run(lambda rate, n: sv.flythrough_trace(rate, count=n), sv.flythrough_rates, "flythrough", sv.fly_duration_s)
+    return {"walk": {"workload": "camera random walk 0.5 chunk / 16 ms (seed 3), view radius 8, LOD 64..2048 px by "
+                                 "distance, Poisson arrivals, 16 ms deadline, whole path per chunk as the batched "
+                                 "workload, 2 engines x 8 Mpx, 200 GB budget with LRU",
+                     "rates": walk},
+            "flythrough": {"workload": f"straight flight east at {sv.fly_step} chunk per 16 ms frame, view radius 8, "
+                                       "a chunk requested when it enters the view or its level of detail gets finer "
+                                       "(every request a chunk not generated before), Poisson arrivals",
+                           "rates": fly}}
 
 
 def world(plain_frames=6):
@@ -307,9 +717,9 @@ def layers(reps=10, maps=64, tuning=None, steps=5):
     print(f"[bench] biome layers: {ms:.3f} ms per {maps} x 512^2 maps, {mpx:.0f} Mpixel/s"
           + (f" (CPU oracle {cpu['mpixel_s']} Mpixel/s)" if cpu else ""), file=sys.stderr, flush=True)
     # the batched workload with its biome maps (chunks and filter halos) from the demo chain
-    cfg, sides, origins, props = workload()
+    cfg, sides, origins, lod, props = workload()
     capi.call("stp_layer_factory_create", arr, len(nodes), 6666, C.byref(reg), side, side, C.byref(h))
-    batch = make_batch(cfg, sides, origins, props, layers=h)
+    batch = make_batch(cfg, sides, origins, lod, props, layers=h)
     try:
         if tuning is not None:
             capi.call("stp_batch_set_tuning", batch, C.byref(tuning))
@@ -388,6 +798,53 @@ def splat(reps=10):
                         "(2 altitude + 1 gradient) rules", "ms_per_area": round(ms, 3), "mpixel_s": round(mpx, 1)}
 
 
+def erosion_roofline(tuning, stage_ms, counters):
+    """Roofline record of the erosion stage's kernel from this run's per-drop counters and its
+    HIP-event time."""
+    pixels, drops, steps, erode, deposit, killed = counters
+    tiled = tuning.ErosionTile > 0
+    kernel = f"erodeTileKERNEL<{tuning.ErosionThreads}>" if tiled else "erosionKERNEL"
+    # the tiled erosion launches once per colour batch; the global-atomic kernel once
+    launches = 4 if tiled else 1
+    launch_s = stage_ms[STAGES.index("erosion")] / 1e3 / launches
+    lds_bytes = 4 * (steps * LDS_ACCESS_STEP + erode * LDS_ACCESS_ERODE + deposit * LDS_ACCESS_DEPOSIT) / launches
+    lds = {"achieved": round(lds_bytes / launch_s / 1e9, 1), "peak": round(LDS_PEAK_GBS, 1), "unit": "GB/s",
+           "frac": round(lds_bytes / launch_s / 1e9 / LDS_PEAK_GBS, 4), "bytes_per_launch": int(lds_bytes),
+           "note": "LDS bank accesses of the drop steps (8 height reads per step, 25 brush-tap atomics per erode "
+                   "step, 4 per deposit step) x 4 B; the peak is 32 banks x 4 B per cycle per CU"}
+    # compulsory HBM bytes: each eroded height read and written once, each drop's tile-list entry
+    # read and its ledger and counters written
+    compulsory = (8 * pixels + 16 * drops) / launches
+    profiled = tiled and (tuning.ErosionTile, tuning.ErosionThreads) == EROSION_PROFILE_SHAPE
+    traffic = EROSION_HBM_BYTES_PER_LAUNCH if profiled else None
+    hbm = {"traffic": traffic, "compulsory_bytes_per_launch": int(compulsory),
+           "traffic_over_compulsory": round(traffic / compulsory, 2) if traffic else None,
+           "achieved": round(traffic / launch_s / 1e9, 1) if traffic else None, "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(traffic / launch_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+           "algorithmic_bytes_per_launch": int((steps * BYTES_STEP + erode * BYTES_ERODE + deposit * BYTES_DEPOSIT)
+                                               / launches),
+           "note": "traffic: rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch on this workload and shape, "
+                   "calibrated on the format kernel; the drop steps' algorithmic height reads and updates "
+                   "(algorithmic_bytes_per_launch) are served by the LDS window, not by HBM"}
+    common = {"kernel": kernel, "launches_per_step": launches, "launch_ms_avg": round(launch_s * 1e3, 4)}
+    if profiled and EROSION_VALU_PER_WAVE_STEP:
+        valu = EROSION_VALU_PER_WAVE_STEP * (steps / 64.0) / launches
+        g = valu / launch_s / 1e9
+        return {"bound": "valu", "achieved": round(g, 1), "peak": VALU_PEAK_GINSTR, "unit": "G VALU instructions/s",
+                "frac": round(g / VALU_PEAK_GINSTR, 4), "traffic": traffic, **common,
+                "valu_per_launch": int(valu), "valu_per_64_drop_steps": EROSION_VALU_PER_WAVE_STEP,
+                "lds": lds, "hbm": hbm,
+                "note": "the kernel's most used resource is VALU instruction issue: its VALU instructions per 64 "
+                        "drop steps (SQ_INSTS_VALU of a rocprofv3 run of this workload and shape) x this run's drop "
+                        "steps, over its launch time, against one wave64 FP32 VALU instruction per cycle per CU "
+                        "(its integer instructions can issue faster, so the fraction is an upper estimate of its "
+                        "VALU use); its LDS pipe waits on bank conflicts (lds), and HBM "
+                        "carries 2.5x the compulsory bytes (hbm)"}
+    return {"bound": "lds", **{k: lds[k] for k in ("achieved", "peak", "unit", "frac")}, "traffic": traffic,
+            **common, "lds": lds, "hbm": hbm,
+            "note": "no instruction profile for this launch shape: LDS bank-access view"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -397,8 +854,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default erosion launch shape")
-    ap.add_argument("--no-served", action="store_true", help="leave out the camera-walk serving replays")
-    ap.add_argument("--no-world", action="store_true", help="leave out the world-pipeline camera walk")
+    ap.add_argument("--no-served", action="store_true", help="leave out the serving replays")
+    ap.add_argument("--no-world", action="store_true", help="leave out the world, splatmap and layer legs")
+    ap.add_argument("--no-configs", action="store_true", help="leave out the per-configuration legs")
+    ap.add_argument("--no-validate", action="store_true", help="leave out the oracle check of the timed run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -407,9 +866,9 @@ def main():
 
     from superterrainplus_amd import capi
     capi.call("stp_engine_initialise", 0)
-    cfg, sides, origins, props = workload()
+    cfg, sides, origins, steps, props = workload()
     print(f"[bench] {len(sides)} chunks, {sum(s * s for s in sides) / 1e6:.1f} Mpixel", file=sys.stderr, flush=True)
-    batch = make_batch(cfg, sides, origins, props)
+    batch = make_batch(cfg, sides, origins, steps, props)
     stream = C.c_void_p()
     capi.call("stp_stream_create", C.byref(stream))
     tuning = capi.BatchTuning()
@@ -449,7 +908,8 @@ def main():
     single_wall = (time.perf_counter() - t1) / args.steps
     counters = (C.c_uint64 * 6)()
     capi.call("stp_batch_counters", batch, counters, stream)
-    pixels, drops, steps, erode, deposit, killed = [int(v) for v in counters]
+    counters = [int(v) for v in counters]
+    pixels, drops, dsteps, erode, deposit, killed = counters
     bins = C.c_uint64()
     capi.call("stp_batch_bin_total", batch, C.byref(bins), stream)
     bins = int(bins.value)
@@ -473,25 +933,22 @@ def main():
         lat = np.array(lat)
         print(f"[bench] plain {plain_mpx:.1f} Mpixel/s", file=sys.stderr, flush=True)
 
-    erosion_bytes = steps * BYTES_STEP + erode * BYTES_ERODE + deposit * BYTES_DEPOSIT
-    # the tiled erosion launches once per colour batch; the global-atomic kernel once
-    kernel = f"erodeTileKERNEL<{tuning.ErosionThreads}>" if tuning.ErosionTile else "erosionKERNEL"
-    launches = 4 if tuning.ErosionTile else 1
-    erosion_s = stage_ms[STAGES.index("erosion")] / 1e3
-    achieved = erosion_bytes / erosion_s / 1e9 if erosion_s > 0 else 0.0
     # every biome of the workload has cfg.fractal.octave octaves
     blend_flop = bins * (cfg.fractal.octave * FLOP_PER_OCTAVE + FLOP_PER_BIN)
     blend_s = stage_ms[STAGES.index("blend")] / 1e3
     blend_tflops = blend_flop / blend_s / 1e12 if blend_s > 0 else 0.0
-    atomics = erode * ATOMICS_ERODE + deposit * ATOMICS_DEPOSIT
+    erosion_s = stage_ms[STAGES.index("erosion")] / 1e3
+    atomics = erode * LDS_ACCESS_ERODE + deposit * LDS_ACCESS_DEPOSIT
     result = {
         "metric": METRIC, "value": round(batched_mpx, 2), "unit": "Mpixel/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {
-            "workload": "batched: 256 chunks, sides {128,256,512,1024} p={.4,.3,.2,.1} seed 11, 4-biome Voronoi "
-                        "(seed 7, cell 256) blend r=16, 8-octave simplex, 0.25 drops/px parallel erosion, UINT16",
+            "workload": "batched: 256 chunks, sides {128,256,512,1024} p={.4,.3,.2,.1} seed 11, each covering a "
+                        "1024-unit slot (LOD 1024/side), 4-biome Voronoi (seed 7, cell 256) blend r=16, 8-octave "
+                        "simplex, 0.25 drops/px parallel erosion, UINT16",
             "chunks": len(sides), "mpixel": round(pixels / 1e6, 3), "drops": drops,
+            "lod_steps": {str(s): 1024 // s for s in sorted(set(sides))},
             "plain_mpixel_s": round(plain_mpx, 2) if plain_mpx else None,
             "batched_vs_plain": round(batched_mpx / plain_mpx, 2) if plain_mpx else None,
             "plain_chunk_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
@@ -501,7 +958,7 @@ def main():
             "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES, stage_ms)},
             "erosion_tuning": {"tile": tuning.ErosionTile, "threads": tuning.ErosionThreads,
                                "autotuned": not args.no_autotune},
-            "erosion_counters": {"steps": steps, "erode_steps": erode, "deposit_steps": deposit, "killed": killed},
+            "erosion_counters": {"steps": dsteps, "erode_steps": erode, "deposit_steps": deposit, "killed": killed},
             "erosion_atomics": {"per_step": atomics, "per_s": round(atomics / erosion_s, 1) if erosion_s > 0 else None,
                                 "note": "LDS integer atomics (25 per erode step, 4 per deposit step) over the "
                                         "erosion stage's event time"},
@@ -511,33 +968,37 @@ def main():
                                "note": "algorithmic FLOP = bins x (octaves x 66 + 3); the peak counts packed "
                                        "FP32, which this build does not emit"},
         },
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TILED_EROSION_TRAFFIC if (tuning.ErosionTile, tuning.ErosionThreads) == (128, 1024) else None,
-                     "kernel": kernel, "launches_per_step": launches,
-                     "launch_ms_avg": round(stage_ms[STAGES.index("erosion")] / launches, 4),
-                     "algorithmic_bytes_per_step": erosion_bytes,
-                     "note": "algorithmic bytes = the drop steps' height reads and updates (32 B per step, "
-                             "+300 B per erode step, +32 B per deposit step); the LDS-tiled kernel serves "
-                             "them from LDS, so they can exceed the HBM peak"},
+        "roofline": erosion_roofline(tuning, stage_ms, counters),
     }
+    threads, avail = host_threads()
+    if not args.no_validate:
+        result["config"]["accuracy"] = validate(cfg, sides, origins, steps, props, batch, threads)
+        worst = max(r["rms_vs_serialized_fp32"] / r["rms_serialized_other_drops"] for r in result["config"]["accuracy"]["chunks"])
+        print(f"[bench] accuracy: pre-erosion bit-exact "
+              f"{all(r['pre_erosion_bit_exact_fp32'] for r in result['config']['accuracy']['chunks'])}, eroded rms "
+              f"<= {worst:.2f} x the serialized floor", file=sys.stderr, flush=True)
+    capi.call("stp_batch_destroy", batch)
+    if not args.no_configs:
+        result["config"]["chunks_512"] = chunks_512(stream)
+        result["config"]["reference_api"] = generator_configs()
+        result["config"]["multi_biome_batch"] = multi_biome_batch(stream)
     if not args.no_served:
-        result["config"]["served"] = {
-            "workload": "camera random walk 0.5 chunk / 16 ms (seed 3), view radius 8, LOD 64..2048 px by distance, "
-                        "Poisson arrivals, 16 ms deadline, whole path per chunk as the batched workload, "
-                        "2 engines x 8 Mpx, 200 GB budget with LRU",
-            "rates": served()}
+        result["config"]["served"] = served()
     if not args.no_world:
         result["config"]["world"] = world()
         result["config"]["splatmap"] = splat()
         result["config"]["biome_layers"] = layers(tuning=tuning)
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        result["cpu_baseline"] = cpu_baseline(cfg, sides, origins, props, args.cpu_seconds, threads)
-        result["config"]["cpu_baseline_fp64_1core"] = cpu_baseline_fp64(cfg, sides, origins, props,
-                                                                         min(8.0, args.cpu_seconds / 2))
-        print(f"[bench] cpu: FP32 {result['cpu_baseline']['value']:.2f} Mpixel/s on {threads} threads, FP64 "
+        result["cpu_baseline"] = cpu_baseline(cfg, sides, origins, steps, props, args.cpu_seconds, threads)
+        result["cpu_baseline"]["host"] = {"os_cpu_count": os.cpu_count(), "affinity": avail,
+                                          "note": "threads = the cores this process may use, at most 16 (one GPU's "
+                                                  "share of the host)"}
+        result["config"]["cpu_baseline_fp64_1core"] = cpu_baseline(cfg, sides, origins, steps, props,
+                                                                   min(8.0, args.cpu_seconds / 2), 1, f64=True)
+        print(f"[bench] cpu: FP32 {result['cpu_baseline']['value']:.2f} Mpixel/s on {threads} threads "
+              f"(os.cpu_count {os.cpu_count()}, affinity {avail}), FP64 "
               f"{result['config']['cpu_baseline_fp64_1core']['value']} Mpixel/s on one", file=sys.stderr, flush=True)
-    capi.call("stp_batch_destroy", batch)
+    capi.call("stp_stream_destroy", stream)
     print(json.dumps(result))
 
 

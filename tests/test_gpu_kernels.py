@@ -125,8 +125,10 @@ def test_erosion_serialized_bit_exact_free_slip(gpu, oracle, table, host_table):
 
 
 def test_erosion_parallel_free_slip_statistics(gpu, oracle, table, host_table):
-    """The free-slip workload: 3x3 chunks of 512x512 merged to 1536x1536, 65536 drops in
-    parallel (FP32 atomics) against the serialized FP32 oracle on the same input.
+    """The free-slip workload: 3x3 chunks of 512x512 at chunk coordinates (-1..1)^2, each sampled
+    at its world offset (cx * 512, cy * 512) with half dimension 256, merged to 1536x1536 by the
+    device merge kernel (bit-exact against the oracle's per-chunk maps and free-slip index mapping),
+    then 65536 drops in parallel (FP32 atomics) against the serialized FP32 oracle on the same input.
 
     Tolerances (parallel and serialized orders are both valid erosions, but the algorithm is
     chaotic, so cells differ once drop paths diverge):
@@ -139,8 +141,15 @@ def test_erosion_parallel_free_slip_statistics(gpu, oracle, table, host_table):
     perm, g32, _ = host_table
     cfg = W.free_slip()
     side = cfg.side * 3
-    d = gpu_noise(table, side, side, (-512.0, -512.0))
+    chunks = [gpu_noise(table, cfg.side, cfg.side, off) for off in cfg.chunk_offsets()]
+    d = capi.DeviceArray((side, side), np.float32)
+    capi.call("stp_nn_merge", (C.c_void_p * 9)(*[c.ptr for c in chunks]), d.vp, 4,
+              C.byref(capi.nn_info((cfg.side, cfg.side), (3, 3))), None)
+    capi.synchronize()
+    want = oracle.nn_merge([oracle.fractal_map(perm, g32, oracle_desc(oracle, cfg.side, cfg.side, off), cfg.side,
+                                               cfg.side) for off in cfg.chunk_offsets()], (3, 3))
     before = d.download().astype(np.float64)
+    assert before.astype(np.float32).tobytes() == want.tobytes()
     s = W.erosion_setting(cfg.drops, cfg.erosion_seed)
     led, info = gpu_erode(d, (512, 512), (3, 3), s, mode=0)
     after = d.download().astype(np.float64)
