@@ -75,14 +75,13 @@ namespace SuperTerrainPlus::STPAlgorithm {
 				return v > 0.0f ? static_cast<int>(v) : static_cast<int>(v - 1.0f);
 			}
 
-			// A corner with w <= 0 contributes 0. The product is evaluated on every lane and
-			// selected, not branched around: the same bits (a NaN w still yields the product, as
-			// the branch would), without an execution-mask branch per corner and octave.
 			__device__ inline float corner(const float dx, const float dy, const float2 g) {
-				const float w = 0.5f - dx * dx - dy * dy;
-				const float w2 = w * w;
-				const float r = w2 * w2 * (g.x * dx + g.y * dy);
-				return w <= 0.0f ? 0.0f : r;
+				float w = 0.5f - dx * dx - dy * dy;
+				if (w <= 0.0f) {
+					return 0.0f;
+				}
+				w *= w;
+				return w * w * (g.x * dx + g.y * dy);
 			}
 
 			// Shared skew/unskew; Fetch(jj, ii, i1, j1) returns the three corner gradients.
