@@ -64,8 +64,8 @@ FP32_PEAK_TFLOPS = 157.3
 # instructions per 64 drop steps (SQ_INSTS_VALU / (drop steps / 64)), and HBM bytes per launch
 # (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on the format kernel).
 EROSION_PROFILE_SHAPE = (128, 1024)
-EROSION_VALU_PER_WAVE_STEP = 186.9
-EROSION_HBM_BYTES_PER_LAUNCH = 359.5e6
+EROSION_VALU_PER_WAVE_STEP = 187.5
+EROSION_HBM_BYTES_PER_LAUNCH = 270.6e6
 # serving loop: two engines of 8 Mpx / 512 chunks each
 SERVER_BATCH_PIXELS, SERVER_BATCH_CHUNKS, SERVER_ENGINES = 8 << 20, 512, 2
 

@@ -1415,28 +1415,35 @@ namespace SuperTerrainPlus::STPBatch {
 		__syncthreads();
 		// No other workgroup of this colour batch touches the window, so the heights in HBM are
 		// still the ones the window was made from: a cell whose fixed-point value is unchanged
-		// keeps its exact FP32 height.
+		// keeps its exact FP32 height. A value q >= WindowExact is stored without that test: a
+		// height h >= 2^-5 has a float spacing of at least 2^-28, so h * 2^28 is an integer of at
+		// most 24 significant bits and fromFixedPoint(toFixedPoint(h)) == h bit for bit. Only the
+		// cells below read the map again.
 		bool outside = false;
 		for (unsigned int base = threadIdx.x; base < cells; base += WindowBatch * Threads) {
+			int q[WindowBatch];
 			float old[WindowBatch];
 			unsigned int at[WindowBatch];
 			WindowCursor k = walk.start(base);
 #pragma unroll
 			for (unsigned int j = 0u; j < WindowBatch; j++) {
-				const WindowCursor w = base + j * Threads < cells ? k : last;
+				const bool in = base + j * Threads < cells;
+				const WindowCursor w = in ? k : last;
 				at[j] = (s.Y0 + w.Y) * side + s.X0 + w.X;
-				old[j] = map[at[j]];
+				q[j] = window[in ? base + j * Threads : cells - 1u];
 				k = walk.next(k);
 			}
 #pragma unroll
 			for (unsigned int j = 0u; j < WindowBatch; j++) {
-				const unsigned int i = base + j * Threads;
-				if (i < cells) {
-					const int q = window[i];
-					if (q != toFixedPoint(old[j])) {
-						map[at[j]] = fromFixedPoint(q);
+				old[j] = q[j] < WindowExact ? map[at[j]] : 0.0f;
+			}
+#pragma unroll
+			for (unsigned int j = 0u; j < WindowBatch; j++) {
+				if (base + j * Threads < cells) {
+					if (q[j] >= WindowExact || q[j] != toFixedPoint(old[j])) {
+						map[at[j]] = fromFixedPoint(q[j]);
 					}
-					outside |= q < WindowLowest || q > WindowHighest;
+					outside |= q[j] < WindowLowest || q[j] > WindowHighest;
 				}
 			}
 		}

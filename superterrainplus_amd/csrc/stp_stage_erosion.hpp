@@ -224,6 +224,11 @@ namespace SuperTerrainPlus::STPStage {
 	static_assert(static_cast<long long>(WindowHighest) == 15ll << (FixedBits - 1), "window bound overflows");
 	static_assert(WindowHighest > (7 << FixedBits) && WindowLowest < 0, "window bounds out of order");
 
+	// 2^-4 in fixed point: from here up every fixed-point value that toFixedPoint makes converts
+	// back to the height it came from, exactly (the float spacing at 2^-5 is 2^-28)
+	constexpr int WindowExact = 1 << (FixedBits - 4);
+	static_assert(WindowExact == 1 << 24, "exact store threshold moved");
+
 	__device__ inline int toFixedPoint(const float h) { return __float2int_rn(h * FixedScale); }
 	__device__ inline float fromFixedPoint(const int q) { return static_cast<float>(q) * FixedStep; }
 
