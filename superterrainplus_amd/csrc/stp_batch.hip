@@ -1729,7 +1729,8 @@ void STPHeightfieldBatch::bind(const std::vector<STPBatchChunk>& chunks, const s
 		c.Crop = slip;
 		const unsigned long long px = static_cast<unsigned long long>(in.Side) * in.Side,
 			spawnPx = static_cast<unsigned long long>(c.SpawnSide) * c.SpawnSide;
-		c.Drops = static_cast<unsigned int>(static_cast<double>(spawnPx) * this->Setting.DropsPerPixel);
+		c.Drops = in.Drops != STPBatchChunk::AutoDrops ? in.Drops
+													   : static_cast<unsigned int>(static_cast<double>(spawnPx) * this->Setting.DropsPerPixel);
 		c.Pixel = pixel;
 		c.Halo = halo;
 		c.Line = line;

@@ -40,6 +40,9 @@ namespace SuperTerrainPlus {
 		// erosion the centre is bit-exact with the chunk generated alone (needs the filter's item
 		// path: at most 8 biomes and a bin capacity of at most 8). 0 = no halo.
 		unsigned int ErosionHalo = 0u;
+		// Raindrops of the chunk; AutoDrops: the spawn area's pixels x DropsPerPixel.
+		static constexpr unsigned int AutoDrops = 0xFFFFFFFFu;
+		unsigned int Drops = AutoDrops;
 	};
 
 	// Arena sizes of a batch that is bound to different chunk lists over its lifetime.
@@ -138,6 +141,9 @@ namespace SuperTerrainPlus {
 		// stage 0 clear both; a run of stage 4 clears bit 2): 1 a histogram overflowed its bin capacity, 2 a
 		// tiled erosion window cell left [WindowLowest, WindowHighest] (synchronises).
 		unsigned int status(hipStream_t stream);
+		// Device address of the status word of bit 2 (nonzero when a tiled erosion window cell left its
+		// range; cleared by every run of stage 4), for work that decides on the device.
+		const unsigned int* windowStatus() const noexcept { return this->Flag.get() + 1; }
 		void readChunk(unsigned int i, float* height, unsigned short* fixed, unsigned short* halo, float* ledger) const;
 
 		// Switches the erosion launch shape (throws STPInvalidArgument if it cannot run here: the

@@ -1,5 +1,6 @@
 // Host classes of the path: nearest-neighbour texture buffer, single histogram filter front-end,
 // heightfield generator and the two built-in diversity generators.
+#include "stp_batch.hpp"
 #include "stp_internal.hpp"
 
 #include <algorithm>
@@ -350,6 +351,7 @@ namespace {
 
 struct STPHeightfieldGenerator::STPImpl {
 	STPRainDropParameter Rain;
+	STPEnvironment::STPRainDropSetting RainSetting;
 	STPNearestNeighbourInformation NoNeighbour, DiversityNeighbour, ErosionNeighbour;
 	STPDiversityGenerator& Diversity;
 	STPSmartDeviceMemory::STPDevice<int> BrushIndex;
@@ -360,14 +362,27 @@ struct STPHeightfieldGenerator::STPImpl {
 	std::vector<hipStream_t> FreeStream;
 	std::atomic<unsigned int> ErodeCounter {0u};
 	std::atomic<int> Mode {0};
+	// Parallel erosion of a square merged map runs on an erosion-only batch (LDS-tiled windows in
+	// colour batches) when the batch has a tiled launch shape for it; every call shares it, in turn.
+	std::mutex EroderLock;
+	bool EroderChecked = false, EroderUsed = false;
+	std::unique_ptr<STPHeightfieldBatch> Eroder;
+	STPSmartDeviceMemory::STPDevice<unsigned int> Gate; // 2 words, see STPHeightfieldKernel::heightGate
+	hipEvent_t EroderDone = nullptr;                     // the last call's work on the eroder
 
 	STPImpl(const STPEnvironment::STPChunkSetting& chunk, const STPEnvironment::STPRainDropSetting& rain,
 		STPDiversityGenerator& diversity) :
-		Rain(toParameter(rain)), NoNeighbour(neighbourInfo(chunk.MapSize, make_uint2(1u, 1u))),
+		Rain(toParameter(rain)), RainSetting(rain), NoNeighbour(neighbourInfo(chunk.MapSize, make_uint2(1u, 1u))),
 		DiversityNeighbour(neighbourInfo(chunk.MapSize, chunk.DiversityNearestNeighbour)),
 		ErosionNeighbour(neighbourInfo(chunk.MapSize, chunk.ErosionNearestNeighbour)), Diversity(diversity) {}
 
 	~STPImpl() {
+		if (this->EroderDone) {
+			// the eroder's arenas may still be in use by the last call's work
+			(void)hipEventSynchronize(this->EroderDone);
+			(void)hipEventDestroy(this->EroderDone);
+		}
+		this->Eroder.reset();
 		for (hipStream_t s : this->FreeStream) {
 			(void)hipStreamDestroy(s);
 		}
@@ -393,6 +408,112 @@ struct STPHeightfieldGenerator::STPImpl {
 	void release(const hipStream_t s) {
 		std::lock_guard<std::mutex> g(this->StreamLock);
 		this->FreeStream.push_back(s);
+	}
+
+	// The erosion-only batch for this generator's merged map, or null where the tiled erosion does
+	// not apply: a merged map that is not square, no lifetime bound (MaxLifetime = 0), or no tile
+	// shape whose window fits in LDS. Called with EroderLock held.
+	STPHeightfieldBatch* eroder() {
+		if (this->EroderChecked) {
+			return this->Eroder.get();
+		}
+		this->EroderChecked = true;
+		const STPNearestNeighbourInformation& nn = this->ErosionNeighbour;
+		const unsigned int side = nn.TotalMapSize.x;
+		if (nn.TotalMapSize.y != side || nn.ChunkNearestNeighbour.x != nn.ChunkNearestNeighbour.y || side > 16384u
+			|| this->Rain.MaxLifetime == 0u || this->Rain.RainDropCount == 0u) {
+			return nullptr;
+		}
+		// the batch's noise and biome stages never run: any valid table; a biome in [0, 1] lets the
+		// batch choose a tiled shape, and the heights the generator hands over are checked on the
+		// device before each erosion (heightGate)
+		const STPEnvironment::STPSimplexNoiseSetting noise {0ull, 8u, 0.0};
+		const std::vector<STPBiomeProperty> biome {STPBiomeProperty {1.0f, 1u, 0.5f, 2.0f, 0.0f, 1.0f}};
+		STPBatchSetting setting {};
+		setting.BiomeSeed = 0ull;
+		setting.BiomeCount = 1u;
+		setting.BiomeCell = 1u;
+		setting.Radius = 2u;
+		setting.BinCapacity = 1u;
+		setting.DropsPerPixel = 0.0f;
+		STPBatchCapacity cap {};
+		cap.Pixels = static_cast<unsigned long long>(side) * side + 4ull;
+		cap.Drops = this->Rain.RainDropCount;
+		cap.Chunks = 1u;
+		cap.MaxSide = side;
+		auto batch = std::make_unique<STPHeightfieldBatch>(noise, biome, this->RainSetting, setting, cap);
+		const STPBatchTuning tuning = batch->latencyTuning();
+		if (tuning.ErosionTile == 0u) {
+			return nullptr;
+		}
+		batch->setTuning(tuning);
+		this->Gate = STPSmartDeviceMemory::makeDevice<unsigned int>(2u);
+		STP_CHECK_HIP(hipEventCreateWithFlags(&this->EroderDone, hipEventDisableTiming));
+		this->Eroder = std::move(batch);
+		return this->Eroder.get();
+	}
+
+	// Parallel erosion of the merged map `map` on the generator's erosion-only batch, enqueued on
+	// `stream`; false (nothing enqueued) where that batch does not apply. The tiled window holds
+	// heights in [0, WindowHeightLimit] (a fixed-point LDS copy), so the batch erodes a copy of the
+	// map while heightGate checks the map's range on the device; when the range check or the
+	// window's own range flag fails, the global-atomic kernel erodes the map itself instead, and
+	// otherwise the copy replaces the map. Both paths run the same drops (seed, stream id, drop
+	// index and spawn rule: STPHeightfieldKernel.cu:93-102).
+	bool erodeTiled(float* const map, const unsigned int sid, const hipStream_t stream) {
+		STPImpl& g = *this;
+		const STPNearestNeighbourInformation& nn = g.ErosionNeighbour;
+		const std::lock_guard<std::mutex> guard(g.EroderLock);
+		STPHeightfieldBatch* const e = g.eroder();
+		if (!e) {
+			return false;
+		}
+		if (g.EroderUsed) {
+			// the previous call's work may still read the batch's arenas and the gate
+			STP_CHECK_HIP(hipStreamWaitEvent(stream, g.EroderDone, 0));
+		}
+		const unsigned int side = nn.TotalMapSize.x;
+		const unsigned long long px = static_cast<unsigned long long>(side) * side;
+		STPBatchChunk chunk {side, 0, 0, 1u, nn.ChunkNearestNeighbour.x};
+		chunk.Drops = g.Rain.RainDropCount;
+		try {
+			e->bind({chunk}, {sid}, stream);
+			STPHeightfieldKernel::heightGate(map, px, 0.0f, STPStage::WindowHeightLimit, g.Gate.get(), stream);
+			STP_CHECK_HIP(hipMemcpyAsync(e->heightOf(0u), map, px * sizeof(float), hipMemcpyDeviceToDevice, stream));
+			// drop binning and the tiled erosion
+			e->runStages(4, 6, stream);
+			STPHeightfieldKernel::copyGateStatus(e->windowStatus(), g.Gate.get(), stream);
+			STPHeightfieldKernel::hydraulicErosion(map, nn, g.Rain, g.BrushIndex.get(), g.BrushWeight.get(), g.BrushSize,
+				sid, STPHeightfieldKernel::STPErosionMode::Parallel, nullptr, nullptr, stream, g.Gate.get());
+			STPHeightfieldKernel::selectUngated(g.Gate.get(), e->heightOf(0u), map, px, stream);
+		} catch (...) {
+			// whatever was enqueued still orders the next call
+			(void)hipEventRecord(g.EroderDone, stream);
+			g.EroderUsed = true;
+			throw;
+		}
+		STP_CHECK_HIP(hipEventRecord(g.EroderDone, stream));
+		g.EroderUsed = true;
+		return true;
+	}
+
+	template <class FB, class XB>
+	void runErosion(const unsigned int sid, const FB& fbuf, const XB& xbuf) {
+		STPImpl& g = *this;
+		const typename FB::STPMergedBuffer fm(fbuf, FB::STPMemoryLocation::DeviceMemory);
+		const typename XB::STPMergedBuffer xm(xbuf, XB::STPMemoryLocation::DeviceMemory);
+		const hipStream_t stream = fbuf.DeviceMemInfo.second;
+		const STPNearestNeighbourInformation& nn = g.ErosionNeighbour;
+		const bool serial = static_cast<STPHeightfieldGenerator::STPErosionMode>(g.Mode.load())
+			== STPHeightfieldGenerator::STPErosionMode::Serialized;
+		if (serial || !this->erodeTiled(fm.getDevice(), sid, stream)) {
+			STPHeightfieldKernel::hydraulicErosion(fm.getDevice(), nn, g.Rain, g.BrushIndex.get(), g.BrushWeight.get(),
+				g.BrushSize, sid,
+				serial ? STPHeightfieldKernel::STPErosionMode::Serialized : STPHeightfieldKernel::STPErosionMode::Parallel,
+				nullptr, nullptr, stream);
+		}
+		STPHeightfieldKernel::formatHeightmap(fm.getDevice(), xm.getDevice(),
+			static_cast<unsigned long long>(nn.TotalMapSize.x) * nn.TotalMapSize.y, stream);
 	}
 };
 
@@ -503,23 +624,6 @@ void STPHeightfieldGenerator::prepareScratch() {
 	(void)scratchOnDevice();
 }
 
-namespace {
-	template <class FB, class XB>
-	void runErosion(STPHeightfieldGenerator::STPErosionMode mode, const STPNearestNeighbourInformation& nn,
-		const STPRainDropParameter& rain, const int* bi, const float* bw, const unsigned int bn, const unsigned int sid,
-		const FB& fbuf, const XB& xbuf) {
-		const typename FB::STPMergedBuffer fm(fbuf, FB::STPMemoryLocation::DeviceMemory);
-		const typename XB::STPMergedBuffer xm(xbuf, XB::STPMemoryLocation::DeviceMemory);
-		const hipStream_t stream = fbuf.DeviceMemInfo.second;
-		STPHeightfieldKernel::hydraulicErosion(fm.getDevice(), nn, rain, bi, bw, bn, sid,
-			mode == STPHeightfieldGenerator::STPErosionMode::Serialized ? STPHeightfieldKernel::STPErosionMode::Serialized
-																	 : STPHeightfieldKernel::STPErosionMode::Parallel,
-			nullptr, nullptr, stream);
-		STPHeightfieldKernel::formatHeightmap(fm.getDevice(), xm.getDevice(),
-			static_cast<unsigned long long>(nn.TotalMapSize.x) * nn.TotalMapSize.y, stream);
-	}
-} // namespace
-
 void STPHeightfieldGenerator::erode(STPHeightFloat_t* const* const heightfield_original,
 	STPHeightFixed_t* const* const heightfield_low) {
 	STPImpl& g = *this->Impl;
@@ -527,8 +631,7 @@ void STPHeightfieldGenerator::erode(STPHeightFloat_t* const* const heightfield_o
 	try {
 		const STPNearestNeighbourHeightFloatRWTextureBuffer fbuf(heightfield_original, g.ErosionNeighbour, {g.Pool, stream});
 		const STPNearestNeighbourHeightFixedWTextureBuffer xbuf(heightfield_low, g.ErosionNeighbour, {g.Pool, stream});
-		runErosion(static_cast<STPErosionMode>(g.Mode.load()), g.ErosionNeighbour, g.Rain, g.BrushIndex.get(),
-			g.BrushWeight.get(), g.BrushSize, g.ErodeCounter++, fbuf, xbuf);
+		g.runErosion(g.ErodeCounter++, fbuf, xbuf);
 		STP_CHECK_HIP(hipStreamSynchronize(stream));
 	} catch (...) {
 		(void)hipStreamSynchronize(stream);
@@ -545,8 +648,7 @@ void STPHeightfieldGenerator::erodeDevice(STPHeightFloat_t* const* const heightf
 	using XB = STPNearestNeighbourHeightFixedWTextureBuffer;
 	const FB fbuf(heightfield_original, g.ErosionNeighbour, {g.Pool, stream}, FB::STPChunkLocation::Device);
 	const XB xbuf(heightfield_low, g.ErosionNeighbour, {g.Pool, stream}, XB::STPChunkLocation::Device);
-	runErosion(static_cast<STPErosionMode>(g.Mode.load()), g.ErosionNeighbour, g.Rain, g.BrushIndex.get(),
-		g.BrushWeight.get(), g.BrushSize, g.ErodeCounter++, fbuf, xbuf);
+	g.runErosion(g.ErodeCounter++, fbuf, xbuf);
 }
 
 void STPHeightfieldGenerator::erodeDeviceNoAlloc(STPHeightFloat_t* const* const heightfield_original,

@@ -79,10 +79,20 @@ namespace SuperTerrainPlus {
 			hipStream_t stream);
 
 		// --- erosion / format (stp_kernels_erosion.hip)
+		// gate (parallel mode only, device words): when not null the drops run only if gate[0] or
+		// gate[1] is nonzero, so the decision can be made on the device by earlier work on `stream`
 		void hydraulicErosion(float* height, const STPNearestNeighbourInformation& nn, const STPRainDropParameter& rd,
 			const int* brush_index, const float* brush_weight, unsigned int brush_size, unsigned int stream_id,
-			STPErosionMode mode, float* ledger, uint2* drop_info, hipStream_t stream);
+			STPErosionMode mode, float* ledger, uint2* drop_info, hipStream_t stream, const unsigned int* gate = nullptr);
 		void formatHeightmap(const float* input, STPHeightFixed_t* output, unsigned long long count, hipStream_t stream);
+		// gate[0] = 1 if a height of map[0, count) lies outside [low, high] or is NaN, else 0; gate[1] = 0
+		void heightGate(const float* map, unsigned long long count, float low, float high, unsigned int* gate,
+			hipStream_t stream);
+		// gate[1] = *status
+		void copyGateStatus(const unsigned int* status, unsigned int* gate, hipStream_t stream);
+		// map[i] = source[i] for i < count, unless gate[0] or gate[1] is nonzero
+		void selectUngated(const unsigned int* gate, const float* source, float* map, unsigned long long count,
+			hipStream_t stream);
 
 		// --- free-slip merge (stp_kernels_nn.hip)
 		void mergeNeighbour(const void* const* chunks, void* merged, unsigned int element_size,

@@ -27,8 +27,13 @@ namespace {
 	constexpr unsigned int ParallelBlock = 256u;
 
 	__global__ void __launch_bounds__(ParallelBlock) erosionParallelKERNEL(float* const map, const ErosionArgs a,
-		const int* const brush_index, const float* const brush_weight, float* const ledger, uint2* const drop_info) {
+		const int* const brush_index, const float* const brush_weight, float* const ledger, uint2* const drop_info,
+		const unsigned int* const gate) {
 		extern __shared__ int2 brush[]; // (index, weight bits)
+		if (gate && (gate[0] | gate[1]) == 0u) {
+			// another path eroded the map: the whole grid leaves at once
+			return;
+		}
 		for (unsigned int k = threadIdx.x; k < a.BrushSize; k += blockDim.x) {
 			brush[k] = make_int2(brush_index[k], __float_as_int(brush_weight[k]));
 		}
@@ -133,12 +138,84 @@ namespace {
 		}
 	}
 
+	// gate words: [0] a height out of range, [1] a status word copied from other work
+	__global__ void gateClearKERNEL(unsigned int* const gate) {
+		gate[0] = 0u;
+		gate[1] = 0u;
+	}
+
+	constexpr unsigned int GateBlock = 256u, GateItems = 16u;
+
+	__global__ void __launch_bounds__(GateBlock) heightGateKERNEL(const float* const map, const unsigned long long count,
+		const float low, const float high, unsigned int* const gate) {
+		const unsigned long long first = static_cast<unsigned long long>(blockIdx.x) * GateBlock * GateItems + threadIdx.x;
+		bool out = false;
+#pragma unroll
+		for (unsigned int k = 0u; k < GateItems; k++) {
+			const unsigned long long i = first + static_cast<unsigned long long>(k) * GateBlock;
+			if (i < count) {
+				const float h = map[i];
+				// NaN fails both comparisons
+				out = out || !(h >= low && h <= high);
+			}
+		}
+		if (__any(out) && (threadIdx.x & 63u) == 0u) {
+			atomicOr(gate, 1u);
+		}
+	}
+
+	__global__ void gateStatusKERNEL(const unsigned int* const status, unsigned int* const gate) { gate[1] = *status; }
+
+	__global__ void __launch_bounds__(256) selectUngatedKERNEL(const unsigned int* const gate, const float* const source,
+		float* const map, const unsigned long long count) {
+		if ((gate[0] | gate[1]) != 0u) {
+			return;
+		}
+		const unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+		if (i < count) {
+			map[i] = source[i];
+		}
+	}
+
 } // namespace
+
+void STPHeightfieldKernel::heightGate(const float* const map, const unsigned long long count, const float low,
+	const float high, unsigned int* const gate, const hipStream_t stream) {
+	if (!map || !gate) {
+		STP_THROW(STPInvalidArgument, "height gate needs a map and a gate");
+	}
+	gateClearKERNEL<<<1, 1, 0, stream>>>(gate);
+	const unsigned long long per = static_cast<unsigned long long>(GateBlock) * GateItems;
+	if (count != 0ull) {
+		heightGateKERNEL<<<static_cast<unsigned int>((count + per - 1ull) / per), GateBlock, 0, stream>>>(map, count, low, high,
+			gate);
+	}
+	STP_CHECK_HIP(hipGetLastError());
+}
+
+void STPHeightfieldKernel::copyGateStatus(const unsigned int* const status, unsigned int* const gate, const hipStream_t stream) {
+	if (!status || !gate) {
+		STP_THROW(STPInvalidArgument, "gate status needs a status word and a gate");
+	}
+	gateStatusKERNEL<<<1, 1, 0, stream>>>(status, gate);
+	STP_CHECK_HIP(hipGetLastError());
+}
+
+void STPHeightfieldKernel::selectUngated(const unsigned int* const gate, const float* const source, float* const map,
+	const unsigned long long count, const hipStream_t stream) {
+	if (!gate || !source || !map) {
+		STP_THROW(STPInvalidArgument, "select needs a gate, a source and a map");
+	}
+	if (count != 0ull) {
+		selectUngatedKERNEL<<<static_cast<unsigned int>((count + 255ull) / 256ull), 256, 0, stream>>>(gate, source, map, count);
+	}
+	STP_CHECK_HIP(hipGetLastError());
+}
 
 void STPHeightfieldKernel::hydraulicErosion(float* const height, const STPNearestNeighbourInformation& nn,
 	const STPRainDropParameter& rd, const int* const brush_index, const float* const brush_weight,
 	const unsigned int brush_size, const unsigned int stream_id, const STPErosionMode mode, float* const ledger,
-	uint2* const drop_info, const hipStream_t stream) {
+	uint2* const drop_info, const hipStream_t stream, const unsigned int* const gate) {
 	if (!height || !brush_index || !brush_weight || brush_size == 0u) {
 		STP_THROW(STPInvalidArgument, "erosion needs a heightmap and a non-empty brush");
 	}
@@ -163,8 +240,11 @@ void STPHeightfieldKernel::hydraulicErosion(float* const height, const STPNeares
 	if (mode == STPErosionMode::Parallel) {
 		const unsigned int grid = (rd.RainDropCount + ParallelBlock - 1u) / ParallelBlock;
 		erosionParallelKERNEL<<<grid, ParallelBlock, brush_size * sizeof(int2), stream>>>(height, a, brush_index,
-			brush_weight, ledger, drop_info);
+			brush_weight, ledger, drop_info, gate);
 	} else {
+		if (gate) {
+			STP_THROW(STPInvalidArgument, "a gated erosion is parallel");
+		}
 		erosionSerialKERNEL<<<1, SerialLanes, 0, stream>>>(height, a, brush_index, brush_weight, ledger, drop_info);
 	}
 	STP_CHECK_HIP(hipGetLastError());
