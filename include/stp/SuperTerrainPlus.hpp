@@ -370,6 +370,7 @@ namespace SuperTerrainPlus {
 	public:
 		STPBiomefieldGenerator(const STPEnvironment::STPSimplexNoiseSetting& noise, std::vector<STPBiomeProperty> table,
 			uint2 map_size, unsigned int interpolation_radius, unsigned int bin_capacity = 16u);
+		~STPBiomefieldGenerator() override;
 		void operator()(const STPNearestNeighbourHeightFloatWTextureBuffer& heightmap_buffer,
 			const STPNearestNeighbourSampleRTextureBuffer& biomemap_buffer, float2 offset) override;
 		const STPAlgorithm::STPPermutation& permutation() const noexcept;
@@ -379,6 +380,12 @@ namespace SuperTerrainPlus {
 		std::vector<STPBiomeProperty> BiomeTable;
 		uint2 MapSize;
 		unsigned int InterpolationRadius, BinCapacity;
+		// The batched path's filter (lines split into segments that run in parallel) for square maps
+		// of at most 8 biomes and bins; false where it does not apply (nothing written then).
+		struct STPFilterEngine;
+		std::unique_ptr<STPFilterEngine> Engine;
+		bool filterBySegments(const STPNearestNeighbourHeightFloatWTextureBuffer& heightmap_buffer,
+			const STPNearestNeighbourSampleRTextureBuffer& biomemap_buffer, float2 offset);
 	};
 
 } // namespace SuperTerrainPlus

@@ -196,6 +196,40 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 	};
 
+	// ExternalBiome batches: the stage-0 tile's halo samples copied from the caller's map. Halo
+	// sample (x, y) of chunk c is origin[(OriginY + y - r) * pitch + OriginX + x - r]; a sample >= limit
+	// (the item path's biome count) is stored as 0, which keeps the per-biome registers of the
+	// filter passes in range, and raises flag[2].
+	__global__ void __launch_bounds__(BiomeTile) externalBiomeKERNEL(const uint2* const tiles, const Chunk* const chunks,
+		const STPSample_t* const origin, const unsigned int pitch, const unsigned int radius, const unsigned int limit,
+		unsigned short* const biome, unsigned int* const flag) {
+		const uint2 t = tiles[blockIdx.x];
+		const Chunk c = chunks[t.x];
+		const unsigned int w = c.Side + 2u * radius, tilesX = (w + BiomeSide - 1u) / BiomeSide;
+		const unsigned int x = (t.y % tilesX) * BiomeSide + threadIdx.x % BiomeSide,
+			y0 = (t.y / tilesX) * BiomeSide + threadIdx.x / BiomeSide;
+		bool bad = false;
+		if (x < w) {
+			const long long col = static_cast<long long>(c.OriginX) + x - radius;
+			for (unsigned int j = 0u; j < BiomeSide; j += BiomeRows) {
+				const unsigned int y = y0 + j;
+				if (y >= w) {
+					break;
+				}
+				const long long row = static_cast<long long>(c.OriginY) + y - radius;
+				unsigned int v = origin[row * static_cast<long long>(pitch) + col];
+				if (v >= limit) {
+					v = 0u;
+					bad = true;
+				}
+				biome[c.Halo + static_cast<size_t>(y) * w + x] = static_cast<unsigned short>(v);
+			}
+		}
+		if (bad) {
+			atomicOr(flag + 2, 1u);
+		}
+	}
+
 	// vcount[col * side + y], vbin[(col * vcap + k) * side + y] (per chunk at Line, Line * vcap)
 	template <unsigned int MaxCap>
 	__global__ void __launch_bounds__(ShfLineBlock) verticalKERNEL(const uint2* const tiles, const Chunk* const chunks,
@@ -1592,6 +1626,9 @@ void STPHeightfieldBatch::init(const STPEnvironment::STPRainDropSetting& rain) {
 	if (setting.BiomeCount == 0u || setting.BiomeCell == 0u || !(setting.DropsPerPixel >= 0.0f)) {
 		STP_THROW(STPInvalidArgument, "invalid batch setting");
 	}
+	if (setting.ExternalBiome && setting.BiomeLayers) {
+		STP_THROW(STPInvalidArgument, "a batch's biome map comes from a layer chain or an external map, not both");
+	}
 	const STPBatchCapacity& cap = this->Capacity;
 	if (cap.Chunks == 0u || cap.Pixels == 0ull || cap.MaxSide < 2u * r.ErosionBrushRadius + 2u || cap.MaxSide > 16384u) {
 		STP_THROW(STPInvalidArgument, "invalid batch capacity");
@@ -1844,6 +1881,42 @@ float* STPHeightfieldBatch::heightOf(const unsigned int i) const {
 		STP_THROW(STPInvalidArgument, "chunk index out of range");
 	}
 	return this->Height.get() + this->Record[i].Pixel;
+}
+
+const unsigned char* STPHeightfieldBatch::binCountOf(const unsigned int i) const {
+	if (i >= this->Record.size()) {
+		STP_THROW(STPInvalidArgument, "chunk index out of range");
+	}
+	return this->BinCount.get() + this->Record[i].Pixel;
+}
+
+const unsigned short* STPHeightfieldBatch::binItemOf(const unsigned int i) const {
+	if (i >= this->Record.size()) {
+		STP_THROW(STPInvalidArgument, "chunk index out of range");
+	}
+	return this->BinItem.get() + this->Record[i].Pixel * this->Setting.BinCapacity;
+}
+
+const float* STPHeightfieldBatch::binWeightOf(const unsigned int i) const {
+	if (i >= this->Record.size()) {
+		STP_THROW(STPInvalidArgument, "chunk index out of range");
+	}
+	return this->BinWeight.get() + this->Record[i].Pixel * this->Setting.BinCapacity;
+}
+
+void STPHeightfieldBatch::setBiomeSource(const STPSample_t* const origin, const unsigned int pitch) {
+	if (!this->Setting.ExternalBiome) {
+		STP_THROW(STPInvalidArgument, "a biome source is for batches with ExternalBiome");
+	}
+	if (!origin || pitch == 0u) {
+		STP_THROW(STPInvalidArgument, "null biome source or zero pitch");
+	}
+	if (origin != this->BiomeSource || pitch != this->BiomeSourcePitch) {
+		// captured stage graphs hold the previous source
+		this->releaseGraphs();
+	}
+	this->BiomeSource = origin;
+	this->BiomeSourcePitch = pitch;
 }
 
 unsigned short* STPHeightfieldBatch::fixedOf(const unsigned int i) const {
@@ -2140,6 +2213,18 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 		}
 		if (this->Setting.BiomeLayers) {
 			this->Setting.BiomeLayers->run(*this->LayerPlan, this->BiomeMap.get(), first, last, this->Setting.BiomeCount, stream);
+			break;
+		}
+		if (this->Setting.ExternalBiome) {
+			if (!this->BiomeSource) {
+				STP_THROW(STPInvalidArgument, "an external-biome batch runs stage 0 after setBiomeSource");
+			}
+			const auto [t, n] = range(0);
+			if (n) {
+				externalBiomeKERNEL<<<n, BiomeTile, 0, stream>>>(t, this->ChunkRecord, this->BiomeSource,
+					this->BiomeSourcePitch, radius, this->itemPath() ? this->Setting.BiomeCount : 0x10000u,
+					this->BiomeMap.get(), this->Flag.get());
+			}
 			break;
 		}
 		const auto [t, n] = range(0);
@@ -2495,13 +2580,16 @@ void STPHeightfieldBatch::check(const hipStream_t stream) {
 		STP_THROW(STPCapacityError, "an eroded height left the range of the LDS-tiled erosion window ([-4, 7.5]); "
 			"erode these chunks with the global-atomic kernel (erosion tile 0)");
 	}
+	if (flag & 4u) {
+		STP_THROW(STPInvalidArgument, "an external biome sample is not below the biome count (it was read as biome 0)");
+	}
 }
 
 unsigned int STPHeightfieldBatch::status(const hipStream_t stream) {
-	unsigned int flag[2] {};
+	unsigned int flag[3] {};
 	STP_CHECK_HIP(hipMemcpyAsync(flag, this->Flag.get(), sizeof(flag), hipMemcpyDeviceToHost, stream));
 	STP_CHECK_HIP(hipStreamSynchronize(stream));
-	return (flag[0] != 0u ? 1u : 0u) | (flag[1] != 0u ? 2u : 0u);
+	return (flag[0] != 0u ? 1u : 0u) | (flag[1] != 0u ? 2u : 0u) | (flag[2] != 0u ? 4u : 0u);
 }
 
 void STPHeightfieldBatch::readChunk(const unsigned int i, float* const height, unsigned short* const fixed,

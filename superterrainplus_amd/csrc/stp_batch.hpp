@@ -22,6 +22,9 @@ namespace SuperTerrainPlus {
 		// null: the synthetic Voronoi biome map; otherwise every chunk's biome map and filter halo come
 		// from this layer chain (ids >= BiomeCount are written as 0 and flagged by the factory)
 		STPDiversity::STPLayerBiomeFactory* BiomeLayers = nullptr;
+		// true (and no layer chain): every chunk's biome map and filter halo are copied from the map
+		// set by STPHeightfieldBatch::setBiomeSource, one sample per pixel (Step is not applied)
+		bool ExternalBiome = false;
 	};
 
 	struct STPBatchChunk {
@@ -110,6 +113,19 @@ namespace SuperTerrainPlus {
 		const STPBatchCapacity& capacity() const noexcept { return this->Capacity; }
 		// Device addresses of chunk i's outputs (valid until the next bind).
 		float* heightOf(unsigned int i) const;
+		// Device addresses of chunk i's filter output (valid until the next bind): per pixel a bin count
+		// and BinCapacity slots. With packedHistogram() a slot is item | count << 3 and the weight is
+		// count * (1 / (2r + 1)^2) (no weight array); otherwise slots hold items and binWeightOf the
+		// weights.
+		const unsigned char* binCountOf(unsigned int i) const;
+		const unsigned short* binItemOf(unsigned int i) const;
+		const float* binWeightOf(unsigned int i) const;
+		bool packedHistogram() const noexcept { return this->packedBins(); }
+		// The external biome map of ExternalBiome batches: halo sample (x, y) of chunk c, for x, y in
+		// [0, Side + 2 Radius), is origin[(c.OriginY + y - Radius) * pitch + c.OriginX + x - Radius]
+		// (device memory; the caller keeps every such sample inside its allocation). On the item path
+		// a sample >= BiomeCount is read as 0 and sets status bit 4.
+		void setBiomeSource(const STPSample_t* origin, unsigned int pitch);
 		unsigned short* fixedOf(unsigned int i) const;
 		// Enqueues copies of every bound chunk's outputs to height[i] / fixed[i] (device memory, one
 		// pointer per chunk) on `stream`, in one launch.
@@ -138,8 +154,9 @@ namespace SuperTerrainPlus {
 		// window cell left its range (synchronises).
 		void check(hipStream_t stream);
 		// Status bits of the runs since the flags were last cleared (construction and every run of
-		// stage 0 clear both; a run of stage 4 clears bit 2): 1 a histogram overflowed its bin capacity, 2 a
-		// tiled erosion window cell left [WindowLowest, WindowHighest] (synchronises).
+		// stage 0 clear them; a run of stage 4 clears bit 2): 1 a histogram overflowed its bin capacity, 2 a
+		// tiled erosion window cell left [WindowLowest, WindowHighest], 4 an external biome sample was not
+		// below BiomeCount on the item path (synchronises).
 		unsigned int status(hipStream_t stream);
 		// Device address of the status word of bit 2 (nonzero when a tiled erosion window cell left its
 		// range; cleared by every run of stage 4), for work that decides on the device.
@@ -217,6 +234,8 @@ namespace SuperTerrainPlus {
 		STPSmartDeviceMemory::STPDevice<float> BinWeight, Ledger;
 		STPSmartDeviceMemory::STPDevice<uint2> DropInfo;
 		STPSmartDeviceMemory::STPDevice<unsigned int> Flag;
+		const STPSample_t* BiomeSource = nullptr;
+		unsigned int BiomeSourcePitch = 0u;
 		STPSmartDeviceMemory::STPDevice<unsigned long long> Counter;
 		// the layer chain's plan of the bound chunks' halo windows (biome maps from a layer chain)
 		std::unique_ptr<STPDiversity::STPLayerPlan> LayerPlan;

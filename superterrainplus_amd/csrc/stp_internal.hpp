@@ -74,6 +74,10 @@ namespace SuperTerrainPlus {
 			unsigned int biome_count, const unsigned char* bin_count, const unsigned short* bin_item,
 			const float* bin_weight, unsigned int bin_stride, float* height, uint2 map_size, float2 offset,
 			hipStream_t stream);
+		// the dense blend over packed slots (item | count << 3, weight = count * norm)
+		void multiBiomeHeightmapPacked(const STPAlgorithm::STPPermutation& perm, const STPBiomeProperty* table_host,
+			unsigned int biome_count, const unsigned char* bin_count, const unsigned short* bin_slot, float norm,
+			unsigned int bin_stride, float* height, uint2 map_size, float2 offset, hipStream_t stream);
 		void voronoiBiomemap(unsigned long long seed, unsigned int biome_count, unsigned int cell_size, long long origin_x,
 			long long origin_y, STPSample_t* out, unsigned int width, unsigned int rows, unsigned int pitch,
 			hipStream_t stream);

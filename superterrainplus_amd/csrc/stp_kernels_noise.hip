@@ -75,9 +75,11 @@ namespace {
 
 	// The same blend over the dense fixed-stride histogram the device filter produces
 	// (bin_count[p], bin_item[p * stride + s], bin_weight[p * stride + s]).
+	// Packed: a slot is item | count << 3 and the weight count * norm (no weight array)
+	template <bool Packed>
 	__global__ void __launch_bounds__(TileX * TileY) multiBiomeDenseKERNEL(const STPPermutation perm,
 		const BiomeTable biomes, const unsigned int biome_count, const unsigned char* const bin_count,
-		const unsigned short* const bin_item, const float* const bin_weight, const unsigned int stride,
+		const unsigned short* const bin_item, const float* const bin_weight, const float norm, const unsigned int stride,
 		float* const height, const uint2 map_size, const float2 offset) {
 		__shared__ Noise::STPStagedPermutation table;
 		__shared__ STPBiomeProperty prop[STPHeightfieldKernel::MaxBiome];
@@ -99,8 +101,13 @@ namespace {
 				break;
 			}
 			const size_t index = x + static_cast<size_t>(y) * map_size.x;
-			height[index] = blendPixel(table, prop, biome_count, bin_count[index], bin_item + index * stride,
-				bin_weight + index * stride, x, y, offset, half);
+			if constexpr (Packed) {
+				height[index] = blendPacked(table, prop, biome_count, bin_count[index], bin_item + index * stride, norm,
+					x, y, offset, half);
+			} else {
+				height[index] = blendPixel(table, prop, biome_count, bin_count[index], bin_item + index * stride,
+					bin_weight + index * stride, x, y, offset, half);
+			}
 		}
 	}
 
@@ -163,8 +170,22 @@ void STPHeightfieldKernel::multiBiomeHeightmapDense(const STPPermutation& perm, 
 	const float2 offset, const hipStream_t stream) {
 	checkPermutation(perm);
 	const BiomeTable t = packBiomeTable(table_host, biome_count);
-	multiBiomeDenseKERNEL<<<tileGrid(map_size.x, map_size.y), dim3(TileX, TileY), 0, stream>>>(perm, t, biome_count,
-		bin_count, bin_item, bin_weight, bin_stride, height, map_size, offset);
+	multiBiomeDenseKERNEL<false><<<tileGrid(map_size.x, map_size.y), dim3(TileX, TileY), 0, stream>>>(perm, t, biome_count,
+		bin_count, bin_item, bin_weight, 0.0f, bin_stride, height, map_size, offset);
+	STP_CHECK_HIP(hipGetLastError());
+}
+
+void STPHeightfieldKernel::multiBiomeHeightmapPacked(const STPPermutation& perm, const STPBiomeProperty* const table_host,
+	const unsigned int biome_count, const unsigned char* const bin_count, const unsigned short* const bin_slot,
+	const float norm, const unsigned int bin_stride, float* const height, const uint2 map_size, const float2 offset,
+	const hipStream_t stream) {
+	checkPermutation(perm);
+	const BiomeTable t = packBiomeTable(table_host, biome_count);
+	if (!bin_count || !bin_slot || !height) {
+		STP_THROW(STPInvalidArgument, "histogram or heightmap pointer is null");
+	}
+	multiBiomeDenseKERNEL<true><<<tileGrid(map_size.x, map_size.y), dim3(TileX, TileY), 0, stream>>>(perm, t, biome_count,
+		bin_count, bin_slot, nullptr, norm, bin_stride, height, map_size, offset);
 	STP_CHECK_HIP(hipGetLastError());
 }
 

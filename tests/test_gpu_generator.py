@@ -134,3 +134,81 @@ def test_generator_biomefield_plugin(gpu, oracle):
     assert d_out.download().tobytes() == want.tobytes()
     del gen
     capi.call("stp_diversity_destroy", C.c_void_p(plugin))
+
+
+@pytest.mark.parametrize("depth", [0.0, 10.0])
+def test_generator_parallel_erosion(gpu, oracle, depth):
+    """Parallel erosion through the drop-in generator (host API, 3x3 free-slip neighbourhood).
+    Heights in [0, 1] (depth 0) erode on the LDS-tiled erosion batch; heights in [10, 11] lie
+    beyond the fixed-point window's range, so the device-side height gate hands the map to the
+    global-atomic kernel (on the window they would saturate near 8 and fail these bars). Both
+    meet the parallel-erosion bars against the serialized oracle run with the same drops (stream
+    id 0 of the generator's first erosion): total height change within 4/sqrt(drops) + 0.12 of
+    the serialized run's, RMS within 1.5x the RMS between two serialized runs with different drops."""
+    log(f"generator parallel erosion, depth {depth}")
+    drops = 4096
+    rain = W.erosion_setting(drops)
+    d = C.c_void_p()
+    biome = capi.BiomeProperty(*W.Fractal(depth=depth).biome())
+    capi.call("stp_diversity_simplex_create", C.byref(NOISE), C.byref(biome), (C.c_uint32 * 2)(MAP, MAP), C.byref(d))
+    try:
+        gen = Generator(d.value, chunk_setting(), rain, mode=0)
+        chunks = [np.zeros((MAP, MAP), np.float32) for _ in range(9)]
+        none_biome = (C.c_void_p * 1)(None)
+        for c, off in zip(chunks, OFFSETS):
+            capi.call("stp_heightfield_generate", C.c_void_p(gen.h), capi.ptr(c), none_biome, off[0], off[1])
+        pre = oracle.nn_merge(chunks, (3, 3))
+        assert depth <= float(pre.min()) and float(pre.max()) <= depth + 1.0
+        fixed = [np.zeros((MAP, MAP), np.uint16) for _ in range(9)]
+        capi.call("stp_heightfield_erode", C.c_void_p(gen.h), (C.c_void_p * 9)(*[c.ctypes.data for c in chunks]),
+                  (C.c_void_p * 9)(*[f.ctypes.data for f in fixed]))
+        del gen
+    finally:
+        capi.call("stp_diversity_destroy", d)
+    par = oracle.nn_merge(chunks, (3, 3)).astype(np.float64)
+    assert (oracle.nn_merge(fixed, (3, 3)) == oracle.format_heightmap(par.astype(np.float32))).all()
+
+    def serialized(seed):
+        e = pre.copy()
+        s = W.erosion_setting(drops, seed=seed)
+        oracle.erode(e, (3, 3), (MAP, MAP), oracle_rain(oracle, s), drops, seed, stream_id=0)
+        return e.astype(np.float64)
+
+    ser, other = serialized(rain.Seed), serialized(rain.Seed + 1000)
+    base = pre.astype(np.float64)
+    moved_par, moved_ser = float((base - par).sum()), float((base - ser).sum())
+    rms = float(np.sqrt(np.mean((par - ser) ** 2)))
+    floor = float(np.sqrt(np.mean((other - ser) ** 2)))
+    print(f"[depth {depth}] height removed {moved_par:.4f} (serialized {moved_ser:.4f}), rms {rms:.3e} "
+          f"vs floor {floor:.3e}", flush=True)
+    assert moved_ser > 0.0
+    assert abs(moved_par / moved_ser - 1.0) <= 4.0 / np.sqrt(drops) + 0.12, (moved_par, moved_ser)
+    assert rms <= 1.5 * floor, (rms, floor)
+
+
+def test_generator_biomefield_plugin_ids_beyond_the_table(gpu, oracle):
+    """Biome maps holding an id the plugin's table lacks (ids 0..4, 4 biomes): the segmented
+    filter refuses the map (its per-biome registers hold ids below the biome count) and the
+    line-by-line filter runs, which keeps id 4 as an item of its own that the blend skips. The
+    oracle indexes the table directly, so its table gets a fifth biome of Depth 0 and Variation 0,
+    whose products add +0.0: the same heights, bit for bit."""
+    log("generator biomefield plugin, ids beyond the table")
+    props = W.biome_table(4, 7)
+    d = C.c_void_p()
+    capi.call("stp_diversity_biomefield_create", C.byref(NOISE), capi.biome_array(props), 4,
+              (C.c_uint32 * 2)(MAP, MAP), 16, 8, C.byref(d))
+    try:
+        gen = Generator(d.value, chunk_setting(diversity_nn=(3, 3)), W.erosion_setting(100), mode=0)
+        biome_chunks = [oracle.voronoi_map(7, 5, 64, (int(off[0]), int(off[1])), MAP, MAP) for off in OFFSETS]
+        assert max(int(b.max()) for b in biome_chunks) == 4
+        out = np.zeros((MAP, MAP), np.float32)
+        capi.call("stp_heightfield_generate", C.c_void_p(gen.h), capi.ptr(out),
+                  (C.c_void_p * 9)(*[b.ctypes.data for b in biome_chunks]), 0.0, 0.0)
+        del gen
+    finally:
+        capi.call("stp_diversity_destroy", d)
+    perm, g32, _ = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+    item, weight, off = oracle.shf(oracle.nn_merge(biome_chunks, (3, 3)), (3, 3), (MAP, MAP), 16)
+    zero = (props[0][0], props[0][1], props[0][2], props[0][3], 0.0, 0.0)
+    want = oracle.multi_biome(perm, g32, list(props) + [zero], item, weight, off, (MAP, MAP), (0.0, 0.0))
+    assert out.tobytes() == want.tobytes()
