@@ -138,6 +138,7 @@ namespace SuperTerrainPlus {
 			std::vector<STPBiomeProperty> table, uint2 map_size, unsigned int interpolation_radius,
 			unsigned int bin_capacity = 16u, const std::string& script = std::string(),
 			const std::vector<std::string>& options = {});
+		~STPRuntimeBiomefieldGenerator() override;
 		void operator()(const STPNearestNeighbourHeightFloatWTextureBuffer& heightmap_buffer,
 			const STPNearestNeighbourSampleRTextureBuffer& biomemap_buffer, float2 offset) override;
 		// Compiler log of the script (may be empty).
@@ -152,6 +153,10 @@ namespace SuperTerrainPlus {
 		std::string Log;
 		STPDeviceRuntimeProgram::STPSmartModule Module;
 		hipFunction_t Kernel = nullptr;
+		// the batched path's filter (lines split into segments) for square maps of at most 8 biomes
+		// and bins, whose histograms are handed to the script in the reference's jagged form
+		struct STPFilterEngine;
+		std::unique_ptr<STPFilterEngine> Engine;
 	};
 
 	// The demo's runtime-compiled splatmap generator (SuperDemo+/World/Biomes/STPSplatmapGenerator.cpp:27-100

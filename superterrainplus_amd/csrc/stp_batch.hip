@@ -1904,6 +1904,52 @@ const float* STPHeightfieldBatch::binWeightOf(const unsigned int i) const {
 	return this->BinWeight.get() + this->Record[i].Pixel * this->Setting.BinCapacity;
 }
 
+std::unique_ptr<STPHeightfieldBatch> STPHeightfieldBatch::makeFilter(const std::vector<STPBiomeProperty>& biome,
+	const unsigned int side, const unsigned int radius, const unsigned int bin_capacity) {
+	if (biome.empty() || biome.size() > 8u || bin_capacity == 0u || bin_capacity > 8u || radius == 0u || radius > 126u
+		|| (radius & 1u) != 0u || side < 6u || side > 16384u) {
+		return nullptr;
+	}
+	const STPEnvironment::STPSimplexNoiseSetting noise {0ull, 8u, 0.0};
+	STPEnvironment::STPRainDropSetting rain {};
+	rain.RainDropCount = 1u;
+	rain.Inertia = 0.05f;
+	rain.SedimentCapacityFactor = 4.0f;
+	rain.minSedimentCapacity = 0.01f;
+	rain.initWaterVolume = 1.0f;
+	rain.minWaterVolume = 0.01f;
+	rain.initSpeed = 1.0f;
+	rain.ErodeSpeed = 0.3f;
+	rain.DepositSpeed = 0.3f;
+	rain.EvaporateSpeed = 0.01f;
+	rain.Gravity = 4.0f;
+	rain.ErosionBrushRadius = 2u;
+	rain.MaxLifetime = 30u;
+	STPBatchSetting setting {};
+	setting.BiomeSeed = 0ull;
+	setting.BiomeCount = static_cast<unsigned int>(biome.size());
+	setting.BiomeCell = 1u;
+	setting.Radius = radius;
+	setting.BinCapacity = bin_capacity;
+	setting.DropsPerPixel = 0.0f;
+	setting.ExternalBiome = true;
+	STPBatchCapacity cap {};
+	cap.Pixels = static_cast<unsigned long long>(side) * side + 4ull;
+	cap.Drops = 1ull;
+	cap.Chunks = 1u;
+	cap.MaxSide = side;
+	return std::make_unique<STPHeightfieldBatch>(noise, biome, rain, setting, cap);
+}
+
+void STPHeightfieldBatch::filterExternal(const STPSample_t* const origin, const unsigned int pitch, const hipStream_t stream) {
+	if (this->Capacity.Chunks != 1u || !this->Setting.ExternalBiome) {
+		STP_THROW(STPInvalidArgument, "filterExternal is for one-chunk batches with ExternalBiome");
+	}
+	this->bind({STPBatchChunk {this->Capacity.MaxSide, 0, 0, 1u, 1u}}, {}, stream);
+	this->setBiomeSource(origin, pitch);
+	this->runStages(0, 3, stream);
+}
+
 void STPHeightfieldBatch::setBiomeSource(const STPSample_t* const origin, const unsigned int pitch) {
 	if (!this->Setting.ExternalBiome) {
 		STP_THROW(STPInvalidArgument, "a biome source is for batches with ExternalBiome");

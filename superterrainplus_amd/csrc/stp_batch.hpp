@@ -7,6 +7,7 @@
 #include "stp_stage_noise.hpp"
 
 #include <array>
+#include <memory>
 #include <vector>
 
 namespace SuperTerrainPlus {
@@ -126,6 +127,15 @@ namespace SuperTerrainPlus {
 		// (device memory; the caller keeps every such sample inside its allocation). On the item path
 		// a sample >= BiomeCount is read as 0 and sets status bit 4.
 		void setBiomeSource(const STPSample_t* origin, unsigned int pitch);
+		// A one-chunk batch that runs the filter passes alone on square maps of `side` (ExternalBiome;
+		// its noise, biome-map and erosion settings are unused), for the per-chunk generators; null
+		// where the item path does not apply (more than 8 biomes or bins, a radius above 126).
+		static std::unique_ptr<STPHeightfieldBatch> makeFilter(const std::vector<STPBiomeProperty>& biome,
+			unsigned int side, unsigned int radius, unsigned int bin_capacity);
+		// Binds the one chunk and enqueues stages 0-2 (halo copy, vertical and horizontal passes) on
+		// `stream`: the chunk's pixel (0, 0) is origin[0] of a sample map of row pitch `pitch`, whose
+		// samples within Radius of the chunk must exist. Read the histograms with binCountOf(0) etc.
+		void filterExternal(const STPSample_t* origin, unsigned int pitch, hipStream_t stream);
 		unsigned short* fixedOf(unsigned int i) const;
 		// Enqueues copies of every bound chunk's outputs to height[i] / fixed[i] (device memory, one
 		// pointer per chunk) on `stream`, in one launch.

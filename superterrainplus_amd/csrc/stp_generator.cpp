@@ -714,7 +714,8 @@ void STPSimplexHeightGenerator::operator()(const STPNearestNeighbourHeightFloatW
 // segments); the blend stays the plugin's own (its offset is a float). Calls take it in turn.
 struct STPBiomefieldGenerator::STPFilterEngine {
 	std::mutex Lock;
-	std::unique_ptr<STPHeightfieldBatch> Batch;
+	bool Checked = false;
+	std::unique_ptr<STPHeightfieldBatch> Batch; // null: the segmented passes do not apply
 };
 
 STPBiomefieldGenerator::STPBiomefieldGenerator(const STPEnvironment::STPSimplexNoiseSetting& noise,
@@ -739,56 +740,28 @@ STPBiomefieldGenerator::~STPBiomefieldGenerator() = default;
 bool STPBiomefieldGenerator::filterBySegments(const STPNearestNeighbourHeightFloatWTextureBuffer& heightmap_buffer,
 	const STPNearestNeighbourSampleRTextureBuffer& biomemap_buffer, const float2 offset) {
 	const unsigned int side = this->MapSize.x, radius = this->InterpolationRadius;
-	if (this->MapSize.y != side || this->BiomeTable.size() > 8u || this->BinCapacity > 8u || radius > 126u
-		|| side > 16384u) {
+	if (this->MapSize.y != side) {
+		return false;
+	}
+	STPFilterEngine& e = *this->Engine;
+	const std::lock_guard<std::mutex> guard(e.Lock);
+	if (!e.Checked) {
+		e.Checked = true;
+		e.Batch = STPHeightfieldBatch::makeFilter(this->BiomeTable, side, radius, this->BinCapacity);
+	}
+	if (!e.Batch) {
 		return false;
 	}
 	const STPNearestNeighbourInformation& bnn = biomemap_buffer.NeighbourInfo;
 	const uint2 start = STPHeightfieldKernel::checkFilterDomain(bnn, radius);
 	const hipStream_t stream = heightmap_buffer.DeviceMemInfo.second;
-	STPFilterEngine& e = *this->Engine;
-	const std::lock_guard<std::mutex> guard(e.Lock);
-	if (!e.Batch) {
-		// the batch's own noise, biome-map and erosion settings are never used: any valid ones
-		const STPEnvironment::STPSimplexNoiseSetting noise {0ull, 8u, 0.0};
-		STPEnvironment::STPRainDropSetting rain {};
-		rain.RainDropCount = 1u;
-		rain.Inertia = 0.05f;
-		rain.SedimentCapacityFactor = 4.0f;
-		rain.minSedimentCapacity = 0.01f;
-		rain.initWaterVolume = 1.0f;
-		rain.minWaterVolume = 0.01f;
-		rain.initSpeed = 1.0f;
-		rain.ErodeSpeed = 0.3f;
-		rain.DepositSpeed = 0.3f;
-		rain.EvaporateSpeed = 0.01f;
-		rain.Gravity = 4.0f;
-		rain.ErosionBrushRadius = 2u;
-		rain.MaxLifetime = 30u;
-		STPBatchSetting setting {};
-		setting.BiomeSeed = 0ull;
-		setting.BiomeCount = static_cast<unsigned int>(this->BiomeTable.size());
-		setting.BiomeCell = 1u;
-		setting.Radius = radius;
-		setting.BinCapacity = this->BinCapacity;
-		setting.DropsPerPixel = 0.0f;
-		setting.ExternalBiome = true;
-		STPBatchCapacity cap {};
-		cap.Pixels = static_cast<unsigned long long>(side) * side + 4ull;
-		cap.Drops = 1ull;
-		cap.Chunks = 1u;
-		cap.MaxSide = side;
-		e.Batch = std::make_unique<STPHeightfieldBatch>(noise, this->BiomeTable, rain, setting, cap);
-	}
 	STPHeightfieldBatch& b = *e.Batch;
 	const STPNearestNeighbourHeightFloatWTextureBuffer::STPMergedBuffer height(heightmap_buffer,
 		STPNearestNeighbourHeightFloatWTextureBuffer::STPMemoryLocation::DeviceMemory);
 	const STPNearestNeighbourSampleRTextureBuffer::STPMergedBuffer biome(biomemap_buffer,
 		STPNearestNeighbourSampleRTextureBuffer::STPMemoryLocation::DeviceMemory);
-	// the filter window's first sample (start - r) is the chunk's halo sample (0, 0)
-	b.bind({STPBatchChunk {side, 0, 0, 1u, 1u}}, {}, stream);
-	b.setBiomeSource(biome.getDevice() + static_cast<size_t>(start.y) * bnn.TotalMapSize.x + start.x, bnn.TotalMapSize.x);
-	b.runStages(0, 3, stream);
+	b.filterExternal(biome.getDevice() + static_cast<size_t>(start.y) * bnn.TotalMapSize.x + start.x, bnn.TotalMapSize.x,
+		stream);
 	const unsigned int count = static_cast<unsigned int>(this->BiomeTable.size());
 	if (b.packedHistogram()) {
 		const unsigned int w = 2u * radius + 1u;
@@ -805,7 +778,7 @@ bool STPBiomefieldGenerator::filterBySegments(const STPNearestNeighbourHeightFlo
 		STP_THROW(STPCapacityError, "a pixel histogram holds more distinct biomes than the bin capacity ("
 			+ std::to_string(this->BinCapacity) + ")");
 	}
-	// a biome id beyond the table: the line-by-line filter below keeps it as an item of its own
+	// a biome id beyond the table: the line-by-line filter keeps it as an item of its own
 	return (status & 4u) == 0u;
 }
 

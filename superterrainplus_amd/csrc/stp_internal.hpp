@@ -129,6 +129,14 @@ namespace SuperTerrainPlus {
 		// Runs vertical + horizontal passes; writes dense bins and (if jagged) the jagged output.
 		void singleHistogramFilter(const STPSample_t* sample, unsigned int pitch, uint2 start, uint2 map_size,
 			unsigned int radius, unsigned int bin_capacity, bool jagged, const STPFilterWorkspace& ws, hipStream_t stream);
+		// The reference's jagged histogram (STPSingleHistogram: bins of all pixels back to back, and
+		// each pixel's first bin) of a dense one: per pixel count[p] slots of bin_capacity, holding
+		// items with `weight` the weights, or (weight null) packed item | count << 3 with weight
+		// count * norm. The jagged arrays live in `workspace` (jaggedWorkspaceBytes).
+		size_t jaggedWorkspaceBytes(unsigned long long pixels, unsigned int bin_capacity);
+		STPAlgorithm::STPSingleHistogram jaggedHistogram(const unsigned char* count, const unsigned short* item,
+			const float* weight, float norm, unsigned int bin_capacity, unsigned int pixels, void* workspace,
+			hipStream_t stream);
 
 	} // namespace STPHeightfieldKernel
 } // namespace SuperTerrainPlus

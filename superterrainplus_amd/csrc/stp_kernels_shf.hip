@@ -118,9 +118,12 @@ namespace {
 		}
 	}
 
+	// Packed: a slot is item | count << 3 and the weight count * norm (no weight array)
+	template <bool Packed>
 	__global__ void __launch_bounds__(ScanBlock) scatterKERNEL(const unsigned char* const count,
-		const unsigned short* const item, const float* const weight, const unsigned int cap, const unsigned int n,
-		const unsigned int* const tile_base, unsigned int* const offset, STPAlgorithm::STPSingleHistogram::STPBin* const bin) {
+		const unsigned short* const item, const float* const weight, const float norm, const unsigned int cap,
+		const unsigned int n, const unsigned int* const tile_base, unsigned int* const offset,
+		STPAlgorithm::STPSingleHistogram::STPBin* const bin) {
 		__shared__ unsigned int waveTotal[ScanBlock / 64u];
 		const size_t base = static_cast<size_t>(blockIdx.x) * ScanTile + threadIdx.x * ScanItems;
 		unsigned int local[ScanItems];
@@ -137,8 +140,14 @@ namespace {
 				offset[p] = o;
 				for (unsigned int b = 0u; b < local[k]; b++) {
 					STPAlgorithm::STPSingleHistogram::STPBin out;
-					out.Item = item[p * cap + b];
-					out.Weight = weight[p * cap + b];
+					if constexpr (Packed) {
+						const unsigned int slot = item[p * cap + b];
+						out.Item = static_cast<unsigned short>(slot & 7u);
+						out.Weight = static_cast<float>(slot >> 3u) * norm;
+					} else {
+						out.Item = item[p * cap + b];
+						out.Weight = weight[p * cap + b];
+					}
 					bin[o + b] = out;
 				}
 				o += local[k];
@@ -238,8 +247,39 @@ void STPHeightfieldKernel::singleHistogramFilter(const STPSample_t* const sample
 		const unsigned int tiles = (pixels + ScanTile - 1u) / ScanTile;
 		tileSumKERNEL<<<tiles, ScanBlock, 0, stream>>>(ws.BinCount, pixels, ws.BlockSum);
 		tileScanKERNEL<<<1, ScanBlock, 0, stream>>>(ws.BlockSum, tiles);
-		scatterKERNEL<<<tiles, ScanBlock, 0, stream>>>(ws.BinCount, ws.BinItem, ws.BinWeight, bin_capacity, pixels,
-			ws.BlockSum, ws.Offset, ws.Bin);
+		scatterKERNEL<false><<<tiles, ScanBlock, 0, stream>>>(ws.BinCount, ws.BinItem, ws.BinWeight, 0.0f, bin_capacity,
+			pixels, ws.BlockSum, ws.Offset, ws.Bin);
 		STP_CHECK_HIP(hipGetLastError());
 	}
+}
+
+size_t STPHeightfieldKernel::jaggedWorkspaceBytes(const unsigned long long pixels, const unsigned int bin_capacity) {
+	const unsigned long long tiles = (pixels + ScanTile - 1u) / ScanTile;
+	return align256((pixels + 1ull) * sizeof(unsigned int)) + align256(tiles * sizeof(unsigned int))
+		+ align256(pixels * bin_capacity * sizeof(STPAlgorithm::STPSingleHistogram::STPBin));
+}
+
+STPAlgorithm::STPSingleHistogram STPHeightfieldKernel::jaggedHistogram(const unsigned char* const count,
+	const unsigned short* const item, const float* const weight, const float norm, const unsigned int bin_capacity,
+	const unsigned int pixels, void* const workspace, const hipStream_t stream) {
+	if (!count || !item || !workspace || pixels == 0u || bin_capacity == 0u || bin_capacity > MaxBinCapacity) {
+		STP_THROW(STPInvalidArgument, "invalid dense histogram");
+	}
+	const unsigned int tiles = (pixels + ScanTile - 1u) / ScanTile;
+	char* const b = static_cast<char*>(workspace);
+	unsigned int* const offset = reinterpret_cast<unsigned int*>(b);
+	unsigned int* const blocksum = reinterpret_cast<unsigned int*>(b + align256((pixels + 1ull) * sizeof(unsigned int)));
+	auto* const bin = reinterpret_cast<STPAlgorithm::STPSingleHistogram::STPBin*>(
+		b + align256((pixels + 1ull) * sizeof(unsigned int)) + align256(static_cast<size_t>(tiles) * sizeof(unsigned int)));
+	tileSumKERNEL<<<tiles, ScanBlock, 0, stream>>>(count, pixels, blocksum);
+	tileScanKERNEL<<<1, ScanBlock, 0, stream>>>(blocksum, tiles);
+	if (weight) {
+		scatterKERNEL<false><<<tiles, ScanBlock, 0, stream>>>(count, item, weight, 0.0f, bin_capacity, pixels, blocksum,
+			offset, bin);
+	} else {
+		scatterKERNEL<true><<<tiles, ScanBlock, 0, stream>>>(count, item, nullptr, norm, bin_capacity, pixels, blocksum,
+			offset, bin);
+	}
+	STP_CHECK_HIP(hipGetLastError());
+	return STPAlgorithm::STPSingleHistogram {bin, offset};
 }

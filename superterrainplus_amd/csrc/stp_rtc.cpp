@@ -8,6 +8,7 @@
 // Plugin: SuperDemo+/World/Biomes/STPBiomefieldGenerator.cpp:44-152 compiles
 // SuperDemo+/Script/STPMultiHeightGenerator.cu at run time, looks its kernel and globals up by
 // lowered name, fills the biome table and launches the kernel after the single histogram filter.
+#include "stp_batch.hpp"
 #include "stp_internal.hpp"
 
 #include <stp/STPDeviceRuntime.hpp>
@@ -15,6 +16,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cstring>
+#include <mutex>
 
 // The library's device headers, embedded at build time (generated source).
 extern "C" {
@@ -301,11 +303,22 @@ const char* STPRuntimeBiomefieldGenerator::builtinScript() noexcept {
 	return STPLibraryMultiHeightScript;
 }
 
+// The filter engine: a one-chunk batch of the map's side with an external biome map (stages 0-2);
+// calls take it in turn.
+struct STPRuntimeBiomefieldGenerator::STPFilterEngine {
+	std::mutex Lock;
+	bool Checked = false;
+	std::vector<STPBiomeProperty> Biome;
+	std::unique_ptr<STPHeightfieldBatch> Batch; // null: the segmented passes do not apply
+};
+
+STPRuntimeBiomefieldGenerator::~STPRuntimeBiomefieldGenerator() = default;
+
 STPRuntimeBiomefieldGenerator::STPRuntimeBiomefieldGenerator(const STPEnvironment::STPSimplexNoiseSetting& noise,
 	std::vector<STPBiomeProperty> table, const uint2 map_size, const unsigned int interpolation_radius,
 	const unsigned int bin_capacity, const std::string& script, const std::vector<std::string>& options) :
 	Table(STPAlgorithm::STPPermutationGenerator::generate(noise)), MapSize(map_size),
-	InterpolationRadius(interpolation_radius), BinCapacity(bin_capacity) {
+	InterpolationRadius(interpolation_radius), BinCapacity(bin_capacity), Engine(std::make_unique<STPFilterEngine>()) {
 	if (table.empty() || map_size.x == 0u || map_size.y == 0u) {
 		STP_THROW(STPInvalidArgument, "the runtime generator needs a biome table and a map size");
 	}
@@ -349,6 +362,7 @@ STPRuntimeBiomefieldGenerator::STPRuntimeBiomefieldGenerator(const STPEnvironmen
 	put("Permutation", &perm, sizeof(perm));
 	put("BiomeCount", &count, sizeof(unsigned int));
 	put("BiomeTable", table.data(), table.size() * sizeof(STPBiomeProperty));
+	this->Engine->Biome = std::move(table);
 }
 
 void STPRuntimeBiomefieldGenerator::operator()(const STPNearestNeighbourHeightFloatWTextureBuffer& heightmap_buffer,
@@ -360,31 +374,73 @@ void STPRuntimeBiomefieldGenerator::operator()(const STPNearestNeighbourHeightFl
 		STP_THROW(STPInvalidArgument, "the map size differs from the size this generator was made for");
 	}
 	const uint2 start = STPHeightfieldKernel::checkFilterDomain(bnn, this->InterpolationRadius);
-	const auto [pool, stream] = heightmap_buffer.DeviceMemInfo;
+	const auto pool = heightmap_buffer.DeviceMemInfo.first;
+	const hipStream_t stream = heightmap_buffer.DeviceMemInfo.second;
 	const STPNearestNeighbourHeightFloatWTextureBuffer::STPMergedBuffer height(heightmap_buffer,
 		STPNearestNeighbourHeightFloatWTextureBuffer::STPMemoryLocation::DeviceMemory);
 	const STPNearestNeighbourSampleRTextureBuffer::STPMergedBuffer biome(biomemap_buffer,
 		STPNearestNeighbourSampleRTextureBuffer::STPMemoryLocation::DeviceMemory);
+	const uint2 m = this->MapSize;
+	const auto launchScript = [&](STPAlgorithm::STPSingleHistogram histogram) {
+		float* out = height.getDevice();
+		float2 off = offset;
+		void* args[] = {&out, &histogram, &off};
+		STP_CHECK_HIP(hipModuleLaunchKernel(this->Kernel, (m.x + ScriptTileX - 1u) / ScriptTileX,
+			(m.y + ScriptTileY - 1u) / ScriptTileY, 1u, ScriptTileX, ScriptTileY, 1u, 0u, stream, args, nullptr));
+	};
+	const auto allocate = [&](const size_t bytes) {
+		void* p = nullptr;
+		if (pool) {
+			STP_CHECK_HIP(hipMallocFromPoolAsync(&p, bytes, pool, stream));
+		} else {
+			STP_CHECK_HIP(hipMallocAsync(&p, bytes, stream));
+		}
+		return p;
+	};
+	{
+		STPFilterEngine& e = *this->Engine;
+		const std::lock_guard<std::mutex> guard(e.Lock);
+		if (!e.Checked) {
+			e.Checked = true;
+			if (m.x == m.y) {
+				e.Batch = STPHeightfieldBatch::makeFilter(e.Biome, m.x, this->InterpolationRadius, this->BinCapacity);
+			}
+		}
+		if (e.Batch) {
+			STPHeightfieldBatch& b = *e.Batch;
+			b.filterExternal(biome.getDevice() + static_cast<size_t>(start.y) * bnn.TotalMapSize.x + start.x,
+				bnn.TotalMapSize.x, stream);
+			const unsigned int pixels = m.x * m.y, w = 2u * this->InterpolationRadius + 1u;
+			void* const jagged = allocate(STPHeightfieldKernel::jaggedWorkspaceBytes(pixels, this->BinCapacity));
+			try {
+				launchScript(STPHeightfieldKernel::jaggedHistogram(b.binCountOf(0u), b.binItemOf(0u),
+					b.packedHistogram() ? nullptr : b.binWeightOf(0u), 1.0f / static_cast<float>(w * w), this->BinCapacity,
+					pixels, jagged, stream));
+			} catch (...) {
+				(void)hipFreeAsync(jagged, stream);
+				throw;
+			}
+			STP_CHECK_HIP(hipFreeAsync(jagged, stream));
+			const unsigned int status = b.status(stream);
+			if (status & 1u) {
+				STP_THROW(STPCapacityError, "a pixel histogram holds more distinct biomes than the bin capacity ("
+					+ std::to_string(this->BinCapacity) + ")");
+			}
+			if ((status & 4u) == 0u) {
+				return;
+			}
+			// a biome id beyond the table: the line-by-line filter keeps it as an item of its own
+		}
+	}
 
 	const size_t bytes =
 		STPHeightfieldKernel::filterWorkspaceBytes(bnn.MapSize, this->InterpolationRadius, this->BinCapacity);
-	void* ws = nullptr;
-	if (pool) {
-		STP_CHECK_HIP(hipMallocFromPoolAsync(&ws, bytes, pool, stream));
-	} else {
-		STP_CHECK_HIP(hipMallocAsync(&ws, bytes, stream));
-	}
+	void* const ws = allocate(bytes);
 	const STPHeightfieldKernel::STPFilterWorkspace carved =
 		STPHeightfieldKernel::carveFilterWorkspace(ws, bnn.MapSize, this->InterpolationRadius, this->BinCapacity);
 	STPHeightfieldKernel::singleHistogramFilter(biome.getDevice(), bnn.TotalMapSize.x, start, bnn.MapSize,
 		this->InterpolationRadius, this->BinCapacity, true, carved, stream);
-	float* out = height.getDevice();
-	STPAlgorithm::STPSingleHistogram histogram {carved.Bin, carved.Offset};
-	float2 off = offset;
-	void* args[] = {&out, &histogram, &off};
-	const uint2 m = this->MapSize;
-	STP_CHECK_HIP(hipModuleLaunchKernel(this->Kernel, (m.x + ScriptTileX - 1u) / ScriptTileX,
-		(m.y + ScriptTileY - 1u) / ScriptTileY, 1u, ScriptTileX, ScriptTileY, 1u, 0u, stream, args, nullptr));
+	launchScript(STPAlgorithm::STPSingleHistogram {carved.Bin, carved.Offset});
 	unsigned int overflow = 0u;
 	STP_CHECK_HIP(hipMemcpyAsync(&overflow, carved.Overflow, sizeof(unsigned int), hipMemcpyDeviceToHost, stream));
 	STP_CHECK_HIP(hipFreeAsync(ws, stream));

@@ -94,6 +94,31 @@ def test_runtime_generator_bit_exact(gpu, oracle):
     assert outs[1].tobytes() == outs[0].tobytes()
 
 
+def test_runtime_generator_ids_beyond_the_table(gpu, oracle):
+    """Biome maps with an id the table lacks (ids 0..4, 4 biomes): the segmented filter refuses the
+    map and the line-by-line filter hands the script a histogram holding id 4, which the built-in
+    script's biome-count guard skips. The oracle's table gets a fifth biome of Depth 0 and
+    Variation 0 (its products add +0.0): the same heights, bit for bit."""
+    log("rtc diversity generator, ids beyond the table")
+    props = W.biome_table(4, 7)
+    plugin = runtime_plugin(props)
+    try:
+        gen = Generator(plugin, chunk_setting(diversity_nn=(3, 3)), W.erosion_setting(100), mode=0)
+        biome_chunks = [oracle.voronoi_map(7, 5, 64, (int(off[0]), int(off[1])), MAP, MAP) for off in OFFSETS]
+        assert max(int(b.max()) for b in biome_chunks) == 4
+        out = np.zeros((MAP, MAP), np.float32)
+        capi.call("stp_heightfield_generate", C.c_void_p(gen.h), capi.ptr(out),
+                  (C.c_void_p * 9)(*[b.ctypes.data for b in biome_chunks]), 128.0, -256.0)
+        del gen
+    finally:
+        capi.call("stp_diversity_destroy", C.c_void_p(plugin))
+    perm, g32, _ = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+    item, weight, off = oracle.shf(oracle.nn_merge(biome_chunks, (3, 3)), (3, 3), (MAP, MAP), 16)
+    zero = (props[0][0], props[0][1], props[0][2], props[0][3], 0.0, 0.0)
+    want = oracle.multi_biome(perm, g32, list(props) + [zero], item, weight, off, (MAP, MAP), (128.0, -256.0))
+    assert out.tobytes() == want.tobytes()
+
+
 def test_runtime_generator_script_errors(gpu):
     props = W.biome_table(4, 7)
     # a script that does not compile: the log comes back in the error
