@@ -475,6 +475,21 @@ def generator_configs(reps=3):
             capi.call("stp_heightfield_erode", g.h, (C.c_void_p * 1)(h.ctypes.data), (C.c_void_p * 1)(f.ctypes.data))
         finally:
             g.close()
+        # the demo's form of the plugin: the multi-height script compiled at run time (hipRTC)
+        rt = C.c_void_p()
+        capi.call("stp_diversity_runtime_create", C.byref(noise), capi.biome_array(props), len(props),
+                  (C.c_uint32 * 2)(mb.side, mb.side), mb.radius, 8, None, (C.c_char_p * 1)(None), 0, C.byref(rt))
+        try:
+            g = Generator(rt.value, mb.side, 3, 1, rain, 0)
+            try:
+                rt_ms = timed(device)
+                capi.call("stp_heightfield_generate_device", g.h, dh.vp, bp, 0.0, 0.0, None)
+                capi.synchronize()
+                rt_pre = dh.download()
+            finally:
+                g.close()
+        finally:
+            capi.call("stp_diversity_destroy", rt)
     finally:
         capi.call("stp_diversity_destroy", d)
     r = mb.radius
@@ -492,10 +507,15 @@ def generator_configs(reps=3):
                           "host_api_mpixel_s": round(px / host_ms / 1e3, 1),
                           "device_api_mpixel_s": round(px / dev_ms / 1e3, 1),
                           "pre_erosion_max_abs_vs_fp64": float(f"{pre_max:.4g}"),
-                          "rms_vs_fp64": float(f"{rms(h, w64):.4g}")}
+                          "rms_vs_fp64": float(f"{rms(h, w64):.4g}"),
+                          "runtime_script_device_api_ms": round(rt_ms, 3),
+                          "runtime_script_device_api_mpixel_s": round(px / rt_ms / 1e3, 1),
+                          "runtime_script_pre_erosion_equal": bool(rt_pre.tobytes() == pre.tobytes())}
     for k, v in out.items():
         print(f"[bench] {k}: host API {v['host_api_mpixel_s']} Mpixel/s, device API {v['device_api_mpixel_s']} "
-              f"Mpixel/s" + (f", rms vs FP64 {v['rms_vs_fp64']}" if "rms_vs_fp64" in v else ""),
+              f"Mpixel/s" + (f", rms vs FP64 {v['rms_vs_fp64']}" if "rms_vs_fp64" in v else "")
+              + (f"; hipRTC script plugin device API {v['runtime_script_device_api_mpixel_s']} Mpixel/s (heights equal: "
+                 f"{v['runtime_script_pre_erosion_equal']})" if "runtime_script_device_api_ms" in v else ""),
               file=sys.stderr, flush=True)
     return out
 

@@ -1232,6 +1232,9 @@ namespace SuperTerrainPlus::STPBatch {
 	// stores, no atomics, and every later batch reads the terrain the earlier ones eroded.
 
 	constexpr unsigned int MaxTiledTaps = 128u;
+	// tiled erosion splits a binding into two stream groups from this many tiles on (about two
+	// rounds of one workgroup per CU for each group's colour launches)
+	constexpr unsigned int SplitErosionTiles = 2048u;
 	// tile sides the autotuner tries, largest first (the default is the largest that fits)
 	constexpr unsigned int ErosionTileCandidates[] = {128u, 112u, 96u, 80u, 72u};
 
@@ -1723,6 +1726,9 @@ void STPHeightfieldBatch::init(const STPEnvironment::STPRainDropSetting& rain) {
 	STP_CHECK_HIP(hipEventCreateWithFlags(&this->TableCopied, hipEventDisableTiming));
 	STP_CHECK_HIP(hipEventCreateWithFlags(&this->OutCopied, hipEventDisableTiming));
 	STP_CHECK_HIP(hipStreamCreateWithFlags(&this->CaptureStream, hipStreamNonBlocking));
+	STP_CHECK_HIP(hipStreamCreateWithFlags(&this->SideStream, hipStreamNonBlocking));
+	STP_CHECK_HIP(hipEventCreateWithFlags(&this->Fork, hipEventDisableTiming));
+	STP_CHECK_HIP(hipEventCreateWithFlags(&this->Join, hipEventDisableTiming));
 }
 
 void STPHeightfieldBatch::bind(const std::vector<STPBatchChunk>& chunks, const std::vector<unsigned int>& stream_id,
@@ -2243,6 +2249,16 @@ STPHeightfieldBatch::~STPHeightfieldBatch() {
 	if (this->CaptureStream) {
 		(void)hipStreamDestroy(this->CaptureStream);
 	}
+	if (this->SideStream) {
+		(void)hipStreamSynchronize(this->SideStream);
+		(void)hipStreamDestroy(this->SideStream);
+	}
+	if (this->Fork) {
+		(void)hipEventDestroy(this->Fork);
+	}
+	if (this->Join) {
+		(void)hipEventDestroy(this->Join);
+	}
 }
 
 void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage, const unsigned int first,
@@ -2438,31 +2454,63 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 				this->DropInfo.get());
 			break;
 		}
-		for (int colour = 0; colour < 4; colour++) {
-			const auto tiles = range(5 + colour);
-			const uint2* const tt = tiles.first;
-			const unsigned int nt = tiles.second;
-			if (nt == 0u) {
-				continue;
+		// the four colour launches of chunks [f, l) on stream `on`
+		const auto colours = [&](const hipStream_t on, const unsigned int f, const unsigned int l) {
+			for (int colour = 0; colour < 4; colour++) {
+				const unsigned int b = this->Range[f].Begin[5 + colour], e = this->Range[l - 1u].End[5 + colour];
+				const uint2* const tt = this->Tile[5 + colour] + b;
+				const unsigned int nt = e - b;
+				if (nt == 0u) {
+					continue;
+				}
+				const auto launch = [&](auto kernel, const unsigned int threads) {
+					kernel<<<nt, threads, this->ErosionLds, on>>>(tt, this->ChunkRecord, args, tile,
+						this->ErosionHalo, this->BrushOffset.get(), this->BrushWeight.get(), this->TileDrops.get(),
+						this->TileFirst.get(), this->DropBin.get(), this->Height.get(), this->Ledger.get(),
+						this->DropInfo.get(), this->Flag.get() + 1);
+				};
+				switch (this->Tuning.ErosionThreads) {
+				case 256u:
+					launch(erodeTileKERNEL<256u>, 256u);
+					break;
+				case 512u:
+					launch(erodeTileKERNEL<512u>, 512u);
+					break;
+				default:
+					launch(erodeTileKERNEL<1024u>, 1024u);
+					break;
+				}
 			}
-			const auto launch = [&](auto kernel, const unsigned int threads) {
-				kernel<<<nt, threads, this->ErosionLds, stream>>>(tt, this->ChunkRecord, args, tile,
-					this->ErosionHalo, this->BrushOffset.get(), this->BrushWeight.get(), this->TileDrops.get(),
-					this->TileFirst.get(), this->DropBin.get(), this->Height.get(), this->Ledger.get(),
-					this->DropInfo.get(), this->Flag.get() + 1);
-			};
-			switch (this->Tuning.ErosionThreads) {
-			case 256u:
-				launch(erodeTileKERNEL<256u>, 256u);
-				break;
-			case 512u:
-				launch(erodeTileKERNEL<512u>, 512u);
-				break;
-			default:
-				launch(erodeTileKERNEL<1024u>, 1024u);
-				break;
+		};
+		// Chunks are separate maps, so two groups of chunks (about half the tiles each) erode on two
+		// streams: a colour launch of one group fills the CUs that the other group's launch leaves
+		// idle in its last round of tiles, instead of every launch waiting out its own last round.
+		const auto tilesOf = [&](const unsigned int i) {
+			unsigned int t = 0u;
+			for (int colour = 0; colour < 4; colour++) {
+				t += this->Range[i].End[5 + colour] - this->Range[i].Begin[5 + colour];
 			}
+			return t;
+		};
+		unsigned int total = 0u;
+		for (unsigned int i = first; i < last; i++) {
+			total += tilesOf(i);
 		}
+		unsigned int mid = first, acc = 0u;
+		while (mid < last && 2u * (acc + tilesOf(mid)) <= total) {
+			acc += tilesOf(mid);
+			mid++;
+		}
+		if (mid == first || mid == last || total < SplitErosionTiles) {
+			colours(stream, first, last);
+			break;
+		}
+		STP_CHECK_HIP(hipEventRecord(this->Fork, stream));
+		STP_CHECK_HIP(hipStreamWaitEvent(this->SideStream, this->Fork, 0));
+		colours(stream, first, mid);
+		colours(this->SideStream, mid, last);
+		STP_CHECK_HIP(hipEventRecord(this->Join, this->SideStream));
+		STP_CHECK_HIP(hipStreamWaitEvent(stream, this->Join, 0));
 		break;
 	}
 	case 6: {

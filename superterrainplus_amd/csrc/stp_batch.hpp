@@ -251,6 +251,10 @@ namespace SuperTerrainPlus {
 		std::unique_ptr<STPDiversity::STPLayerPlan> LayerPlan;
 
 		hipStream_t CaptureStream = nullptr;
+		// the tiled erosion of a second group of chunks runs on SideStream, forked from and joined
+		// back into the stage's stream with these events
+		hipStream_t SideStream = nullptr;
+		hipEvent_t Fork = nullptr, Join = nullptr;
 		hipGraph_t Graph = nullptr;
 		hipGraphExec_t GraphExec = nullptr;
 		hipGraph_t StageGraph[StageCount] {};
