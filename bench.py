@@ -59,13 +59,15 @@ LDS_ACCESS_STEP, LDS_ACCESS_ERODE, LDS_ACCESS_DEPOSIT = 8, 25, 4
 # product and the sum); one bin of a pixel costs octave * 66 + 3
 FLOP_PER_OCTAVE, FLOP_PER_BIN = 66, 3
 FP32_PEAK_TFLOPS = 157.3
-# Measured on this workload with rocprofv3 for the LDS-tiled erosion kernel at tile 128 x 1024
-# threads (profiles/erosion_sq_counters.md, profiles/erosion_hbm_traffic.md): VALU wave64
-# instructions per 64 drop steps (SQ_INSTS_VALU / (drop steps / 64)), and HBM bytes per launch
-# (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on the format kernel).
-EROSION_PROFILE_SHAPE = (128, 1024)
-EROSION_VALU_PER_WAVE_STEP = 187.5
-EROSION_HBM_BYTES_PER_LAUNCH = 270.6e6
+# Measured on this workload with rocprofv3 for the LDS-tiled erosion kernel at the launch shapes
+# the autotuner picks from (tile, threads) (profiles/erosion_sq_counters.md,
+# profiles/erosion_hbm_traffic.md): VALU wave64 instructions per 64 drop steps (SQ_INSTS_VALU /
+# (drop steps / 64)), and HBM bytes per colour phase (FETCH_SIZE x 2 + WRITE_SIZE of its launches,
+# calibrated on the format kernel).
+EROSION_PROFILES = {
+    (128, 1024): {"valu_per_64_drop_steps": 187.5, "hbm_bytes_per_phase": 270.8e6},
+    (72, 512): {"valu_per_64_drop_steps": 203.6, "hbm_bytes_per_phase": 443.1e6},
+}
 # serving loop: two engines of 8 Mpx / 512 chunks each
 SERVER_BATCH_PIXELS, SERVER_BATCH_CHUNKS, SERVER_ENGINES = 8 << 20, 512, 2
 
@@ -824,7 +826,9 @@ def erosion_roofline(tuning, stage_ms, counters):
     pixels, drops, steps, erode, deposit, killed = counters
     tiled = tuning.ErosionTile > 0
     kernel = f"erodeTileKERNEL<{tuning.ErosionThreads}>" if tiled else "erosionKERNEL"
-    # the tiled erosion launches once per colour batch; the global-atomic kernel once
+    # the tiled erosion runs four colour phases (on this workload each phase is two concurrent
+    # launches, one per chunk group and stream, each lasting about the phase); the global-atomic
+    # kernel is one launch. Figures "per launch" below are per phase.
     launches = 4 if tiled else 1
     launch_s = stage_ms[STAGES.index("erosion")] / 1e3 / launches
     lds_bytes = 4 * (steps * LDS_ACCESS_STEP + erode * LDS_ACCESS_ERODE + deposit * LDS_ACCESS_DEPOSIT) / launches
@@ -835,8 +839,9 @@ def erosion_roofline(tuning, stage_ms, counters):
     # compulsory HBM bytes: each eroded height read and written once, each drop's tile-list entry
     # read and its ledger and counters written
     compulsory = (8 * pixels + 16 * drops) / launches
-    profiled = tiled and (tuning.ErosionTile, tuning.ErosionThreads) == EROSION_PROFILE_SHAPE
-    traffic = EROSION_HBM_BYTES_PER_LAUNCH if profiled else None
+    profile = EROSION_PROFILES.get((tuning.ErosionTile, tuning.ErosionThreads)) if tiled else None
+    profiled = profile is not None
+    traffic = profile["hbm_bytes_per_phase"] if profiled else None
     hbm = {"traffic": traffic, "compulsory_bytes_per_launch": int(compulsory),
            "traffic_over_compulsory": round(traffic / compulsory, 2) if traffic else None,
            "achieved": round(traffic / launch_s / 1e9, 1) if traffic else None, "peak": HBM_PEAK_GBS,
@@ -846,20 +851,23 @@ def erosion_roofline(tuning, stage_ms, counters):
            "note": "traffic: rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch on this workload and shape, "
                    "calibrated on the format kernel; the drop steps' algorithmic height reads and updates "
                    "(algorithmic_bytes_per_launch) are served by the LDS window, not by HBM"}
-    common = {"kernel": kernel, "launches_per_step": launches, "launch_ms_avg": round(launch_s * 1e3, 4)}
-    if profiled and EROSION_VALU_PER_WAVE_STEP:
-        valu = EROSION_VALU_PER_WAVE_STEP * (steps / 64.0) / launches
+    common = {"kernel": kernel, "launch_shape": [tuning.ErosionTile, tuning.ErosionThreads],
+              "launches_per_step": launches, "launch_ms_avg": round(launch_s * 1e3, 4)}
+    if profiled:
+        per64 = profile["valu_per_64_drop_steps"]
+        valu = per64 * (steps / 64.0) / launches
         g = valu / launch_s / 1e9
         return {"bound": "valu", "achieved": round(g, 1), "peak": VALU_PEAK_GINSTR, "unit": "G VALU instructions/s",
                 "frac": round(g / VALU_PEAK_GINSTR, 4), "traffic": traffic, **common,
-                "valu_per_launch": int(valu), "valu_per_64_drop_steps": EROSION_VALU_PER_WAVE_STEP,
+                "valu_per_launch": int(valu), "valu_per_64_drop_steps": per64,
                 "lds": lds, "hbm": hbm,
                 "note": "the kernel's most used resource is VALU instruction issue: its VALU instructions per 64 "
                         "drop steps (SQ_INSTS_VALU of a rocprofv3 run of this workload and shape) x this run's drop "
                         "steps, over its launch time, against one wave64 FP32 VALU instruction per cycle per CU "
                         "(its integer instructions can issue faster, so the fraction is an upper estimate of its "
-                        "VALU use); its LDS pipe waits on bank conflicts (lds), and HBM "
-                        "carries 2.5x the compulsory bytes (hbm)"}
+                        "VALU use); its LDS pipe waits on bank conflicts (lds); hbm gives the counted traffic "
+                        "against the compulsory bytes. A launch here is a colour phase: two concurrent launches "
+                        "of half the tiles each on this workload"}
     return {"bound": "lds", **{k: lds[k] for k in ("achieved", "peak", "unit", "frac")}, "traffic": traffic,
             **common, "lds": lds, "hbm": hbm,
             "note": "no instruction profile for this launch shape: LDS bank-access view"}
@@ -874,6 +882,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default erosion launch shape")
+    ap.add_argument("--erosion-shape", default=None, metavar="TILE,THREADS",
+                    help="erode with this launch shape instead of the autotuned one (e.g. 72,512)")
     ap.add_argument("--no-served", action="store_true", help="leave out the serving replays")
     ap.add_argument("--no-world", action="store_true", help="leave out the world, splatmap and layer legs")
     ap.add_argument("--no-configs", action="store_true", help="leave out the per-configuration legs")
@@ -892,12 +902,17 @@ def main():
     stream = C.c_void_p()
     capi.call("stp_stream_create", C.byref(stream))
     tuning = capi.BatchTuning()
-    if args.no_autotune:
+    autotuned = not args.no_autotune and not args.erosion_shape
+    if args.erosion_shape:
+        tile, threads = (int(v) for v in args.erosion_shape.split(","))
+        tuning = capi.BatchTuning(tile, threads)
+        capi.call("stp_batch_set_tuning", batch, C.byref(tuning))
+    elif args.no_autotune:
         capi.call("stp_batch_get_tuning", batch, C.byref(tuning))
     else:
         capi.call("stp_batch_autotune", batch, 2, stream, C.byref(tuning))
     print(f"[bench] erosion tile {tuning.ErosionTile}, {tuning.ErosionThreads} threads"
-          f"{'' if args.no_autotune else ' (autotuned)'}", file=sys.stderr, flush=True)
+          f"{' (autotuned)' if autotuned else ''}", file=sys.stderr, flush=True)
 
     # timed region: one graph per stage, HIP events recorded on the stream between the graph
     # launches (so the erosion kernel's duration is measured in the same run as the throughput)
@@ -977,7 +992,7 @@ def main():
             "single_graph_mpixel_s": round(pixels / single_wall / 1e6, 2),
             "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES, stage_ms)},
             "erosion_tuning": {"tile": tuning.ErosionTile, "threads": tuning.ErosionThreads,
-                               "autotuned": not args.no_autotune},
+                               "autotuned": autotuned},
             "erosion_counters": {"steps": dsteps, "erode_steps": erode, "deposit_steps": deposit, "killed": killed},
             "erosion_atomics": {"per_step": atomics, "per_s": round(atomics / erosion_s, 1) if erosion_s > 0 else None,
                                 "note": "LDS integer atomics (25 per erode step, 4 per deposit step) over the "
