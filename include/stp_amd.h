@@ -360,9 +360,11 @@ int stp_batch_create(const stp_simplex_noise_setting* noise, const stp_biome_pro
 	const stp_raindrop_setting* raindrop, const stp_batch_setting* setting, const stp_batch_chunk* h_chunks,
 	uint32_t chunk_count, stp_heightfield_batch** out);
 int stp_batch_destroy(stp_heightfield_batch* batch);
-/* mode 0: replay of one captured hipGraph (asynchronous); 1: the same launches without a graph
- * (asynchronous); 2: plain -- one chunk per launch sequence with a host synchronisation after each
- * chunk, the reference's calling pattern (synchronous). */
+/* mode 0: replay of one captured hipGraph (asynchronous; a binding of at least 8 Mi pixels runs as
+ * two chunk groups whose whole pipelines are concurrent chains of the graph; with timing on, one
+ * graph per stage with events between, the groups' stages in step); 1: the same launches without
+ * a graph (asynchronous); 2: plain -- one chunk per launch sequence with a host synchronisation
+ * after each chunk, the reference's calling pattern (synchronous). */
 int stp_batch_run(stp_heightfield_batch* batch, int mode, void* stream);
 /* Record HIP events between the 7 stages (biome, vertical filter, horizontal filter, blend,
  * erosion drop binning, erosion, format) of later runs. */
@@ -401,9 +403,9 @@ typedef struct {
 /* STP_ERROR_INVALID_ARGUMENT if the shape cannot run on this batch. */
 int stp_batch_set_tuning(stp_heightfield_batch* batch, const stp_batch_tuning* tuning);
 int stp_batch_get_tuning(stp_heightfield_batch* batch, stp_batch_tuning* out);
-/* Autotuner: times every candidate shape with eager runs of the whole batch on `stream`
- * (erosion stage time, `repeat` timed runs after one warm-up each), keeps the fastest and writes
- * it to `out` (optional). Synchronous. */
+/* Autotuner: times every candidate shape with graph runs of the whole batch on `stream` (the
+ * pipeline as stp_batch_run runs it, `repeat` timed runs after one warm-up each), keeps the
+ * fastest and writes it to `out` (optional). Synchronous. */
 int stp_batch_autotune(stp_heightfield_batch* batch, uint32_t repeat, void* stream, stp_batch_tuning* out);
 /* Copies chunk `index`'s outputs to host memory (each pointer optional): heights and UINT16
  * (side^2), halo biome map ((side + 2 * Radius)^2), per-drop final sediment (drops; one value per

@@ -1235,6 +1235,10 @@ namespace SuperTerrainPlus::STPBatch {
 	// tiled erosion splits a binding into two stream groups from this many tiles on (about two
 	// rounds of one workgroup per CU for each group's colour launches)
 	constexpr unsigned int SplitErosionTiles = 2048u;
+	// A graph run of a binding of at least this many pixels runs two chunk groups' whole pipelines
+	// as two concurrent chains: one group's latency-bound filter passes share the GPU with the other
+	// group's issue-bound blend and erosion, and each fills the other's launch tails.
+	constexpr unsigned long long PipelinePixels = 8ull << 20;
 	// tile sides the autotuner tries, largest first (the default is the largest that fits)
 	constexpr unsigned int ErosionTileCandidates[] = {128u, 112u, 96u, 80u, 72u};
 
@@ -2206,30 +2210,46 @@ STPBatchTuning STPHeightfieldBatch::autotune(const hipStream_t stream, const uns
 	if (candidate.empty()) {
 		candidate.push_back(STPBatchTuning {0u, ErosionBlock});
 	}
+	// Each shape is scored on whole graph runs (the mode a caller runs, with two concurrent chunk
+	// chains on large bindings): the tuning changes only the erosion binning and erosion, but how
+	// well they share the GPU with the other chain's stages depends on the shape.
 	STPBatchTuning best = candidate.front();
 	float bestMs = 0.0f;
-	this->Timing = true;
+	hipEvent_t begin = nullptr, end = nullptr;
+	this->Timing = false;
 	try {
+		STP_CHECK_HIP(hipEventCreate(&begin));
+		STP_CHECK_HIP(hipEventCreate(&end));
 		for (const STPBatchTuning& t : candidate) {
 			this->setTuning(t);
-			float ms = 0.0f;
-			for (unsigned int k = 0u; k <= repeat; k++) {
-				this->run(STPRunMode::Eager, stream);
-				STP_CHECK_HIP(hipStreamSynchronize(stream));
-				if (k > 0u) { // the first run warms up
-					const std::vector<float> st = this->stageTimes();
-					ms += st[4] + st[5];
-				}
+			// the first run captures the graph and warms up
+			this->run(STPRunMode::Graph, stream);
+			STP_CHECK_HIP(hipEventRecord(begin, stream));
+			for (unsigned int k = 0u; k < std::max(repeat, 1u); k++) {
+				this->run(STPRunMode::Graph, stream);
 			}
+			STP_CHECK_HIP(hipEventRecord(end, stream));
+			STP_CHECK_HIP(hipEventSynchronize(end));
+			float ms = 0.0f;
+			STP_CHECK_HIP(hipEventElapsedTime(&ms, begin, end));
 			if (bestMs == 0.0f || ms < bestMs) {
 				bestMs = ms;
 				best = t;
 			}
 		}
 	} catch (...) {
+		(void)hipStreamSynchronize(stream);
+		if (begin) {
+			(void)hipEventDestroy(begin);
+		}
+		if (end) {
+			(void)hipEventDestroy(end);
+		}
 		this->Timing = timing;
 		throw;
 	}
+	(void)hipEventDestroy(begin);
+	(void)hipEventDestroy(end);
 	this->Timing = timing;
 	this->setTuning(best);
 	return best;
@@ -2261,8 +2281,23 @@ STPHeightfieldBatch::~STPHeightfieldBatch() {
 	}
 }
 
+unsigned int STPHeightfieldBatch::pipelineSplit() const noexcept {
+	const unsigned int n = static_cast<unsigned int>(this->Record.size());
+	if (n < 2u || this->Pixels < PipelinePixels) {
+		return 0u;
+	}
+	unsigned long long acc = 0ull;
+	for (unsigned int i = 0u; i + 1u < n; i++) {
+		acc += static_cast<unsigned long long>(this->Record[i].Side) * this->Record[i].Side;
+		if (2ull * acc >= this->Pixels) {
+			return i + 1u;
+		}
+	}
+	return n - 1u;
+}
+
 void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage, const unsigned int first,
-	const unsigned int last, const bool clear_flag) {
+	const unsigned int last, const bool clear_flag, const bool split_erosion) {
 	const unsigned int radius = this->Setting.Radius, cap = this->Setting.BinCapacity;
 	const auto range = [&](const int s) {
 		const unsigned int b = this->Range[first].Begin[s], e = this->Range[last - 1u].End[s];
@@ -2501,7 +2536,7 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 			acc += tilesOf(mid);
 			mid++;
 		}
-		if (mid == first || mid == last || total < SplitErosionTiles) {
+		if (!split_erosion || mid == first || mid == last || total < SplitErosionTiles) {
 			colours(stream, first, last);
 			break;
 		}
@@ -2541,11 +2576,28 @@ void STPHeightfieldBatch::enqueue(const hipStream_t stream, const unsigned int f
 
 hipGraphExec_t STPHeightfieldBatch::capture(const int stage_begin, const int stage_end, hipGraph_t& graph) {
 	const unsigned int n = static_cast<unsigned int>(this->Record.size());
+	const unsigned int split = stage_begin == 0 && stage_end == StageCount ? this->pipelineSplit() : 0u;
 	STP_CHECK_HIP(hipStreamBeginCapture(this->CaptureStream, hipStreamCaptureModeThreadLocal));
 	bool ok = true;
 	try {
-		for (int s = stage_begin; s < stage_end; s++) {
-			this->enqueueStage(this->CaptureStream, s, 0u, n, true);
+		if (split != 0u) {
+			// the flags every stage 0 and 4 would clear, cleared once for both chains
+			zeroKERNEL<<<1, 64, 0, this->CaptureStream>>>(this->Flag.get(), 4u);
+			STP_CHECK_HIP(hipGetLastError());
+			STP_CHECK_HIP(hipEventRecord(this->Fork, this->CaptureStream));
+			STP_CHECK_HIP(hipStreamWaitEvent(this->SideStream, this->Fork, 0));
+			for (int s = 0; s < StageCount; s++) {
+				this->enqueueStage(this->CaptureStream, s, 0u, split, false, false);
+			}
+			for (int s = 0; s < StageCount; s++) {
+				this->enqueueStage(this->SideStream, s, split, n, false, false);
+			}
+			STP_CHECK_HIP(hipEventRecord(this->Join, this->SideStream));
+			STP_CHECK_HIP(hipStreamWaitEvent(this->CaptureStream, this->Join, 0));
+		} else {
+			for (int s = stage_begin; s < stage_end; s++) {
+				this->enqueueStage(this->CaptureStream, s, 0u, n, true);
+			}
 		}
 	} catch (...) {
 		ok = false;

@@ -178,8 +178,8 @@ namespace SuperTerrainPlus {
 		// rebuilt at the next graph run.
 		void setTuning(const STPBatchTuning& tuning);
 		const STPBatchTuning& tuning() const noexcept { return this->Tuning; }
-		// Times every candidate launch shape on this batch (eager runs on `stream`, erosion stage
-		// time) and keeps the fastest. Returns the chosen shape.
+		// Times every candidate launch shape on this batch (graph runs on `stream`, the whole
+		// pipeline) and keeps the fastest. Returns the chosen shape.
 		STPBatchTuning autotune(hipStream_t stream, unsigned int repeat = 2u);
 		// The shape a new batch starts with: the largest tile whose window fits in LDS.
 		STPBatchTuning defaultTuning() const;
@@ -264,7 +264,12 @@ namespace SuperTerrainPlus {
 		hipEvent_t TableCopied = nullptr, OutCopied = nullptr;
 		std::vector<double> Latency;
 
-		void enqueueStage(hipStream_t stream, int stage, unsigned int first, unsigned int last, bool clear_flag);
+		// split_erosion: the tiled erosion of a large range may fork its second chunk group onto SideStream
+		void enqueueStage(hipStream_t stream, int stage, unsigned int first, unsigned int last, bool clear_flag,
+			bool split_erosion = true);
+		// The first chunk of the second of two chunk groups (about half the pixels each) whose whole
+		// pipelines a graph run captures as two concurrent chains; 0 where the binding is too small.
+		unsigned int pipelineSplit() const noexcept;
 		void enqueue(hipStream_t stream, unsigned int first, unsigned int last, bool timed, bool clear_flag);
 		hipGraphExec_t capture(int stage_begin, int stage_end, hipGraph_t& graph);
 		void releaseGraphs() noexcept;

@@ -283,6 +283,38 @@ def test_batch_status_clear_after_tiled_erosion(gpu):
     assert status.value == 0
 
 
+def test_batch_pipelined_graph(gpu, oracle):
+    """A binding of at least 8 Mi pixels replays as two chunk groups (split after about half the
+    pixels: chunks 0-4 and 5-8 here) whose whole pipelines are concurrent chains of the graph.
+    Without drops the chunks on both sides of the split are bit-exact against the oracle, in two
+    replays; with drops every chunk's ledger balances its heights to 2e-4 relative and the status
+    words stay clear."""
+    log("batch pipelined graph, 9 chunks of 1024^2")
+    sides = [1024] * 9
+    origins = [((k % 3 - 1) * 1024 + 37, (k // 3 - 1) * 1024 - 11) for k in range(9)]
+    pre = Batch(0.0, sides=sides, origins=origins)
+    for replay in range(2):
+        pre.run(0)
+        for i in (0, 4, 5, 8):
+            h, f, b = pre.read(i)
+            halo, want = oracle_chunk(oracle, i, sides, origins)
+            assert (b == halo).all(), (replay, i)
+            assert h.tobytes() == want.tobytes(), (replay, i)
+            assert (f == oracle.format_heightmap(want)).all(), (replay, i)
+    dpp = 0.25
+    er = Batch(dpp, sides=sides, origins=origins)
+    assert er.tuning()[0] > 0
+    er.run(0)
+    for i, side in enumerate(sides):
+        before, after = pre.read(i)[0], er.read(i)[0]
+        moved = float(er.ledger(i, int(side * side * dpp)).astype(np.float64).sum())
+        assert moved > 0.0, i
+        assert abs((after.astype(np.float64).sum() - before.astype(np.float64).sum()) + moved) <= 2e-4 * moved, i
+    status = C.c_uint32()
+    capi.call("stp_batch_status", C.c_void_p(er.h), C.byref(status), None)
+    assert status.value == 0
+
+
 def test_batch_plain_matches_graph_statistically(gpu, oracle):
     """The plain per-chunk run and the graph run erode with the same drops; only the order of
     concurrent atomic updates differs. Yardstick as above: two serialized runs with different
