@@ -361,10 +361,10 @@ int stp_batch_create(const stp_simplex_noise_setting* noise, const stp_biome_pro
 	uint32_t chunk_count, stp_heightfield_batch** out);
 int stp_batch_destroy(stp_heightfield_batch* batch);
 /* mode 0: replay of one captured hipGraph (asynchronous; a binding of at least 8 Mi pixels runs as
- * two chunk groups whose whole pipelines are concurrent chains of the graph; with timing on, one
- * graph per stage with events between, the groups' stages in step); 1: the same launches without
- * a graph (asynchronous); 2: plain -- one chunk per launch sequence with a host synchronisation
- * after each chunk, the reference's calling pattern (synchronous). */
+ * four chunk groups of about equal pixels whose whole pipelines are concurrent chains of the
+ * graph; with timing on, one graph per stage with events between, so no stage overlaps another);
+ * 1: the same launches without a graph (asynchronous); 2: plain -- one chunk per launch sequence
+ * with a host synchronisation after each chunk, the reference's calling pattern (synchronous). */
 int stp_batch_run(stp_heightfield_batch* batch, int mode, void* stream);
 /* Record HIP events between the 7 stages (biome, vertical filter, horizontal filter, blend,
  * erosion drop binning, erosion, format) of later runs. */

@@ -251,10 +251,12 @@ namespace SuperTerrainPlus {
 		std::unique_ptr<STPDiversity::STPLayerPlan> LayerPlan;
 
 		hipStream_t CaptureStream = nullptr;
-		// the tiled erosion of a second group of chunks runs on SideStream, forked from and joined
-		// back into the stage's stream with these events
-		hipStream_t SideStream = nullptr;
-		hipEvent_t Fork = nullptr, Join = nullptr;
+		// Work forked from a stage's stream and joined back into it with these events: the tiled
+		// erosion's second chunk group (Side[0]), and the chains of a graph run's chunk groups after
+		// the first (Side[0..PipelineGroups - 2]).
+		static constexpr int SideStreams = 3;
+		hipStream_t Side[SideStreams] {};
+		hipEvent_t Fork = nullptr, Join[SideStreams] {};
 		hipGraph_t Graph = nullptr;
 		hipGraphExec_t GraphExec = nullptr;
 		hipGraph_t StageGraph[StageCount] {};
@@ -264,12 +266,12 @@ namespace SuperTerrainPlus {
 		hipEvent_t TableCopied = nullptr, OutCopied = nullptr;
 		std::vector<double> Latency;
 
-		// split_erosion: the tiled erosion of a large range may fork its second chunk group onto SideStream
+		// split_erosion: the tiled erosion of a large range may fork its second chunk group onto Side[0]
 		void enqueueStage(hipStream_t stream, int stage, unsigned int first, unsigned int last, bool clear_flag,
 			bool split_erosion = true);
-		// The first chunk of the second of two chunk groups (about half the pixels each) whose whole
-		// pipelines a graph run captures as two concurrent chains; 0 where the binding is too small.
-		unsigned int pipelineSplit() const noexcept;
+		// The chunk groups (about equal pixels each, [bound[g], bound[g + 1])) whose whole pipelines a
+		// graph run captures as concurrent chains; a single group where the binding is too small.
+		std::vector<unsigned int> pipelineGroups() const;
 		void enqueue(hipStream_t stream, unsigned int first, unsigned int last, bool timed, bool clear_flag);
 		hipGraphExec_t capture(int stage_begin, int stage_end, hipGraph_t& graph);
 		void releaseGraphs() noexcept;
