@@ -171,7 +171,7 @@ def validate(cfg, sides, origins, steps, props, batch, threads):
     * before erosion (the same chunks in a batch without drops): bit-exact against the FP32
       oracle, max-abs and RMS against the FP64 oracle;
     * eroded (the timed batch's own output): RMS against the serialized FP32 oracle erosion with
-      the same drops, the floor it is judged by (RMS between serialized runs with different drops),
+      the same drops, the floor it is assessed by (RMS between serialized runs with different drops),
       RMS against the serialized FP64 erosion of the FP64 map, the serialized FP32 run's own RMS
       against that FP64 run, the mass-ledger residual and the sediment moved against serialized."""
     from superterrainplus_amd import capi, workloads as W
@@ -241,7 +241,7 @@ def validate(cfg, sides, origins, steps, props, batch, threads):
             "note": "pre-erosion heights are deterministic and meet 1e-4 RMS against FP64 with room to spare; "
                     "eroded heights cannot: the serialized FP32 oracle itself differs from the FP64 one by "
                     "rms_serialized_fp32_vs_fp64 (erosion is chaotic: one rounding moves a drop's path), so "
-                    "parallel erosion is judged against the serialized FP32 run with the same drops, by "
+                    "parallel erosion is assessed against the serialized FP32 run with the same drops, by "
                     "rms <= 1.5 x rms_serialized_other_drops, the mass ledger and the sediment moved"}
 
 

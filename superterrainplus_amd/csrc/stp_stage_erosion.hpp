@@ -115,7 +115,7 @@ namespace SuperTerrainPlus::STPStage {
 			&& static_cast<unsigned int>(d.py) + 1u <= a.Total.y - 1u;
 	}
 
-	// Fast = the hardware square root (1 ulp) of the parallel modes, which are judged
+	// Fast = the hardware square root (1 ulp) of the parallel modes, which are assessed
 	// statistically; the serialized kernel keeps the correctly rounded sqrtf.
 	template <bool Fast = false>
 	__device__ inline void advance(const ErosionArgs& a, DropState& d, const float deltaHeight) {
@@ -484,7 +484,7 @@ namespace SuperTerrainPlus::STPStage {
 	// with a third of the operations: with top = mix(hx, hy, cx) and bottom = mix(hz, hw, cx) the
 	// height is mix(top, bottom, cy), the x slope mix(hy - hx, hw - hz, cy), and the y slope
 	// (hz - hx)(1 - cx) + (hw - hy) cx is exactly bottom - top. Its roundings differ from the
-	// reference's expanded form by an ulp; parallel erosion is judged statistically, and the
+	// reference's expanded form by an ulp; parallel erosion is assessed statistically, and the
 	// serialized kernel keeps the reference's expression (heightGradient).
 	// The heights are read unscaled and the results stay unscaled: the caller applies the surface's
 	// Unit, a power of two, where it multiplies or subtracts anyway (exact: scaling by a power of
@@ -547,7 +547,7 @@ namespace SuperTerrainPlus::STPStage {
 			d.dx = (-hg.y) * gradientScale + d.dx * inertia;
 			d.dy = (-hg.z) * gradientScale + d.dy * inertia;
 			// parallel mode normalises with the hardware reciprocal square root (1 ulp): its result
-			// is judged statistically; the serialized kernel keeps the reference's division
+			// is assessed statistically; the serialized kernel keeps the reference's division
 			const float length2 = d.dx * d.dx + d.dy * d.dy;
 			if (length2 != 0.0f) {
 				const float inv = __builtin_amdgcn_rsqf(length2);

@@ -9,7 +9,7 @@ erode) -> SuperTerrain+/SuperTerrain+/Private/GPGPU/STPHeightfieldKernel.cu:37-5
 
 Bars:
   * filter + blend: bit-exact against the FP32 oracle, <= 1e-5 max-abs against the FP64 oracle;
-  * erosion (parallel, judged as every parallel erosion here): mass balance
+  * erosion (parallel, assessed as every parallel erosion here): mass balance
     sum(h_after - h_before) = -sum(ledger) to 2e-4 relative; sediment moved within
     4/sqrt(drops) + 0.12 of the serialized FP32 oracle run with the same drops (0.12: the
     concurrency deficit of parallel drops, which all start on the unmodified terrain); RMS against
