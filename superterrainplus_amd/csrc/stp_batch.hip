@@ -1238,8 +1238,9 @@ namespace SuperTerrainPlus::STPBatch {
 	// A graph run of a binding of at least this many pixels runs PipelineGroups chunk groups' whole
 	// pipelines as concurrent chains: one group's latency-bound filter passes share the GPU with
 	// another group's issue-bound blend and erosion, and each fills the others' launch tails
-	// (batched workload: one chain 4.72 ms per run, two 4.37, three 4.32, four 4.31; four is also
-	// best on 64 chunks of 512^2).
+	// (batched workload: one chain 4.72 ms per run, two 4.37, three 4.32, four 4.31, six 4.63, eight
+	// 4.60: beyond the four hardware queues of a process chains wait on each other; four is also best
+	// on 64 chunks of 512^2).
 	constexpr unsigned long long PipelinePixels = 8ull << 20;
 	constexpr unsigned int PipelineGroups = 4u;
 	// tile sides the autotuner tries, largest first (the default is the largest that fits)
