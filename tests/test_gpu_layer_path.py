@@ -91,6 +91,37 @@ def test_batch_with_demo_layer_biomes_bit_exact(gpu, oracle):
         capi.lib().stp_layer_factory_destroy(f)
 
 
+def test_batch_with_layer_biomes_pipelined_graph(gpu):
+    """A binding above 8 Mi pixels with layer-chain biomes: its graph run evaluates the chain for four
+    chunk groups concurrently (each map's nodes in its own part of the plan's arena). Every chunk's
+    halo biome map and heights equal those of the plain run, which takes the chunks one at a time."""
+    log("batch with layer biomes, pipelined graph (9 chunks of 1024^2)")
+    layers, _, _ = factory()
+    try:
+        sides = [1024] * 9
+        origins = [((k % 3 - 1) * 1024 + 13, (k // 3 - 1) * 1024 - 5) for k in range(9)]
+        h = make_batch(layers, sides, origins, [1] * 9)
+        try:
+            out = {}
+            for mode in (0, 2):
+                capi.call("stp_batch_run", h, mode, None)
+                capi.synchronize()
+                got = []
+                for i, side in enumerate(sides):
+                    hh = np.zeros((side, side), np.float32)
+                    bb = np.zeros((side + 2 * RADIUS, side + 2 * RADIUS), np.uint16)
+                    capi.call("stp_batch_read_chunk", h, i, capi.ptr(hh), None, capi.ptr(bb), None)
+                    got.append((hh, bb))
+                out[mode] = got
+            for i in range(len(sides)):
+                assert (out[0][i][1] == out[2][i][1]).all(), i
+                assert out[0][i][0].tobytes() == out[2][i][0].tobytes(), i
+        finally:
+            capi.lib().stp_batch_destroy(h)
+    finally:
+        capi.lib().stp_layer_factory_destroy(layers)
+
+
 def test_layer_ids_beyond_the_biome_table_are_flagged(gpu, oracle):
     """A chain of ocean-temperature noise draws ids 0, 2 and 5 of a registry with distinct ids; with
     4 biomes, id 5 is replaced by 0 and flagged."""
