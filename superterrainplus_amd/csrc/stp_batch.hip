@@ -510,6 +510,17 @@ namespace SuperTerrainPlus::STPBatch {
 	constexpr unsigned int NoRun = 0xFFFFFFFFu;
 	constexpr unsigned int ItemStageRows = 16u;
 	static_assert(ItemStageRows == VPrefetch, "the vertical item pass stages the rows it fetches at once");
+	// The vertical item pass stages a row's slots two to an LDS word. A slot is item | count << 8
+	// (item < 8, count <= 2r + 1 <= 255), so bits 3-7 are free; slot 0 carries the row's kept bin
+	// count there (at most 8). Per workgroup: ceil(vcap / 2) words per row and line.
+	__host__ __device__ constexpr unsigned int verticalItemWords(const unsigned int vcap) {
+		return (vcap + 1u) / 2u * ItemStageRows * 65u;
+	}
+	__device__ inline unsigned int verticalItemKept(const unsigned int word) { return (word >> 3) & 15u; }
+	__device__ inline unsigned short verticalItemSlot(const unsigned int word, const unsigned int k) {
+		const unsigned int v = (word >> (16u * (k & 1u))) & 0xFFFFu;
+		return static_cast<unsigned short>(k == 0u ? v & 0xFF07u : v);
+	}
 
 	// Per-biome window state of one line.
 	template <unsigned int MaxB>
@@ -825,8 +836,9 @@ namespace SuperTerrainPlus::STPBatch {
 		unsigned char* const vcount, const unsigned int* const state, unsigned int* const overflow) {
 		extern __shared__ unsigned int lds[];
 		constexpr unsigned int R = ItemStageRows;
-		unsigned int* const stage = lds;                      // [k][row][65]
-		unsigned int* const stageN = stage + vcap * R * 65u;  // [row][65]
+		// [slot pair][row][65]: slots 2p and 2p + 1 in the low and high halves, the kept bin count
+		// in bits 3-6 of slot 0 (verticalItemSlot)
+		unsigned int* const stage = lds;
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
 		const unsigned int lane = threadIdx.x, side = c.Side, ncols = side + 2u * radius;
@@ -885,11 +897,17 @@ namespace SuperTerrainPlus::STPBatch {
 					unsigned int item[MaxB], count[MaxB];
 					const unsigned int n = line.order(item, count);
 					over = over || n > vcap;
-					stageN[i * 65u + lane] = min(n, vcap);
+					const unsigned int kept = min(n, vcap);
 #pragma unroll
-					for (unsigned int k = 0u; k < MaxB; k++) {
-						if (k < n && k < vcap) {
-							stage[(k * R + i) * 65u + lane] = item[k] | (count[k] << 8);
+					for (unsigned int p = 0u; p < (MaxB + 1u) / 2u; p++) {
+						if (2u * p < vcap) {
+							const unsigned int k0 = 2u * p, k1 = k0 + 1u;
+							unsigned int lo = k0 < kept ? (item[k0] | (count[k0] << 8)) : 0u;
+							const unsigned int hi = k1 < kept ? (item[k1] | (count[k1] << 8)) : 0u;
+							if (p == 0u) {
+								lo |= kept << 3;
+							}
+							stage[(p * R + i) * 65u + lane] = lo | (hi << 16);
 						}
 					}
 				}
@@ -903,10 +921,11 @@ namespace SuperTerrainPlus::STPBatch {
 					const unsigned int cc = i / Q, q = i % Q;
 					const unsigned int cg = (t.y % nb) * ShfLineBlock + cc;
 					if (cg < ncols) {
-						unsigned int n[4];
+						unsigned int w0[4], n[4];
 #pragma unroll
 						for (unsigned int r = 0u; r < 4u; r++) {
-							n[r] = stageN[(4u * q + r) * 65u + cc];
+							w0[r] = stage[(4u * q + r) * 65u + cc];
+							n[r] = verticalItemKept(w0[r]);
 						}
 						const size_t y = static_cast<size_t>(y0) + 4u * q;
 						*reinterpret_cast<uchar4*>(outN + static_cast<size_t>(cg) * side + y) = make_uchar4(
@@ -914,15 +933,26 @@ namespace SuperTerrainPlus::STPBatch {
 							static_cast<unsigned char>(n[3]));
 						const unsigned int most = max(max(n[0], n[1]), max(n[2], n[3]));
 #pragma unroll
-						for (unsigned int k = 0u; k < MaxB; k++) {
-							if (k < most) {
-								unsigned short v[4];
+						for (unsigned int p = 0u; p < (MaxB + 1u) / 2u; p++) {
+							if (2u * p < most) {
+								unsigned int w[4];
 #pragma unroll
 								for (unsigned int r = 0u; r < 4u; r++) {
-									v[r] = k < n[r] ? static_cast<unsigned short>(stage[(k * R + 4u * q + r) * 65u + cc]) : 0u;
+									w[r] = p == 0u ? w0[r] : stage[(p * R + 4u * q + r) * 65u + cc];
 								}
-								*reinterpret_cast<ushort4*>(outBin + (static_cast<size_t>(cg) * vcap + k) * side + y) =
-									make_ushort4(v[0], v[1], v[2], v[3]);
+#pragma unroll
+								for (unsigned int h = 0u; h < 2u; h++) {
+									const unsigned int k = 2u * p + h;
+									if (k < most) {
+										unsigned short v[4];
+#pragma unroll
+										for (unsigned int r = 0u; r < 4u; r++) {
+											v[r] = k < n[r] ? verticalItemSlot(w[r], k) : 0u;
+										}
+										*reinterpret_cast<ushort4*>(outBin + (static_cast<size_t>(cg) * vcap + k) * side + y) =
+											make_ushort4(v[0], v[1], v[2], v[3]);
+									}
+								}
 							}
 						}
 					}
@@ -940,13 +970,14 @@ namespace SuperTerrainPlus::STPBatch {
 				const unsigned int cc = i / R, yy = i % R;
 				const unsigned int cg = (t.y % nb) * ShfLineBlock + cc;
 				if (cg < ncols && yy < rows) {
-					const unsigned int n = stageN[yy * 65u + cc];
+					const unsigned int w0 = stage[yy * 65u + cc], n = verticalItemKept(w0);
 					const size_t y = static_cast<size_t>(y0) + yy;
 					outN[static_cast<size_t>(cg) * side + y] = static_cast<unsigned char>(n);
 #pragma unroll
 					for (unsigned int k = 0u; k < MaxB; k++) {
 						if (k < n) {
-							outBin[(static_cast<size_t>(cg) * vcap + k) * side + y] = static_cast<unsigned short>(stage[(k * R + yy) * 65u + cc]);
+							const unsigned int w = k < 2u ? w0 : stage[((k / 2u) * R + yy) * 65u + cc];
+							outBin[(static_cast<size_t>(cg) * vcap + k) * side + y] = verticalItemSlot(w, k);
 						}
 					}
 				}
@@ -2352,7 +2383,7 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 					this->VCount.get(), this->Flag.get());
 			};
 			if (this->itemPath()) {
-				const unsigned int words = (this->VCap * ItemStageRows + ItemStageRows) * 65u;
+				const unsigned int words = verticalItemWords(this->VCap);
 				unsigned int* const state = this->SegState.get();
 				// the item path's column histograms are 16-bit slots (item | count << 8)
 				unsigned short* const vbin = reinterpret_cast<unsigned short*>(this->VBin.get());

@@ -283,6 +283,49 @@ def test_batch_status_clear_after_tiled_erosion(gpu):
     assert status.value == 0
 
 
+RAGGED_SIDES = [8, 9, 13, 37, 71, 73, 145]
+RAGGED_ORIGINS = [(-3, 5), (1000, -1000), (-4097, 33), (250, 250), (-71, -73), (7777, -1), (0, 145)]
+
+
+@pytest.mark.parametrize("tuning", [(72, 256), (0, 256)], ids=["tile72", "global"])
+def test_batch_ragged_sides(gpu, oracle, tuning):
+    """Chunks of the smallest side the erosion brush allows (2r + 2 = 8 at r = 3), odd sides, and
+    sides one below, one above and twice the erosion tile plus one (partial tiles on both edges):
+    without drops every chunk is bit-exact against the oracle in graph and plain mode; with drops
+    (0.25 per pixel, 16 on the smallest chunk) each chunk's ledger balances its heights to 2e-4
+    relative, the counters add up, the status words stay clear and the eroded map formats
+    bit-exactly."""
+    log(f"batch ragged sides {tuning}")
+    pre = Batch(0.0, sides=RAGGED_SIDES, origins=RAGGED_ORIGINS)
+    before = {}
+    for mode in (0, 2):
+        pre.run(mode)
+        for i in range(len(RAGGED_SIDES)):
+            gh, gf, gb = pre.read(i)
+            halo, h = oracle_chunk(oracle, i, RAGGED_SIDES, RAGGED_ORIGINS)
+            assert (gb == halo).all(), (mode, i)
+            assert gh.tobytes() == h.tobytes(), (mode, i)
+            assert (gf == oracle.format_heightmap(h)).all(), (mode, i)
+            before[i] = gh
+    dpp = 0.25
+    b = Batch(dpp, sides=RAGGED_SIDES, origins=RAGGED_ORIGINS)
+    assert b.tuning(*tuning) == tuning
+    b.run(0)
+    drops_total = sum(int(s * s * dpp) for s in RAGGED_SIDES)
+    cnt = b.counters()
+    assert cnt[0] == sum(s * s for s in RAGGED_SIDES) and cnt[1] == drops_total
+    assert 0 < cnt[2] <= 30 * drops_total and cnt[5] <= drops_total
+    status = C.c_uint32(7)
+    capi.call("stp_batch_status", C.c_void_p(b.h), C.byref(status), None)
+    assert status.value == 0
+    for i, s in enumerate(RAGGED_SIDES):
+        after, fixed, _ = b.read(i)
+        moved = float(b.ledger(i, int(s * s * dpp)).astype(np.float64).sum())
+        assert np.isfinite(after).all(), i
+        assert abs((after.astype(np.float64).sum() - before[i].astype(np.float64).sum()) + moved) <= 2e-4 * moved + 1e-6, i
+        assert (fixed == oracle.format_heightmap(after)).all(), i
+
+
 def test_batch_pipelined_graph(gpu, oracle):
     """A binding of at least 8 Mi pixels replays as two chunk groups (split after about half the
     pixels: chunks 0-4 and 5-8 here) whose whole pipelines are concurrent chains of the graph.
@@ -407,3 +450,10 @@ def test_batch_rejects_bad_settings(gpu):
         Batch(0.25, radius=3)
     with pytest.raises(capi.STPError):
         Batch(0.25, sides=[4], origins=[(0, 0)])
+    # one chunk below the brush's 2r + 2 in a binding of valid ones, a side beyond 16384, no chunks
+    with pytest.raises(capi.STPError):
+        Batch(0.25, sides=[64, 7], origins=[(0, 0), (64, 0)])
+    with pytest.raises(capi.STPError):
+        Batch(0.25, sides=[16385], origins=[(0, 0)])
+    with pytest.raises(capi.STPError):
+        Batch(0.25, sides=[], origins=[])
