@@ -122,8 +122,9 @@ namespace SuperTerrainPlus::STPBatch {
 	//
 	// The vertical pass runs one lane per column and the horizontal pass one lane per row
 	// (stp_stage_shf.hpp holds the accumulators and the order rules). Between the two passes the
-	// column histograms are stored TRANSPOSED, [column][slot][row], so that the horizontal pass,
-	// whose lanes are consecutive rows, reads them with coalesced loads. Both passes go through
+	// column histograms are stored TRANSPOSED, [column][slot][row] (the item passes: [column][row]
+	// slot groups, itemSlots), so that the horizontal pass, whose lanes are consecutive rows, reads
+	// them with coalesced loads. Both passes go through
 	// LDS so that every global access of a wave is a contiguous run: the vertical pass stages
 	// `rows` output rows and writes them column by column; the horizontal pass loads blocks of
 	// `block` columns (entering and leaving) before sliding over them, and stages its dense
@@ -510,16 +511,35 @@ namespace SuperTerrainPlus::STPBatch {
 	constexpr unsigned int NoRun = 0xFFFFFFFFu;
 	constexpr unsigned int ItemStageRows = 16u;
 	static_assert(ItemStageRows == VPrefetch, "the vertical item pass stages the rows it fetches at once");
-	// The vertical item pass stages a row's slots two to an LDS word. A slot is item | count << 8
-	// (item < 8, count <= 2r + 1 <= 255), so bits 3-7 are free; slot 0 carries the row's kept bin
-	// count there (at most 8). Per workgroup: ceil(vcap / 2) words per row and line.
+	// Column histograms of the item path: per (column, row) a group of itemSlots(vcap) 16-bit
+	// slots at vbin[(Line + col * side + y) * slots], slot k = item | count << 8 of the window's
+	// k-th bin (item < 8, count <= 2r + 1 <= 255) and 0 past its last bin (a bin's count is >= 1),
+	// so one 8- or 16-byte load gives a lane a column's whole histogram and its bin count.
+	__host__ __device__ constexpr unsigned int itemSlots(const unsigned int vcap) { return vcap <= 4u ? 4u : 8u; }
+	// The vertical item pass stages a row's slots two to an LDS word (slots 2p, 2p + 1 in the low
+	// and high halves, as in the group): itemSlots(vcap) / 2 words per row and line.
 	__host__ __device__ constexpr unsigned int verticalItemWords(const unsigned int vcap) {
-		return (vcap + 1u) / 2u * ItemStageRows * 65u;
+		return itemSlots(vcap) / 2u * ItemStageRows * 65u;
 	}
-	__device__ inline unsigned int verticalItemKept(const unsigned int word) { return (word >> 3) & 15u; }
-	__device__ inline unsigned short verticalItemSlot(const unsigned int word, const unsigned int k) {
-		const unsigned int v = (word >> (16u * (k & 1u))) & 0xFFFFu;
-		return static_cast<unsigned short>(k == 0u ? v & 0xFF07u : v);
+	// slot k of a group's words
+	__device__ inline unsigned int itemSlot(const unsigned int* const w, const unsigned int k) {
+		return (w[k / 2u] >> (16u * (k & 1u))) & 0xFFFFu;
+	}
+	// the first L slots of the group at g (8- or 16-byte aligned: the group's own size)
+	template <unsigned int L>
+	__device__ inline void loadItemGroup(const unsigned int* const g, unsigned int (&w)[L / 2u]) {
+		static_assert(L == 4u || L == 8u, "slot groups of 4 or 8");
+		if constexpr (L == 4u) {
+			const uint2 v = *reinterpret_cast<const uint2*>(g);
+			w[0] = v.x;
+			w[1] = v.y;
+		} else {
+			const uint4 v = *reinterpret_cast<const uint4*>(g);
+			w[0] = v.x;
+			w[1] = v.y;
+			w[2] = v.z;
+			w[3] = v.w;
+		}
 	}
 
 	// Per-biome window state of one line.
@@ -783,10 +803,12 @@ namespace SuperTerrainPlus::STPBatch {
 	}
 
 	// run summaries of the horizontal pass's segments (all but each line's last)
-	template <unsigned int MaxB>
+	// L: slots a lane loads per column group, min(MaxB, itemSlots(vcap)) (a window holds at most
+	// MaxB biomes, so the slots past MaxB are 0)
+	template <unsigned int MaxB, unsigned int L>
 	__global__ void __launch_bounds__(ShfLineBlock) horizontalSummaryKERNEL(const uint2* const tiles,
-		const Chunk* const chunks, const unsigned int radius, const unsigned int vcap, const unsigned short* const vbin,
-		const unsigned char* const vcount, unsigned int* const state) {
+		const Chunk* const chunks, const unsigned int radius, const unsigned int vcap, const unsigned int* const vbin,
+		unsigned int* const state) {
 		constexpr unsigned int P = HItemPrefetch;
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
@@ -798,27 +820,24 @@ namespace SuperTerrainPlus::STPBatch {
 		const uint2 xr = segmentRange(c, seg);
 		const bool active = y < side;
 		const unsigned int yc = min(y, side - 1u);
-		const unsigned short* const inBin = vbin + c.Line * vcap;
-		const unsigned char* const inN = vcount + c.Line;
+		const unsigned int gw = itemSlots(vcap) / 2u; // words per group
+		const unsigned int* const inBin = vbin + c.Line * gw;
 		RunSummary<MaxB> sum;
 		for (unsigned int x0 = xr.x; x0 < xr.y; x0 += P) {
-			unsigned int n[P], b[P][MaxB];
+			unsigned int b[P][L / 2u];
 #pragma unroll
 			for (unsigned int j = 0u; j < P; j++) {
 				const size_t col = min(x0 + j, xr.y - 1u);
-				n[j] = inN[col * side + yc];
-#pragma unroll
-				for (unsigned int k = 0u; k < MaxB; k++) {
-					b[j][k] = k < vcap ? inBin[(col * vcap + k) * side + yc] : 0u;
-				}
+				loadItemGroup<L>(inBin + (col * side + yc) * gw, b[j]);
 			}
 #pragma unroll
 			for (unsigned int j = 0u; j < P; j++) {
 				if (active && x0 + j < xr.y) {
 #pragma unroll
-					for (unsigned int k = 0u; k < MaxB; k++) {
-						if (k < n[j]) {
-							sum.occur(b[j][k] & 0xFFu, static_cast<int>(x0 + j), k, radius);
+					for (unsigned int k = 0u; k < L; k++) {
+						const unsigned int v = itemSlot(b[j], k);
+						if (v != 0u) {
+							sum.occur(v & 0xFFu, static_cast<int>(x0 + j), k, radius);
 						}
 					}
 				}
@@ -829,15 +848,15 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 	}
 
-	// vertical pass, vcount[col * side + y], vbin[(col * vcap + k) * side + y] as verticalKERNEL
+	// vertical pass: the slot group of (col, y) at vbin + (Line + col * side + y) * itemSlots(vcap) / 2
+	// words (itemSlots)
 	template <unsigned int MaxB>
 	__global__ void __launch_bounds__(ShfLineBlock) verticalItemKERNEL(const uint2* const tiles, const Chunk* const chunks,
-		const unsigned int radius, const unsigned int vcap, const unsigned short* const biome, unsigned short* const vbin,
-		unsigned char* const vcount, const unsigned int* const state, unsigned int* const overflow) {
+		const unsigned int radius, const unsigned int vcap, const unsigned short* const biome, unsigned int* const vbin,
+		const unsigned int* const state, unsigned int* const overflow) {
 		extern __shared__ unsigned int lds[];
 		constexpr unsigned int R = ItemStageRows;
-		// [slot pair][row][65]: slots 2p and 2p + 1 in the low and high halves, the kept bin count
-		// in bits 3-6 of slot 0 (verticalItemSlot)
+		// [slot pair][row][65] (verticalItemWords)
 		unsigned int* const stage = lds;
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
@@ -871,10 +890,8 @@ namespace SuperTerrainPlus::STPBatch {
 				}
 			}
 		}
-		unsigned short* const outBin = vbin + c.Line * vcap;
-		unsigned char* const outN = vcount + c.Line;
-		// 4-row vector stores when every output row block starts 4-aligned in the arenas
-		const bool quads = side % 4u == 0u && yr.x % 4u == 0u && c.Line % 4u == 0u;
+		const unsigned int gw = itemSlots(vcap) / 2u; // words per group
+		unsigned int* const out = vbin + c.Line * gw;
 		// software pipelined: the rows of the next block are fetched before this block is processed
 		unsigned int enter[VPrefetch], leave[VPrefetch];
 		fetch(enter, static_cast<long long>(yr.x) + 2 * static_cast<long long>(radius));
@@ -900,85 +917,34 @@ namespace SuperTerrainPlus::STPBatch {
 					const unsigned int kept = min(n, vcap);
 #pragma unroll
 					for (unsigned int p = 0u; p < (MaxB + 1u) / 2u; p++) {
-						if (2u * p < vcap) {
+						if (p < gw) {
 							const unsigned int k0 = 2u * p, k1 = k0 + 1u;
-							unsigned int lo = k0 < kept ? (item[k0] | (count[k0] << 8)) : 0u;
+							const unsigned int lo = k0 < kept ? (item[k0] | (count[k0] << 8)) : 0u;
 							const unsigned int hi = k1 < kept ? (item[k1] | (count[k1] << 8)) : 0u;
-							if (p == 0u) {
-								lo |= kept << 3;
-							}
 							stage[(p * R + i) * 65u + lane] = lo | (hi << 16);
 						}
 					}
 				}
 			}
 			__syncthreads();
-			if (quads && rows == R) {
-				// lane -> (column, 4 consecutive rows): 4- and 8-byte stores; a slot plane is written for the
-				// rows of a quad when one of them uses the slot (the others get zeros, never read)
-				constexpr unsigned int Q = R / 4u;
-				for (unsigned int i = lane; i < ShfLineBlock * Q; i += ShfLineBlock) {
-					const unsigned int cc = i / Q, q = i % Q;
-					const unsigned int cg = (t.y % nb) * ShfLineBlock + cc;
-					if (cg < ncols) {
-						unsigned int w0[4], n[4];
-#pragma unroll
-						for (unsigned int r = 0u; r < 4u; r++) {
-							w0[r] = stage[(4u * q + r) * 65u + cc];
-							n[r] = verticalItemKept(w0[r]);
-						}
-						const size_t y = static_cast<size_t>(y0) + 4u * q;
-						*reinterpret_cast<uchar4*>(outN + static_cast<size_t>(cg) * side + y) = make_uchar4(
-							static_cast<unsigned char>(n[0]), static_cast<unsigned char>(n[1]), static_cast<unsigned char>(n[2]),
-							static_cast<unsigned char>(n[3]));
-						const unsigned int most = max(max(n[0], n[1]), max(n[2], n[3]));
-#pragma unroll
-						for (unsigned int p = 0u; p < (MaxB + 1u) / 2u; p++) {
-							if (2u * p < most) {
-								unsigned int w[4];
-#pragma unroll
-								for (unsigned int r = 0u; r < 4u; r++) {
-									w[r] = p == 0u ? w0[r] : stage[(p * R + 4u * q + r) * 65u + cc];
-								}
-#pragma unroll
-								for (unsigned int h = 0u; h < 2u; h++) {
-									const unsigned int k = 2u * p + h;
-									if (k < most) {
-										unsigned short v[4];
-#pragma unroll
-										for (unsigned int r = 0u; r < 4u; r++) {
-											v[r] = k < n[r] ? verticalItemSlot(w[r], k) : 0u;
-										}
-										*reinterpret_cast<ushort4*>(outBin + (static_cast<size_t>(cg) * vcap + k) * side + y) =
-											make_ushort4(v[0], v[1], v[2], v[3]);
-									}
-								}
-							}
-						}
-					}
-				}
-				__syncthreads();
-#pragma unroll
-				for (unsigned int i = 0u; i < VPrefetch; i++) {
-					enter[i] = nextEnter[i];
-					leave[i] = nextLeave[i];
-				}
-				continue;
-			}
-			// consecutive lanes take consecutive rows of one column
+			// lane -> (column, row): consecutive lanes store the consecutive groups of one column
+			// (16 rows of 8 or 16 bytes), slots 0 .. 7 of a group past MaxB stay 0
 			for (unsigned int i = lane; i < ShfLineBlock * R; i += ShfLineBlock) {
 				const unsigned int cc = i / R, yy = i % R;
 				const unsigned int cg = (t.y % nb) * ShfLineBlock + cc;
 				if (cg < ncols && yy < rows) {
-					const unsigned int w0 = stage[yy * 65u + cc], n = verticalItemKept(w0);
-					const size_t y = static_cast<size_t>(y0) + yy;
-					outN[static_cast<size_t>(cg) * side + y] = static_cast<unsigned char>(n);
+					unsigned int w[4] = {};
 #pragma unroll
-					for (unsigned int k = 0u; k < MaxB; k++) {
-						if (k < n) {
-							const unsigned int w = k < 2u ? w0 : stage[((k / 2u) * R + yy) * 65u + cc];
-							outBin[(static_cast<size_t>(cg) * vcap + k) * side + y] = verticalItemSlot(w, k);
+					for (unsigned int p = 0u; p < (MaxB + 1u) / 2u; p++) {
+						if (p < gw) {
+							w[p] = stage[(p * R + yy) * 65u + cc];
 						}
+					}
+					unsigned int* const g = out + (static_cast<size_t>(cg) * side + y0 + yy) * gw;
+					if (gw == 2u) {
+						*reinterpret_cast<uint2*>(g) = make_uint2(w[0], w[1]);
+					} else {
+						*reinterpret_cast<uint4*>(g) = make_uint4(w[0], w[1], w[2], w[3]);
 					}
 				}
 			}
@@ -1000,12 +966,12 @@ namespace SuperTerrainPlus::STPBatch {
 	// horizontal pass, dense output as horizontalKERNEL; Packed: a slot holds item | count << 3 in
 	// bin_item and bin_weight is not written (the weight is count * (1 / (2r + 1)^2), the window's
 	// sum being (2r + 1)^2 at every pixel)
-	template <unsigned int MaxB, bool Packed>
+	// L as horizontalSummaryKERNEL
+	template <unsigned int MaxB, unsigned int L, bool Packed>
 	__global__ void __launch_bounds__(ShfLineBlock) horizontalItemKERNEL(const uint2* const tiles,
 		const Chunk* const chunks, const unsigned int radius, const unsigned int vcap, const unsigned int cap,
-		const unsigned short* const vbin, const unsigned char* const vcount, unsigned char* const bin_count,
-		unsigned short* const bin_item, float* const bin_weight, const unsigned int* const state,
-		unsigned int* const overflow) {
+		const unsigned int* const vbin, unsigned char* const bin_count, unsigned short* const bin_item,
+		float* const bin_weight, const unsigned int* const state, unsigned int* const overflow) {
 		extern __shared__ unsigned int lds[];
 		constexpr unsigned int P = HItemPrefetch;
 		// staged pixels, [px][word][65]: word 0 = bin count | items (3 bits each) << 4, word 1 = the
@@ -1021,8 +987,8 @@ namespace SuperTerrainPlus::STPBatch {
 		// this block's segment of the outputs; an unsplit line is the one segment [0, side)
 		const uint2 xr = segmentRange(c, seg);
 		const int xBegin = static_cast<int>(xr.x), xEnd = static_cast<int>(xr.y);
-		const unsigned short* const inBin = vbin + c.Line * vcap;
-		const unsigned char* const inN = vcount + c.Line;
+		const unsigned int gw = itemSlots(vcap) / 2u; // words per group
+		const unsigned int* const inBin = vbin + c.Line * gw;
 		const unsigned int yc = min(y, side - 1u);
 		const auto colIndex = [&](const int col) {
 			return static_cast<size_t>(col < 0 ? 0 : (col >= static_cast<int>(ncols) ? static_cast<int>(ncols) - 1 : col));
@@ -1035,18 +1001,13 @@ namespace SuperTerrainPlus::STPBatch {
 		const int first = xBegin - 2 * static_cast<int>(radius);
 		for (int s0 = first; s0 < xEnd; s0 += static_cast<int>(P)) {
 			// column histograms of the block: entering s0 + 2r + j, leaving s0 - 1 + j
-			unsigned int eN[P], lN[P], eB[P][MaxB], lB[P][MaxB];
+			unsigned int eB[P][L / 2u], lB[P][L / 2u];
 #pragma unroll
 			for (unsigned int j = 0u; j < P; j++) {
 				const size_t ce = colIndex(s0 + 2 * static_cast<int>(radius) + static_cast<int>(j)),
 					cl = colIndex(s0 - 1 + static_cast<int>(j));
-				eN[j] = inN[ce * side + yc];
-				lN[j] = inN[cl * side + yc];
-#pragma unroll
-				for (unsigned int k = 0u; k < MaxB; k++) {
-					eB[j][k] = k < vcap ? inBin[(ce * vcap + k) * side + yc] : 0u;
-					lB[j][k] = k < vcap ? inBin[(cl * vcap + k) * side + yc] : 0u;
-				}
+				loadItemGroup<L>(inBin + (ce * side + yc) * gw, eB[j]);
+				loadItemGroup<L>(inBin + (cl * side + yc) * gw, lB[j]);
 			}
 			const int sEnd = min(s0 + static_cast<int>(P), xEnd);
 #pragma unroll
@@ -1055,16 +1016,18 @@ namespace SuperTerrainPlus::STPBatch {
 				if (active && s < sEnd) {
 					const int ce = s + 2 * static_cast<int>(radius);
 #pragma unroll
-					for (unsigned int k = 0u; k < MaxB; k++) {
-						if (k < eN[j]) {
-							line.enter(eB[j][k] & 0xFFu, ce, eB[j][k] >> 8, k, radius);
+					for (unsigned int k = 0u; k < L; k++) {
+						const unsigned int v = itemSlot(eB[j], k);
+						if (v != 0u) {
+							line.enter(v & 0xFFu, ce, v >> 8, k, radius);
 						}
 					}
 					if (s >= xBegin + 1) {
 #pragma unroll
-						for (unsigned int k = 0u; k < MaxB; k++) {
-							if (k < lN[j]) {
-								line.leave(lB[j][k] & 0xFFu, lB[j][k] >> 8);
+						for (unsigned int k = 0u; k < L; k++) {
+							const unsigned int v = itemSlot(lB[j], k);
+							if (v != 0u) {
+								line.leave(v & 0xFFu, v >> 8);
 							}
 						}
 					}
@@ -1719,8 +1682,11 @@ void STPHeightfieldBatch::init(const STPEnvironment::STPRainDropSetting& rain) {
 	this->Height = STPSmartDeviceMemory::makeDevice<float>(pixel);
 	this->Fixed = STPSmartDeviceMemory::makeDevice<unsigned short>(pixel);
 	this->BiomeMap = STPSmartDeviceMemory::makeDevice<unsigned short>(halo);
-	this->VBin = STPSmartDeviceMemory::makeDevice<unsigned int>(line * this->VCap);
-	this->VCount = STPSmartDeviceMemory::makeDevice<unsigned char>(line);
+	// the register path's slots are words, the item path's groups itemSlots(VCap) 16-bit slots
+	this->VBin = STPSmartDeviceMemory::makeDevice<unsigned int>(
+		line * std::max(this->VCap, this->itemPath() ? itemSlots(this->VCap) / 2u : 0u));
+	// the item path reads a column's bin count off its slot group
+	this->VCount = STPSmartDeviceMemory::makeDevice<unsigned char>(this->itemPath() ? 1ull : line);
 	// slots of 3 words for each of at most 8 biomes: a few-line binding holds at most
 	// SegmentTarget slots, the long-line split at most sum (side / LongSegment + 1) (side + 2r)
 	// (a chunk with a free-slip halo splits its lines into three regions: at most 2 more segments)
@@ -2385,15 +2351,14 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 			if (this->itemPath()) {
 				const unsigned int words = verticalItemWords(this->VCap);
 				unsigned int* const state = this->SegState.get();
-				// the item path's column histograms are 16-bit slots (item | count << 8)
-				unsigned short* const vbin = reinterpret_cast<unsigned short*>(this->VBin.get());
+				// the item path's column histograms are groups of 16-bit slots (itemSlots)
 				const auto item = [&](auto summary, auto carry, auto filter) {
 					if (this->Segmented) {
 						summary<<<n, ShfLineBlock, 0, stream>>>(t, this->ChunkRecord, radius, this->BiomeMap.get(), state);
 						carry<<<n, ShfLineBlock, 0, stream>>>(t, this->ChunkRecord, radius, 2u * radius, state);
 					}
 					filter<<<n, ShfLineBlock, words * sizeof(unsigned int), stream>>>(t, this->ChunkRecord, radius,
-						this->VCap, this->BiomeMap.get(), vbin, this->VCount.get(), state, this->Flag.get());
+						this->VCap, this->BiomeMap.get(), this->VBin.get(), state, this->Flag.get());
 				};
 				if (this->Setting.BiomeCount <= 4u) {
 					item(verticalSummaryKERNEL<4u>, segmentCarryKERNEL<4u>, verticalItemKERNEL<4u>);
@@ -2432,28 +2397,35 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 			if (this->itemPath()) {
 				const unsigned int words = HItemPrefetch * horizontalItemWords(cap) * 65u;
 				unsigned int* const state = this->SegState.get();
-				const unsigned short* const vbin = reinterpret_cast<const unsigned short*>(this->VBin.get());
+				const unsigned int* const vbin = this->VBin.get();
 				const auto item = [&](auto summary, auto carry, auto filter) {
 					if (this->Segmented) {
-						summary<<<n, ShfLineBlock, 0, stream>>>(t, this->ChunkRecord, radius, this->VCap, vbin,
-							this->VCount.get(), state);
+						summary<<<n, ShfLineBlock, 0, stream>>>(t, this->ChunkRecord, radius, this->VCap, vbin, state);
 						carry<<<n, ShfLineBlock, 0, stream>>>(t, this->ChunkRecord, radius, 0u, state);
 					}
 					filter<<<n, ShfLineBlock, words * sizeof(unsigned int), stream>>>(t, this->ChunkRecord, radius,
-						this->VCap, cap, vbin, this->VCount.get(), this->BinCount.get(), this->BinItem.get(),
-						this->BinWeight.get(), state, this->Flag.get());
+						this->VCap, cap, vbin, this->BinCount.get(), this->BinItem.get(), this->BinWeight.get(), state,
+						this->Flag.get());
 				};
+				// slots loaded per column group: a window holds at most 4 biomes below 5, and a group
+				// holds 4 slots when vcap <= 4
 				const bool packed = this->packedBins();
 				if (this->Setting.BiomeCount <= 4u) {
 					if (packed) {
-						item(horizontalSummaryKERNEL<4u>, segmentCarryKERNEL<4u>, horizontalItemKERNEL<4u, true>);
+						item(horizontalSummaryKERNEL<4u, 4u>, segmentCarryKERNEL<4u>, horizontalItemKERNEL<4u, 4u, true>);
 					} else {
-						item(horizontalSummaryKERNEL<4u>, segmentCarryKERNEL<4u>, horizontalItemKERNEL<4u, false>);
+						item(horizontalSummaryKERNEL<4u, 4u>, segmentCarryKERNEL<4u>, horizontalItemKERNEL<4u, 4u, false>);
+					}
+				} else if (itemSlots(this->VCap) == 4u) {
+					if (packed) {
+						item(horizontalSummaryKERNEL<8u, 4u>, segmentCarryKERNEL<8u>, horizontalItemKERNEL<8u, 4u, true>);
+					} else {
+						item(horizontalSummaryKERNEL<8u, 4u>, segmentCarryKERNEL<8u>, horizontalItemKERNEL<8u, 4u, false>);
 					}
 				} else if (packed) {
-					item(horizontalSummaryKERNEL<8u>, segmentCarryKERNEL<8u>, horizontalItemKERNEL<8u, true>);
+					item(horizontalSummaryKERNEL<8u, 8u>, segmentCarryKERNEL<8u>, horizontalItemKERNEL<8u, 8u, true>);
 				} else {
-					item(horizontalSummaryKERNEL<8u>, segmentCarryKERNEL<8u>, horizontalItemKERNEL<8u, false>);
+					item(horizontalSummaryKERNEL<8u, 8u>, segmentCarryKERNEL<8u>, horizontalItemKERNEL<8u, 8u, false>);
 				}
 				break;
 			}

@@ -95,15 +95,18 @@ def test_batch_without_erosion_bit_exact(gpu, oracle, cap):
             assert all((x == y).all() for x, y in zip(results[mode][i], results[0][i])), (mode, i)
 
 
-@pytest.mark.parametrize("count", [4, 8])
-def test_batch_split_lines_bit_exact(gpu, oracle, count):
+# (biomes, bin capacity): the column histograms' slot groups hold 8 slots at capacity 8 and 4 at
+# capacity 4, of which the horizontal pass loads 4 with at most 4 biomes (the chunks below hold at
+# most 4 bins per window)
+@pytest.mark.parametrize("count,cap", [(4, 8), (8, 8), (8, 4)])
+def test_batch_split_lines_bit_exact(gpu, oracle, count, cap):
     """A binding of few lines splits every filter line of its large chunks into segments (here
     16 of 64 outputs for the 1024-pixel chunk, 5 uneven ones for the 333-pixel chunk), each
     resumed from the state folded over the segments before it: bit-exact against the oracle, for
     the 4- and the 8-biome kernels, in graph and plain mode."""
-    log(f"batch split lines count={count}")
+    log(f"batch split lines count={count} cap={cap}")
     sides, origins = [1024, 333], [(-5000, 700), (123, -456)]
-    b = Batch(0.0, sides=sides, origins=origins, count=count)
+    b = Batch(0.0, sides=sides, origins=origins, count=count, cap=cap)
     for mode in (0, 2):
         b.run(mode)
         for i in range(len(sides)):
