@@ -559,25 +559,31 @@ namespace SuperTerrainPlus::STPBatch {
 		__device__ inline void resume(const unsigned int* const state, const Chunk& c, const unsigned int lines,
 			const unsigned int seg, const unsigned int line);
 		// biome `it` occurs at position `pos` with count `c`, as slot `slot` of that position
+		// Straight-line selects over the biomes: written as branches, the compiler turns the updates
+		// into a branch tree on `it`, which a wave whose lanes hold different biomes walks in full.
 		__device__ inline void enter(const unsigned int it, const int pos, const unsigned int c, const unsigned int slot,
 			const unsigned int radius) {
+			const int gap = pos - 2 * static_cast<int>(radius) - 1;
+			const unsigned int key = static_cast<unsigned int>(pos) * 64u + slot;
 #pragma unroll
 			for (unsigned int i = 0u; i < MaxB; i++) {
-				if (i == it) {
-					if (this->Last[i] < pos - 2 * static_cast<int>(radius) - 1) {
-						this->Key[i] = static_cast<unsigned int>(pos) * 64u + slot;
-					}
-					this->Last[i] = pos;
-					this->Count[i] += c;
-				}
+				const bool hit = i == it;
+				this->Key[i] = hit && this->Last[i] < gap ? key : this->Key[i];
+				this->Last[i] = hit ? pos : this->Last[i];
+				this->Count[i] += hit ? c : 0u;
 			}
 		}
 		__device__ inline void leave(const unsigned int it, const unsigned int c) {
 #pragma unroll
 			for (unsigned int i = 0u; i < MaxB; i++) {
-				if (i == it) {
-					this->Count[i] -= c;
-				}
+				this->Count[i] -= i == it ? c : 0u;
+			}
+		}
+		// biome `it` occurs again at `pos` with its count and run unchanged
+		__device__ inline void touch(const unsigned int it, const int pos) {
+#pragma unroll
+			for (unsigned int i = 0u; i < MaxB; i++) {
+				this->Last[i] = i == it ? pos : this->Last[i];
 			}
 		}
 		// present biomes in order: item[k], count[k] for k < returned n
@@ -682,18 +688,16 @@ namespace SuperTerrainPlus::STPBatch {
 				this->Last[i] = -0x40000000;
 			}
 		}
+		// straight-line selects, as ItemLine::enter
 		__device__ inline void occur(const unsigned int it, const int pos, const unsigned int slot, const unsigned int radius) {
+			const int gap = pos - 2 * static_cast<int>(radius) - 1;
+			const unsigned int key = static_cast<unsigned int>(pos) * 64u + slot;
 #pragma unroll
 			for (unsigned int i = 0u; i < MaxB; i++) {
-				if (i == it) {
-					const unsigned int key = static_cast<unsigned int>(pos) * 64u + slot;
-					if (this->First[i] == NoRun) {
-						this->First[i] = key;
-					} else if (this->Last[i] < pos - 2 * static_cast<int>(radius) - 1) {
-						this->Inner[i] = key;
-					}
-					this->Last[i] = pos;
-				}
+				const bool hit = i == it, fresh = this->First[i] == NoRun;
+				this->Inner[i] = hit && !fresh && this->Last[i] < gap ? key : this->Inner[i];
+				this->First[i] = hit && fresh ? key : this->First[i];
+				this->Last[i] = hit ? pos : this->Last[i];
 			}
 		}
 		__device__ inline void store(unsigned int* const state, const Chunk& c, const unsigned int lines,
@@ -821,14 +825,16 @@ namespace SuperTerrainPlus::STPBatch {
 		const bool active = y < side;
 		const unsigned int yc = min(y, side - 1u);
 		const unsigned int gw = itemSlots(vcap) / 2u; // words per group
-		const unsigned int* const inBin = vbin + c.Line * gw;
+		// the lane's row of the chunk's groups; a column steps by side * gw words (wave-uniform)
+		const unsigned int* const inRow = vbin + (c.Line + yc) * gw;
+		const size_t colStride = static_cast<size_t>(side) * gw;
 		RunSummary<MaxB> sum;
 		for (unsigned int x0 = xr.x; x0 < xr.y; x0 += P) {
 			unsigned int b[P][L / 2u];
 #pragma unroll
 			for (unsigned int j = 0u; j < P; j++) {
 				const size_t col = min(x0 + j, xr.y - 1u);
-				loadItemGroup<L>(inBin + (col * side + yc) * gw, b[j]);
+				loadItemGroup<L>(inRow + col * colStride, b[j]);
 			}
 #pragma unroll
 			for (unsigned int j = 0u; j < P; j++) {
@@ -988,8 +994,10 @@ namespace SuperTerrainPlus::STPBatch {
 		const uint2 xr = segmentRange(c, seg);
 		const int xBegin = static_cast<int>(xr.x), xEnd = static_cast<int>(xr.y);
 		const unsigned int gw = itemSlots(vcap) / 2u; // words per group
-		const unsigned int* const inBin = vbin + c.Line * gw;
 		const unsigned int yc = min(y, side - 1u);
+		// the lane's row of the chunk's groups; a column steps by side * gw words (wave-uniform)
+		const unsigned int* const inRow = vbin + (c.Line + yc) * gw;
+		const size_t colStride = static_cast<size_t>(side) * gw;
 		const auto colIndex = [&](const int col) {
 			return static_cast<size_t>(col < 0 ? 0 : (col >= static_cast<int>(ncols) ? static_cast<int>(ncols) - 1 : col));
 		};
@@ -998,6 +1006,9 @@ namespace SuperTerrainPlus::STPBatch {
 			line.resume(state, c, side, seg, y);
 		}
 		bool over = false;
+		// the staged words of the lane's last pixel: bin count | items, 1 / sum (unpacked only), counts
+		unsigned int outHead = 0u, outPacked[(MaxB + 1u) / 2u] = {};
+		float outNorm = 0.0f;
 		const int first = xBegin - 2 * static_cast<int>(radius);
 		for (int s0 = first; s0 < xEnd; s0 += static_cast<int>(P)) {
 			// column histograms of the block: entering s0 + 2r + j, leaving s0 - 1 + j
@@ -1006,56 +1017,88 @@ namespace SuperTerrainPlus::STPBatch {
 			for (unsigned int j = 0u; j < P; j++) {
 				const size_t ce = colIndex(s0 + 2 * static_cast<int>(radius) + static_cast<int>(j)),
 					cl = colIndex(s0 - 1 + static_cast<int>(j));
-				loadItemGroup<L>(inBin + (ce * side + yc) * gw, eB[j]);
-				loadItemGroup<L>(inBin + (cl * side + yc) * gw, lB[j]);
+				loadItemGroup<L>(inRow + ce * colStride, eB[j]);
+				loadItemGroup<L>(inRow + cl * colStride, lB[j]);
 			}
 			const int sEnd = min(s0 + static_cast<int>(P), xEnd);
 #pragma unroll
 			for (unsigned int j = 0u; j < P; j++) {
 				const int s = s0 + static_cast<int>(j);
-				if (active && s < sEnd) {
+				if (s < sEnd) {
 					const int ce = s + 2 * static_cast<int>(radius);
+					// An entering column group equal to the leaving one (inside a biome's area) leaves
+					// every count and run key as it was (each of its items occurs in the leaving column,
+					// at s - 1, no further back than the gap a new run needs): only the items' last
+					// occurrences move, and the pixel's bins are the previous pixel's. Taken when every
+					// lane of the wave can.
+					bool same = true;
 #pragma unroll
-					for (unsigned int k = 0u; k < L; k++) {
-						const unsigned int v = itemSlot(eB[j], k);
-						if (v != 0u) {
-							line.enter(v & 0xFFu, ce, v >> 8, k, radius);
-						}
+					for (unsigned int w = 0u; w < L / 2u; w++) {
+						same = same && eB[j][w] == lB[j][w];
 					}
-					if (s >= xBegin + 1) {
+					if (s >= xBegin + 1 && __all(!active || same)) {
+						if (active) {
+#pragma unroll
+							for (unsigned int k = 0u; k < L; k++) {
+								const unsigned int v = itemSlot(eB[j], k);
+								if (v != 0u) {
+									line.touch(v & 0xFFu, ce);
+								}
+							}
+						}
+					} else if (active) {
 #pragma unroll
 						for (unsigned int k = 0u; k < L; k++) {
-							const unsigned int v = itemSlot(lB[j], k);
+							const unsigned int v = itemSlot(eB[j], k);
 							if (v != 0u) {
-								line.leave(v & 0xFFu, v >> 8);
+								line.enter(v & 0xFFu, ce, v >> 8, k, radius);
+							}
+						}
+						if (s >= xBegin + 1) {
+#pragma unroll
+							for (unsigned int k = 0u; k < L; k++) {
+								const unsigned int v = itemSlot(lB[j], k);
+								if (v != 0u) {
+									line.leave(v & 0xFFu, v >> 8);
+								}
+							}
+						}
+						if (s >= xBegin) {
+							unsigned int item[MaxB], count[MaxB];
+							const unsigned int n = line.order(item, count);
+							over = over || n > cap;
+							if (!Packed) {
+								unsigned int sum = 0u;
+#pragma unroll
+								for (unsigned int k = 0u; k < MaxB; k++) {
+									sum += k < n ? count[k] : 0u;
+								}
+								outNorm = 1.0f / static_cast<float>(sum);
+							}
+							outHead = min(n, cap);
+#pragma unroll
+							for (unsigned int w = 0u; w < (MaxB + 1u) / 2u; w++) {
+								outPacked[w] = 0u;
+							}
+#pragma unroll
+							for (unsigned int k = 0u; k < MaxB; k++) {
+								if (k < n && k < cap) {
+									outHead |= item[k] << (4u + 3u * k);
+									outPacked[k / 2u] |= count[k] << (16u * (k & 1u));
+								}
 							}
 						}
 					}
-					if (s >= xBegin) {
-						unsigned int item[MaxB], count[MaxB];
-						const unsigned int n = line.order(item, count);
-						over = over || n > cap;
-						unsigned int sum = 0u;
-#pragma unroll
-						for (unsigned int k = 0u; k < MaxB; k++) {
-							sum += k < n ? count[k] : 0u;
-						}
-						const float norm = 1.0f / static_cast<float>(sum);
-						unsigned int head = min(n, cap), packed[(MaxB + 1u) / 2u] = {};
-#pragma unroll
-						for (unsigned int k = 0u; k < MaxB; k++) {
-							if (k < n && k < cap) {
-								head |= item[k] << (4u + 3u * k);
-								packed[k / 2u] |= count[k] << (16u * (k & 1u));
-							}
-						}
+					if (active && s >= xBegin) {
 						unsigned int* const at = stage + j * words * 65u + lane;
-						at[0] = head;
-						at[65] = __float_as_uint(norm);
+						at[0] = outHead;
+						if (!Packed) {
+							at[65] = __float_as_uint(outNorm);
+						}
 #pragma unroll
 						for (unsigned int w = 0u; w < (MaxB + 1u) / 2u; w++) {
 							if (2u * w < cap) {
-								at[(2u + w) * 65u] = packed[w];
+								at[(2u + w) * 65u] = outPacked[w];
 							}
 						}
 					}
