@@ -75,9 +75,11 @@ def oracle_chunk(oracle, i, sides=SIDES, origins=ORIGINS, count=COUNT):
     return halo, h
 
 
-# histogram capacities: 8 keeps both filter passes' accumulators in registers, 16 keeps the
-# vertical one in registers and the horizontal one in LDS, 33 puts both in LDS
-@pytest.mark.parametrize("cap", [8, 16, 33])
+# histogram capacities: 3 and 8 take the biome-indexed item passes (3: an odd capacity, whose
+# dense output the horizontal pass stores slot by slot; these chunks hold at most 3 bins per
+# window), 16 keeps the vertical accumulator in registers and the horizontal one in LDS, 33 puts
+# both in LDS
+@pytest.mark.parametrize("cap", [3, 8, 16, 33])
 def test_batch_without_erosion_bit_exact(gpu, oracle, cap):
     log(f"batch no erosion cap={cap}")
     b = Batch(0.0, cap=cap)
