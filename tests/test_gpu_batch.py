@@ -288,6 +288,63 @@ def test_batch_status_clear_after_tiled_erosion(gpu):
     assert status.value == 0
 
 
+def random_binding(seed):
+    """A seeded binding: 1-8 biomes, an even filter radius 2-24, a bin capacity from the biome
+    count up to 8 (the item passes; a window never holds more bins than biomes), 3-6 chunks of
+    sides 8-300 at levels of detail 1, 2 or 4 and origins anywhere in +-2^20."""
+    rng = np.random.default_rng(seed)
+    count = int(rng.integers(1, 9))
+    radius = 2 * int(rng.integers(1, 13))
+    cap = int(rng.integers(count, 9))
+    cell = int(rng.integers(16, 200))
+    n = int(rng.integers(3, 7))
+    sides = [int(v) for v in rng.integers(8, 301, n)]
+    steps = [int(v) for v in rng.choice([1, 2, 4], n)]
+    origins = [(int(a), int(b)) for a, b in rng.integers(-(1 << 20), 1 << 20, (n, 2))]
+    return count, radius, cap, cell, sides, steps, origins
+
+
+@pytest.mark.parametrize("seed", [101, 202, 303, 404])
+def test_batch_random_bindings_bit_exact(gpu, oracle, seed):
+    """Seeded random bindings through the item passes (every biome count, slot groups of 4 and 8,
+    odd capacities, segmented and unsplit lines, levels of detail): every chunk's biome map,
+    heights and UINT16 copy are bit-exact against the oracle in graph and plain mode."""
+    count, radius, cap, cell, sides, steps, origins = random_binding(seed)
+    log(f"batch random binding seed={seed}: {count} biomes r={radius} cap={cap} cell={cell} sides={sides} steps={steps}")
+    setting = capi.BatchSetting(SEED, count, cell, radius, cap, 0.0)
+    chunks = (capi.BatchChunk * len(sides))(*[capi.BatchChunk(s, (C.c_int32 * 2)(*o), st)
+                                               for s, o, st in zip(sides, origins, steps)])
+    h = C.c_void_p()
+    capi.call("stp_batch_create", C.byref(NOISE), capi.biome_array(props(count)), count,
+              C.byref(W.erosion_setting(1)), C.byref(setting), chunks, len(sides), C.byref(h))
+    try:
+        perm, g32, _ = oracle.permutation(42, W.DISTRIBUTION, W.GRADIENT_OFFSET, W.ENGINE_MT19937)
+        want = []
+        for side, o, st in zip(sides, origins, steps):
+            w = side + 2 * radius
+            halo = oracle.voronoi_map(SEED, count, cell, (o[0] - radius * st, o[1] - radius * st), w, w, step=st)
+            item, weight, off = oracle.shf_window(halo, (radius, radius), (side, side), radius)
+            half = side * st / 2.0
+            hm = oracle.multi_biome(perm, g32, props(count), item, weight, off, (side, side),
+                                    (o[0] + half, o[1] + half), step=st)
+            want.append((halo, hm))
+        for mode in (0, 2):
+            capi.call("stp_batch_run", h, mode, None)
+            capi.synchronize()
+            for i, side in enumerate(sides):
+                w = side + 2 * radius
+                gh = np.zeros((side, side), np.float32)
+                gf = np.zeros((side, side), np.uint16)
+                gb = np.zeros((w, w), np.uint16)
+                capi.call("stp_batch_read_chunk", h, i, capi.ptr(gh), capi.ptr(gf), capi.ptr(gb), None)
+                halo, hm = want[i]
+                assert (gb == halo).all(), (mode, i)
+                assert gh.tobytes() == hm.tobytes(), (mode, i)
+                assert (gf == oracle.format_heightmap(hm)).all(), (mode, i)
+    finally:
+        capi.lib().stp_batch_destroy(h)
+
+
 RAGGED_SIDES = [8, 9, 13, 37, 71, 73, 145]
 RAGGED_ORIGINS = [(-3, 5), (1000, -1000), (-4097, 33), (250, 250), (-71, -73), (7777, -1), (0, 145)]
 
