@@ -258,14 +258,14 @@ def time_batch(batch, steps, stream, mode=0):
 
 
 def prefer_profiled_shape(batch, tuning, stream, runs=8, slack=1.02):
-    """The autotuner's erosion launch shape, or a shape with a measured instruction profile
-    (EROSION_PROFILES) when one runs the batch within `slack` of it: the roofline record needs the
-    profile, and the autotuner's own timing (two runs per shape) separates shapes this close only
-    by noise. Graph replays timed over `runs` runs each, best of two rounds."""
+    """The erosion launch shape the bench runs: the autotuner's shape and every shape with a
+    measured instruction profile (EROSION_PROFILES; the roofline record needs one) are timed again
+    as graph replays over `runs` runs each, best of two rounds, since the autotuner's own timing
+    (two runs per shape) separates shapes this close only by noise. The fastest profiled shape is
+    kept when it runs the batch within `slack` of the autotuner's shape."""
     from superterrainplus_amd import capi
-    if (tuning.ErosionTile, tuning.ErosionThreads) in EROSION_PROFILES:
-        return tuning
-    shapes = [(tuning.ErosionTile, tuning.ErosionThreads)] + list(EROSION_PROFILES)
+    shapes = [(tuning.ErosionTile, tuning.ErosionThreads)] + [k for k in EROSION_PROFILES
+                                                               if k != (tuning.ErosionTile, tuning.ErosionThreads)]
     best = {}
     for _ in range(2):
         for shape in shapes:
@@ -274,7 +274,7 @@ def prefer_profiled_shape(batch, tuning, stream, runs=8, slack=1.02):
             best[shape] = min(best.get(shape, ms), ms)
     pick = min(EROSION_PROFILES, key=lambda k: best[k])
     chosen = pick if best[pick] <= slack * best[shapes[0]] else shapes[0]
-    print(f"[bench] autotuned erosion shape {shapes[0]} has no profile: " + ", ".join(
+    print(f"[bench] erosion shape: autotuned {shapes[0]}; " + ", ".join(
         f"{k} {v:.3f} ms" for k, v in best.items()) + f"; keeping {chosen}", file=sys.stderr, flush=True)
     out = capi.BatchTuning(*chosen)
     capi.call("stp_batch_set_tuning", batch, C.byref(out))
@@ -940,7 +940,7 @@ def main():
           f"{' (autotuned)' if autotuned else ''}", file=sys.stderr, flush=True)
 
     # Timed region: replays of the batch's one captured graph (the mode stp_batch_run(0) runs: on
-    # this binding two chunk groups whose whole pipelines are concurrent chains), K steps bracketed
+    # this binding four chunk groups whose whole pipelines are concurrent chains), K steps bracketed
     # by synchronisations; this is `value`.
     for _ in range(args.warmup):
         capi.call("stp_batch_run", batch, 0, stream)
@@ -1017,7 +1017,7 @@ def main():
             "plain_chunk_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                                        "p99": round(float(np.percentile(lat, 99)), 3)} if len(lat) else None,
             "batched_step_latency_ms": round(ms_per_step, 3),
-            "timed_region": "replays of the batch's one captured graph (two chunk groups as concurrent chains)",
+            "timed_region": "replays of the batch's one captured graph (four chunk groups as concurrent chains)",
             "staged_mpixel_s": round(pixels / staged_wall / 1e6, 2),
             "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES, stage_ms)},
             "stage_ms_note": "from K more runs with one graph per stage and HIP events between (stages in step, "

@@ -486,24 +486,17 @@ namespace SuperTerrainPlus::STPBatch {
 	// registers, and sorts the present biomes by key when it emits -- O(MaxB) register work per
 	// step instead of searching and shifting bins.
 
-	// ascending sorting network over (key, value) pairs, every index static
+	// ascending sorting network over keys, every index static: odd-even transposition, N rounds of
+	// adjacent compare-exchanges as min / max
 	template <unsigned int N>
-	__device__ inline void sortPairs(unsigned int (&key)[N], unsigned int (&val)[N]) {
-		const auto swap = [&](const unsigned int a, const unsigned int b) {
-			const bool x = key[b] < key[a];
-			const unsigned int ka = x ? key[b] : key[a], kb = x ? key[a] : key[b];
-			const unsigned int va = x ? val[b] : val[a], vb = x ? val[a] : val[b];
-			key[a] = ka;
-			key[b] = kb;
-			val[a] = va;
-			val[b] = vb;
-		};
-		// odd-even transposition: N rounds of adjacent compare-exchanges
+	__device__ inline void sortKeys(unsigned int (&key)[N]) {
 #pragma unroll
 		for (unsigned int round = 0u; round < N; round++) {
 #pragma unroll
 			for (unsigned int a = round & 1u; a + 1u < N; a += 2u) {
-				swap(a, a + 1u);
+				const unsigned int lo = min(key[a], key[a + 1u]), hi = max(key[a], key[a + 1u]);
+				key[a] = lo;
+				key[a + 1u] = hi;
 			}
 		}
 	}
@@ -586,25 +579,34 @@ namespace SuperTerrainPlus::STPBatch {
 				this->Last[i] = i == it ? pos : this->Last[i];
 			}
 		}
-		// present biomes in order: item[k], count[k] for k < returned n
+		// Present biomes in order: item[k], count[k] for k < returned n. A run key is below 2^21
+		// (position * 64 + slot, positions below 2^15), so the biome id rides in its low 3 bits and
+		// the keys sort alone, with min / max (distinct biomes never share a key).
+		// CountBits (8 or 16) bounds the window counts: they are packed 32 / CountBits to a word and
+		// each sorted slot's count is a bit-field extract at its biome id.
+		template <unsigned int CountBits>
 		__device__ inline unsigned int order(unsigned int (&item)[MaxB], unsigned int (&count)[MaxB]) const {
-			unsigned int key[MaxB], n = 0u;
+			static_assert(CountBits == 8u || CountBits == 16u, "counts packed as bytes or halves");
+			constexpr unsigned int PerWord = 32u / CountBits, Words = (MaxB + PerWord - 1u) / PerWord;
+			unsigned int key[MaxB], packed[Words] = {}, n = 0u;
 #pragma unroll
 			for (unsigned int i = 0u; i < MaxB; i++) {
 				const bool present = this->Count[i] > 0u;
-				key[i] = present ? this->Key[i] : NoRun;
-				item[i] = i;
+				key[i] = present ? (this->Key[i] << 3) | i : NoRun;
 				n += present ? 1u : 0u;
+				packed[i / PerWord] |= this->Count[i] << (CountBits * (i % PerWord));
 			}
-			sortPairs(key, item);
+			sortKeys(key);
 #pragma unroll
 			for (unsigned int k = 0u; k < MaxB; k++) {
-				unsigned int c = 0u;
+				const unsigned int it = key[k] & 7u;
+				unsigned int w = packed[0];
 #pragma unroll
-				for (unsigned int i = 0u; i < MaxB; i++) {
-					c = item[k] == i ? this->Count[i] : c;
+				for (unsigned int q = 1u; q < Words; q++) {
+					w = it / PerWord == q ? packed[q] : w;
 				}
-				count[k] = c;
+				item[k] = it;
+				count[k] = __builtin_amdgcn_ubfe(w, CountBits * (it % PerWord), CountBits);
 			}
 			return n;
 		}
@@ -918,7 +920,8 @@ namespace SuperTerrainPlus::STPBatch {
 						line.leave(leave[i], 1u);
 					}
 					unsigned int item[MaxB], count[MaxB];
-					const unsigned int n = line.order(item, count);
+					// a column window's counts are at most 2r + 1 <= 255
+					const unsigned int n = line.template order<8u>(item, count);
 					over = over || n > vcap;
 					const unsigned int kept = min(n, vcap);
 #pragma unroll
@@ -1065,7 +1068,8 @@ namespace SuperTerrainPlus::STPBatch {
 						}
 						if (s >= xBegin) {
 							unsigned int item[MaxB], count[MaxB];
-							const unsigned int n = line.order(item, count);
+							// a window's counts are at most (2r + 1)^2 <= 65025
+							const unsigned int n = line.template order<16u>(item, count);
 							over = over || n > cap;
 							if (!Packed) {
 								unsigned int sum = 0u;
@@ -2256,9 +2260,9 @@ STPBatchTuning STPHeightfieldBatch::autotune(const hipStream_t stream, const uns
 	if (candidate.empty()) {
 		candidate.push_back(STPBatchTuning {0u, ErosionBlock});
 	}
-	// Each shape is scored on whole graph runs (the mode a caller runs, with two concurrent chunk
+	// Each shape is scored on whole graph runs (the mode a caller runs, with four concurrent chunk
 	// chains on large bindings): the tuning changes only the erosion binning and erosion, but how
-	// well they share the GPU with the other chain's stages depends on the shape.
+	// well they share the GPU with the other chains' stages depends on the shape.
 	STPBatchTuning best = candidate.front();
 	float bestMs = 0.0f;
 	hipEvent_t begin = nullptr, end = nullptr;
