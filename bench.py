@@ -291,7 +291,8 @@ def chunks_512(stream, count=64, steps=10, plain_steps=2):
     b = make_batch(cfg, sides, origins, [1] * count, props)
     try:
         tuning = capi.BatchTuning()
-        capi.call("stp_batch_autotune", b, 2, stream, C.byref(tuning))
+        # six timed graph runs per shape: two leave the pick between close shapes to noise
+        capi.call("stp_batch_autotune", b, 6, stream, C.byref(tuning))
         graph_ms = time_batch(b, steps, stream)
         plain_ms = time_batch(b, plain_steps, stream, mode=2)
         lat = (C.c_double * count)()

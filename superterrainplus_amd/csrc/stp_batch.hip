@@ -1079,6 +1079,8 @@ namespace SuperTerrainPlus::STPBatch {
 								}
 								outNorm = 1.0f / static_cast<float>(sum);
 							}
+							// every slot packed: the store phase reads slot k only for k below the
+							// staged bin count min(n, cap) (items 3 bits, counts 16 bits each)
 							outHead = min(n, cap);
 #pragma unroll
 							for (unsigned int w = 0u; w < (MaxB + 1u) / 2u; w++) {
@@ -1086,10 +1088,8 @@ namespace SuperTerrainPlus::STPBatch {
 							}
 #pragma unroll
 							for (unsigned int k = 0u; k < MaxB; k++) {
-								if (k < n && k < cap) {
-									outHead |= item[k] << (4u + 3u * k);
-									outPacked[k / 2u] |= count[k] << (16u * (k & 1u));
-								}
+								outHead |= item[k] << (4u + 3u * k);
+								outPacked[k / 2u] |= count[k] << (16u * (k & 1u));
 							}
 						}
 					}
