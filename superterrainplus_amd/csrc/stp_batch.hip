@@ -969,8 +969,12 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 	}
 
-	// LDS words per staged pixel of horizontalItemKERNEL
-	__host__ __device__ constexpr unsigned int horizontalItemWords(const unsigned int cap) { return 2u + (cap + 1u) / 2u; }
+	// LDS words per staged pixel of horizontalItemKERNEL: the head word, the 1 / sum word unless
+	// Packed (whose weights the blend rebuilds), then the counts two to a word
+	__host__ __device__ constexpr unsigned int horizontalCountWord(const bool packed) { return packed ? 1u : 2u; }
+	__host__ __device__ constexpr unsigned int horizontalItemWords(const unsigned int cap, const bool packed) {
+		return horizontalCountWord(packed) + (cap + 1u) / 2u;
+	}
 
 	// horizontal pass, dense output as horizontalKERNEL; Packed: a slot holds item | count << 3 in
 	// bin_item and bin_weight is not written (the weight is count * (1 / (2r + 1)^2), the window's
@@ -984,8 +988,9 @@ namespace SuperTerrainPlus::STPBatch {
 		extern __shared__ unsigned int lds[];
 		constexpr unsigned int P = HItemPrefetch;
 		// staged pixels, [px][word][65]: word 0 = bin count | items (3 bits each) << 4, word 1 = the
-		// weight factor 1 / sum, words 2.. = the counts, two 16-bit halves per word
-		const unsigned int words = horizontalItemWords(cap);
+		// weight factor 1 / sum (not Packed), then the counts, two 16-bit halves per word
+		constexpr unsigned int C = horizontalCountWord(Packed);
+		const unsigned int words = horizontalItemWords(cap, Packed);
 		unsigned int* const stage = lds;
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
@@ -1102,7 +1107,7 @@ namespace SuperTerrainPlus::STPBatch {
 #pragma unroll
 						for (unsigned int w = 0u; w < (MaxB + 1u) / 2u; w++) {
 							if (2u * w < cap) {
-								at[(2u + w) * 65u] = outPacked[w];
+								at[(C + w) * 65u] = outPacked[w];
 							}
 						}
 					}
@@ -1125,10 +1130,10 @@ namespace SuperTerrainPlus::STPBatch {
 							const size_t p = static_cast<size_t>(y0 + r) * side + static_cast<size_t>(xa) + px;
 							const unsigned int* const at = stage + (jb + px) * words * 65u + r;
 							const unsigned int head = at[0], n = head & 15u;
-							const float norm = __uint_as_float(at[65]);
+							const float norm = Packed ? 0.0f : __uint_as_float(at[65]);
 							bin_count[c.Pixel + p] = static_cast<unsigned char>(n);
 							for (unsigned int q = 0u; q < cap; q += 4u) {
-								const unsigned int lo = at[(2u + q / 2u) * 65u], hi = at[(3u + q / 2u) * 65u];
+								const unsigned int lo = at[(C + q / 2u) * 65u], hi = at[(C + 1u + q / 2u) * 65u];
 								const unsigned int cnt[4] = {lo & 0xFFFFu, lo >> 16, hi & 0xFFFFu, hi >> 16};
 								unsigned short it[4];
 								float w[4];
@@ -1160,7 +1165,7 @@ namespace SuperTerrainPlus::STPBatch {
 							const unsigned int head = at[0], n = head & 15u;
 							// slots past the pixel's bin count are never read: not stored
 							if (k < n) {
-								const unsigned int count = (at[(2u + k / 2u) * 65u] >> (16u * (k & 1u))) & 0xFFFFu;
+								const unsigned int count = (at[(C + k / 2u) * 65u] >> (16u * (k & 1u))) & 0xFFFFu;
 								const unsigned int item = (head >> (4u + 3u * k)) & 7u;
 								bin_item[(c.Pixel + p) * cap + k] = static_cast<unsigned short>(Packed ? item | count << 3u : item);
 								if (!Packed) {
@@ -1438,7 +1443,10 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 	};
 
-	template <unsigned int Threads>
+	// Taps: the brush size the drops are compiled for, 0 for any (a 25-tap drop alone holds 88
+	// VGPRs instead of the 115 every brush size needs, which leaves an erosion CU room for other
+	// kernels' waves)
+	template <unsigned int Threads, unsigned int Taps>
 	__global__ void __launch_bounds__(Threads) erodeTileKERNEL(const uint2* const tiles, const Chunk* const chunks,
 		const ErosionArgs args, const unsigned int tile, const unsigned int halo, const int2* const brush_offset,
 		const float* const brush_weight, const unsigned int* const tile_drops, const unsigned int* const tile_first,
@@ -1496,7 +1504,11 @@ namespace SuperTerrainPlus::STPBatch {
 		const unsigned int first = tile_first[g];
 		for (unsigned int k = threadIdx.x; k < n; k += Threads) {
 			// results in tile order (consecutive lanes write consecutive words), within the chunk's range
-			erodeDropAnyBrush(s, a, brush, bin[first + k], first + k, ledger, drop_info);
+			if constexpr (Taps == 0u) {
+				erodeDropAnyBrush(s, a, brush, bin[first + k], first + k, ledger, drop_info);
+			} else {
+				erodeDrop<LocalSurface, Taps>(s, a, brush, bin[first + k], first + k, ledger, drop_info);
+			}
 		}
 		__syncthreads();
 		// No other workgroup of this colour batch touches the window, so the heights in HBM are
@@ -2056,7 +2068,9 @@ void STPHeightfieldBatch::releaseGraphs() noexcept {
 namespace {
 	template <unsigned int Threads>
 	void allowLds(const unsigned int bytes) {
-		STP_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&erodeTileKERNEL<Threads>),
+		STP_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&erodeTileKERNEL<Threads, 0u>),
+			hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+		STP_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&erodeTileKERNEL<Threads, 25u>),
 			hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
 	}
 } // namespace
@@ -2442,7 +2456,7 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 					this->BinCount.get(), this->BinItem.get(), this->BinWeight.get(), this->Flag.get());
 			};
 			if (this->itemPath()) {
-				const unsigned int words = HItemPrefetch * horizontalItemWords(cap) * 65u;
+				const unsigned int words = HItemPrefetch * horizontalItemWords(cap, this->packedBins()) * 65u;
 				unsigned int* const state = this->SegState.get();
 				const unsigned int* const vbin = this->VBin.get();
 				const auto item = [&](auto summary, auto carry, auto filter) {
@@ -2564,15 +2578,16 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 						this->TileFirst.get(), this->DropBin.get(), this->Height.get(), this->Ledger.get(),
 						this->DropInfo.get(), this->Flag.get() + 1);
 				};
+				const bool taps25 = args.BrushSize == 25u;
 				switch (this->Tuning.ErosionThreads) {
 				case 256u:
-					launch(erodeTileKERNEL<256u>, 256u);
+					taps25 ? launch(erodeTileKERNEL<256u, 25u>, 256u) : launch(erodeTileKERNEL<256u, 0u>, 256u);
 					break;
 				case 512u:
-					launch(erodeTileKERNEL<512u>, 512u);
+					taps25 ? launch(erodeTileKERNEL<512u, 25u>, 512u) : launch(erodeTileKERNEL<512u, 0u>, 512u);
 					break;
 				default:
-					launch(erodeTileKERNEL<1024u>, 1024u);
+					taps25 ? launch(erodeTileKERNEL<1024u, 25u>, 1024u) : launch(erodeTileKERNEL<1024u, 0u>, 1024u);
 					break;
 				}
 			}
