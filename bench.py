@@ -59,14 +59,14 @@ LDS_ACCESS_STEP, LDS_ACCESS_ERODE, LDS_ACCESS_DEPOSIT = 8, 25, 4
 # product and the sum); one bin of a pixel costs octave * 66 + 3
 FLOP_PER_OCTAVE, FLOP_PER_BIN = 66, 3
 FP32_PEAK_TFLOPS = 157.3
-# Measured on this workload with rocprofv3 for the LDS-tiled erosion kernel at the launch shapes
-# the autotuner picks from (tile, threads) (profiles/erosion_sq_counters.md,
+# Measured on this workload with rocprofv3 for the LDS-tiled erosion kernel (its 25-tap
+# instantiation) at the launch shapes the autotuner picks from (tile, threads) (profiles/erosion_sq_counters.md,
 # profiles/erosion_hbm_traffic.md): VALU wave64 instructions per 64 drop steps (SQ_INSTS_VALU /
 # (drop steps / 64)), and HBM bytes per colour phase (FETCH_SIZE x 2 + WRITE_SIZE of its launches,
 # calibrated on the format kernel).
 EROSION_PROFILES = {
-    (128, 1024): {"valu_per_64_drop_steps": 187.5, "hbm_bytes_per_phase": 271.1e6},
-    (72, 512): {"valu_per_64_drop_steps": 203.6, "hbm_bytes_per_phase": 443.2e6},
+    (128, 1024): {"valu_per_64_drop_steps": 187.3, "hbm_bytes_per_phase": 271.0e6},
+    (72, 512): {"valu_per_64_drop_steps": 203.4, "hbm_bytes_per_phase": 443.2e6},
 }
 # serving loop: two engines of 8 Mpx / 512 chunks each
 SERVER_BATCH_PIXELS, SERVER_BATCH_CHUNKS, SERVER_ENGINES = 8 << 20, 512, 2
@@ -850,7 +850,8 @@ def erosion_roofline(tuning, stage_ms, counters):
     HIP-event time."""
     pixels, drops, steps, erode, deposit, killed = counters
     tiled = tuning.ErosionTile > 0
-    kernel = f"erodeTileKERNEL<{tuning.ErosionThreads}>" if tiled else "erosionKERNEL"
+    # the workload's brush radius 3 has 25 taps, for which the tiled kernel has an instantiation
+    kernel = f"erodeTileKERNEL<{tuning.ErosionThreads}, 25>" if tiled else "erosionKERNEL"
     # the tiled erosion runs four colour phases (on this workload each phase is two concurrent
     # launches, one per chunk group and stream, each lasting about the phase); the global-atomic
     # kernel is one launch. Figures "per launch" below are per phase.
