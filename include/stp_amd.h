@@ -393,8 +393,9 @@ int stp_batch_bin_total(stp_heightfield_batch* batch, uint64_t* h_out, void* str
  * window that fits in LDS, and every biome's heights (Depth to Depth + Variation) within [0, 4]:
  * the window holds heights in 32-bit fixed point, and below 0 the erode update can drive them
  * without bound. A batch outside that range erodes with 0. 0: every drop works on the heightmap in
- * device memory with global atomics. ErosionThreads: threads of one tile workgroup, 256, 512 or
- * 1024. */
+ * device memory with global atomics. ErosionThreads: threads of one tile workgroup, 256, 512, 768
+ * or 1024 (768: two tile-72 windows per CU at six waves per SIMD; offered by the autotuner for the
+ * 25-tap brush, whose drop fits the 80 registers per lane that leaves). */
 typedef struct {
 	uint32_t ErosionTile;
 	uint32_t ErosionThreads;

@@ -1443,11 +1443,10 @@ namespace SuperTerrainPlus::STPBatch {
 		}
 	};
 
-	// Taps: the brush size the drops are compiled for, 0 for any (a 25-tap drop alone holds 88
-	// VGPRs instead of the 115 every brush size needs, which leaves an erosion CU room for other
-	// kernels' waves)
+	// Taps: the brush size the drops are compiled for, 0 for any (a 25-tap drop alone holds 85
+	// VGPRs instead of the 115 every brush size needs)
 	template <unsigned int Threads, unsigned int Taps>
-	__global__ void __launch_bounds__(Threads) erodeTileKERNEL(const uint2* const tiles, const Chunk* const chunks,
+	__device__ __forceinline__ void erodeTile(const uint2* const tiles, const Chunk* const chunks,
 		const ErosionArgs args, const unsigned int tile, const unsigned int halo, const int2* const brush_offset,
 		const float* const brush_weight, const unsigned int* const tile_drops, const unsigned int* const tile_first,
 		const unsigned int* const bin, float* const height, float* const ledger, uint2* const drop_info,
@@ -1549,6 +1548,29 @@ namespace SuperTerrainPlus::STPBatch {
 			*status = 1u;
 		}
 	}
+
+#define STP_ERODE_TILE_PARAMS const uint2* const tiles, const Chunk* const chunks, const ErosionArgs args, \
+	const unsigned int tile, const unsigned int halo, const int2* const brush_offset, const float* const brush_weight, \
+	const unsigned int* const tile_drops, const unsigned int* const tile_first, const unsigned int* const bin, \
+	float* const height, float* const ledger, uint2* const drop_info, unsigned int* const status
+#define STP_ERODE_TILE_ARGS tiles, chunks, args, tile, halo, brush_offset, brush_weight, tile_drops, tile_first, bin, \
+	height, ledger, drop_info, status
+
+	template <unsigned int Threads, unsigned int Taps>
+	__global__ void __launch_bounds__(Threads) erodeTileKERNEL(STP_ERODE_TILE_PARAMS) {
+		erodeTile<Threads, Taps>(STP_ERODE_TILE_ARGS);
+	}
+
+	// 768 threads: two workgroups of a tile-72 window (78 KB each) share a CU only at six waves per
+	// SIMD, which caps a wave at 80 VGPRs (the 25-tap drop fits them)
+	template <unsigned int Taps>
+	__global__ void __launch_bounds__(768) __attribute__((amdgpu_waves_per_eu(6))) erodeTile768KERNEL(
+		STP_ERODE_TILE_PARAMS) {
+		erodeTile<768u, Taps>(STP_ERODE_TILE_ARGS);
+	}
+
+#undef STP_ERODE_TILE_ARGS
+#undef STP_ERODE_TILE_PARAMS
 
 	// chunk blockIdx.y's outputs -> out[chunk] (heights), out[chunks + chunk] (UINT16): its map
 	// without the Crop pixels of free-slip halo on every side, (Side - 2 Crop)^2 row-major
@@ -2066,12 +2088,20 @@ void STPHeightfieldBatch::releaseGraphs() noexcept {
 }
 
 namespace {
+	template <class Kernel>
+	void allowLds(Kernel* const kernel, const unsigned int bytes) {
+		STP_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+			hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+	}
 	template <unsigned int Threads>
 	void allowLds(const unsigned int bytes) {
-		STP_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&erodeTileKERNEL<Threads, 0u>),
-			hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
-		STP_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&erodeTileKERNEL<Threads, 25u>),
-			hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+		if constexpr (Threads == 768u) {
+			allowLds(&erodeTile768KERNEL<0u>, bytes);
+			allowLds(&erodeTile768KERNEL<25u>, bytes);
+		} else {
+			allowLds(&erodeTileKERNEL<Threads, 0u>, bytes);
+			allowLds(&erodeTileKERNEL<Threads, 25u>, bytes);
+		}
 	}
 } // namespace
 
@@ -2139,8 +2169,8 @@ STPBatchTuning STPHeightfieldBatch::latencyTuning() const {
 
 void STPHeightfieldBatch::setTuning(const STPBatchTuning& tuning) {
 	const unsigned int th = tuning.ErosionThreads;
-	if (th != 256u && th != 512u && th != 1024u) {
-		STP_THROW(STPInvalidArgument, "erosion threads must be 256, 512 or 1024");
+	if (th != 256u && th != 512u && th != 768u && th != 1024u) {
+		STP_THROW(STPInvalidArgument, "erosion threads must be 256, 512, 768 or 1024");
 	}
 	if (tuning.ErosionTile != 0u
 		&& (tuning.ErosionTile < 16u || tuning.ErosionTile > 4096u || this->erosionLdsBytes(tuning.ErosionTile) == 0u)) {
@@ -2176,6 +2206,9 @@ void STPHeightfieldBatch::buildTileTables(const hipStream_t stream) {
 			break;
 		case 512u:
 			allowLds<512u>(this->ErosionLds);
+			break;
+		case 768u:
+			allowLds<768u>(this->ErosionLds);
 			break;
 		default:
 			allowLds<1024u>(this->ErosionLds);
@@ -2267,8 +2300,12 @@ STPBatchTuning STPHeightfieldBatch::autotune(const hipStream_t stream, const uns
 		if (this->erosionLdsBytes(tile) == 0u) {
 			continue;
 		}
-		for (const unsigned int th : {256u, 512u, 1024u}) {
-			candidate.push_back(STPBatchTuning {tile, th});
+		for (const unsigned int th : {256u, 512u, 768u, 1024u}) {
+			// at 768 threads a drop gets 80 VGPRs, which the 25-tap drop fits and the drop of every
+			// brush size does not (31 registers spilled)
+			if (th != 768u || this->BrushSize == 25u) {
+				candidate.push_back(STPBatchTuning {tile, th});
+			}
 		}
 	}
 	if (candidate.empty()) {
@@ -2585,6 +2622,9 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 					break;
 				case 512u:
 					taps25 ? launch(erodeTileKERNEL<512u, 25u>, 512u) : launch(erodeTileKERNEL<512u, 0u>, 512u);
+					break;
+				case 768u:
+					taps25 ? launch(erodeTile768KERNEL<25u>, 768u) : launch(erodeTile768KERNEL<0u>, 768u);
 					break;
 				default:
 					taps25 ? launch(erodeTileKERNEL<1024u, 25u>, 1024u) : launch(erodeTileKERNEL<1024u, 0u>, 1024u);

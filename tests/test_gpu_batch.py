@@ -162,11 +162,12 @@ def test_batch_graph_replays_are_stable(gpu):
 # erosion launch shapes: the default LDS-tiled one (128-cell tiles: the 256-cell chunk has four,
 # the others one), the smallest tile the colour batches allow at lifetime 30 and r = 3 (72: the
 # 96- and 128-cell chunks have four tiles, the last ones cut short, windows clipped at the chunk
-# borders), and the global-atomic kernel
-TUNINGS = [None, (72, 256), (0, 256)]
+# borders), the same tile with 768-thread workgroups (the 25-tap kernel at six waves per SIMD), and
+# the global-atomic kernel
+TUNINGS = [None, (72, 256), (72, 768), (0, 256)]
 
 
-@pytest.mark.parametrize("tuning", TUNINGS, ids=["default", "tile72", "global"])
+@pytest.mark.parametrize("tuning", TUNINGS, ids=["default", "tile72", "tile72x768", "global"])
 def test_batch_erosion_against_serialized_oracle(gpu, oracle, tuning):
     """Per chunk: parallel erosion inside the batch vs the serialized FP32 oracle erosion of the
     same pre-erosion heightfield, same seed and stream id (= chunk index).
@@ -349,7 +350,7 @@ RAGGED_SIDES = [8, 9, 13, 37, 71, 73, 145]
 RAGGED_ORIGINS = [(-3, 5), (1000, -1000), (-4097, 33), (250, 250), (-71, -73), (7777, -1), (0, 145)]
 
 
-@pytest.mark.parametrize("tuning", [(72, 256), (0, 256)], ids=["tile72", "global"])
+@pytest.mark.parametrize("tuning", [(72, 256), (72, 768), (0, 256)], ids=["tile72", "tile72x768", "global"])
 def test_batch_ragged_sides(gpu, oracle, tuning):
     """Chunks of the smallest side the erosion brush allows (2r + 2 = 8 at r = 3), odd sides, and
     sides one below, one above and twice the erosion tile plus one (partial tiles on both edges):
