@@ -67,6 +67,7 @@ FP32_PEAK_TFLOPS = 157.3
 EROSION_PROFILES = {
     (128, 1024): {"valu_per_64_drop_steps": 187.3, "hbm_bytes_per_phase": 271.0e6},
     (72, 512): {"valu_per_64_drop_steps": 203.4, "hbm_bytes_per_phase": 443.2e6},
+    (72, 768): {"valu_per_64_drop_steps": 208.2, "hbm_bytes_per_phase": 443.2e6},
 }
 # serving loop: two engines of 8 Mpx / 512 chunks each
 SERVER_BATCH_PIXELS, SERVER_BATCH_CHUNKS, SERVER_ENGINES = 8 << 20, 512, 2
@@ -850,8 +851,14 @@ def erosion_roofline(tuning, stage_ms, counters):
     HIP-event time."""
     pixels, drops, steps, erode, deposit, killed = counters
     tiled = tuning.ErosionTile > 0
-    # the workload's brush radius 3 has 25 taps, for which the tiled kernel has an instantiation
-    kernel = f"erodeTileKERNEL<{tuning.ErosionThreads}, 25>" if tiled else "erosionKERNEL"
+    # the workload's brush radius 3 has 25 taps, for which the tiled kernels have an instantiation
+    # (768-thread workgroups run a kernel of their own)
+    if not tiled:
+        kernel = "erosionKERNEL"
+    elif tuning.ErosionThreads == 768:
+        kernel = "erodeTile768KERNEL<25>"
+    else:
+        kernel = f"erodeTileKERNEL<{tuning.ErosionThreads}, 25>"
     # the tiled erosion runs four colour phases (on this workload each phase is two concurrent
     # launches, one per chunk group and stream, each lasting about the phase); the global-atomic
     # kernel is one launch. Figures "per launch" below are per phase.
