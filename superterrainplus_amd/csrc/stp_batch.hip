@@ -2302,7 +2302,7 @@ STPBatchTuning STPHeightfieldBatch::autotune(const hipStream_t stream, const uns
 		}
 		for (const unsigned int th : {256u, 512u, 768u, 1024u}) {
 			// at 768 threads a drop gets 80 VGPRs, which the 25-tap drop fits and the drop of every
-			// brush size does not (31 registers spilled)
+			// brush size does not (it spills)
 			if (th != 768u || this->BrushSize == 25u) {
 				candidate.push_back(STPBatchTuning {tile, th});
 			}
@@ -2505,8 +2505,8 @@ void STPHeightfieldBatch::enqueueStage(const hipStream_t stream, const int stage
 						this->VCap, cap, vbin, this->BinCount.get(), this->BinItem.get(), this->BinWeight.get(), state,
 						this->Flag.get());
 				};
-				// slots loaded per column group: a window holds at most 4 biomes below 5, and a group
-				// holds 4 slots when vcap <= 4
+				// slots loaded per column group: with at most 4 biomes a window holds at most 4 bins,
+				// and a group holds 4 slots when vcap <= 4
 				const bool packed = this->packedBins();
 				if (this->Setting.BiomeCount <= 4u) {
 					if (packed) {
