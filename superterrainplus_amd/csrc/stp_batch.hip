@@ -1202,23 +1202,21 @@ namespace SuperTerrainPlus::STPBatch {
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
 		const unsigned int tilesX = (c.Side + BlendX - 1u) / BlendX;
-		// The workgroup covers 64 x 16 pixels; wave w (threadIdx.y) takes columns 16w .. 16w + 15,
-		// and iteration k its 16 x 4 block of rows 4k .. 4k + 3 (lane l: column l % 16, row l / 16).
-		// A wave runs each iteration for as long as its lane with the most bins: a 16 x 4 block
-		// straddles the edge of a biome boundary's band of 2-bin pixels less often than a 64-pixel
-		// row, which took 13% more bin iterations on the batched workload.
-		static_assert(BlendX == 64u && BlendY == 4u && BlendPixel == 4u, "the 16 x 4 wave blocks assume 64 x 16 tiles");
-		const unsigned int x = (t.y % tilesX) * BlendX + threadIdx.y * 16u + (threadIdx.x & 15u);
-		const unsigned int y0 = (t.y / tilesX) * (BlendY * BlendPixel) + (threadIdx.x >> 4u);
-		if (x >= c.Side) {
-			return;
-		}
+		// The workgroup covers 64 x 16 pixels; wave w (threadIdx.y) takes the 16 x 16 square of columns
+		// 16w .. 16w + 15, and iteration k its 8 x 8 quarter (k % 2, k / 2) (lane l: column l % 8, row
+		// l / 8). A wave runs each iteration for as long as its lane with the most bins: a compact
+		// block straddles the edge of a biome boundary's band of 2-bin pixels less often (a 64-pixel
+		// row took 13% more bin iterations than a 16 x 4 block on the batched workload, which took
+		// 1.2% more blend time than an 8 x 8 block).
+		static_assert(BlendX == 64u && BlendY == 4u && BlendPixel == 4u, "the 8 x 8 wave blocks assume 64 x 16 tiles");
+		const unsigned int xb = (t.y % tilesX) * BlendX + threadIdx.y * 16u + (threadIdx.x & 7u);
+		const unsigned int yb = (t.y / tilesX) * (BlendY * BlendPixel) + (threadIdx.x >> 3u);
 		const float half = static_cast<float>(c.Side * c.Step) / 2.0f;
 		// pixel x of the chunk samples world coordinate Origin + x * Step
 		const float2 offset = make_float2(static_cast<float>(c.OriginX) + half, static_cast<float>(c.OriginY) + half);
 		for (unsigned int k = 0u; k < BlendPixel; k++) {
-			const unsigned int y = y0 + 4u * k;
-			if (y < c.Side) {
+			const unsigned int x = xb + 8u * (k & 1u), y = yb + 8u * (k >> 1u);
+			if (x < c.Side && y < c.Side) {
 				const unsigned long long i = c.Pixel + static_cast<unsigned long long>(y) * c.Side + x;
 				if (Packed) {
 					// the horizontal pass's 1 / sum, the window (2r + 1)^2 being every pixel's sum
