@@ -65,8 +65,8 @@ FP32_PEAK_TFLOPS = 157.3
 # (drop steps / 64)), and HBM bytes per colour phase (FETCH_SIZE x 2 + WRITE_SIZE of its launches,
 # calibrated on the format kernel).
 EROSION_PROFILES = {
-    (128, 1024): {"valu_per_64_drop_steps": 187.3, "hbm_bytes_per_phase": 271.0e6},
-    (72, 512): {"valu_per_64_drop_steps": 203.4, "hbm_bytes_per_phase": 443.2e6},
+    (128, 1024): {"valu_per_64_drop_steps": 189.5, "hbm_bytes_per_phase": 271.0e6},
+    (72, 512): {"valu_per_64_drop_steps": 205.6, "hbm_bytes_per_phase": 443.2e6},
     (72, 768): {"valu_per_64_drop_steps": 208.2, "hbm_bytes_per_phase": 443.2e6},
 }
 # serving loop: two engines of 8 Mpx / 512 chunks each
