@@ -1450,6 +1450,7 @@ namespace SuperTerrainPlus::STPBatch {
 		const unsigned int* const bin, float* const height, float* const ledger, uint2* const drop_info,
 		unsigned int* const status) {
 		extern __shared__ int window[];
+		__shared__ int2 brush[MaxTiledTaps];
 		const uint2 t = tiles[blockIdx.x];
 		const Chunk c = chunks[t.x];
 		const unsigned int g = c.TileBase + t.y, n = tile_drops[g];
@@ -1491,26 +1492,19 @@ namespace SuperTerrainPlus::STPBatch {
 				}
 			}
 		}
+		for (unsigned int k = threadIdx.x; k < args.BrushSize; k += Threads) {
+			const int2 o = brush_offset[k];
+			brush[k] = make_int2(o.y * static_cast<int>(s.RW) + o.x, __float_as_int(brush_weight[k] * LocalSurface::WeightScale));
+		}
+		__syncthreads();
 		const ErosionArgs a = chunkArgs(args, c);
 		const unsigned int first = tile_first[g];
-		// results in tile order (consecutive lanes write consecutive words), within the chunk's range
-		if constexpr (Taps == 0u) {
-			// the brush as (window offset, weight) pairs; the shaped brushes of the other
-			// instantiations are compile-time, and without this array their window starts at LDS
-			// offset 0, within reach of the height reads' immediate offsets
-			__shared__ int2 brush[MaxTiledTaps];
-			for (unsigned int k = threadIdx.x; k < args.BrushSize; k += Threads) {
-				const int2 o = brush_offset[k];
-				brush[k] = make_int2(o.y * static_cast<int>(s.RW) + o.x, __float_as_int(brush_weight[k] * LocalSurface::WeightScale));
-			}
-			__syncthreads();
-			for (unsigned int k = threadIdx.x; k < n; k += Threads) {
+		for (unsigned int k = threadIdx.x; k < n; k += Threads) {
+			// results in tile order (consecutive lanes write consecutive words), within the chunk's range
+			if constexpr (Taps == 0u) {
 				erodeDropAnyBrush(s, a, brush, bin[first + k], first + k, ledger, drop_info);
-			}
-		} else {
-			__syncthreads();
-			for (unsigned int k = threadIdx.x; k < n; k += Threads) {
-				erodeDrop<LocalSurface, Taps>(s, a, nullptr, bin[first + k], first + k, ledger, drop_info);
+			} else {
+				erodeDrop<LocalSurface, Taps>(s, a, brush, bin[first + k], first + k, ledger, drop_info);
 			}
 		}
 		__syncthreads();
