@@ -486,6 +486,52 @@ def test_batch_tuning(gpu):
     assert np.isfinite(again).all() and abs(again.mean() - first.mean()) < 5e-3
 
 
+def test_batch_768_threads_any_brush(gpu):
+    """768-thread workgroups run a kernel capped at 80 VGPRs: the 25-tap drop fits it, the drop of
+    every other brush size spills, so the autotuner offers 768 threads only for the 25-tap brush.
+    With a 45-tap brush (r = 4) the autotuner never returns 768 threads, and the shape still runs
+    when set by hand: its ledger balances the heights and the status words stay clear."""
+    log("batch 768 threads with a 45-tap brush")
+    dpp = 0.25
+    rain = W.erosion_setting(1)
+    rain.ErosionBrushRadius = 4
+    setting = capi.BatchSetting(SEED, COUNT, CELL, RADIUS, 8, dpp)
+    chunks = (capi.BatchChunk * len(SIDES))(*[capi.BatchChunk(s, (C.c_int32 * 2)(*o)) for s, o in zip(SIDES, ORIGINS)])
+    pre_setting = capi.BatchSetting(SEED, COUNT, CELL, RADIUS, 8, 0.0)
+    handles = []
+    try:
+        for st in (pre_setting, setting):
+            h = C.c_void_p()
+            capi.call("stp_batch_create", C.byref(NOISE), capi.biome_array(props()), COUNT, C.byref(rain),
+                      C.byref(st), chunks, len(SIDES), C.byref(h))
+            handles.append(h)
+        pre, b = handles
+        out = capi.BatchTuning()
+        capi.call("stp_batch_autotune", b, 1, None, C.byref(out))
+        assert out.ErosionThreads != 768 and out.ErosionTile > 0
+        capi.call("stp_batch_set_tuning", b, C.byref(capi.BatchTuning(72, 768)))
+        for h in handles:
+            capi.call("stp_batch_run", h, 0, None)
+        capi.synchronize()
+        status = C.c_uint32(7)
+        capi.call("stp_batch_status", b, C.byref(status), None)
+        assert status.value == 0
+        for i, side in enumerate(SIDES):
+            before = np.zeros((side, side), np.float32)
+            after = np.zeros((side, side), np.float32)
+            capi.call("stp_batch_read_chunk", pre, i, capi.ptr(before), None, None, None)
+            capi.call("stp_batch_read_chunk", b, i, capi.ptr(after), None, None, None)
+            drops = int(side * side * dpp)
+            ledger = np.zeros(drops, np.float32)
+            capi.call("stp_batch_read_chunk", b, i, None, None, None, capi.ptr(ledger))
+            moved = float(ledger.astype(np.float64).sum())
+            assert np.isfinite(after).all(), i
+            assert abs((after.astype(np.float64).sum() - before.astype(np.float64).sum()) + moved) <= 2e-4 * moved + 1e-6, i
+    finally:
+        for h in handles:
+            capi.lib().stp_batch_destroy(h)
+
+
 def test_batch_tiled_needs_lifetime(gpu):
     """MaxLifetime = 0 (unlimited drops, the reference's behaviour) cannot be tiled: the batch
     starts on the global-atomic kernel and refuses tiled shapes."""
